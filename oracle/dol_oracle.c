@@ -1,0 +1,1069 @@
+/*
+ * dol_oracle.c -- CPU restatement of DistanceOcclusionLib's distance-field
+ * path.  TEST INFRASTRUCTURE ONLY (see dol_oracle.h): it exists so the HIP
+ * library can be checked, and so bench.py can time a CPU baseline.  Nothing
+ * in the product library links, loads or calls it.
+ *
+ * Every function names the reference lines it restates.  Where the reference
+ * relies on two's-complement wrap (int16 stores of out-of-range deltas,
+ * DistanceOcclusionLib.cpp:797-866; int32 overflow of the squared length in
+ * Closer, DistanceOcclusionLib.cpp:624-630), the wrap is done here in unsigned
+ * arithmetic so that it is defined behaviour with the same result.
+ */
+#include "dol_oracle.h"
+
+#include <float.h>
+#include <stddef.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------ utils */
+
+static inline int mask_stride(int w) { return (w + 31) >> 5; }
+
+/* bits of word j that are pixels (32j + i < w); ITER_SET_BITS_BEGIN's
+ * "break at x >= w" (DistanceOcclusionLib.cpp:56-63) is the same as masking. */
+static inline uint32_t valid_bits(int w, int j)
+{
+    int n = w - 32 * j;
+    if (n >= 32) return 0xffffffffu;
+    if (n <= 0) return 0u;
+    return (1u << n) - 1u;
+}
+
+static inline int16_t wrap16(int v) { return (int16_t)(uint16_t)(uint32_t)v; }
+
+int32_t oc_len2_2(const int16_t* c)
+{
+    uint32_t a = (uint32_t)((int32_t)c[0] * c[0]);
+    uint32_t b = (uint32_t)((int32_t)c[1] * c[1]);
+    return (int32_t)(a + b);
+}
+
+int32_t oc_len2_3(const int16_t* c)
+{
+    uint32_t a = (uint32_t)((int32_t)c[0] * c[0]);
+    uint32_t b = (uint32_t)((int32_t)c[1] * c[1]);
+    uint32_t e = (uint32_t)((int32_t)c[2] * c[2]);
+    return (int32_t)(a + b + e);
+}
+
+static inline int lowest_bit(uint32_t m) { return __builtin_ctz(m); }
+
+/* ------------------------------------------------------------- generators */
+
+/* LCG of DistanceOcclusionLib.cpp:28-32 */
+uint32_t oc_rng(uint32_t* seed, int limit)
+{
+    *seed = (uint32_t)((uint64_t)(*seed) * 1103515245u) + 12345u;
+    return (uint32_t)(((uint64_t)(*seed) * (uint64_t)(uint32_t)limit) >> 32);
+}
+
+/* DistanceOcclusionLib.cpp:95-113: draws x then y, ORs the bit in */
+void oc_mask_add_points_2d(int w, int h, int count, uint32_t seed, uint32_t* mask)
+{
+    int ws = mask_stride(w);
+    for (int n = 0; n < count; n++) {
+        int x = (int)oc_rng(&seed, w);
+        int y = (int)oc_rng(&seed, h);
+        mask[(size_t)y * ws + (x >> 5)] |= 1u << (x & 31);
+    }
+}
+
+/* DistanceOcclusionLib.cpp:1224-1244: draws x, y, z */
+void oc_mask_add_points_3d(int w, int h, int d, int count, uint32_t seed, uint32_t* mask)
+{
+    int ws = mask_stride(w);
+    for (int n = 0; n < count; n++) {
+        int x = (int)oc_rng(&seed, w);
+        int y = (int)oc_rng(&seed, h);
+        int z = (int)oc_rng(&seed, d);
+        mask[((size_t)z * h + y) * ws + (x >> 5)] |= 1u << (x & 31);
+    }
+}
+
+/* inclusive horizontal span [x0, x1] on rows [y0, y1]
+ * (FillMaskBlock, DistanceOcclusionLib.cpp:131-156).  The reference is only
+ * defined for w >= 2, h >= 3 (d >= 3): below that its block bounds go
+ * negative and it writes outside the mask, so negative bounds are clipped. */
+static void fill_span(int ws, int x0, int x1, int y0, int y1, uint32_t* mask)
+{
+    if (x0 < 0 || x1 < 0) return;
+    for (int y = y0 < 0 ? 0 : y0; y <= y1; y++) {
+        uint32_t* row = mask + (size_t)y * ws;
+        for (int x = x0; x <= x1; x++) row[x >> 5] |= 1u << (x & 31);
+    }
+}
+
+/* DistanceOcclusionLib.cpp:159-179 */
+void oc_mask_add_block_2d(int w, int h, int sides, uint32_t* mask)
+{
+    int ws = mask_stride(w);
+    int x0 = w / 4, x1 = x0 + w / 2 - 1;
+    int y0 = h / 3, y1 = y0 + h / 3 - 1;
+    if (sides > 4) { fill_span(ws, x0, x1, y0, y1, mask); return; }
+    if (sides > 0) fill_span(ws, x0, x1, y0, y0, mask);
+    if (sides > 1) fill_span(ws, x0, x0, y0, y1, mask);
+    if (sides > 2) fill_span(ws, x0, x1, y1, y1, mask);
+    if (sides > 3) fill_span(ws, x1, x1, y0, y1, mask);
+}
+
+static void fill_box3(int ws, int h, int x0, int x1, int y0, int y1, int z0, int z1,
+                      uint32_t* mask)
+{
+    for (int z = z0 < 0 ? 0 : z0; z <= z1; z++) fill_span(ws, x0, x1, y0, y1, mask + (size_t)z * ws * h);
+}
+
+/* DistanceOcclusionLib.cpp:1259-1287 */
+void oc_mask_add_block_3d(int w, int h, int d, int sides, uint32_t* mask)
+{
+    int ws = mask_stride(w);
+    int x0 = w / 4, x1 = x0 + w / 2 - 1;
+    int y0 = h / 3, y1 = y0 + h / 3 - 1;
+    int z0 = d / 3, z1 = z0 + d / 3 - 1;
+    if (sides > 6) { fill_box3(ws, h, x0, x1, y0, y1, z0, z1, mask); return; }
+    if (sides > 0) fill_box3(ws, h, x0, x1, y0, y1, z0, z0, mask);
+    if (sides > 1) fill_box3(ws, h, x0, x0, y0, y1, z0, z1, mask);
+    if (sides > 2) fill_box3(ws, h, x0, x1, y0, y0, z0, z1, mask);
+    if (sides > 3) fill_box3(ws, h, x1, x1, y0, y1, z0, z1, mask);
+    if (sides > 4) fill_box3(ws, h, x0, x1, y1, y1, z0, z1, mask);
+    if (sides > 5) fill_box3(ws, h, x0, x1, y0, y1, z1, z1, mask);
+}
+
+/* BitMaskFromImageFile's threshold, DistanceOcclusionTool.cpp:646-665:
+ * occupied iff (v >= threshold) XOR lessThan; padding bits are zero. */
+void oc_mask_from_bytes(int w, int rows, const uint8_t* bytes, int threshold, int less_than,
+                        uint32_t* mask)
+{
+    int ws = mask_stride(w);
+    for (int y = 0; y < rows; y++)
+        for (int j = 0; j < ws; j++) {
+            uint32_t m = 0;
+            for (int i = 0; i < 32 && 32 * j + i < w; i++) {
+                int v = bytes[(size_t)y * w + 32 * j + i];
+                if ((v >= threshold) != (less_than != 0)) m |= 1u << i;
+            }
+            mask[(size_t)y * ws + j] = m;
+        }
+}
+
+/* ---------------------------------------------------------- initialisation */
+
+/* DistanceOcclusionLib.cpp:224-254 (2D) */
+void oc_init_deltas_2d(int w, int h, const uint32_t* mask, int16_t* delta, int invert)
+{
+    int ws = mask_stride(w);
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            int bit = (mask[(size_t)y * ws + (x >> 5)] >> (x & 31)) & 1;
+            int16_t v = (bit ^ (invert != 0)) ? 0 : OC_MAX_DELTA2;
+            int16_t* c = delta + 2 * ((size_t)y * w + x);
+            c[0] = v; c[1] = v;
+        }
+}
+
+/* DistanceOcclusionLib.cpp:1289-1321 (3D) */
+void oc_init_deltas_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta, int invert)
+{
+    int ws = mask_stride(w);
+    for (int z = 0; z < d; z++)
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                size_t r = (size_t)z * h + y;
+                int bit = (mask[r * ws + (x >> 5)] >> (x & 31)) & 1;
+                int16_t v = (bit ^ (invert != 0)) ? 0 : OC_MAX_DELTA3;
+                int16_t* c = delta + 3 * (r * w + x);
+                c[0] = v; c[1] = v; c[2] = v;
+            }
+}
+
+/* InitFromBitMask<T>, DistanceOcclusionLib.cpp:183-222 */
+#define INIT_DIST_BODY(T)                                                           \
+    int ws = mask_stride(w);                                                        \
+    for (int y = 0; y < h; y++)                                                     \
+        for (int x = 0; x < w; x++) {                                               \
+            int bit = (mask[(size_t)y * ws + (x >> 5)] >> (x & 31)) & 1;            \
+            dist[(size_t)y * w + x] = bit ? (T)0 : max_d;                           \
+        }
+
+void oc_init_distances_f32(int w, int h, const uint32_t* mask, float* dist, float max_d)
+{ INIT_DIST_BODY(float) }
+void oc_init_distances_i32(int w, int h, const uint32_t* mask, int32_t* dist, int32_t max_d)
+{ INIT_DIST_BODY(int32_t) }
+void oc_init_distances_i16(int w, int h, const uint32_t* mask, int16_t* dist, int16_t max_d)
+{ INIT_DIST_BODY(int16_t) }
+
+/* ------------------------------------------------------ border init (2D) */
+
+static inline void put2(int16_t* c, int x, int y) { c[0] = (int16_t)x; c[1] = (int16_t)y; }
+
+/* Combo border initialisation, DistanceOcclusionLib.cpp:504-599 and 604-619.
+ * Row pairs (y, y+1) in order; per 32-bit word three passes over set bits in
+ * ascending order: corners (written only while the cell is still unset, i.e.
+ * x == kMaxDelta2), then horizontal-change cells, then vertical-change
+ * cells.  Later writes win, which is what decides ties.  The last row then
+ * gets its horizontal changes only. */
+void oc_border_2d(int w, int h, const uint32_t* mask, int16_t* delta)
+{
+    const int ws = mask_stride(w);
+    for (size_t i = 0; i < (size_t)w * h; i++) put2(delta + 2 * i, OC_MAX_DELTA2, OC_MAX_DELTA2);
+    if (w <= 0 || h <= 0) return;
+
+    for (int y = 0; y + 1 < h; y++) {
+        const uint32_t* ra = mask + (size_t)y * ws;
+        const uint32_t* rb = ra + ws;
+        int16_t* da = delta + 2 * (size_t)y * w;
+        int16_t* db = da + 2 * (size_t)w;
+        uint32_t carry_a = ra[0] & 1u, carry_b = rb[0] & 1u;   /* x = -1 replicates x = 0 */
+
+        for (int j = 0; j < ws; j++) {
+            const uint32_t a = ra[j], b = rb[j];
+            const uint32_t a_prev = (a << 1) | carry_a;          /* pixel x-1 of row y   */
+            const uint32_t b_prev = (b << 1) | carry_b;          /* pixel x-1 of row y+1 */
+            carry_a = a >> 31;
+            carry_b = b >> 31;
+            const uint32_t lim = valid_bits(w, j);
+            const uint32_t vert = a ^ b;                 /* change between rows at x   */
+            const uint32_t horz = a ^ a_prev;            /* change x-1 -> x on row y   */
+            const uint32_t vprev = a_prev ^ b_prev;      /* change between rows at x-1 */
+            uint32_t corner = (vert ^ vprev) & lim;
+            uint32_t hbits = horz & lim;
+            uint32_t vbits = vert & lim;
+            const int base = 32 * j;
+
+            while (corner) {
+                int i = lowest_bit(corner); corner &= corner - 1;
+                int dx = (vprev >> i) & 1, dy = (horz >> i) & 1;
+                int16_t* c = (dy ? db : da) + 2 * (base + i + dx - 1);
+                if (c[0] == OC_MAX_DELTA2) put2(c, 1 - 2 * dx, 1 - 2 * dy);
+            }
+            while (hbits) {
+                int i = lowest_bit(hbits); hbits &= hbits - 1;
+                put2(da + 2 * (base + i - 1), +1, 0);
+                put2(da + 2 * (base + i), -1, 0);
+            }
+            while (vbits) {
+                int i = lowest_bit(vbits); vbits &= vbits - 1;
+                put2(da + 2 * (base + i), 0, +1);
+                put2(db + 2 * (base + i), 0, -1);
+            }
+        }
+    }
+
+    /* last row: horizontal changes only (DistanceOcclusionLib.cpp:571-598) */
+    {
+        const uint32_t* r = mask + (size_t)(h - 1) * ws;
+        int16_t* dr = delta + 2 * (size_t)(h - 1) * w;
+        uint32_t carry = r[0] & 1u;
+        for (int j = 0; j < ws; j++) {
+            uint32_t a = r[j];
+            uint32_t hbits = (a ^ ((a << 1) | carry)) & valid_bits(w, j);
+            carry = a >> 31;
+            while (hbits) {
+                int i = lowest_bit(hbits); hbits &= hbits - 1;
+                put2(dr + 2 * (32 * j + i - 1), +1, 0);
+                put2(dr + 2 * (32 * j + i), -1, 0);
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------ border init (3D) */
+
+static inline void put3(int16_t* c, int x, int y, int z)
+{
+    c[0] = (int16_t)x; c[1] = (int16_t)y; c[2] = (int16_t)z;
+}
+
+/* face and corner writes may replace an unset cell or a diagonal one */
+static inline int replaceable3(const int16_t* c)
+{
+    return c[0] == OC_MAX_DELTA3 || (abs(c[0]) + abs(c[1]) + abs(c[2]) > 1);
+}
+
+/* Combo border initialisation, DistanceOcclusionLib.cpp:1820-2150 and
+ * 2154-2173.  Slice pairs (z, z+1) in order; within each, row pairs (y, y+1);
+ * per 32-bit word, passes in the order: box corners (only while unset),
+ * x-face diagonals, y-face diagonals, z-face diagonals (each replacing unset
+ * or diagonal cells), then the x, y and z edge cells (unconditional).  The
+ * last row of each slice pair gets y-face diagonals and x/z edges; the last
+ * slice gets z-face diagonals and x/y edges, and its last row x edges. */
+void oc_border_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta)
+{
+    const int ws = mask_stride(w);
+    const size_t slice_words = (size_t)ws * h;
+    const size_t slice_cells = (size_t)w * h;
+    for (size_t i = 0; i < slice_cells * d; i++)
+        put3(delta + 3 * i, OC_MAX_DELTA3, OC_MAX_DELTA3, OC_MAX_DELTA3);
+    if (w <= 0 || h <= 0 || d <= 0) return;
+
+#define CELL3(X, Y, Z) (delta + 3 * ((size_t)(Z) * slice_cells + (size_t)(Y) * w + (X)))
+
+    for (int z = 0; z + 1 < d; z++) {
+        const uint32_t* s0 = mask + (size_t)z * slice_words;
+        const uint32_t* s1 = s0 + slice_words;
+
+        for (int y = 0; y + 1 < h; y++) {
+            const uint32_t* r00 = s0 + (size_t)y * ws;      /* (y,   z)   */
+            const uint32_t* r01 = r00 + ws;                 /* (y+1, z)   */
+            const uint32_t* r10 = s1 + (size_t)y * ws;      /* (y,   z+1) */
+            const uint32_t* r11 = r10 + ws;                 /* (y+1, z+1) */
+            uint32_t c00 = r00[0] & 1u, c01 = r01[0] & 1u, c10 = r10[0] & 1u, c11 = r11[0] & 1u;
+
+            for (int j = 0; j < ws; j++) {
+                /* suffix 1 = pixel x, suffix 0 = pixel x-1 */
+                const uint32_t p001 = r00[j], p011 = r01[j], p101 = r10[j], p111 = r11[j];
+                const uint32_t p000 = (p001 << 1) | c00, p010 = (p011 << 1) | c01;
+                const uint32_t p100 = (p101 << 1) | c10, p110 = (p111 << 1) | c11;
+                c00 = p001 >> 31; c01 = p011 >> 31; c10 = p101 >> 31; c11 = p111 >> 31;
+
+                const uint32_t ex = p000 ^ p001, ey = p000 ^ p010, ez = p000 ^ p100;
+                const uint32_t ezx = p001 ^ p101, eyz = p100 ^ p110, exy = p010 ^ p011;
+                const uint32_t fx = ey ^ eyz, fy = ez ^ ezx, fz = ex ^ exy;
+                const uint32_t eyx = p001 ^ p011, eyxz = p101 ^ p111;
+                const uint32_t fxx = eyx ^ eyxz;
+                const uint32_t bc = fz ^ (eyz ^ eyxz);
+                const uint32_t lim = valid_bits(w, j);
+                const int base = 32 * j;
+                uint32_t m;
+
+                m = bc & lim;
+                while (m) {
+                    int i = lowest_bit(m); m &= m - 1;
+                    int dx = (fx >> i) & 1, dy = (fy >> i) & 1, dz = (fz >> i) & 1;
+                    int16_t* c = CELL3(base + i - 1 + dx, y + dy, z + dz);
+                    if (c[0] == OC_MAX_DELTA3) put3(c, 1 - 2 * dx, 1 - 2 * dy, 1 - 2 * dz);
+                }
+                m = fxx & lim;
+                while (m) {
+                    int i = lowest_bit(m); m &= m - 1;
+                    int dy = (ezx >> i) & 1, dz = (eyx >> i) & 1;
+                    int16_t* c = CELL3(base + i, y + dy, z + dz);
+                    if (replaceable3(c)) put3(c, 0, 1 - 2 * dy, 1 - 2 * dz);
+                }
+                m = fy & lim;
+                while (m) {
+                    int i = lowest_bit(m); m &= m - 1;
+                    int dx = (ez >> i) & 1, dz = (ex >> i) & 1;
+                    int16_t* c = CELL3(base + i - 1 + dx, y, z + dz);
+                    if (replaceable3(c)) put3(c, 1 - 2 * dx, 0, 1 - 2 * dz);
+                }
+                m = fz & lim;
+                while (m) {
+                    int i = lowest_bit(m); m &= m - 1;
+                    int dx = (ey >> i) & 1, dy = (ex >> i) & 1;
+                    int16_t* c = CELL3(base + i - 1 + dx, y + dy, z);
+                    if (replaceable3(c)) put3(c, 1 - 2 * dx, 1 - 2 * dy, 0);
+                }
+                m = ex & lim;
+                while (m) {
+                    int i = lowest_bit(m); m &= m - 1;
+                    put3(CELL3(base + i - 1, y, z), +1, 0, 0);
+                    put3(CELL3(base + i, y, z), -1, 0, 0);
+                }
+                m = eyx & lim;
+                while (m) {
+                    int i = lowest_bit(m); m &= m - 1;
+                    put3(CELL3(base + i, y, z), 0, +1, 0);
+                    put3(CELL3(base + i, y + 1, z), 0, -1, 0);
+                }
+                m = ezx & lim;
+                while (m) {
+                    int i = lowest_bit(m); m &= m - 1;
+                    put3(CELL3(base + i, y, z), 0, 0, +1);
+                    put3(CELL3(base + i, y, z + 1), 0, 0, -1);
+                }
+            }
+        }
+
+        /* last row of the slice pair (DistanceOcclusionLib.cpp:1993-2050) */
+        {
+            const int y = h - 1;
+            const uint32_t* r00 = s0 + (size_t)y * ws;
+            const uint32_t* r10 = s1 + (size_t)y * ws;
+            uint32_t c00 = r00[0] & 1u, c10 = r10[0] & 1u;
+            for (int j = 0; j < ws; j++) {
+                const uint32_t p001 = r00[j], p101 = r10[j];
+                const uint32_t p000 = (p001 << 1) | c00, p100 = (p101 << 1) | c10;
+                c00 = p001 >> 31; c10 = p101 >> 31;
+                const uint32_t ex = p000 ^ p001, ez = p000 ^ p100, ezx = p001 ^ p101;
+                const uint32_t fy = ez ^ ezx;
+                const uint32_t lim = valid_bits(w, j);
+                const int base = 32 * j;
+                uint32_t m;
+
+                m = fy & lim;
+                while (m) {
+                    int i = lowest_bit(m); m &= m - 1;
+                    int dx = (ez >> i) & 1, dz = (ex >> i) & 1;
+                    int16_t* c = CELL3(base + i - 1 + dx, y, z + dz);
+                    if (replaceable3(c)) put3(c, 1 - 2 * dx, 0, 1 - 2 * dz);
+                }
+                m = ex & lim;
+                while (m) {
+                    int i = lowest_bit(m); m &= m - 1;
+                    put3(CELL3(base + i - 1, y, z), +1, 0, 0);
+                    put3(CELL3(base + i, y, z), -1, 0, 0);
+                }
+                m = ezx & lim;
+                while (m) {
+                    int i = lowest_bit(m); m &= m - 1;
+                    put3(CELL3(base + i, y, z), 0, 0, +1);
+                    put3(CELL3(base + i, y, z + 1), 0, 0, -1);
+                }
+            }
+        }
+    }
+
+    /* last slice (DistanceOcclusionLib.cpp:2053-2149) */
+    {
+        const int z = d - 1;
+        const uint32_t* s0 = mask + (size_t)z * slice_words;
+        for (int y = 0; y + 1 < h; y++) {
+            const uint32_t* r00 = s0 + (size_t)y * ws;
+            const uint32_t* r01 = r00 + ws;
+            uint32_t c00 = r00[0] & 1u, c01 = r01[0] & 1u;
+            for (int j = 0; j < ws; j++) {
+                const uint32_t p001 = r00[j], p011 = r01[j];
+                const uint32_t p000 = (p001 << 1) | c00, p010 = (p011 << 1) | c01;
+                c00 = p001 >> 31; c01 = p011 >> 31;
+                const uint32_t ex = p000 ^ p001, ey = p000 ^ p010, eyx = p001 ^ p011;
+                const uint32_t fz = ey ^ eyx;
+                const uint32_t lim = valid_bits(w, j);
+                const int base = 32 * j;
+                uint32_t m;
+
+                m = fz & lim;
+                while (m) {
+                    int i = lowest_bit(m); m &= m - 1;
+                    int dx = (ey >> i) & 1, dy = (ex >> i) & 1;
+                    int16_t* c = CELL3(base + i - 1 + dx, y + dy, z);
+                    if (replaceable3(c)) put3(c, 1 - 2 * dx, 1 - 2 * dy, 0);
+                }
+                m = ex & lim;
+                while (m) {
+                    int i = lowest_bit(m); m &= m - 1;
+                    put3(CELL3(base + i - 1, y, z), +1, 0, 0);
+                    put3(CELL3(base + i, y, z), -1, 0, 0);
+                }
+                m = eyx & lim;
+                while (m) {
+                    int i = lowest_bit(m); m &= m - 1;
+                    put3(CELL3(base + i, y, z), 0, +1, 0);
+                    put3(CELL3(base + i, y + 1, z), 0, -1, 0);
+                }
+            }
+        }
+        {
+            const int y = h - 1;
+            const uint32_t* r00 = s0 + (size_t)y * ws;
+            uint32_t c00 = r00[0] & 1u;
+            for (int j = 0; j < ws; j++) {
+                const uint32_t p001 = r00[j];
+                const uint32_t ex = ((p001 << 1) | c00) ^ p001;
+                c00 = p001 >> 31;
+                uint32_t m = ex & valid_bits(w, j);
+                while (m) {
+                    int i = lowest_bit(m); m &= m - 1;
+                    put3(CELL3(32 * j + i - 1, y, z), +1, 0, 0);
+                    put3(CELL3(32 * j + i, y, z), -1, 0, 0);
+                }
+            }
+        }
+    }
+#undef CELL3
+}
+
+/* ------------------------------------------------------------ jump flood */
+
+/* FloorPow2, DistanceOcclusionLib.cpp:875-884 */
+int oc_floor_pow2(int v)
+{
+    uint32_t u = (uint32_t)v;
+    u |= u >> 1; u |= u >> 2; u |= u >> 4; u |= u >> 8; u |= u >> 16;
+    return (int)((int32_t)(u + 1u) >> 1);
+}
+
+/* one 2D pass, DistanceOcclusionLib.cpp:782-873.  Tap order: self; (-s,0),
+ * (-s,-s), (-s,+s) when x-s >= 0; (+s,0), (+s,-s), (+s,+s) when x+s < w;
+ * (0,-s); (0,+s).  Strict ">" keeps the earliest of equal candidates; no
+ * candidate is skipped, and candidates wrap to int16. */
+static void jfa2_rows(int step, int w, int h, const int16_t* in, int16_t* out, int cw,
+                      int y0, int y1)
+{
+    const int off = cw * step;
+    for (int y = y0; y < y1; y++) {
+        for (int x = 0; x < w; x++) {
+            int16_t best[2];
+            const int16_t* self = in + 2 * ((size_t)y * w + x);
+            best[0] = self[0]; best[1] = self[1];
+            int32_t bl = oc_len2_2(best);
+#define JFA2_TAP(DX, DY)                                                          \
+            {                                                                     \
+                const int16_t* s = in + 2 * ((size_t)(y + (DY) * step) * w + x + (DX) * step); \
+                int16_t c[2] = { wrap16(s[0] + (DX) * off), wrap16(s[1] + (DY) * off) }; \
+                int32_t cl = oc_len2_2(c);                                        \
+                if (bl > cl) { best[0] = c[0]; best[1] = c[1]; bl = cl; }         \
+            }
+            if (x - step >= 0) {
+                JFA2_TAP(-1, 0)
+                if (y - step >= 0) JFA2_TAP(-1, -1)
+                if (y + step < h) JFA2_TAP(-1, +1)
+            }
+            if (x + step < w) {
+                JFA2_TAP(+1, 0)
+                if (y - step >= 0) JFA2_TAP(+1, -1)
+                if (y + step < h) JFA2_TAP(+1, +1)
+            }
+            if (y - step >= 0) JFA2_TAP(0, -1)
+            if (y + step < h) JFA2_TAP(0, +1)
+#undef JFA2_TAP
+            int16_t* o = out + 2 * ((size_t)y * w + x);
+            o[0] = best[0]; o[1] = best[1];
+        }
+    }
+}
+
+void oc_jfa_pass_2d(int step, int w, int h, const int16_t* in, int16_t* out, int cw)
+{
+    jfa2_rows(step, w, h, in, out, cw, 0, h);
+}
+
+static void jfa2_pass_mt(int step, int w, int h, const int16_t* in, int16_t* out, int cw,
+                         int threads)
+{
+#ifdef _OPENMP
+    if (threads > 1) {
+        int chunk = (h + threads - 1) / threads;
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (int t = 0; t < threads; t++) {
+            int y0 = t * chunk, y1 = y0 + chunk < h ? y0 + chunk : h;
+            if (y0 < y1) jfa2_rows(step, w, h, in, out, cw, y0, y1);
+        }
+        return;
+    }
+#endif
+    (void)threads;
+    jfa2_rows(step, w, h, in, out, cw, 0, h);
+}
+
+/* driver, DistanceOcclusionLib.cpp:887-908: steps FloorPow2(max(w,h)-1) down
+ * to 1, ping-ponging; an odd pass count ends with one more step-1 pass back
+ * into the caller's buffer. */
+void oc_jfa_2d_mt(int w, int h, int16_t* delta, int cw, int threads)
+{
+    size_t n = (size_t)w * h;
+    int16_t* tmp = (int16_t*)malloc(n * 2 * sizeof(int16_t) + 4);
+    int16_t* a = delta;
+    int16_t* b = tmp;
+    for (int step = oc_floor_pow2((w >= h ? w : h) - 1); step > 0; step /= 2) {
+        jfa2_pass_mt(step, w, h, a, b, cw, threads);
+        int16_t* t = a; a = b; b = t;
+    }
+    if (a != delta) jfa2_pass_mt(1, w, h, a, delta, cw, threads);
+    free(tmp);
+}
+
+void oc_jfa_2d(int w, int h, int16_t* delta, int cw) { oc_jfa_2d_mt(w, h, delta, cw, 1); }
+
+/* one 3D pass, DistanceOcclusionLib.cpp:2398-2447: 27 taps, loops z offset
+ * (outer), y, x (inner), each over {-s, 0, +s} clipped at the volume edge;
+ * a source whose x is kMaxDelta3 is skipped; strict ">". */
+static void jfa3_slices(int step, int w, int h, int d, const int16_t* in, int16_t* out, int cw,
+                        int z0, int z1)
+{
+    const size_t sl = (size_t)w * h;
+    for (int z = z0; z < z1; z++) {
+        int kc0 = (z - step) >= 0 ? -step : 0, kc1 = (z + step) < d ? step : 0;
+        for (int y = 0; y < h; y++) {
+            int jc0 = (y - step) >= 0 ? -step : 0, jc1 = (y + step) < h ? step : 0;
+            for (int x = 0; x < w; x++) {
+                int ic0 = (x - step) >= 0 ? -step : 0, ic1 = (x + step) < w ? step : 0;
+                const size_t idx = (size_t)z * sl + (size_t)y * w + x;
+                int16_t best[3] = { in[3 * idx], in[3 * idx + 1], in[3 * idx + 2] };
+                int32_t bl = oc_len2_3(best);
+                for (int kc = kc0; kc <= kc1; kc += step)
+                    for (int jc = jc0; jc <= jc1; jc += step)
+                        for (int ic = ic0; ic <= ic1; ic += step) {
+                            const int16_t* s = in + 3 * (idx + (ptrdiff_t)kc * (ptrdiff_t)sl
+                                                         + (ptrdiff_t)jc * w + ic);
+                            if (s[0] == OC_MAX_DELTA3) continue;
+                            int16_t c[3] = { wrap16(s[0] + cw * ic), wrap16(s[1] + cw * jc),
+                                             wrap16(s[2] + cw * kc) };
+                            int32_t cl = oc_len2_3(c);
+                            if (bl > cl) { best[0] = c[0]; best[1] = c[1]; best[2] = c[2]; bl = cl; }
+                        }
+                out[3 * idx] = best[0]; out[3 * idx + 1] = best[1]; out[3 * idx + 2] = best[2];
+            }
+        }
+    }
+}
+
+void oc_jfa_pass_3d(int step, int w, int h, int d, const int16_t* in, int16_t* out, int cw)
+{
+    jfa3_slices(step, w, h, d, in, out, cw, 0, d);
+}
+
+static void jfa3_pass_mt(int step, int w, int h, int d, const int16_t* in, int16_t* out, int cw,
+                         int threads)
+{
+#ifdef _OPENMP
+    if (threads > 1) {
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+        for (int z = 0; z < d; z++) jfa3_slices(step, w, h, d, in, out, cw, z, z + 1);
+        return;
+    }
+#endif
+    (void)threads;
+    jfa3_slices(step, w, h, d, in, out, cw, 0, d);
+}
+
+/* driver, DistanceOcclusionLib.cpp:2450-2472 */
+void oc_jfa_3d_mt(int w, int h, int d, int16_t* delta, int cw, int threads)
+{
+    size_t n = (size_t)w * h * d;
+    int16_t* tmp = (int16_t*)malloc(n * 3 * sizeof(int16_t) + 8);
+    int16_t* a = delta;
+    int16_t* b = tmp;
+    int m = w >= h ? w : h;
+    m = m >= d ? m : d;
+    for (int step = oc_floor_pow2(m - 1); step > 0; step /= 2) {
+        jfa3_pass_mt(step, w, h, d, a, b, cw, threads);
+        int16_t* t = a; a = b; b = t;
+    }
+    if (a != delta) jfa3_pass_mt(1, w, h, d, a, delta, cw, threads);
+    free(tmp);
+}
+
+void oc_jfa_3d(int w, int h, int d, int16_t* delta, int cw) { oc_jfa_3d_mt(w, h, d, delta, cw, 1); }
+
+/* ------------------------------------------------------------ Felzenszwalb */
+
+/* 1D lower envelope of parabolas in float, DistanceOcclusionLib.cpp:1026-1062.
+ * Same operation order as the reference so the float results are identical:
+ * q*q and 2*(q-v) are formed in int and converted, the envelope pops while
+ * s <= z[k], and the scan advances while z[k+1] < q. */
+static void felz_1d(int n, size_t m, const float* f, float* out, int* v, float* z)
+{
+    int k = 0;
+    v[0] = 0;
+    z[0] = -FLT_MAX;
+    z[1] = +FLT_MAX;
+    for (int q = 1; q < n; q++) {
+        float fq = f[m * q] + (float)(q * q);
+        float fv = f[m * v[k]] + (float)(v[k] * v[k]);
+        float s = (fq - fv) / (float)(2 * (q - v[k]));
+        while (s <= z[k]) {
+            k--;
+            fv = f[m * v[k]] + (float)(v[k] * v[k]);
+            s = (fq - fv) / (float)(2 * (q - v[k]));
+        }
+        k++;
+        v[k] = q;
+        z[k] = s;
+        z[k + 1] = +FLT_MAX;
+    }
+    k = 0;
+    for (int q = 0; q < n; q++) {
+        while (z[k + 1] < (float)q) k++;
+        out[q] = (float)((q - v[k]) * (q - v[k])) + f[m * v[k]];
+    }
+}
+
+/* column pass into a transposed temporary, then row pass back
+ * (DistanceOcclusionLib.cpp:1065-1085) */
+void oc_felzenszwalb_2d(int w, int h, float* dist)
+{
+    if (w <= 0 || h <= 0) return;
+    int n = w >= h ? w : h;
+    float* td = (float*)malloc(sizeof(float) * (size_t)w * h);
+    int* v = (int*)malloc(sizeof(int) * (size_t)n);
+    float* z = (float*)malloc(sizeof(float) * ((size_t)n + 1));
+    for (int x = 0; x < w; x++) felz_1d(h, (size_t)w, dist + x, td + (size_t)x * h, v, z);
+    for (int y = 0; y < h; y++) felz_1d(w, (size_t)h, td + y, dist + (size_t)y * w, v, z);
+    free(td); free(v); free(z);
+}
+
+/* ------------------------------------------------------------ brute force */
+
+/* DistanceOcclusionLib.cpp:1088-1110: every boundary cell (|x| < cw and
+ * |y| < cw), in raster order, relaxes every cell in place. */
+void oc_brute_force_2d(int w, int h, int16_t* delta, int cw)
+{
+    for (int i = 0; i < h; i++)
+        for (int j = 0; j < w; j++) {
+            int16_t* cell = delta + 2 * ((size_t)i * w + j);
+            if (abs(cell[0]) >= cw || abs(cell[1]) >= cw) continue;
+            for (int y = 0; y < h; y++)
+                for (int x = 0; x < w; x++) {
+                    int16_t cand[2] = { wrap16((j - x) * cw + cell[0]), wrap16((i - y) * cw + cell[1]) };
+                    int16_t* c2 = delta + 2 * ((size_t)y * w + x);
+                    if (oc_len2_2(c2) > oc_len2_2(cand)) { c2[0] = cand[0]; c2[1] = cand[1]; }
+                }
+        }
+}
+
+/* DistanceOcclusionLib.cpp:2476-2505: as 2D, but the candidate is the plain
+ * cell offset -- cw and the boundary cell's own delta are not used. */
+void oc_brute_force_3d(int w, int h, int d, int16_t* delta, int cw)
+{
+    for (int k = 0; k < d; k++)
+        for (int j = 0; j < h; j++)
+            for (int i = 0; i < w; i++) {
+                const int16_t* cell = delta + 3 * (((size_t)k * h + j) * w + i);
+                if (abs(cell[0]) >= cw || abs(cell[1]) >= cw || abs(cell[2]) >= cw) continue;
+                for (int z = 0; z < d; z++)
+                    for (int y = 0; y < h; y++)
+                        for (int x = 0; x < w; x++) {
+                            int16_t* c2 = delta + 3 * (((size_t)z * h + y) * w + x);
+                            int32_t d0 = (i - x) * (i - x) + (j - y) * (j - y) + (k - z) * (k - z);
+                            if (d0 < oc_len2_3(c2)) put3(c2, i - x, j - y, k - z);
+                        }
+            }
+}
+
+/* ------------------------------------------------- exact squared EDT (ext.) */
+
+/* Not in the reference (which only has the float 2D form above).  The
+ * textbook separable transform in exact 64-bit integer arithmetic: per-row
+ * nearest seed distance, then the lower envelope of parabolas along each
+ * further axis with intersections compared by cross-multiplication.
+ * Values of INT64_MAX mean "no seed on this line". */
+#define EDT_NONE INT64_MAX
+
+static void edt_envelope_1d(int n, const int64_t* f, size_t fs, int64_t* out, size_t os,
+                            int* v)
+{
+    int k = -1;
+    for (int q = 0; q < n; q++) {
+        int64_t fq = f[fs * q];
+        if (fq == EDT_NONE) continue;
+        int64_t Fq = fq + (int64_t)q * q;
+        while (k >= 1) {
+            int a = v[k - 1], b = v[k];
+            int64_t Fa = f[fs * a] + (int64_t)a * a, Fb = f[fs * b] + (int64_t)b * b;
+            /* pop b when its interval is empty: s(b,q) <= s(a,b) */
+            if ((Fq - Fb) * (int64_t)(b - a) <= (Fb - Fa) * (int64_t)(q - b)) k--;
+            else break;
+        }
+        v[++k] = q;
+    }
+    if (k < 0) {
+        for (int q = 0; q < n; q++) out[os * q] = EDT_NONE;
+        return;
+    }
+    int p = 0;
+    for (int q = 0; q < n; q++) {
+        for (;;) {
+            if (p + 1 > k) break;
+            int64_t c0 = f[fs * v[p]] + (int64_t)(q - v[p]) * (q - v[p]);
+            int64_t c1 = f[fs * v[p + 1]] + (int64_t)(q - v[p + 1]) * (q - v[p + 1]);
+            if (c1 <= c0) p++;
+            else break;
+        }
+        out[os * q] = f[fs * v[p]] + (int64_t)(q - v[p]) * (q - v[p]);
+    }
+}
+
+static void edt_rows(int w, size_t rows, const uint32_t* mask, int invert, int64_t* g)
+{
+    int ws = mask_stride(w);
+    for (size_t r = 0; r < rows; r++) {
+        const uint32_t* row = mask + r * ws;
+        int64_t* gr = g + r * w;
+        int last = -1;
+        for (int x = 0; x < w; x++) {
+            int bit = ((row[x >> 5] >> (x & 31)) & 1) ^ (invert != 0);
+            if (bit) last = x;
+            gr[x] = last < 0 ? EDT_NONE : (int64_t)(x - last);
+        }
+        last = -1;
+        for (int x = w - 1; x >= 0; x--) {
+            int bit = ((row[x >> 5] >> (x & 31)) & 1) ^ (invert != 0);
+            if (bit) last = x;
+            if (last >= 0 && (gr[x] == EDT_NONE || gr[x] > last - x)) gr[x] = last - x;
+        }
+        for (int x = 0; x < w; x++)
+            if (gr[x] != EDT_NONE) gr[x] = gr[x] * gr[x];
+    }
+}
+
+static void edt_store(size_t n, const int64_t* g, int32_t* d2)
+{
+    for (size_t i = 0; i < n; i++)
+        d2[i] = (g[i] == EDT_NONE || g[i] > INT32_MAX) ? OC_EDT_INF : (int32_t)g[i];
+}
+
+void oc_edt_2d(int w, int h, const uint32_t* mask, int invert, int32_t* d2)
+{
+    if (w <= 0 || h <= 0) return;
+    size_t n = (size_t)w * h;
+    int64_t* g = (int64_t*)malloc(n * sizeof(int64_t));
+    int64_t* t = (int64_t*)malloc(n * sizeof(int64_t));
+    int* v = (int*)malloc(sizeof(int) * (size_t)(w > h ? w : h));
+    edt_rows(w, (size_t)h, mask, invert, g);
+    for (int x = 0; x < w; x++) edt_envelope_1d(h, g + x, (size_t)w, t + x, (size_t)w, v);
+    edt_store(n, t, d2);
+    free(g); free(t); free(v);
+}
+
+void oc_edt_3d(int w, int h, int d, const uint32_t* mask, int invert, int32_t* d2)
+{
+    if (w <= 0 || h <= 0 || d <= 0) return;
+    size_t sl = (size_t)w * h, n = sl * d;
+    int64_t* g = (int64_t*)malloc(n * sizeof(int64_t));
+    int64_t* t = (int64_t*)malloc(n * sizeof(int64_t));
+    int mx = w > h ? w : h;
+    mx = mx > d ? mx : d;
+    int* v = (int*)malloc(sizeof(int) * (size_t)mx);
+    edt_rows(w, (size_t)h * d, mask, invert, g);
+    for (int z = 0; z < d; z++)
+        for (int x = 0; x < w; x++)
+            edt_envelope_1d(h, g + z * sl + x, (size_t)w, t + z * sl + x, (size_t)w, v);
+    for (size_t i = 0; i < sl; i++) edt_envelope_1d(d, t + i, sl, g + i, sl, v);
+    edt_store(n, g, d2);
+    free(g); free(t); free(v);
+}
+
+void oc_edt_brute_2d(int w, int h, const uint32_t* mask, int invert, int32_t* d2)
+{
+    int ws = mask_stride(w);
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            int64_t best = EDT_NONE;
+            for (int sy = 0; sy < h; sy++)
+                for (int sx = 0; sx < w; sx++) {
+                    int bit = ((mask[(size_t)sy * ws + (sx >> 5)] >> (sx & 31)) & 1) ^ (invert != 0);
+                    if (!bit) continue;
+                    int64_t e = (int64_t)(sx - x) * (sx - x) + (int64_t)(sy - y) * (sy - y);
+                    if (e < best) best = e;
+                }
+            d2[(size_t)y * w + x] = best == EDT_NONE ? OC_EDT_INF : (int32_t)best;
+        }
+}
+
+void oc_edt_brute_3d(int w, int h, int d, const uint32_t* mask, int invert, int32_t* d2)
+{
+    int ws = mask_stride(w);
+    for (int z = 0; z < d; z++)
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                int64_t best = EDT_NONE;
+                for (int sz = 0; sz < d; sz++)
+                    for (int sy = 0; sy < h; sy++)
+                        for (int sx = 0; sx < w; sx++) {
+                            size_t r = (size_t)sz * h + sy;
+                            int bit = ((mask[r * ws + (sx >> 5)] >> (sx & 31)) & 1) ^ (invert != 0);
+                            if (!bit) continue;
+                            int64_t e = (int64_t)(sx - x) * (sx - x) + (int64_t)(sy - y) * (sy - y)
+                                      + (int64_t)(sz - z) * (sz - z);
+                            if (e < best) best = e;
+                        }
+                d2[((size_t)z * h + y) * w + x] = best == EDT_NONE ? OC_EDT_INF : (int32_t)best;
+            }
+}
+
+/* float distance of an exact squared distance: sqrtf of the int converted to
+ * float; "no seed" maps to sqrtf(FLT_MAX), which is what the reference's
+ * float path yields for a cell that no seed reaches (DistanceOcclusionTool.cpp:753-754). */
+float oc_edt_float(int32_t d2)
+{
+    if (d2 == OC_EDT_INF) return sqrtf(FLT_MAX);
+    return sqrtf((float)d2);
+}
+
+/* ------------------------------------------------------------- encoders */
+
+/* EncodeU8, DistanceOcclusionLib.h:137-141 */
+uint8_t oc_encode_u8(float v)
+{
+    v = (v < 0.0f) ? 0.0f : ((v > 1.0f) ? 1.0f : v);
+    return (uint8_t)lrintf(v * 255);
+}
+
+/* EncodeU8Signed, DistanceOcclusionTool.cpp:71-75 */
+uint8_t oc_encode_u8_signed(float v)
+{
+    v = (v < -1.0f) ? -1.0f : ((v > 1.0f) ? 1.0f : v);
+    return (uint8_t)lrintf(128 + v * 127);
+}
+
+/* Dist, DistanceOcclusionTool.cpp:54-57 (int32 squared length, as in Dist2) */
+void oc_dist_2d(int n, const int16_t* delta, float* out)
+{
+    for (int i = 0; i < n; i++) out[i] = sqrtf((float)oc_len2_2(delta + 2 * i));
+}
+
+static inline int mask_bit(const uint32_t* mask, int ws, size_t row, int x)
+{
+    return (mask[row * ws + (x >> 5)] >> (x & 31)) & 1;
+}
+
+/* signed distance in cells from half-cell deltas of the border method:
+ * -/+ 0.5 * sqrtf(|delta|^2), negative inside (set bit) */
+void oc_signed_dist_2d(int w, int h, const uint32_t* mask, const int16_t* delta, float* out)
+{
+    int ws = mask_stride(w);
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            size_t i = (size_t)y * w + x;
+            float d = 0.5f * sqrtf((float)oc_len2_2(delta + 2 * i));
+            out[i] = mask_bit(mask, ws, (size_t)y, x) ? -d : d;
+        }
+}
+
+void oc_signed_dist_3d(int w, int h, int d, const uint32_t* mask, const int16_t* delta, float* out)
+{
+    int ws = mask_stride(w);
+    for (int z = 0; z < d; z++)
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                size_t r = (size_t)z * h + y, i = r * w + x;
+                float dd = 0.5f * sqrtf((float)oc_len2_3(delta + 3 * i));
+                out[i] = mask_bit(mask, ws, r, x) ? -dd : dd;
+            }
+}
+
+/* standard signed image of the border method, DistanceOcclusionTool.cpp:1242-1271;
+ * max_len is the value passed there (twice the tool's maxLen). */
+void oc_sdf_u8_2d(int w, int h, const uint32_t* mask, const int16_t* delta, float max_len, uint8_t* out)
+{
+    int ws = mask_stride(w);
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            size_t i = (size_t)y * w + x;
+            float d = sqrtf((float)oc_len2_2(delta + 2 * i)) / max_len;
+            out[i] = oc_encode_u8_signed(mask_bit(mask, ws, (size_t)y, x) ? -d : d);
+        }
+}
+
+/* DistanceOcclusionTool.cpp:1369-1404 */
+void oc_sdf_u8_3d(int w, int h, int d, const uint32_t* mask, const int16_t* delta, float max_len,
+                  uint8_t* out)
+{
+    int ws = mask_stride(w);
+    for (int z = 0; z < d; z++)
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                size_t r = (size_t)z * h + y, i = r * w + x;
+                float dd = sqrtf((float)oc_len2_3(delta + 3 * i)) / max_len;
+                out[i] = oc_encode_u8_signed(mask_bit(mask, ws, r, x) ? -dd : dd);
+            }
+}
+
+/* delta image, DistanceOcclusionTool.cpp:1281-1288 */
+void oc_sdf_delta_rgba_2d(int w, int h, const int16_t* delta, float max_len, uint8_t* out)
+{
+    for (size_t i = 0; i < (size_t)w * h; i++) {
+        out[4 * i + 0] = oc_encode_u8_signed(delta[2 * i] / max_len);
+        out[4 * i + 1] = oc_encode_u8_signed(delta[2 * i + 1] / max_len);
+        out[4 * i + 2] = 0;
+        out[4 * i + 3] = 255;
+    }
+}
+
+/* red/green image, DistanceOcclusionTool.cpp:1289-1307: the distance goes
+ * to channel (bit), the other of the first two is zero. */
+void oc_sdf_rg_rgba_2d(int w, int h, const uint32_t* mask, const int16_t* delta, float max_len,
+                       uint8_t* out)
+{
+    int ws = mask_stride(w);
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            size_t i = (size_t)y * w + x;
+            int b = mask_bit(mask, ws, (size_t)y, x);
+            out[4 * i + b] = oc_encode_u8(sqrtf((float)oc_len2_2(delta + 2 * i)) / max_len);
+            out[4 * i + (b ^ 1)] = 0;
+            out[4 * i + 2] = 0;
+            out[4 * i + 3] = 255;
+        }
+}
+
+/* two-field signed image, DistanceOcclusionTool.cpp:1035-1036 */
+void oc_sdf_two_field_u8_2d(int w, int h, const int16_t* d0, const int16_t* d1, float max_len,
+                            uint8_t* out)
+{
+    for (size_t i = 0; i < (size_t)w * h; i++)
+        out[i] = oc_encode_u8_signed((sqrtf((float)oc_len2_2(d0 + 2 * i))
+                                      - sqrtf((float)oc_len2_2(d1 + 2 * i))) / max_len);
+}
+
+/* ------------------------------------------------------------- invariants */
+
+/* CheckDeltas, DistanceOcclusionTool.cpp:371-402: each delta must land in
+ * bounds on a boundary cell.  The sum is formed in int16 as there. */
+long oc_check_deltas_2d(int w, int h, const int16_t* delta, int cw)
+{
+    long bad = 0;
+    for (int j = 0; j < h; j++)
+        for (int i = 0; i < w; i++) {
+            const int16_t* c = delta + 2 * ((size_t)j * w + i);
+            int cx = wrap16(c[0] + i * cw), cy = wrap16(c[1] + j * cw);
+            if (cx < 0 || cx >= cw * w || cy < 0 || cy >= cw * h) { bad++; continue; }
+            const int16_t* r = delta + 2 * ((size_t)(cy / cw) * w + cx / cw);
+            if (!(abs(r[0]) < cw && abs(r[1]) < cw)) bad++;
+        }
+    return bad;
+}
+
+/* DistanceOcclusionTool.cpp:404-439 */
+long oc_check_deltas_3d(int w, int h, int d, const int16_t* delta, int cw)
+{
+    long bad = 0;
+    for (int k = 0; k < d; k++)
+        for (int j = 0; j < h; j++)
+            for (int i = 0; i < w; i++) {
+                const int16_t* c = delta + 3 * (((size_t)k * h + j) * w + i);
+                int cx = wrap16(c[0] + i * cw), cy = wrap16(c[1] + j * cw), cz = wrap16(c[2] + k * cw);
+                if (cx < 0 || cx >= cw * w || cy < 0 || cy >= cw * h || cz < 0 || cz >= cw * d) {
+                    bad++;
+                    continue;
+                }
+                const int16_t* r = delta + 3 * (((size_t)(cz / cw) * h + cy / cw) * w + cx / cw);
+                if (!(abs(r[0]) < cw && abs(r[1]) < cw && abs(r[2]) < cw)) bad++;
+            }
+    return bad;
+}
+
+/* --------------------------------------------- continuous seeds (ext.) */
+
+/* Nearest seed to each cell centre (x, y), in float64; ties go to the lowest
+ * seed index.  seeds holds n (x, y) float pairs. */
+void oc_seed_nearest_2d(int w, int h, int n, const float* seeds, int32_t* idx, double* dist)
+{
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            double best = INFINITY;
+            int bi = -1;
+            for (int k = 0; k < n; k++) {
+                double dx = (double)x - (double)seeds[2 * k], dy = (double)y - (double)seeds[2 * k + 1];
+                double e = dx * dx + dy * dy;
+                if (e < best) { best = e; bi = k; }
+            }
+            idx[(size_t)y * w + x] = bi;
+            dist[(size_t)y * w + x] = sqrt(best);
+        }
+}
+
+/* d(sum w*d)/dp_k = sum over cells of seed k of -w (c - p_k) / d; zero where d == 0 */
+void oc_seed_grad_2d(int w, int h, int n, const float* seeds, const int32_t* idx,
+                     const float* weight, double* grad)
+{
+    for (int k = 0; k < 2 * n; k++) grad[k] = 0.0;
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            size_t i = (size_t)y * w + x;
+            int k = idx[i];
+            if (k < 0) continue;
+            double dx = (double)x - (double)seeds[2 * k], dy = (double)y - (double)seeds[2 * k + 1];
+            double dd = sqrt(dx * dx + dy * dy);
+            if (dd == 0.0) continue;
+            grad[2 * k] -= (double)weight[i] * dx / dd;
+            grad[2 * k + 1] -= (double)weight[i] * dy / dd;
+        }
+}

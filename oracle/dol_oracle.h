@@ -1,0 +1,99 @@
+/*
+ * dol_oracle.h -- CPU restatement of DistanceOcclusionLib's distance-field
+ * path.  TEST INFRASTRUCTURE ONLY: this is the checker that the HIP library is
+ * compared against.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it; the product library never links or calls it.
+ *
+ * Layouts follow DistanceOcclusionLib.h:16-57:
+ *   bitmask : rows padded to ceil(w/32) uint32 words, bit i of word j is
+ *             pixel x = 32j + i (LSB first); slices of ceil(w/32)*h words.
+ *   delta2  : int16 {x, y} per cell, row-major, cell index y*w + x.
+ *   delta3  : int16 {x, y, z} per cell (6 bytes, packed), index z*w*h + y*w + x.
+ *   kMaxDelta2 = 32767, kMaxDelta3 = 25000 (DistanceOcclusionLib.h:56-57).
+ */
+#ifndef DOL_ORACLE_H
+#define DOL_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OC_MAX_DELTA2 32767
+#define OC_MAX_DELTA3 25000
+#define OC_EDT_INF    0x7fffffff   /* squared distance where no seed exists */
+
+/* generators and byte -> bit (Lib:28-32, 95-179, 1224-1287; Tool:634-670) */
+uint32_t oc_rng(uint32_t* seed, int limit);
+void oc_mask_add_points_2d(int w, int h, int count, uint32_t seed, uint32_t* mask);
+void oc_mask_add_points_3d(int w, int h, int d, int count, uint32_t seed, uint32_t* mask);
+void oc_mask_add_block_2d(int w, int h, int sides, uint32_t* mask);
+void oc_mask_add_block_3d(int w, int h, int d, int sides, uint32_t* mask);
+void oc_mask_from_bytes(int w, int rows, const uint8_t* bytes, int threshold, int less_than,
+                        uint32_t* mask);
+
+/* initialisation (Lib:181-254, 1289-1321) */
+void oc_init_deltas_2d(int w, int h, const uint32_t* mask, int16_t* delta, int invert);
+void oc_init_deltas_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta, int invert);
+void oc_init_distances_f32(int w, int h, const uint32_t* mask, float* dist, float max_d);
+void oc_init_distances_i32(int w, int h, const uint32_t* mask, int32_t* dist, int32_t max_d);
+void oc_init_distances_i16(int w, int h, const uint32_t* mask, int16_t* dist, int16_t max_d);
+
+/* signed border initialisation, Combo write order (Lib:504-619, 1820-2173) */
+void oc_border_2d(int w, int h, const uint32_t* mask, int16_t* delta);
+void oc_border_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta);
+
+/* jump flooding (Lib:780-908, 2396-2472) */
+int  oc_floor_pow2(int v);
+void oc_jfa_pass_2d(int step, int w, int h, const int16_t* in, int16_t* out, int cw);
+void oc_jfa_pass_3d(int step, int w, int h, int d, const int16_t* in, int16_t* out, int cw);
+void oc_jfa_2d(int w, int h, int16_t* delta, int cw);
+void oc_jfa_3d(int w, int h, int d, int16_t* delta, int cw);
+void oc_jfa_2d_mt(int w, int h, int16_t* delta, int cw, int threads);
+void oc_jfa_3d_mt(int w, int h, int d, int16_t* delta, int cw, int threads);
+
+/* Felzenszwalb float squared distances, in place (Lib:1018-1085) */
+void oc_felzenszwalb_2d(int w, int h, float* dist);
+
+/* brute force (Lib:1087-1110, 2475-2505) */
+void oc_brute_force_2d(int w, int h, int16_t* delta, int cw);
+void oc_brute_force_3d(int w, int h, int d, int16_t* delta, int cw);
+
+/* exact squared EDT to set bits (invert: to clear bits); OC_EDT_INF if none */
+void oc_edt_2d(int w, int h, const uint32_t* mask, int invert, int32_t* d2);
+void oc_edt_3d(int w, int h, int d, const uint32_t* mask, int invert, int32_t* d2);
+void oc_edt_brute_2d(int w, int h, const uint32_t* mask, int invert, int32_t* d2);
+void oc_edt_brute_3d(int w, int h, int d, const uint32_t* mask, int invert, int32_t* d2);
+float oc_edt_float(int32_t d2);
+
+/* distances and encoders (Tool:46-75, 678-698, 1029-1040, 1240-1331, 1367-1469) */
+int32_t oc_len2_2(const int16_t* c);
+int32_t oc_len2_3(const int16_t* c);
+uint8_t oc_encode_u8(float v);
+uint8_t oc_encode_u8_signed(float v);
+void oc_dist_2d(int n, const int16_t* delta, float* out);
+void oc_signed_dist_2d(int w, int h, const uint32_t* mask, const int16_t* delta, float* out);
+void oc_signed_dist_3d(int w, int h, int d, const uint32_t* mask, const int16_t* delta, float* out);
+void oc_sdf_u8_2d(int w, int h, const uint32_t* mask, const int16_t* delta, float max_len, uint8_t* out);
+void oc_sdf_u8_3d(int w, int h, int d, const uint32_t* mask, const int16_t* delta, float max_len,
+                  uint8_t* out);
+void oc_sdf_delta_rgba_2d(int w, int h, const int16_t* delta, float max_len, uint8_t* out);
+void oc_sdf_rg_rgba_2d(int w, int h, const uint32_t* mask, const int16_t* delta, float max_len,
+                       uint8_t* out);
+void oc_sdf_two_field_u8_2d(int w, int h, const int16_t* d0, const int16_t* d1, float max_len,
+                            uint8_t* out);
+
+/* invariants of Tool:371-439 (CheckDeltas); returns number of failing cells */
+long oc_check_deltas_2d(int w, int h, const int16_t* delta, int cw);
+long oc_check_deltas_3d(int w, int h, int d, const int16_t* delta, int cw);
+
+/* continuous seeds (extension): nearest seed index + distance, float64 */
+void oc_seed_nearest_2d(int w, int h, int n, const float* seeds, int32_t* idx, double* dist);
+void oc_seed_grad_2d(int w, int h, int n, const float* seeds, const int32_t* idx,
+                     const float* weight, double* grad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,55 @@
+// ref_capi.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// A C-ABI face over the reference library, DistanceOcclusionLib.cpp, compiled
+// from its own sources under /root/reference by oracle/Makefile into
+// oracle/_ref/libdolref.so.  Tests use it to pin the oracle restatement (and,
+// through the oracle, the HIP library) to the reference bit for bit; bench.py
+// may time it as the CPU baseline.  Nothing here is product code.
+//
+// Each entry point forwards to the DOL:: function it is named after
+// (DistanceOcclusionLib.h:21-98); delta arrays are passed as int16 triples or
+// pairs, which is the cCellDelta2/cCellDelta3 layout (DistanceOcclusionLib.h:41-53).
+#include "DistanceOcclusionLib.h"
+
+using namespace DOL;
+
+extern "C" {
+
+void ref_mask_add_points_2d(int w, int h, int n, uint32_t seed, uint32_t* mask)
+{ BitMaskAddPoints(w, h, n, seed, mask); }
+void ref_mask_add_points_3d(int w, int h, int d, int n, uint32_t seed, uint32_t* mask)
+{ BitMaskAddPoints(w, h, d, n, seed, mask); }
+void ref_mask_add_block_2d(int w, int h, int sides, uint32_t* mask)
+{ BitMaskAddBlock(w, h, sides, mask); }
+void ref_mask_add_block_3d(int w, int h, int d, int sides, uint32_t* mask)
+{ BitMaskAddBlock(w, h, d, sides, mask); }
+
+void ref_init_deltas_2d(int w, int h, const uint32_t* mask, int16_t* delta, int invert)
+{ InitDeltasFromBitMask(w, h, mask, reinterpret_cast<cCellDelta2*>(delta), invert != 0); }
+void ref_init_deltas_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta, int invert)
+{ InitDeltasFromBitMask(w, h, d, mask, reinterpret_cast<cCellDelta3*>(delta), invert != 0); }
+void ref_border_2d(int w, int h, const uint32_t* mask, int16_t* delta)
+{ InitBorderDeltasFromBitMask(w, h, mask, reinterpret_cast<cCellDelta2*>(delta)); }
+void ref_border_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta)
+{ InitBorderDeltasFromBitMask(w, h, d, mask, reinterpret_cast<cCellDelta3*>(delta)); }
+void ref_init_distances_f32(int w, int h, const uint32_t* mask, float* dist, float max_d)
+{ InitDistancesFromBitMask(w, h, mask, dist, max_d); }
+void ref_init_distances_i32(int w, int h, const uint32_t* mask, int32_t* dist, int32_t max_d)
+{ InitDistancesFromBitMask(w, h, mask, dist, max_d); }
+void ref_init_distances_i16(int w, int h, const uint32_t* mask, int16_t* dist, int16_t max_d)
+{ InitDistancesFromBitMask(w, h, mask, dist, max_d); }
+
+void ref_jfa_2d(int w, int h, int16_t* delta, int cw)
+{ JumpFlood(w, h, reinterpret_cast<cCellDelta2*>(delta), cw); }
+void ref_jfa_3d(int w, int h, int d, int16_t* delta, int cw)
+{ JumpFlood(w, h, d, reinterpret_cast<cCellDelta3*>(delta), cw); }
+void ref_felzenszwalb_2d(int w, int h, float* dist)
+{ Felzenszwalb(w, h, dist); }
+void ref_brute_force_2d(int w, int h, int16_t* delta, int cw)
+{ BruteForce(w, h, reinterpret_cast<cCellDelta2*>(delta), cw); }
+void ref_brute_force_3d(int w, int h, int d, int16_t* delta, int cw)
+{ BruteForce(w, h, d, reinterpret_cast<cCellDelta3*>(delta), cw); }
+
+uint8_t ref_encode_u8(float v) { return EncodeU8(v); }
+
+}

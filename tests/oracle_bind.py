@@ -1,0 +1,327 @@
+"""ctypes bindings for the test oracle (oracle/liboracle.so) and, when it was
+built, the compiled reference (oracle/_ref/libdolref.so).
+
+Test infrastructure only: tests, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use this module as the checker; the product library never
+does.  Arrays are numpy; delta fields are int16 arrays of shape (..., 2) or
+(..., 3), which is the cCellDelta2/cCellDelta3 layout
+(DistanceOcclusionLib.h:41-53).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libdolref.so")
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_U = ctypes.c_uint32
+_F = ctypes.c_float
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def mask_stride(w):
+    return (w + 31) // 32
+
+
+def empty_mask(*dims):
+    """dims = (w, h) or (w, h, d); returns a zeroed uint32 mask."""
+    w = dims[0]
+    rows = int(np.prod(dims[1:]))
+    return np.zeros(mask_stride(w) * rows, dtype=np.uint32)
+
+
+def unpack_mask(mask, *dims):
+    """bitmask -> bool array of shape (d, h, w) or (h, w)."""
+    w = dims[0]
+    rows = int(np.prod(dims[1:]))
+    ws = mask_stride(w)
+    m = mask.reshape(rows, ws)
+    bits = ((m[:, :, None] >> np.arange(32, dtype=np.uint32)) & 1).astype(bool)
+    bits = bits.reshape(rows, ws * 32)[:, :w]
+    return bits.reshape(tuple(reversed(dims)))
+
+
+def pack_mask(bits):
+    """bool array (..., w) -> LSB-first padded uint32 words."""
+    bits = np.asarray(bits, dtype=bool)
+    w = bits.shape[-1]
+    rows = bits.reshape(-1, w)
+    ws = mask_stride(w)
+    pad = np.zeros((rows.shape[0], ws * 32), dtype=np.uint64)
+    pad[:, :w] = rows
+    pad = pad.reshape(rows.shape[0], ws, 32)
+    words = (pad << np.arange(32, dtype=np.uint64)).sum(axis=2).astype(np.uint32)
+    return words.reshape(-1)
+
+
+class Oracle:
+    """The CPU restatement (oracle/dol_oracle.c)."""
+
+    def __init__(self, path=ORACLE_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (run make -C oracle)")
+        L = ctypes.CDLL(path)
+        self.L = L
+        L.oc_edt_float.restype = ctypes.c_float
+        L.oc_edt_float.argtypes = [ctypes.c_int32]
+        L.oc_encode_u8.restype = ctypes.c_uint8
+        L.oc_encode_u8.argtypes = [_F]
+        L.oc_encode_u8_signed.restype = ctypes.c_uint8
+        L.oc_encode_u8_signed.argtypes = [_F]
+        L.oc_check_deltas_2d.restype = ctypes.c_long
+        L.oc_check_deltas_3d.restype = ctypes.c_long
+        L.oc_floor_pow2.restype = _I
+        L.oc_rng.restype = _U
+
+    # generators
+    def mask_add_points_2d(self, w, h, n, seed, mask):
+        self.L.oc_mask_add_points_2d(_I(w), _I(h), _I(n), _U(seed), _ptr(mask))
+
+    def mask_add_points_3d(self, w, h, d, n, seed, mask):
+        self.L.oc_mask_add_points_3d(_I(w), _I(h), _I(d), _I(n), _U(seed), _ptr(mask))
+
+    def mask_add_block_2d(self, w, h, sides, mask):
+        self.L.oc_mask_add_block_2d(_I(w), _I(h), _I(sides), _ptr(mask))
+
+    def mask_add_block_3d(self, w, h, d, sides, mask):
+        self.L.oc_mask_add_block_3d(_I(w), _I(h), _I(d), _I(sides), _ptr(mask))
+
+    def mask_from_bytes(self, w, rows, data, threshold=128, less_than=True):
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        mask = np.zeros(mask_stride(w) * rows, dtype=np.uint32)
+        self.L.oc_mask_from_bytes(_I(w), _I(rows), _ptr(data), _I(threshold), _I(int(less_than)), _ptr(mask))
+        return mask
+
+    # init
+    def init_deltas_2d(self, w, h, mask, invert=False):
+        out = np.empty((h, w, 2), dtype=np.int16)
+        self.L.oc_init_deltas_2d(_I(w), _I(h), _ptr(mask), _ptr(out), _I(int(invert)))
+        return out
+
+    def init_deltas_3d(self, w, h, d, mask, invert=False):
+        out = np.empty((d, h, w, 3), dtype=np.int16)
+        self.L.oc_init_deltas_3d(_I(w), _I(h), _I(d), _ptr(mask), _ptr(out), _I(int(invert)))
+        return out
+
+    def init_distances_f32(self, w, h, mask, max_d=np.finfo(np.float32).max):
+        out = np.empty((h, w), dtype=np.float32)
+        self.L.oc_init_distances_f32(_I(w), _I(h), _ptr(mask), _ptr(out), _F(max_d))
+        return out
+
+    def border_2d(self, w, h, mask):
+        out = np.empty((h, w, 2), dtype=np.int16)
+        self.L.oc_border_2d(_I(w), _I(h), _ptr(mask), _ptr(out))
+        return out
+
+    def border_3d(self, w, h, d, mask):
+        out = np.empty((d, h, w, 3), dtype=np.int16)
+        self.L.oc_border_3d(_I(w), _I(h), _I(d), _ptr(mask), _ptr(out))
+        return out
+
+    # propagation (in place on a copy)
+    def jfa_2d(self, w, h, delta, cw=1, threads=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self.L.oc_jfa_2d_mt(_I(w), _I(h), _ptr(out), _I(cw), _I(threads))
+        return out
+
+    def jfa_3d(self, w, h, d, delta, cw=1, threads=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self.L.oc_jfa_3d_mt(_I(w), _I(h), _I(d), _ptr(out), _I(cw), _I(threads))
+        return out
+
+    def jfa_pass_2d(self, step, w, h, delta, cw=1):
+        src = np.ascontiguousarray(delta, dtype=np.int16)
+        out = np.empty_like(src)
+        self.L.oc_jfa_pass_2d(_I(step), _I(w), _I(h), _ptr(src), _ptr(out), _I(cw))
+        return out
+
+    def jfa_pass_3d(self, step, w, h, d, delta, cw=1):
+        src = np.ascontiguousarray(delta, dtype=np.int16)
+        out = np.empty_like(src)
+        self.L.oc_jfa_pass_3d(_I(step), _I(w), _I(h), _I(d), _ptr(src), _ptr(out), _I(cw))
+        return out
+
+    def felzenszwalb_2d(self, w, h, dist):
+        out = np.ascontiguousarray(dist, dtype=np.float32).copy()
+        self.L.oc_felzenszwalb_2d(_I(w), _I(h), _ptr(out))
+        return out
+
+    def brute_force_2d(self, w, h, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self.L.oc_brute_force_2d(_I(w), _I(h), _ptr(out), _I(cw))
+        return out
+
+    def brute_force_3d(self, w, h, d, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self.L.oc_brute_force_3d(_I(w), _I(h), _I(d), _ptr(out), _I(cw))
+        return out
+
+    # exact EDT
+    def edt_2d(self, w, h, mask, invert=False):
+        out = np.empty((h, w), dtype=np.int32)
+        self.L.oc_edt_2d(_I(w), _I(h), _ptr(mask), _I(int(invert)), _ptr(out))
+        return out
+
+    def edt_3d(self, w, h, d, mask, invert=False):
+        out = np.empty((d, h, w), dtype=np.int32)
+        self.L.oc_edt_3d(_I(w), _I(h), _I(d), _ptr(mask), _I(int(invert)), _ptr(out))
+        return out
+
+    def edt_brute_2d(self, w, h, mask, invert=False):
+        out = np.empty((h, w), dtype=np.int32)
+        self.L.oc_edt_brute_2d(_I(w), _I(h), _ptr(mask), _I(int(invert)), _ptr(out))
+        return out
+
+    def edt_brute_3d(self, w, h, d, mask, invert=False):
+        out = np.empty((d, h, w), dtype=np.int32)
+        self.L.oc_edt_brute_3d(_I(w), _I(h), _I(d), _ptr(mask), _I(int(invert)), _ptr(out))
+        return out
+
+    # encoders
+    def signed_dist_2d(self, w, h, mask, delta):
+        out = np.empty((h, w), dtype=np.float32)
+        self.L.oc_signed_dist_2d(_I(w), _I(h), _ptr(mask), _ptr(np.ascontiguousarray(delta)), _ptr(out))
+        return out
+
+    def signed_dist_3d(self, w, h, d, mask, delta):
+        out = np.empty((d, h, w), dtype=np.float32)
+        self.L.oc_signed_dist_3d(_I(w), _I(h), _I(d), _ptr(mask), _ptr(np.ascontiguousarray(delta)), _ptr(out))
+        return out
+
+    def sdf_u8_2d(self, w, h, mask, delta, max_len):
+        out = np.empty((h, w), dtype=np.uint8)
+        self.L.oc_sdf_u8_2d(_I(w), _I(h), _ptr(mask), _ptr(np.ascontiguousarray(delta)), _F(max_len), _ptr(out))
+        return out
+
+    def sdf_u8_3d(self, w, h, d, mask, delta, max_len):
+        out = np.empty((d, h, w), dtype=np.uint8)
+        self.L.oc_sdf_u8_3d(_I(w), _I(h), _I(d), _ptr(mask), _ptr(np.ascontiguousarray(delta)), _F(max_len), _ptr(out))
+        return out
+
+    def sdf_delta_rgba_2d(self, w, h, delta, max_len):
+        out = np.empty((h, w, 4), dtype=np.uint8)
+        self.L.oc_sdf_delta_rgba_2d(_I(w), _I(h), _ptr(np.ascontiguousarray(delta)), _F(max_len), _ptr(out))
+        return out
+
+    def sdf_rg_rgba_2d(self, w, h, mask, delta, max_len):
+        out = np.empty((h, w, 4), dtype=np.uint8)
+        self.L.oc_sdf_rg_rgba_2d(_I(w), _I(h), _ptr(mask), _ptr(np.ascontiguousarray(delta)), _F(max_len), _ptr(out))
+        return out
+
+    def dist_2d(self, delta):
+        delta = np.ascontiguousarray(delta, dtype=np.int16)
+        n = delta.size // 2
+        out = np.empty(delta.shape[:-1], dtype=np.float32)
+        self.L.oc_dist_2d(_I(n), _ptr(delta), _ptr(out))
+        return out
+
+    def check_deltas_2d(self, w, h, delta, cw=1):
+        return self.L.oc_check_deltas_2d(_I(w), _I(h), _ptr(np.ascontiguousarray(delta)), _I(cw))
+
+    def check_deltas_3d(self, w, h, d, delta, cw=1):
+        return self.L.oc_check_deltas_3d(_I(w), _I(h), _I(d), _ptr(np.ascontiguousarray(delta)), _I(cw))
+
+    def seed_nearest_2d(self, w, h, seeds):
+        seeds = np.ascontiguousarray(seeds, dtype=np.float32)
+        idx = np.empty((h, w), dtype=np.int32)
+        dist = np.empty((h, w), dtype=np.float64)
+        self.L.oc_seed_nearest_2d(_I(w), _I(h), _I(seeds.shape[0]), _ptr(seeds), _ptr(idx), _ptr(dist))
+        return idx, dist
+
+    def seed_grad_2d(self, w, h, seeds, idx, weight):
+        seeds = np.ascontiguousarray(seeds, dtype=np.float32)
+        grad = np.empty((seeds.shape[0], 2), dtype=np.float64)
+        self.L.oc_seed_grad_2d(_I(w), _I(h), _I(seeds.shape[0]), _ptr(seeds),
+                               _ptr(np.ascontiguousarray(idx, dtype=np.int32)),
+                               _ptr(np.ascontiguousarray(weight, dtype=np.float32)), _ptr(grad))
+        return grad
+
+
+class Reference:
+    """The reference library compiled from its own sources (oracle/_ref)."""
+
+    def __init__(self, path=REF_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.L = ctypes.CDLL(path)
+
+    def mask_add_points_2d(self, w, h, n, seed, mask):
+        self.L.ref_mask_add_points_2d(_I(w), _I(h), _I(n), _U(seed), _ptr(mask))
+
+    def mask_add_points_3d(self, w, h, d, n, seed, mask):
+        self.L.ref_mask_add_points_3d(_I(w), _I(h), _I(d), _I(n), _U(seed), _ptr(mask))
+
+    def mask_add_block_2d(self, w, h, sides, mask):
+        self.L.ref_mask_add_block_2d(_I(w), _I(h), _I(sides), _ptr(mask))
+
+    def mask_add_block_3d(self, w, h, d, sides, mask):
+        self.L.ref_mask_add_block_3d(_I(w), _I(h), _I(d), _I(sides), _ptr(mask))
+
+    def init_deltas_2d(self, w, h, mask, invert=False):
+        out = np.empty((h, w, 2), dtype=np.int16)
+        self.L.ref_init_deltas_2d(_I(w), _I(h), _ptr(mask), _ptr(out), _I(int(invert)))
+        return out
+
+    def init_deltas_3d(self, w, h, d, mask, invert=False):
+        out = np.empty((d, h, w, 3), dtype=np.int16)
+        self.L.ref_init_deltas_3d(_I(w), _I(h), _I(d), _ptr(mask), _ptr(out), _I(int(invert)))
+        return out
+
+    def init_distances_f32(self, w, h, mask, max_d=np.finfo(np.float32).max):
+        out = np.empty((h, w), dtype=np.float32)
+        self.L.ref_init_distances_f32(_I(w), _I(h), _ptr(mask), _ptr(out), _F(max_d))
+        return out
+
+    def border_2d(self, w, h, mask):
+        out = np.empty((h, w, 2), dtype=np.int16)
+        self.L.ref_border_2d(_I(w), _I(h), _ptr(mask), _ptr(out))
+        return out
+
+    def border_3d(self, w, h, d, mask):
+        out = np.empty((d, h, w, 3), dtype=np.int16)
+        self.L.ref_border_3d(_I(w), _I(h), _I(d), _ptr(mask), _ptr(out))
+        return out
+
+    def jfa_2d(self, w, h, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self.L.ref_jfa_2d(_I(w), _I(h), _ptr(out), _I(cw))
+        return out
+
+    def jfa_3d(self, w, h, d, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self.L.ref_jfa_3d(_I(w), _I(h), _I(d), _ptr(out), _I(cw))
+        return out
+
+    def felzenszwalb_2d(self, w, h, dist):
+        out = np.ascontiguousarray(dist, dtype=np.float32).copy()
+        self.L.ref_felzenszwalb_2d(_I(w), _I(h), _ptr(out))
+        return out
+
+    def brute_force_2d(self, w, h, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self.L.ref_brute_force_2d(_I(w), _I(h), _ptr(out), _I(cw))
+        return out
+
+    def brute_force_3d(self, w, h, d, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self.L.ref_brute_force_3d(_I(w), _I(h), _I(d), _ptr(out), _I(cw))
+        return out
+
+
+def load_oracle():
+    return Oracle()
+
+
+def load_reference():
+    """The compiled reference, or None where it was not built."""
+    try:
+        return Reference()
+    except (FileNotFoundError, OSError):
+        return None
