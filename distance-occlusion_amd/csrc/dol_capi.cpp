@@ -1,0 +1,928 @@
+// dol_capi.cpp -- host runtime and C ABI of libdolhip (include/dol_hip.h).
+//
+// Host-pointer entry points keep the reference's synchronous, in-place
+// contract (DistanceOcclusionLib.h:61-98): copy in, run the kernels, copy
+// back, return.  Device-pointer entry points (_dev) enqueue on the caller's
+// stream and return.  Failures are reported as status codes with a message
+// in dolhip_last_error(); nothing falls back to the CPU.
+#include "../../include/dol_hip.h"
+#include "dol_common.hpp"
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace dolhip {
+
+// kernel launchers (dol_init.hip, dol_jfa.hip, dol_edt.hip, dol_encode.hip)
+void launch_mask_from_bytes(int w, int rows, const uint8_t* bytes, int threshold, int less_than,
+                            uint32_t* mask, hipStream_t s);
+void launch_init_deltas_2d(int w, int h, const uint32_t* mask, int16_t* delta, int invert,
+                           hipStream_t s);
+void launch_init_deltas_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta, int invert,
+                           hipStream_t s);
+template <class T>
+void launch_init_dist(int w, int h, const uint32_t* mask, T* dist, T max_d, hipStream_t s);
+void launch_border_2d(int w, int h, const uint32_t* mask, int16_t* delta, hipStream_t s);
+void launch_border_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta, hipStream_t s);
+void launch_jfa2d_pass(int step, int w, int h, const int16_t* in, int16_t* out, int cw,
+                       hipStream_t s);
+void launch_jfa3d_pass(int step, int w, int h, int d, const int16_t* in, int16_t* out, int cw,
+                       hipStream_t s);
+size_t felz_workspace(int w, int h);
+void launch_felzenszwalb_2d(int w, int h, float* dist, void* work, hipStream_t s);
+size_t edt_workspace(long long cells, int dims);
+void launch_edt_2d(int w, int h, const uint32_t* mask, int invert, int32_t* d2, float* fl,
+                   void* work, hipStream_t s);
+void launch_edt_3d(int w, int h, int d, const uint32_t* mask, int invert, int32_t* d2, float* fl,
+                   void* work, hipStream_t s);
+void launch_dist_2d(size_t n, const int16_t* delta, float* out, hipStream_t s);
+void launch_signed_2d(int w, int h, const uint32_t* mask, const int16_t* delta, float max_len,
+                      void* out, int mode, hipStream_t s);
+void launch_signed_3d(int w, int h, int d, const uint32_t* mask, const int16_t* delta,
+                      float max_len, void* out, int mode, hipStream_t s);
+void launch_sdf_delta_rgba(int w, int h, const int16_t* delta, float max_len, uint8_t* out,
+                           hipStream_t s);
+void launch_sdf_rg_rgba(int w, int h, const uint32_t* mask, const int16_t* delta, float max_len,
+                        uint8_t* out, hipStream_t s);
+void launch_signed_edt(size_t n, const int32_t* dout, const int32_t* din, float* sdf,
+                       hipStream_t s);
+
+// ------------------------------------------------------------------ errors
+
+static thread_local std::string g_error;
+
+void set_error(const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_error = buf;
+}
+
+static int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+static int fail_hip(hipError_t e, const char* what)
+{
+    if (e == hipErrorOutOfMemory)
+        return fail(DOLHIP_ERR_OUT_OF_MEMORY, "%s: %s", what, hipGetErrorString(e));
+    return fail(DOLHIP_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+int check_launch(const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail_hip(e, what);
+    return DOLHIP_OK;
+}
+
+#define TRY_HIP(x)                                   \
+    do {                                             \
+        hipError_t e_ = (x);                         \
+        if (e_ != hipSuccess) return fail_hip(e_, #x); \
+    } while (0)
+
+#define TRY(x)                  \
+    do {                        \
+        int r_ = (x);           \
+        if (r_ != DOLHIP_OK) return r_; \
+    } while (0)
+
+// ------------------------------------------------------------------ device
+
+static int g_device_ok = -1;   // -1 unknown, 0 none, 1 usable
+static std::mutex g_mutex;
+
+static int require_device()
+{
+    std::lock_guard<std::mutex> lk(g_mutex);
+    if (g_device_ok < 0) {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+            (void)hipGetLastError();
+            g_device_ok = 0;
+        } else {
+            hipDeviceProp_t prop;
+            int dev = 0;
+            (void)hipGetDevice(&dev);
+            g_device_ok = (hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+                           std::strncmp(prop.gcnArchName, "gfx950", 6) == 0)
+                              ? 1
+                              : 0;
+        }
+    }
+    if (!g_device_ok)
+        return fail(DOLHIP_ERR_NO_DEVICE, "no gfx950 (MI355X) device is available");
+    return DOLHIP_OK;
+}
+
+// per-device work buffer that grows as needed
+struct WorkBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+static std::vector<WorkBuf> g_work;
+
+static int library_work(size_t bytes, void** out)
+{
+    int dev = 0;
+    TRY_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_mutex);
+    if ((int)g_work.size() <= dev) g_work.resize(dev + 1);
+    WorkBuf& wb = g_work[dev];
+    if (wb.bytes < bytes) {
+        if (wb.ptr) {
+            TRY_HIP(hipDeviceSynchronize());
+            TRY_HIP(hipFree(wb.ptr));
+            wb.ptr = nullptr;
+            wb.bytes = 0;
+        }
+        TRY_HIP(hipMalloc(&wb.ptr, bytes));
+        wb.bytes = bytes;
+    }
+    *out = wb.ptr;
+    return DOLHIP_OK;
+}
+
+static int resolve_work(void* given, size_t bytes, void** out)
+{
+    if (given) {
+        *out = given;
+        return DOLHIP_OK;
+    }
+    return library_work(bytes, out);
+}
+
+// RAII device buffer for the host-pointer entry points
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t bytes)
+    {
+        hipError_t e = hipMalloc(&p, bytes ? bytes : 4);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail_hip(e, "hipMalloc");
+        }
+        return DOLHIP_OK;
+    }
+    int upload(const void* src, size_t bytes)
+    {
+        TRY(alloc(bytes));
+        if (bytes) TRY_HIP(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+        return DOLHIP_OK;
+    }
+    int download(void* dst, size_t bytes)
+    {
+        if (bytes) TRY_HIP(hipMemcpy(dst, p, bytes, hipMemcpyDeviceToHost));
+        return DOLHIP_OK;
+    }
+    template <class T>
+    T* as() { return (T*)p; }
+};
+
+// ------------------------------------------------------------------ profiling
+
+struct ProfRec {
+    std::string name;
+    hipEvent_t a, b;
+};
+static bool g_prof = false;
+static std::vector<ProfRec> g_prof_recs;
+
+ProfileScope::ProfileScope(const char* name, hipStream_t s) : name_(name), s_(s), ev0_(nullptr)
+{
+    if (!g_prof) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, s);
+    ev0_ = e;
+}
+
+ProfileScope::~ProfileScope()
+{
+    if (!ev0_) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, s_);
+    std::lock_guard<std::mutex> lk(g_mutex);
+    g_prof_recs.push_back({name_, (hipEvent_t)ev0_, e});
+}
+
+static void prof_clear()
+{
+    for (auto& r : g_prof_recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    g_prof_recs.clear();
+}
+
+// ------------------------------------------------------------------ helpers
+
+static inline hipStream_t as_stream(void* s) { return (hipStream_t)s; }
+
+static int check_dims(int w, int h, int d = 1)
+{
+    if (w < 1 || h < 1 || d < 1)
+        return fail(DOLHIP_ERR_INVALID_ARG, "dimensions must be positive (w=%d h=%d d=%d)", w, h, d);
+    if ((long long)w * h * d > (1LL << 34))
+        return fail(DOLHIP_ERR_INVALID_ARG, "volume too large (%d x %d x %d)", w, h, d);
+    if (w > 32767 || h > 32767 || d > 32767)
+        return fail(DOLHIP_ERR_INVALID_ARG, "each dimension must be below 32768 (int16 deltas)");
+    return DOLHIP_OK;
+}
+
+static int check_ptr(const void* p, const char* name)
+{
+    if (!p) return fail(DOLHIP_ERR_INVALID_ARG, "%s is null", name);
+    return DOLHIP_OK;
+}
+
+static int check_cw(int cw)
+{
+    if (cw < 1 || cw > 2)
+        return fail(DOLHIP_ERR_INVALID_ARG, "cw must be 1 (cell units) or 2 (half-cell units), got %d", cw);
+    return DOLHIP_OK;
+}
+
+static inline size_t mask_bytes(int w, long long rows) { return (size_t)mask_stride(w) * rows * 4; }
+
+static int finish(hipStream_t s, const char* what)
+{
+    TRY(check_launch(what));
+    (void)s;
+    return DOLHIP_OK;
+}
+
+static int finish_sync(const char* what)
+{
+    TRY(check_launch(what));
+    TRY_HIP(hipDeviceSynchronize());
+    return DOLHIP_OK;
+}
+
+// FloorPow2 of DistanceOcclusionLib.cpp:875-884
+static int floor_pow2(int v)
+{
+    uint32_t u = (uint32_t)v;
+    u |= u >> 1; u |= u >> 2; u |= u >> 4; u |= u >> 8; u |= u >> 16;
+    return (int)((int32_t)(u + 1u) >> 1);
+}
+
+// pass schedule of DistanceOcclusionLib.cpp:887-908 / 2450-2472
+static void jfa2d_schedule(int w, int h, int16_t* delta, int cw, int16_t* tmp, hipStream_t s)
+{
+    int16_t* a = delta;
+    int16_t* b = tmp;
+    for (int step = floor_pow2((w >= h ? w : h) - 1); step > 0; step /= 2) {
+        launch_jfa2d_pass(step, w, h, a, b, cw, s);
+        int16_t* t = a; a = b; b = t;
+    }
+    if (a != delta) launch_jfa2d_pass(1, w, h, a, delta, cw, s);
+}
+
+static void jfa3d_schedule(int w, int h, int d, int16_t* delta, int cw, int16_t* tmp, hipStream_t s)
+{
+    int16_t* a = delta;
+    int16_t* b = tmp;
+    int m = w >= h ? w : h;
+    m = m >= d ? m : d;
+    for (int step = floor_pow2(m - 1); step > 0; step /= 2) {
+        launch_jfa3d_pass(step, w, h, d, a, b, cw, s);
+        int16_t* t = a; a = b; b = t;
+    }
+    if (a != delta) launch_jfa3d_pass(1, w, h, d, a, delta, cw, s);
+}
+
+template <class T>
+int init_dist_host(int w, int h, const uint32_t* mask, T* dist, T max_d)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(dist, "dist"));
+    TRY(require_device());
+    DevBuf m, o;
+    TRY(m.upload(mask, mask_bytes(w, h)));
+    TRY(o.alloc((size_t)w * h * sizeof(T)));
+    launch_init_dist<T>(w, h, m.as<uint32_t>(), o.as<T>(), max_d, 0);
+    TRY(finish_sync("init_distances"));
+    return o.download(dist, (size_t)w * h * sizeof(T));
+}
+
+}  // namespace dolhip
+
+using namespace dolhip;
+
+extern "C" {
+
+const char* dolhip_last_error(void) { return g_error.c_str(); }
+const char* dolhip_version(void) { return "dolhip 0.1 gfx950"; }
+
+int dolhip_device_count(void)
+{
+    if (require_device() != DOLHIP_OK) return 0;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int dolhip_profile_enable(int enable)
+{
+    std::lock_guard<std::mutex> lk(g_mutex);
+    prof_clear();
+    g_prof = enable != 0;
+    return DOLHIP_OK;
+}
+
+int dolhip_profile_query(const char* prefix, double* avg_ms, int* count)
+{
+    TRY(check_ptr(prefix, "prefix"));
+    TRY_HIP(hipDeviceSynchronize());
+    std::lock_guard<std::mutex> lk(g_mutex);
+    double total = 0.0;
+    int n = 0;
+    size_t pl = std::strlen(prefix);
+    for (auto& r : g_prof_recs) {
+        if (r.name.compare(0, pl, prefix) != 0) continue;
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            total += ms;
+            n++;
+        }
+    }
+    if (avg_ms) *avg_ms = n ? total / n : 0.0;
+    if (count) *count = n;
+    return DOLHIP_OK;
+}
+
+// ---- masks
+
+int dolhip_mask_from_bytes_dev(int w, int rows, const uint8_t* bytes, int threshold,
+                               int less_than, uint32_t* mask, void* stream)
+{
+    TRY(check_dims(w, rows));
+    TRY(check_ptr(bytes, "bytes"));
+    TRY(check_ptr(mask, "mask"));
+    TRY(require_device());
+    launch_mask_from_bytes(w, rows, bytes, threshold, less_than, mask, as_stream(stream));
+    return finish(as_stream(stream), "mask_from_bytes");
+}
+
+int dolhip_mask_from_bytes(int w, int rows, const uint8_t* bytes, int threshold, int less_than,
+                           uint32_t* mask)
+{
+    TRY(check_dims(w, rows));
+    TRY(check_ptr(bytes, "bytes"));
+    TRY(check_ptr(mask, "mask"));
+    TRY(require_device());
+    DevBuf b, m;
+    TRY(b.upload(bytes, (size_t)w * rows));
+    TRY(m.alloc(mask_bytes(w, rows)));
+    launch_mask_from_bytes(w, rows, b.as<uint8_t>(), threshold, less_than, m.as<uint32_t>(), 0);
+    TRY(finish_sync("mask_from_bytes"));
+    return m.download(mask, mask_bytes(w, rows));
+}
+
+// ---- initialisation
+
+int dolhip_init_deltas_2d_dev(int w, int h, const uint32_t* mask, int16_t* delta, int invert,
+                              void* stream)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(delta, "delta"));
+    TRY(require_device());
+    launch_init_deltas_2d(w, h, mask, delta, invert, as_stream(stream));
+    return finish(as_stream(stream), "init_deltas_2d");
+}
+
+int dolhip_init_deltas_2d(int w, int h, const uint32_t* mask, int16_t* delta, int invert)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(delta, "delta"));
+    TRY(require_device());
+    DevBuf m, o;
+    TRY(m.upload(mask, mask_bytes(w, h)));
+    TRY(o.alloc((size_t)w * h * 4));
+    launch_init_deltas_2d(w, h, m.as<uint32_t>(), o.as<int16_t>(), invert, 0);
+    TRY(finish_sync("init_deltas_2d"));
+    return o.download(delta, (size_t)w * h * 4);
+}
+
+int dolhip_init_deltas_3d_dev(int w, int h, int d, const uint32_t* mask, int16_t* delta,
+                              int invert, void* stream)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(delta, "delta"));
+    TRY(require_device());
+    launch_init_deltas_3d(w, h, d, mask, delta, invert, as_stream(stream));
+    return finish(as_stream(stream), "init_deltas_3d");
+}
+
+int dolhip_init_deltas_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta, int invert)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(delta, "delta"));
+    TRY(require_device());
+    const size_t cells = (size_t)w * h * d;
+    DevBuf m, o;
+    TRY(m.upload(mask, mask_bytes(w, (long long)h * d)));
+    TRY(o.alloc(cells * 6));
+    launch_init_deltas_3d(w, h, d, m.as<uint32_t>(), o.as<int16_t>(), invert, 0);
+    TRY(finish_sync("init_deltas_3d"));
+    return o.download(delta, cells * 6);
+}
+
+int dolhip_init_border_deltas_2d_dev(int w, int h, const uint32_t* mask, int16_t* delta,
+                                     void* stream)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(delta, "delta"));
+    TRY(require_device());
+    launch_border_2d(w, h, mask, delta, as_stream(stream));
+    return finish(as_stream(stream), "border_2d");
+}
+
+int dolhip_init_border_deltas_2d(int w, int h, const uint32_t* mask, int16_t* delta)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(delta, "delta"));
+    TRY(require_device());
+    DevBuf m, o;
+    TRY(m.upload(mask, mask_bytes(w, h)));
+    TRY(o.alloc((size_t)w * h * 4));
+    launch_border_2d(w, h, m.as<uint32_t>(), o.as<int16_t>(), 0);
+    TRY(finish_sync("border_2d"));
+    return o.download(delta, (size_t)w * h * 4);
+}
+
+int dolhip_init_border_deltas_3d_dev(int w, int h, int d, const uint32_t* mask, int16_t* delta,
+                                     void* stream)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(delta, "delta"));
+    TRY(require_device());
+    launch_border_3d(w, h, d, mask, delta, as_stream(stream));
+    return finish(as_stream(stream), "border_3d");
+}
+
+int dolhip_init_border_deltas_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(delta, "delta"));
+    TRY(require_device());
+    const size_t cells = (size_t)w * h * d;
+    DevBuf m, o;
+    TRY(m.upload(mask, mask_bytes(w, (long long)h * d)));
+    TRY(o.alloc(cells * 6));
+    launch_border_3d(w, h, d, m.as<uint32_t>(), o.as<int16_t>(), 0);
+    TRY(finish_sync("border_3d"));
+    return o.download(delta, cells * 6);
+}
+
+int dolhip_init_distances_f32(int w, int h, const uint32_t* mask, float* dist, float max_d)
+{ return init_dist_host<float>(w, h, mask, dist, max_d); }
+int dolhip_init_distances_i32(int w, int h, const uint32_t* mask, int32_t* dist, int32_t max_d)
+{ return init_dist_host<int32_t>(w, h, mask, dist, max_d); }
+int dolhip_init_distances_i16(int w, int h, const uint32_t* mask, int16_t* dist, int16_t max_d)
+{ return init_dist_host<int16_t>(w, h, mask, dist, max_d); }
+
+int dolhip_init_distances_f32_dev(int w, int h, const uint32_t* mask, float* dist, float max_d,
+                                  void* stream)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(dist, "dist"));
+    TRY(require_device());
+    launch_init_dist<float>(w, h, mask, dist, max_d, as_stream(stream));
+    return finish(as_stream(stream), "init_distances");
+}
+
+// ---- propagation
+
+size_t dolhip_jump_flood_2d_workspace(int w, int h)
+{
+    return (w < 1 || h < 1) ? 0 : (size_t)w * h * 4;
+}
+
+int dolhip_jump_flood_2d_dev(int w, int h, int16_t* delta, int cw, void* work, void* stream)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_cw(cw));
+    TRY(require_device());
+    void* tmp = nullptr;
+    TRY(resolve_work(work, dolhip_jump_flood_2d_workspace(w, h), &tmp));
+    jfa2d_schedule(w, h, delta, cw, (int16_t*)tmp, as_stream(stream));
+    return finish(as_stream(stream), "jump_flood_2d");
+}
+
+int dolhip_jump_flood_2d(int w, int h, int16_t* delta, int cw)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_cw(cw));
+    TRY(require_device());
+    const size_t bytes = (size_t)w * h * 4;
+    DevBuf a, t;
+    TRY(a.upload(delta, bytes));
+    TRY(t.alloc(bytes));
+    jfa2d_schedule(w, h, a.as<int16_t>(), cw, t.as<int16_t>(), 0);
+    TRY(finish_sync("jump_flood_2d"));
+    return a.download(delta, bytes);
+}
+
+size_t dolhip_jump_flood_3d_workspace(int w, int h, int d)
+{
+    return (w < 1 || h < 1 || d < 1) ? 0 : (size_t)w * h * d * 6;
+}
+
+int dolhip_jump_flood_3d_dev(int w, int h, int d, int16_t* delta, int cw, void* work,
+                             void* stream)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_cw(cw));
+    TRY(require_device());
+    void* tmp = nullptr;
+    TRY(resolve_work(work, dolhip_jump_flood_3d_workspace(w, h, d), &tmp));
+    jfa3d_schedule(w, h, d, delta, cw, (int16_t*)tmp, as_stream(stream));
+    return finish(as_stream(stream), "jump_flood_3d");
+}
+
+int dolhip_jump_flood_3d(int w, int h, int d, int16_t* delta, int cw)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_cw(cw));
+    TRY(require_device());
+    const size_t bytes = (size_t)w * h * d * 6;
+    DevBuf a, t;
+    TRY(a.upload(delta, bytes));
+    TRY(t.alloc(bytes));
+    jfa3d_schedule(w, h, d, a.as<int16_t>(), cw, t.as<int16_t>(), 0);
+    TRY(finish_sync("jump_flood_3d"));
+    return a.download(delta, bytes);
+}
+
+size_t dolhip_felzenszwalb_2d_workspace(int w, int h)
+{
+    return (w < 1 || h < 1) ? 0 : felz_workspace(w, h);
+}
+
+int dolhip_felzenszwalb_2d_dev(int w, int h, float* dist, void* work, void* stream)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(dist, "dist"));
+    TRY(require_device());
+    void* tmp = nullptr;
+    TRY(resolve_work(work, felz_workspace(w, h), &tmp));
+    launch_felzenszwalb_2d(w, h, dist, tmp, as_stream(stream));
+    return finish(as_stream(stream), "felzenszwalb_2d");
+}
+
+int dolhip_felzenszwalb_2d(int w, int h, float* dist)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(dist, "dist"));
+    TRY(require_device());
+    const size_t bytes = (size_t)w * h * 4;
+    DevBuf a, t;
+    TRY(a.upload(dist, bytes));
+    TRY(t.alloc(felz_workspace(w, h)));
+    launch_felzenszwalb_2d(w, h, a.as<float>(), t.p, 0);
+    TRY(finish_sync("felzenszwalb_2d"));
+    return a.download(dist, bytes);
+}
+
+// ---- exact EDT
+
+size_t dolhip_edt_2d_workspace(int w, int h)
+{
+    return (w < 1 || h < 1) ? 0 : edt_workspace((long long)w * h, 2);
+}
+
+size_t dolhip_edt_3d_workspace(int w, int h, int d)
+{
+    return (w < 1 || h < 1 || d < 1) ? 0 : edt_workspace((long long)w * h * d, 3);
+}
+
+int dolhip_edt_2d_dev(int w, int h, const uint32_t* mask, int invert, int32_t* d2, float* dist,
+                      void* work, void* stream)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(mask, "mask"));
+    if (!d2 && !dist) return fail(DOLHIP_ERR_INVALID_ARG, "d2 and dist are both null");
+    TRY(require_device());
+    void* tmp = nullptr;
+    TRY(resolve_work(work, dolhip_edt_2d_workspace(w, h), &tmp));
+    launch_edt_2d(w, h, mask, invert, d2, dist, tmp, as_stream(stream));
+    return finish(as_stream(stream), "edt_2d");
+}
+
+int dolhip_edt_2d(int w, int h, const uint32_t* mask, int invert, int32_t* d2, float* dist)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(mask, "mask"));
+    if (!d2 && !dist) return fail(DOLHIP_ERR_INVALID_ARG, "d2 and dist are both null");
+    TRY(require_device());
+    const size_t cells = (size_t)w * h;
+    DevBuf m, o2, of, t;
+    TRY(m.upload(mask, mask_bytes(w, h)));
+    if (d2) TRY(o2.alloc(cells * 4));
+    if (dist) TRY(of.alloc(cells * 4));
+    TRY(t.alloc(dolhip_edt_2d_workspace(w, h)));
+    launch_edt_2d(w, h, m.as<uint32_t>(), invert, o2.as<int32_t>(), of.as<float>(), t.p, 0);
+    TRY(finish_sync("edt_2d"));
+    if (d2) TRY(o2.download(d2, cells * 4));
+    if (dist) TRY(of.download(dist, cells * 4));
+    return DOLHIP_OK;
+}
+
+int dolhip_edt_3d_dev(int w, int h, int d, const uint32_t* mask, int invert, int32_t* d2,
+                      float* dist, void* work, void* stream)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(mask, "mask"));
+    if (!d2 && !dist) return fail(DOLHIP_ERR_INVALID_ARG, "d2 and dist are both null");
+    TRY(require_device());
+    void* tmp = nullptr;
+    TRY(resolve_work(work, dolhip_edt_3d_workspace(w, h, d), &tmp));
+    launch_edt_3d(w, h, d, mask, invert, d2, dist, tmp, as_stream(stream));
+    return finish(as_stream(stream), "edt_3d");
+}
+
+int dolhip_edt_3d(int w, int h, int d, const uint32_t* mask, int invert, int32_t* d2,
+                  float* dist)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(mask, "mask"));
+    if (!d2 && !dist) return fail(DOLHIP_ERR_INVALID_ARG, "d2 and dist are both null");
+    TRY(require_device());
+    const size_t cells = (size_t)w * h * d;
+    DevBuf m, o2, of, t;
+    TRY(m.upload(mask, mask_bytes(w, (long long)h * d)));
+    if (d2) TRY(o2.alloc(cells * 4));
+    if (dist) TRY(of.alloc(cells * 4));
+    TRY(t.alloc(dolhip_edt_3d_workspace(w, h, d)));
+    launch_edt_3d(w, h, d, m.as<uint32_t>(), invert, o2.as<int32_t>(), of.as<float>(), t.p, 0);
+    TRY(finish_sync("edt_3d"));
+    if (d2) TRY(o2.download(d2, cells * 4));
+    if (dist) TRY(of.download(dist, cells * 4));
+    return DOLHIP_OK;
+}
+
+// two fields (to set bits, to clear bits) in the work area after the EDT's own
+static int signed_edt_dev(int w, int h, int d, const uint32_t* mask, float* sdf, void* work,
+                          hipStream_t s)
+{
+    const long long cells = (long long)w * h * d;
+    const size_t field = ((size_t)cells * 4 + 255) & ~(size_t)255;
+    const size_t inner = d == 1 ? dolhip_edt_2d_workspace(w, h) : dolhip_edt_3d_workspace(w, h, d);
+    void* tmp = nullptr;
+    TRY(resolve_work(work, inner + 2 * field, &tmp));
+    int32_t* fout = (int32_t*)((char*)tmp + inner);
+    int32_t* fin = (int32_t*)((char*)tmp + inner + field);
+    if (d == 1) {
+        launch_edt_2d(w, h, mask, 0, fout, nullptr, tmp, s);
+        launch_edt_2d(w, h, mask, 1, fin, nullptr, tmp, s);
+    } else {
+        launch_edt_3d(w, h, d, mask, 0, fout, nullptr, tmp, s);
+        launch_edt_3d(w, h, d, mask, 1, fin, nullptr, tmp, s);
+    }
+    launch_signed_edt((size_t)cells, fout, fin, sdf, s);
+    return check_launch("signed_edt");
+}
+
+int dolhip_signed_edt_2d_dev(int w, int h, const uint32_t* mask, float* sdf, void* work,
+                             void* stream)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(sdf, "sdf"));
+    TRY(require_device());
+    return signed_edt_dev(w, h, 1, mask, sdf, work, as_stream(stream));
+}
+
+int dolhip_signed_edt_3d_dev(int w, int h, int d, const uint32_t* mask, float* sdf, void* work,
+                             void* stream)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(sdf, "sdf"));
+    TRY(require_device());
+    return signed_edt_dev(w, h, d, mask, sdf, work, as_stream(stream));
+}
+
+static int signed_edt_host(int w, int h, int d, const uint32_t* mask, float* sdf)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(sdf, "sdf"));
+    TRY(require_device());
+    const size_t cells = (size_t)w * h * d;
+    DevBuf m, o;
+    TRY(m.upload(mask, mask_bytes(w, (long long)h * d)));
+    TRY(o.alloc(cells * 4));
+    TRY(signed_edt_dev(w, h, d, m.as<uint32_t>(), o.as<float>(), nullptr, 0));
+    TRY(finish_sync("signed_edt"));
+    return o.download(sdf, cells * 4);
+}
+
+int dolhip_signed_edt_2d(int w, int h, const uint32_t* mask, float* sdf)
+{ return signed_edt_host(w, h, 1, mask, sdf); }
+int dolhip_signed_edt_3d(int w, int h, int d, const uint32_t* mask, float* sdf)
+{ return signed_edt_host(w, h, d, mask, sdf); }
+
+// ---- distances and encoders
+
+int dolhip_dist_2d_dev(int n, const int16_t* delta, float* out, void* stream)
+{
+    if (n < 0) return fail(DOLHIP_ERR_INVALID_ARG, "n must be >= 0");
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_ptr(out, "out"));
+    TRY(require_device());
+    if (n) launch_dist_2d((size_t)n, delta, out, as_stream(stream));
+    return finish(as_stream(stream), "dist_2d");
+}
+
+static int signed_dev(int w, int h, int d, const uint32_t* mask, const int16_t* delta,
+                      float max_len, void* out, int mode, void* stream)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_ptr(out, "out"));
+    if (mode == 1 && !(max_len > 0.0f)) return fail(DOLHIP_ERR_INVALID_ARG, "max_len must be > 0");
+    TRY(require_device());
+    if (d == 1 && mode >= 0)
+        launch_signed_2d(w, h, mask, delta, max_len, out, mode, as_stream(stream));
+    else
+        launch_signed_3d(w, h, d, mask, delta, max_len, out, mode, as_stream(stream));
+    return finish(as_stream(stream), "signed");
+}
+
+static int signed_host(int w, int h, int d, bool three, const uint32_t* mask,
+                       const int16_t* delta, float max_len, void* out, int mode)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_ptr(out, "out"));
+    if (mode == 1 && !(max_len > 0.0f)) return fail(DOLHIP_ERR_INVALID_ARG, "max_len must be > 0");
+    TRY(require_device());
+    const size_t cells = (size_t)w * h * d;
+    const size_t obytes = cells * (mode == 0 ? 4 : 1);
+    DevBuf m, dl, o;
+    TRY(m.upload(mask, mask_bytes(w, (long long)h * d)));
+    TRY(dl.upload(delta, cells * (three ? 6 : 4)));
+    TRY(o.alloc(obytes));
+    if (!three)
+        launch_signed_2d(w, h, m.as<uint32_t>(), dl.as<int16_t>(), max_len, o.p, mode, 0);
+    else
+        launch_signed_3d(w, h, d, m.as<uint32_t>(), dl.as<int16_t>(), max_len, o.p, mode, 0);
+    TRY(finish_sync("signed"));
+    return o.download(out, obytes);
+}
+
+int dolhip_signed_distance_2d_dev(int w, int h, const uint32_t* mask, const int16_t* delta,
+                                  float* out, void* stream)
+{ return signed_dev(w, h, 1, mask, delta, 1.0f, out, 0, stream); }
+
+int dolhip_signed_distance_2d(int w, int h, const uint32_t* mask, const int16_t* delta,
+                              float* out)
+{ return signed_host(w, h, 1, false, mask, delta, 1.0f, out, 0); }
+
+int dolhip_signed_distance_3d_dev(int w, int h, int d, const uint32_t* mask,
+                                  const int16_t* delta, float* out, void* stream)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_ptr(out, "out"));
+    TRY(require_device());
+    launch_signed_3d(w, h, d, mask, delta, 1.0f, out, 0, as_stream(stream));
+    return finish(as_stream(stream), "signed_dist_3d");
+}
+
+int dolhip_signed_distance_3d(int w, int h, int d, const uint32_t* mask, const int16_t* delta,
+                              float* out)
+{ return signed_host(w, h, d, true, mask, delta, 1.0f, out, 0); }
+
+int dolhip_encode_sdf_u8_2d_dev(int w, int h, const uint32_t* mask, const int16_t* delta,
+                                float max_len, uint8_t* out, void* stream)
+{ return signed_dev(w, h, 1, mask, delta, max_len, out, 1, stream); }
+
+int dolhip_encode_sdf_u8_2d(int w, int h, const uint32_t* mask, const int16_t* delta,
+                            float max_len, uint8_t* out)
+{ return signed_host(w, h, 1, false, mask, delta, max_len, out, 1); }
+
+int dolhip_encode_sdf_u8_3d_dev(int w, int h, int d, const uint32_t* mask,
+                                const int16_t* delta, float max_len, uint8_t* out, void* stream)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_ptr(out, "out"));
+    if (!(max_len > 0.0f)) return fail(DOLHIP_ERR_INVALID_ARG, "max_len must be > 0");
+    TRY(require_device());
+    launch_signed_3d(w, h, d, mask, delta, max_len, out, 1, as_stream(stream));
+    return finish(as_stream(stream), "encode_sdf_u8_3d");
+}
+
+int dolhip_encode_sdf_u8_3d(int w, int h, int d, const uint32_t* mask, const int16_t* delta,
+                            float max_len, uint8_t* out)
+{ return signed_host(w, h, d, true, mask, delta, max_len, out, 1); }
+
+int dolhip_encode_sdf_delta_rgba_2d(int w, int h, const int16_t* delta, float max_len,
+                                    uint8_t* out)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_ptr(out, "out"));
+    if (!(max_len > 0.0f)) return fail(DOLHIP_ERR_INVALID_ARG, "max_len must be > 0");
+    TRY(require_device());
+    const size_t cells = (size_t)w * h;
+    DevBuf dl, o;
+    TRY(dl.upload(delta, cells * 4));
+    TRY(o.alloc(cells * 4));
+    launch_sdf_delta_rgba(w, h, dl.as<int16_t>(), max_len, o.as<uint8_t>(), 0);
+    TRY(finish_sync("encode_sdf_delta"));
+    return o.download(out, cells * 4);
+}
+
+int dolhip_encode_sdf_rg_rgba_2d(int w, int h, const uint32_t* mask, const int16_t* delta,
+                                 float max_len, uint8_t* out)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_ptr(out, "out"));
+    if (!(max_len > 0.0f)) return fail(DOLHIP_ERR_INVALID_ARG, "max_len must be > 0");
+    TRY(require_device());
+    const size_t cells = (size_t)w * h;
+    DevBuf m, dl, o;
+    TRY(m.upload(mask, mask_bytes(w, h)));
+    TRY(dl.upload(delta, cells * 4));
+    TRY(o.alloc(cells * 4));
+    launch_sdf_rg_rgba(w, h, m.as<uint32_t>(), dl.as<int16_t>(), max_len, o.as<uint8_t>(), 0);
+    TRY(finish_sync("encode_sdf_rg"));
+    return o.download(out, cells * 4);
+}
+
+// ---- signed pipelines
+
+int dolhip_signed_border_2d_dev(int w, int h, const uint32_t* mask, int16_t* delta, void* work,
+                                void* stream)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(delta, "delta"));
+    TRY(require_device());
+    void* tmp = nullptr;
+    TRY(resolve_work(work, dolhip_jump_flood_2d_workspace(w, h), &tmp));
+    launch_border_2d(w, h, mask, delta, as_stream(stream));
+    jfa2d_schedule(w, h, delta, 2, (int16_t*)tmp, as_stream(stream));
+    return finish(as_stream(stream), "signed_border_2d");
+}
+
+int dolhip_signed_border_3d_dev(int w, int h, int d, const uint32_t* mask, int16_t* delta,
+                                void* work, void* stream)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(delta, "delta"));
+    TRY(require_device());
+    void* tmp = nullptr;
+    TRY(resolve_work(work, dolhip_jump_flood_3d_workspace(w, h, d), &tmp));
+    launch_border_3d(w, h, d, mask, delta, as_stream(stream));
+    jfa3d_schedule(w, h, d, delta, 2, (int16_t*)tmp, as_stream(stream));
+    return finish(as_stream(stream), "signed_border_3d");
+}
+
+}  // extern "C"

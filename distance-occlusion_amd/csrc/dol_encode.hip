@@ -1,0 +1,187 @@
+// dol_encode.hip -- distances and image encoders of the reference tool
+// (DistanceOcclusionTool.cpp:46-75, 678-698, 1029-1040, 1240-1331, 1367-1469),
+// computed on the device so results leave the GPU as float or 8-bit images.
+//
+// Float semantics follow the reference exactly: the squared length is an
+// int32, converted to float and passed to a correctly rounded sqrtf; divides
+// are correctly rounded; 128 + v * 127 is a separate multiply and add (no
+// FMA); lrintf is round-half-to-even.
+#include "dol_common.hpp"
+
+#include <cfloat>
+
+namespace dolhip {
+
+// EncodeU8Signed, DistanceOcclusionTool.cpp:71-75
+__device__ __forceinline__ uint8_t encode_u8_signed(float v)
+{
+    v = (v < -1.0f) ? -1.0f : ((v > 1.0f) ? 1.0f : v);
+    return (uint8_t)(int)__builtin_rintf(__fadd_rn(128.0f, __fmul_rn(v, 127.0f)));
+}
+
+// EncodeU8, DistanceOcclusionLib.h:137-141
+__device__ __forceinline__ uint8_t encode_u8(float v)
+{
+    v = (v < 0.0f) ? 0.0f : ((v > 1.0f) ? 1.0f : v);
+    return (uint8_t)(int)__builtin_rintf(__fmul_rn(v, 255.0f));
+}
+
+__device__ __forceinline__ float dist_of(int l2) { return sqrt_rn((float)l2); }
+
+__global__ void __launch_bounds__(256) k_dist_2d(size_t n, const uint32_t* __restrict__ delta,
+                                                 float* __restrict__ out)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t v = delta[i];
+    out[i] = dist_of(len2(unpack2x(v), unpack2y(v)));
+}
+
+void launch_dist_2d(size_t n, const int16_t* delta, float* out, hipStream_t s)
+{
+    ProfileScope p("dist_2d", s);
+    hipLaunchKernelGGL(k_dist_2d, dim3(grid1d(n, 256)), dim3(256), 0, s, n, (const uint32_t*)delta,
+                       out);
+}
+
+// mode 0: signed float distance in cells, (bit ? -0.5 : 0.5) * Dist
+// mode 1: EncodeU8Signed((bit ? -Dist : Dist) / max_len)
+template <int MODE>
+__global__ void __launch_bounds__(256) k_signed_2d(int w, int xb, const uint32_t* __restrict__ mask,
+                                                   const uint32_t* __restrict__ delta, float max_len,
+                                                   void* __restrict__ out)
+{
+    const int y = blockIdx.x / xb;
+    const int x = (blockIdx.x - y * xb) * 256 + threadIdx.x;
+    if (x >= w) return;
+    const int ws = mask_stride(w);
+    const size_t i = (size_t)y * w + x;
+    const uint32_t v = delta[i];
+    const int bit = mask_bit(mask, ws, (size_t)y, x);
+    const float dd = dist_of(len2(unpack2x(v), unpack2y(v)));
+    if (MODE == 0) {
+        const float h = __fmul_rn(0.5f, dd);
+        ((float*)out)[i] = bit ? -h : h;
+    } else {
+        const float q = __fdiv_rn(dd, max_len);
+        ((uint8_t*)out)[i] = encode_u8_signed(bit ? -q : q);
+    }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_signed_3d(int w, int xb, const uint32_t* __restrict__ mask,
+                                                   const int16_t* __restrict__ delta, float max_len,
+                                                   void* __restrict__ out)
+{
+    const int row = blockIdx.x / xb;
+    const int x = (blockIdx.x - row * xb) * 256 + threadIdx.x;
+    if (x >= w) return;
+    const int ws = mask_stride(w);
+    const size_t i = (size_t)row * w + x;
+    const int16_t* c = delta + 3 * i;
+    const int bit = mask_bit(mask, ws, (size_t)row, x);
+    const float dd = dist_of(len2(c[0], c[1], c[2]));
+    if (MODE == 0) {
+        const float h = __fmul_rn(0.5f, dd);
+        ((float*)out)[i] = bit ? -h : h;
+    } else {
+        const float q = __fdiv_rn(dd, max_len);
+        ((uint8_t*)out)[i] = encode_u8_signed(bit ? -q : q);
+    }
+}
+
+void launch_signed_2d(int w, int h, const uint32_t* mask, const int16_t* delta, float max_len,
+                      void* out, int mode, hipStream_t s)
+{
+    int xb = (w + 255) / 256;
+    ProfileScope p(mode == 0 ? "signed_dist_2d" : "encode_sdf_u8_2d", s);
+    if (mode == 0)
+        hipLaunchKernelGGL(k_signed_2d<0>, dim3((unsigned)(xb * h)), dim3(256), 0, s, w, xb, mask,
+                           (const uint32_t*)delta, max_len, out);
+    else
+        hipLaunchKernelGGL(k_signed_2d<1>, dim3((unsigned)(xb * h)), dim3(256), 0, s, w, xb, mask,
+                           (const uint32_t*)delta, max_len, out);
+}
+
+void launch_signed_3d(int w, int h, int d, const uint32_t* mask, const int16_t* delta,
+                      float max_len, void* out, int mode, hipStream_t s)
+{
+    int xb = (w + 255) / 256;
+    unsigned blocks = (unsigned)((size_t)xb * h * d);
+    ProfileScope p(mode == 0 ? "signed_dist_3d" : "encode_sdf_u8_3d", s);
+    if (mode == 0)
+        hipLaunchKernelGGL(k_signed_3d<0>, dim3(blocks), dim3(256), 0, s, w, xb, mask, delta,
+                           max_len, out);
+    else
+        hipLaunchKernelGGL(k_signed_3d<1>, dim3(blocks), dim3(256), 0, s, w, xb, mask, delta,
+                           max_len, out);
+}
+
+// delta image (DistanceOcclusionTool.cpp:1281-1288): x and y of the delta
+// over max_len, signed-encoded, blue 0, alpha 255
+__global__ void __launch_bounds__(256) k_sdf_delta_rgba(size_t n, const uint32_t* __restrict__ delta,
+                                                        float max_len, uint32_t* __restrict__ out)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t v = delta[i];
+    uint32_t r = encode_u8_signed(__fdiv_rn((float)unpack2x(v), max_len));
+    uint32_t g = encode_u8_signed(__fdiv_rn((float)unpack2y(v), max_len));
+    out[i] = r | (g << 8) | (255u << 24);
+}
+
+// red/green image (DistanceOcclusionTool.cpp:1289-1307): distance in the
+// channel given by the mask bit
+__global__ void __launch_bounds__(256) k_sdf_rg_rgba(int w, int xb, const uint32_t* __restrict__ mask,
+                                                     const uint32_t* __restrict__ delta, float max_len,
+                                                     uint32_t* __restrict__ out)
+{
+    const int y = blockIdx.x / xb;
+    const int x = (blockIdx.x - y * xb) * 256 + threadIdx.x;
+    if (x >= w) return;
+    const size_t i = (size_t)y * w + x;
+    const uint32_t v = delta[i];
+    const int bit = mask_bit(mask, mask_stride(w), (size_t)y, x);
+    const uint32_t e = encode_u8(__fdiv_rn(dist_of(len2(unpack2x(v), unpack2y(v))), max_len));
+    out[i] = (bit ? (e << 8) : e) | (255u << 24);
+}
+
+void launch_sdf_delta_rgba(int w, int h, const int16_t* delta, float max_len, uint8_t* out,
+                           hipStream_t s)
+{
+    size_t n = (size_t)w * h;
+    ProfileScope p("encode_sdf_delta", s);
+    hipLaunchKernelGGL(k_sdf_delta_rgba, dim3(grid1d(n, 256)), dim3(256), 0, s, n,
+                       (const uint32_t*)delta, max_len, (uint32_t*)out);
+}
+
+void launch_sdf_rg_rgba(int w, int h, const uint32_t* mask, const int16_t* delta, float max_len,
+                        uint8_t* out, hipStream_t s)
+{
+    int xb = (w + 255) / 256;
+    ProfileScope p("encode_sdf_rg", s);
+    hipLaunchKernelGGL(k_sdf_rg_rgba, dim3((unsigned)(xb * h)), dim3(256), 0, s, w, xb, mask,
+                       (const uint32_t*)delta, max_len, (uint32_t*)out);
+}
+
+// two exact fields -> signed distance sqrt(out) - sqrt(in)
+__global__ void __launch_bounds__(256) k_signed_edt(size_t n, const int32_t* __restrict__ dout,
+                                                    const int32_t* __restrict__ din,
+                                                    float* __restrict__ sdf)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t a = dout[i], b = din[i];
+    const float fa = a == kEdtInf ? sqrt_rn(FLT_MAX) : sqrt_rn((float)a);
+    const float fb = b == kEdtInf ? sqrt_rn(FLT_MAX) : sqrt_rn((float)b);
+    sdf[i] = __fsub_rn(fa, fb);
+}
+
+void launch_signed_edt(size_t n, const int32_t* dout, const int32_t* din, float* sdf,
+                       hipStream_t s)
+{
+    ProfileScope p("signed_edt_combine", s);
+    hipLaunchKernelGGL(k_signed_edt, dim3(grid1d(n, 256)), dim3(256), 0, s, n, dout, din, sdf);
+}
+
+}  // namespace dolhip

@@ -1,0 +1,434 @@
+// dol_init.hip -- mask packing and delta/distance initialisation for gfx950.
+//
+// Border initialisation: the reference builds the signed border seeds by
+// scattering writes from 2x2 (2x2x2) mask windows, row pair after row pair,
+// word after word, and lets later writes win (SetBorderCellsCombo,
+// DistanceOcclusionLib.cpp:504-599 and 1820-2150).  Here every thread owns one
+// output cell and replays, in that same global order, only the writes that
+// land on its cell.  The windows that can reach cell (x, y[, z]) are those at
+// bit b in {x, x+1}, row pair q in {y-1, y} and slice pair p in {z-1, z}, so a
+// thread needs just the 3x3(x3) pixels around it; cells whose neighbourhood is
+// uniform receive no write and stay at kMaxDelta.
+#include "dol_common.hpp"
+
+namespace dolhip {
+
+// ------------------------------------------------------------ byte -> bit
+
+// One thread per padded pixel; 32 consecutive threads form one mask word,
+// gathered with a wavefront ballot (threshold rule of
+// DistanceOcclusionTool.cpp:662).
+__global__ void __launch_bounds__(256) k_mask_from_bytes(int w, int ws, size_t total,
+                                                         const uint8_t* __restrict__ bytes,
+                                                         int threshold, int less_than,
+                                                         uint32_t* __restrict__ mask)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int pred = 0;
+    if (t < total) {
+        size_t row = t / (size_t)(ws * 32);
+        int x = (int)(t - row * (size_t)(ws * 32));
+        if (x < w) {
+            int v = bytes[row * (size_t)w + x];
+            pred = ((v >= threshold) != (less_than != 0));
+        }
+    }
+    unsigned long long b = __ballot(pred);
+    int lane = threadIdx.x & 63;
+    if ((lane & 31) == 0 && t < total) {
+        uint32_t word = lane == 0 ? (uint32_t)(b & 0xffffffffull) : (uint32_t)(b >> 32);
+        mask[t >> 5] = word;
+    }
+}
+
+void launch_mask_from_bytes(int w, int rows, const uint8_t* bytes, int threshold, int less_than,
+                            uint32_t* mask, hipStream_t s)
+{
+    int ws = mask_stride(w);
+    size_t total = (size_t)rows * ws * 32;
+    ProfileScope p("mask_from_bytes", s);
+    hipLaunchKernelGGL(k_mask_from_bytes, dim3(grid1d(total, 256)), dim3(256), 0, s, w, ws, total,
+                       bytes, threshold, less_than, mask);
+}
+
+// ------------------------------------------------------------ unsigned init
+
+// rows x ceil(w/256) blocks; block b covers row b / xb, columns (b % xb)*256 + tid
+__global__ void __launch_bounds__(256) k_init_deltas_2d(int w, int xb, const uint32_t* __restrict__ mask,
+                                                        uint32_t* __restrict__ delta, uint32_t invert)
+{
+    int row = blockIdx.x / xb;
+    int x = (blockIdx.x - row * xb) * 256 + threadIdx.x;
+    if (x >= w) return;
+    int ws = mask_stride(w);
+    uint32_t bit = ((mask[(size_t)row * ws + (x >> 5)] >> (x & 31)) & 1u) ^ invert;
+    delta[(size_t)row * w + x] = bit ? 0u : pack2(kMaxDelta2, kMaxDelta2);
+}
+
+__global__ void __launch_bounds__(256) k_init_deltas_3d(int w, int xb, const uint32_t* __restrict__ mask,
+                                                        int16_t* __restrict__ delta, uint32_t invert)
+{
+    int row = blockIdx.x / xb;
+    int x = (blockIdx.x - row * xb) * 256 + threadIdx.x;
+    if (x >= w) return;
+    int ws = mask_stride(w);
+    uint32_t bit = ((mask[(size_t)row * ws + (x >> 5)] >> (x & 31)) & 1u) ^ invert;
+    int16_t v = bit ? (int16_t)0 : (int16_t)kMaxDelta3;
+    int16_t* c = delta + 3 * ((size_t)row * w + x);
+    c[0] = v; c[1] = v; c[2] = v;
+}
+
+void launch_init_deltas_2d(int w, int h, const uint32_t* mask, int16_t* delta, int invert,
+                           hipStream_t s)
+{
+    int xb = (w + 255) / 256;
+    ProfileScope p("init_deltas_2d", s);
+    hipLaunchKernelGGL(k_init_deltas_2d, dim3((unsigned)(xb * h)), dim3(256), 0, s, w, xb, mask,
+                       (uint32_t*)delta, invert ? 1u : 0u);
+}
+
+void launch_init_deltas_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta, int invert,
+                           hipStream_t s)
+{
+    int xb = (w + 255) / 256;
+    ProfileScope p("init_deltas_3d", s);
+    hipLaunchKernelGGL(k_init_deltas_3d, dim3((unsigned)((size_t)xb * h * d)), dim3(256), 0, s, w,
+                       xb, mask, delta, invert ? 1u : 0u);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_init_dist(int w, int xb, const uint32_t* __restrict__ mask,
+                                                   T* __restrict__ dist, T max_d)
+{
+    int row = blockIdx.x / xb;
+    int x = (blockIdx.x - row * xb) * 256 + threadIdx.x;
+    if (x >= w) return;
+    int ws = mask_stride(w);
+    uint32_t bit = (mask[(size_t)row * ws + (x >> 5)] >> (x & 31)) & 1u;
+    dist[(size_t)row * w + x] = bit ? (T)0 : max_d;
+}
+
+template <class T>
+void launch_init_dist(int w, int h, const uint32_t* mask, T* dist, T max_d, hipStream_t s)
+{
+    int xb = (w + 255) / 256;
+    ProfileScope p("init_distances", s);
+    hipLaunchKernelGGL(k_init_dist<T>, dim3((unsigned)(xb * h)), dim3(256), 0, s, w, xb, mask,
+                       dist, max_d);
+}
+template void launch_init_dist<float>(int, int, const uint32_t*, float*, float, hipStream_t);
+template void launch_init_dist<int32_t>(int, int, const uint32_t*, int32_t*, int32_t, hipStream_t);
+template void launch_init_dist<int16_t>(int, int, const uint32_t*, int16_t*, int16_t, hipStream_t);
+
+// ------------------------------------------------------------ border init 2D
+
+// pixel with the x = -1 column replicating x = 0 (the "lastBit" seed of
+// DistanceOcclusionLib.cpp:516-517)
+__device__ __forceinline__ int px2(const uint32_t* __restrict__ mask, int ws, int x, int y)
+{
+    return mask_bit(mask, ws, (size_t)y, x < 0 ? 0 : x);
+}
+
+struct Cell2 {
+    int x, y;
+    __device__ void set(int vx, int vy) { x = vx; y = vy; }
+    __device__ void set_if_unset(int vx, int vy)
+    {
+        if (x == kMaxDelta2) { x = vx; y = vy; }
+    }
+};
+
+// Writes to cell (x, y) in the reference's order.  For row pair p the window
+// at bit b has  a = P(b, p), b' = P(b, p+1), and the same one column left;
+//   vert = a ^ b', horz = a ^ a_left, vprev = a_left ^ b'_left,
+//   corner = vert ^ vprev; a corner goes to (b + vprev - 1, p + horz).
+__global__ void __launch_bounds__(256) k_border_2d(int w, int h, int xb, const uint32_t* __restrict__ mask,
+                                                   uint32_t* __restrict__ delta)
+{
+    const int y = blockIdx.x / xb;
+    const int x = (blockIdx.x - y * xb) * 256 + threadIdx.x;
+    if (x >= w) return;
+    const int ws = mask_stride(w);
+    const bool right = x + 1 < w;
+    const bool word_end = (x & 31) == 31;
+
+    // 3x3 pixels: c[r][k] for row y-1+r, column x-1+k (right column 0 past w)
+    int c[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        int yy = y - 1 + r;
+        bool ok = yy >= 0 && yy < h;
+        c[r][0] = ok ? px2(mask, ws, x - 1, yy) : 0;
+        c[r][1] = ok ? px2(mask, ws, x, yy) : 0;
+        c[r][2] = (ok && right) ? px2(mask, ws, x + 1, yy) : 0;
+    }
+
+    Cell2 s{kMaxDelta2, kMaxDelta2};
+    const int any = (c[0][0] | c[0][1] | c[0][2] | c[1][0] | c[1][1] | c[1][2] | c[2][0] | c[2][1] | c[2][2]);
+    const int all = (c[0][0] & c[0][1] & c[1][0] & c[1][1] & c[2][0] & c[2][1]);
+    // rows out of range read as 0 and never take part in a window below, so
+    // a uniform check over the rows in range is enough
+    bool uniform;
+    {
+        int lo = y >= 1 ? 0 : 1, hi = y + 1 < h ? 2 : 1;
+        int ref = c[1][1];
+        uniform = true;
+        for (int r = lo; r <= hi; r++)
+            for (int k = 0; k < 3; k++)
+                if ((k < 2 || right) && c[r][k] != ref) uniform = false;
+        (void)any; (void)all;
+    }
+    if (!uniform) {
+        // pair p = y - 1: this cell is its lower row (r = 1), window rows 0/1
+        if (y >= 1) {
+            const int vert0 = c[0][1] ^ c[1][1], horz0 = c[0][1] ^ c[0][0];
+            const int vprev0 = c[0][0] ^ c[1][0], corner0 = vert0 ^ vprev0;
+            const int vert1 = c[0][2] ^ c[1][2], horz1 = c[0][2] ^ c[0][1];
+            const int vprev1 = vert0, corner1 = vert1 ^ vprev1;
+            const bool e1 = corner0 && vprev0 && horz0;
+            const bool e2 = right && corner1 && !vprev1 && horz1;
+            const bool e3 = vert0;
+            if (!word_end) {
+                if (e1) s.set_if_unset(-1, -1);
+                if (e2) s.set_if_unset(+1, -1);
+                if (e3) s.set(0, -1);
+            } else {
+                if (e1) s.set_if_unset(-1, -1);
+                if (e3) s.set(0, -1);
+                if (e2) s.set_if_unset(+1, -1);
+            }
+        }
+        if (y + 1 < h) {
+            // pair p = y: this cell is its upper row, window rows 1/2
+            const int vert0 = c[1][1] ^ c[2][1], horz0 = c[1][1] ^ c[1][0];
+            const int vprev0 = c[1][0] ^ c[2][0], corner0 = vert0 ^ vprev0;
+            const int vert1 = c[1][2] ^ c[2][2], horz1 = c[1][2] ^ c[1][1];
+            const int vprev1 = vert0, corner1 = vert1 ^ vprev1;
+            const bool f1 = corner0 && vprev0 && !horz0;
+            const bool f2 = right && corner1 && !vprev1 && !horz1;
+            const bool f3 = horz0;
+            const bool f4 = right && horz1;
+            const bool f5 = vert0;
+            if (!word_end) {
+                if (f1) s.set_if_unset(-1, +1);
+                if (f2) s.set_if_unset(+1, +1);
+                if (f3) s.set(-1, 0);
+                if (f4) s.set(+1, 0);
+                if (f5) s.set(0, +1);
+            } else {
+                if (f1) s.set_if_unset(-1, +1);
+                if (f3) s.set(-1, 0);
+                if (f5) s.set(0, +1);
+                if (f2) s.set_if_unset(+1, +1);
+                if (f4) s.set(+1, 0);
+            }
+        } else {
+            // last row: horizontal changes only (DistanceOcclusionLib.cpp:571-598)
+            if (c[1][1] ^ c[1][0]) s.set(-1, 0);
+            if (right && (c[1][2] ^ c[1][1])) s.set(+1, 0);
+        }
+    }
+    delta[(size_t)y * w + x] = pack2(s.x, s.y);
+}
+
+void launch_border_2d(int w, int h, const uint32_t* mask, int16_t* delta, hipStream_t s)
+{
+    int xb = (w + 255) / 256;
+    ProfileScope p("border_2d", s);
+    hipLaunchKernelGGL(k_border_2d, dim3((unsigned)(xb * h)), dim3(256), 0, s, w, h, xb, mask,
+                       (uint32_t*)delta);
+}
+
+// ------------------------------------------------------------ border init 3D
+
+struct Cell3 {
+    int x, y, z;
+    __device__ void set(int vx, int vy, int vz) { x = vx; y = vy; z = vz; }
+};
+
+// Loop kinds of one window, in the reference's order
+// (DistanceOcclusionLib.cpp:1899-1973)
+enum : int { L_BC = 1, L_FXX = 2, L_FY = 4, L_FZ = 8, L_EX = 16, L_EYX = 32, L_EZX = 64 };
+
+// Pixel cube around the cell: bit (dz*9 + dy*3 + dx) is pixel
+// (x-1+dx, y-1+dy, z-1+dz), x = -1 replicating x = 0; outside the volume 0.
+__device__ __forceinline__ int cube_bit(uint32_t cube, int dx, int dy, int dz)
+{
+    return (int)((cube >> (dz * 9 + dy * 3 + dx)) & 1u);
+}
+
+struct Win3 {
+    int ex, ey, ez, ezx, eyx, fx, fy, fz, fxx, bc;
+};
+
+// window whose "x-1" column is cube column kx (0 or 1), rows ky/ky+1,
+// slices kz/kz+1 (DistanceOcclusionLib.cpp:1862-1894)
+__device__ __forceinline__ Win3 window3(uint32_t cube, int kx, int ky, int kz)
+{
+    const int p000 = cube_bit(cube, kx, ky, kz), p001 = cube_bit(cube, kx + 1, ky, kz);
+    const int p010 = cube_bit(cube, kx, ky + 1, kz), p011 = cube_bit(cube, kx + 1, ky + 1, kz);
+    const int p100 = cube_bit(cube, kx, ky, kz + 1), p101 = cube_bit(cube, kx + 1, ky, kz + 1);
+    const int p110 = cube_bit(cube, kx, ky + 1, kz + 1), p111 = cube_bit(cube, kx + 1, ky + 1, kz + 1);
+    Win3 r;
+    r.ex = p000 ^ p001;
+    r.ey = p000 ^ p010;
+    r.ez = p000 ^ p100;
+    r.ezx = p001 ^ p101;
+    const int eyz = p100 ^ p110, exy = p010 ^ p011;
+    r.fx = r.ey ^ eyz;
+    r.fy = r.ez ^ r.ezx;
+    r.fz = r.ex ^ exy;
+    r.eyx = p001 ^ p011;
+    const int eyxz = p101 ^ p111;
+    r.fxx = r.eyx ^ eyxz;
+    r.bc = r.fz ^ eyz ^ eyxz;
+    return r;
+}
+
+// Apply loop `kind` of window (b, q, p) to the cell (tx, ty, tz relative
+// offsets: the cell sits at (b - bx, q - qy, p - pz) for window origin
+// offsets bx in {0, 1} (b = x or x+1), etc.  Coordinates here are relative to
+// the window: the cell's position inside the 2x2x2 box is (cx, cy, cz) with
+// cx in {0: column b-1, 1: column b}, cy in {0: row q, 1: row q+1},
+// cz in {0: slice p, 1: slice p+1}.
+__device__ __forceinline__ void apply3(Cell3& s, const Win3& v, int kind, int cx, int cy, int cz)
+{
+    switch (kind) {
+    case L_BC:
+        if (v.bc && cx == v.fx && cy == v.fy && cz == v.fz && s.x == kMaxDelta3)
+            s.set(1 - 2 * v.fx, 1 - 2 * v.fy, 1 - 2 * v.fz);
+        break;
+    case L_FXX:
+        if (v.fxx && cx == 1 && cy == v.ezx && cz == v.eyx &&
+            (s.x == kMaxDelta3 || abs(s.x) + abs(s.y) + abs(s.z) > 1))
+            s.set(0, 1 - 2 * v.ezx, 1 - 2 * v.eyx);
+        break;
+    case L_FY:
+        if (v.fy && cx == v.ez && cy == 0 && cz == v.ex &&
+            (s.x == kMaxDelta3 || abs(s.x) + abs(s.y) + abs(s.z) > 1))
+            s.set(1 - 2 * v.ez, 0, 1 - 2 * v.ex);
+        break;
+    case L_FZ:
+        if (v.fz && cx == v.ey && cy == v.ex && cz == 0 &&
+            (s.x == kMaxDelta3 || abs(s.x) + abs(s.y) + abs(s.z) > 1))
+            s.set(1 - 2 * v.ey, 1 - 2 * v.ex, 0);
+        break;
+    case L_EX:
+        if (v.ex && cy == 0 && cz == 0) {
+            if (cx == 0) s.set(+1, 0, 0);
+            else s.set(-1, 0, 0);
+        }
+        break;
+    case L_EYX:
+        if (v.eyx && cx == 1 && cz == 0) {
+            if (cy == 0) s.set(0, +1, 0);
+            else s.set(0, -1, 0);
+        }
+        break;
+    case L_EZX:
+        if (v.ezx && cx == 1 && cy == 0) {
+            if (cz == 0) s.set(0, 0, +1);
+            else s.set(0, 0, -1);
+        }
+        break;
+    }
+}
+
+// One row-pair step (fixed q, p) for the windows at b = x and b = x + 1 with
+// the loop kinds in `kinds`, ordered as the reference's word loop orders
+// them: loop-major when both bits share a word, bit-major across a word end.
+__device__ __forceinline__ void row_step3(Cell3& s, uint32_t cube, int kinds, bool right,
+                                          bool word_end, int ky, int kz, int cy, int cz)
+{
+    // window at b = x: columns x-1, x -> kx = 0, cell is column b (cx = 1)
+    // window at b = x+1: columns x, x+1 -> kx = 1, cell is column b-1 (cx = 0)
+    const Win3 w0 = window3(cube, 0, ky, kz);
+    const Win3 w1 = window3(cube, 1, ky, kz);
+    if (!word_end) {
+#pragma unroll
+        for (int k = 1; k <= L_EZX; k <<= 1) {
+            if (!(kinds & k)) continue;
+            apply3(s, w0, k, 1, cy, cz);
+            if (right) apply3(s, w1, k, 0, cy, cz);
+        }
+    } else {
+#pragma unroll
+        for (int k = 1; k <= L_EZX; k <<= 1)
+            if (kinds & k) apply3(s, w0, k, 1, cy, cz);
+        if (right) {
+#pragma unroll
+            for (int k = 1; k <= L_EZX; k <<= 1)
+                if (kinds & k) apply3(s, w1, k, 0, cy, cz);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_border_3d(int w, int h, int d, int xb,
+                                                   const uint32_t* __restrict__ mask,
+                                                   int16_t* __restrict__ delta)
+{
+    const int row = blockIdx.x / xb;           // z*h + y
+    const int x = (blockIdx.x - row * xb) * 256 + threadIdx.x;
+    if (x >= w) return;
+    const int z = row / h, y = row - z * h;
+    const int ws = mask_stride(w);
+    const bool right = x + 1 < w;
+    const bool word_end = (x & 31) == 31;
+
+    // gather the 3x3x3 pixel cube and the bits that lie inside the volume
+    uint32_t cube = 0, inside = 0;
+    for (int dz = 0; dz < 3; dz++) {
+        const int zz = z - 1 + dz;
+        if (zz < 0 || zz >= d) continue;
+        for (int dy = 0; dy < 3; dy++) {
+            const int yy = y - 1 + dy;
+            if (yy < 0 || yy >= h) continue;
+            const uint32_t* r = mask + ((size_t)zz * h + yy) * ws;
+            const int xl = x > 0 ? x - 1 : 0;
+            const int b0 = (int)((r[xl >> 5] >> (xl & 31)) & 1u);
+            const int b1 = (int)((r[x >> 5] >> (x & 31)) & 1u);
+            const int b2 = right ? (int)((r[(x + 1) >> 5] >> ((x + 1) & 31)) & 1u) : 0;
+            const int base = dz * 9 + dy * 3;
+            cube |= (uint32_t)(b0 | (b1 << 1) | (b2 << 2)) << base;
+            inside |= (right ? 7u : 3u) << base;
+        }
+    }
+    Cell3 s{kMaxDelta3, kMaxDelta3, kMaxDelta3};
+    const uint32_t in_set = cube & inside;
+    if (in_set != 0 && in_set != inside) {
+        const int all = L_BC | L_FXX | L_FY | L_FZ | L_EX | L_EYX | L_EZX;
+        // slice pairs p = z-1 (cell in slice p+1, cz = 1, kz = 0) then p = z
+        for (int pp = 0; pp < 2; pp++) {
+            const int p = z - 1 + pp;
+            if (p < 0 || p + 1 >= d) continue;
+            const int kz = pp, cz = 1 - pp;
+            for (int qq = 0; qq < 2; qq++) {
+                const int q = y - 1 + qq;
+                if (q < 0 || q + 1 >= h) continue;
+                row_step3(s, cube, all, right, word_end, qq, kz, 1 - qq, cz);
+            }
+            if (y == h - 1)   // last row of the pair: y faces, x and z edges
+                row_step3(s, cube, L_FY | L_EX | L_EZX, right, word_end, 1, kz, 0, cz);
+        }
+        if (z == d - 1) {     // last slice: z faces, x and y edges
+            for (int qq = 0; qq < 2; qq++) {
+                const int q = y - 1 + qq;
+                if (q < 0 || q + 1 >= h) continue;
+                row_step3(s, cube, L_FZ | L_EX | L_EYX, right, word_end, qq, 1, 1 - qq, 0);
+            }
+            if (y == h - 1) row_step3(s, cube, L_EX, right, word_end, 1, 1, 0, 0);
+        }
+    }
+    int16_t* c = delta + 3 * ((size_t)row * w + x);
+    c[0] = (int16_t)s.x; c[1] = (int16_t)s.y; c[2] = (int16_t)s.z;
+}
+
+void launch_border_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta, hipStream_t s)
+{
+    int xb = (w + 255) / 256;
+    ProfileScope p("border_3d", s);
+    hipLaunchKernelGGL(k_border_3d, dim3((unsigned)((size_t)xb * h * d)), dim3(256), 0, s, w, h, d,
+                       xb, mask, delta);
+}
+
+}  // namespace dolhip

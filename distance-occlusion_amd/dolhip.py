@@ -1,0 +1,319 @@
+"""Python face of libdolhip (include/dol_hip.h) for tests, benchmarks and
+torch users.
+
+Host-array methods take and return numpy arrays with the reference's layouts
+(DistanceOcclusionLib.h:16-57): bitmasks as flat uint32 words, 2D deltas as
+int16 (h, w, 2), 3D deltas as int16 (d, h, w, 3).  `*_dev` methods take
+torch tensors (or raw device addresses) already on the GPU and a stream
+(default: torch's current stream) and return without synchronising.
+
+There is no CPU fallback: loading fails if libdolhip.so was not built, and
+every compute call raises DolHipError when no gfx950 device is usable.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdolhip.so")
+
+OK = 0
+ERR_INVALID_ARG = 1
+ERR_OUT_OF_MEMORY = 2
+ERR_DEVICE = 3
+ERR_NO_DEVICE = 4
+
+MAX_DELTA2 = 32767
+MAX_DELTA3 = 25000
+EDT_INF = 0x7FFFFFFF
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+_SZ = ctypes.c_size_t
+
+
+class DolHipError(RuntimeError):
+    def __init__(self, code, name, msg):
+        super().__init__(f"{name} failed ({code}): {msg}")
+        self.code = code
+
+
+# name -> (argtypes, restype); _P for every pointer and the stream
+_SIGS = {
+    "dolhip_last_error": ([], ctypes.c_char_p),
+    "dolhip_version": ([], ctypes.c_char_p),
+    "dolhip_device_count": ([], _I),
+    "dolhip_profile_enable": ([_I], _I),
+    "dolhip_profile_query": ([ctypes.c_char_p, _P, _P], _I),
+    "dolhip_mask_from_bytes": ([_I, _I, _P, _I, _I, _P], _I),
+    "dolhip_mask_from_bytes_dev": ([_I, _I, _P, _I, _I, _P, _P], _I),
+    "dolhip_init_deltas_2d": ([_I, _I, _P, _P, _I], _I),
+    "dolhip_init_deltas_2d_dev": ([_I, _I, _P, _P, _I, _P], _I),
+    "dolhip_init_deltas_3d": ([_I, _I, _I, _P, _P, _I], _I),
+    "dolhip_init_deltas_3d_dev": ([_I, _I, _I, _P, _P, _I, _P], _I),
+    "dolhip_init_border_deltas_2d": ([_I, _I, _P, _P], _I),
+    "dolhip_init_border_deltas_2d_dev": ([_I, _I, _P, _P, _P], _I),
+    "dolhip_init_border_deltas_3d": ([_I, _I, _I, _P, _P], _I),
+    "dolhip_init_border_deltas_3d_dev": ([_I, _I, _I, _P, _P, _P], _I),
+    "dolhip_init_distances_f32": ([_I, _I, _P, _P, _F], _I),
+    "dolhip_init_distances_f32_dev": ([_I, _I, _P, _P, _F, _P], _I),
+    "dolhip_init_distances_i32": ([_I, _I, _P, _P, ctypes.c_int32], _I),
+    "dolhip_init_distances_i16": ([_I, _I, _P, _P, ctypes.c_int16], _I),
+    "dolhip_jump_flood_2d": ([_I, _I, _P, _I], _I),
+    "dolhip_jump_flood_2d_workspace": ([_I, _I], _SZ),
+    "dolhip_jump_flood_2d_dev": ([_I, _I, _P, _I, _P, _P], _I),
+    "dolhip_jump_flood_3d": ([_I, _I, _I, _P, _I], _I),
+    "dolhip_jump_flood_3d_workspace": ([_I, _I, _I], _SZ),
+    "dolhip_jump_flood_3d_dev": ([_I, _I, _I, _P, _I, _P, _P], _I),
+    "dolhip_felzenszwalb_2d": ([_I, _I, _P], _I),
+    "dolhip_felzenszwalb_2d_workspace": ([_I, _I], _SZ),
+    "dolhip_felzenszwalb_2d_dev": ([_I, _I, _P, _P, _P], _I),
+    "dolhip_edt_2d": ([_I, _I, _P, _I, _P, _P], _I),
+    "dolhip_edt_2d_workspace": ([_I, _I], _SZ),
+    "dolhip_edt_2d_dev": ([_I, _I, _P, _I, _P, _P, _P, _P], _I),
+    "dolhip_edt_3d": ([_I, _I, _I, _P, _I, _P, _P], _I),
+    "dolhip_edt_3d_workspace": ([_I, _I, _I], _SZ),
+    "dolhip_edt_3d_dev": ([_I, _I, _I, _P, _I, _P, _P, _P, _P], _I),
+    "dolhip_signed_edt_2d": ([_I, _I, _P, _P], _I),
+    "dolhip_signed_edt_2d_dev": ([_I, _I, _P, _P, _P, _P], _I),
+    "dolhip_signed_edt_3d": ([_I, _I, _I, _P, _P], _I),
+    "dolhip_signed_edt_3d_dev": ([_I, _I, _I, _P, _P, _P, _P], _I),
+    "dolhip_dist_2d_dev": ([_I, _P, _P, _P], _I),
+    "dolhip_signed_distance_2d": ([_I, _I, _P, _P, _P], _I),
+    "dolhip_signed_distance_2d_dev": ([_I, _I, _P, _P, _P, _P], _I),
+    "dolhip_signed_distance_3d": ([_I, _I, _I, _P, _P, _P], _I),
+    "dolhip_signed_distance_3d_dev": ([_I, _I, _I, _P, _P, _P, _P], _I),
+    "dolhip_encode_sdf_u8_2d": ([_I, _I, _P, _P, _F, _P], _I),
+    "dolhip_encode_sdf_u8_2d_dev": ([_I, _I, _P, _P, _F, _P, _P], _I),
+    "dolhip_encode_sdf_u8_3d": ([_I, _I, _I, _P, _P, _F, _P], _I),
+    "dolhip_encode_sdf_u8_3d_dev": ([_I, _I, _I, _P, _P, _F, _P, _P], _I),
+    "dolhip_encode_sdf_delta_rgba_2d": ([_I, _I, _P, _F, _P], _I),
+    "dolhip_encode_sdf_rg_rgba_2d": ([_I, _I, _P, _P, _F, _P], _I),
+    "dolhip_signed_border_2d_dev": ([_I, _I, _P, _P, _P, _P], _I),
+    "dolhip_signed_border_3d_dev": ([_I, _I, _I, _P, _P, _P, _P], _I),
+    "dolhip_batch_create": ([_I, _P, _P, _P, _I, _I, _P], _I),
+    "dolhip_batch_destroy": ([_P], _I),
+    "dolhip_batch_bytes": ([_P], ctypes.c_int64),
+    "dolhip_batch_sdf_u8_dev": ([_P, _P, _P, _I, _P], _I),
+    "dolhip_glyph_sdf_u8_dev": ([_I, _I, _P, _I, _I, _P, _P, _P], _I),
+    "dolhip_batch_config": ([_P, _P, _P], _I),
+    "dolhip_batch_tune": ([_P, _P, _P, _P], _I),
+    "dolhip_seed_workspace": ([_I, _I, _I], _SZ),
+    "dolhip_seed_nearest_2d_dev": ([_I, _I, _I, _P, _P, _P, _P, _P], _I),
+    "dolhip_seed_grad_2d_dev": ([_I, _I, _I, _P, _P, _P, _P, _P, _P], _I),
+}
+
+
+def _addr(x):
+    """numpy array / torch tensor / int / None -> c_void_p."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return ctypes.c_void_p(x)
+    if isinstance(x, np.ndarray):
+        return ctypes.c_void_p(x.ctypes.data)
+    if hasattr(x, "data_ptr"):
+        return ctypes.c_void_p(x.data_ptr())
+    raise TypeError(f"cannot take the address of {type(x)}")
+
+
+def _stream(stream):
+    if stream is None:
+        try:
+            import torch
+
+            if torch.cuda.is_available():
+                return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        except ImportError:
+            pass
+        return None
+    if isinstance(stream, int):
+        return ctypes.c_void_p(stream)
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def mask_stride(w):
+    return (w + 31) // 32
+
+
+class DolHip:
+    def __init__(self, path=LIB_PATH):
+        if not os.path.exists(path):
+            raise ImportError(
+                f"{path} is not built: run python distance-occlusion_amd/build.py "
+                "(there is no CPU fallback)")
+        self.path = path
+        self.L = ctypes.CDLL(path)
+        self.missing = []
+        for name, (args, res) in _SIGS.items():
+            try:
+                fn = getattr(self.L, name)
+            except AttributeError:
+                self.missing.append(name)
+                continue
+            fn.argtypes = args
+            fn.restype = res
+
+    # ---- plumbing
+    def last_error(self):
+        return self.L.dolhip_last_error().decode()
+
+    def _call(self, name, *args):
+        rc = getattr(self.L, name)(*args)
+        if rc != OK:
+            raise DolHipError(rc, name, self.last_error())
+        return rc
+
+    def device_count(self):
+        return self.L.dolhip_device_count()
+
+    def profile_enable(self, on=True):
+        self._call("dolhip_profile_enable", int(on))
+
+    def profile_query(self, prefix):
+        avg = ctypes.c_double(0.0)
+        cnt = ctypes.c_int(0)
+        self._call("dolhip_profile_query", prefix.encode(), ctypes.byref(avg), ctypes.byref(cnt))
+        return avg.value, cnt.value
+
+    # ---- host-array entry points
+    def mask_from_bytes(self, w, rows, data, threshold=128, less_than=True):
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        assert data.size == w * rows
+        mask = np.zeros(mask_stride(w) * rows, dtype=np.uint32)
+        self._call("dolhip_mask_from_bytes", w, rows, _addr(data), threshold, int(less_than), _addr(mask))
+        return mask
+
+    def init_deltas_2d(self, w, h, mask, invert=False):
+        out = np.empty((h, w, 2), dtype=np.int16)
+        self._call("dolhip_init_deltas_2d", w, h, _addr(np.ascontiguousarray(mask, np.uint32)), _addr(out), int(invert))
+        return out
+
+    def init_deltas_3d(self, w, h, d, mask, invert=False):
+        out = np.empty((d, h, w, 3), dtype=np.int16)
+        self._call("dolhip_init_deltas_3d", w, h, d, _addr(np.ascontiguousarray(mask, np.uint32)), _addr(out), int(invert))
+        return out
+
+    def init_border_deltas_2d(self, w, h, mask):
+        out = np.empty((h, w, 2), dtype=np.int16)
+        self._call("dolhip_init_border_deltas_2d", w, h, _addr(np.ascontiguousarray(mask, np.uint32)), _addr(out))
+        return out
+
+    def init_border_deltas_3d(self, w, h, d, mask):
+        out = np.empty((d, h, w, 3), dtype=np.int16)
+        self._call("dolhip_init_border_deltas_3d", w, h, d, _addr(np.ascontiguousarray(mask, np.uint32)), _addr(out))
+        return out
+
+    def init_distances_f32(self, w, h, mask, max_d=float(np.finfo(np.float32).max)):
+        out = np.empty((h, w), dtype=np.float32)
+        self._call("dolhip_init_distances_f32", w, h, _addr(np.ascontiguousarray(mask, np.uint32)), _addr(out), max_d)
+        return out
+
+    def init_distances_i32(self, w, h, mask, max_d=2**31 - 1):
+        out = np.empty((h, w), dtype=np.int32)
+        self._call("dolhip_init_distances_i32", w, h, _addr(np.ascontiguousarray(mask, np.uint32)), _addr(out), max_d)
+        return out
+
+    def init_distances_i16(self, w, h, mask, max_d=32767):
+        out = np.empty((h, w), dtype=np.int16)
+        self._call("dolhip_init_distances_i16", w, h, _addr(np.ascontiguousarray(mask, np.uint32)), _addr(out), max_d)
+        return out
+
+    def jump_flood_2d(self, w, h, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self._call("dolhip_jump_flood_2d", w, h, _addr(out), cw)
+        return out
+
+    def jump_flood_3d(self, w, h, d, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self._call("dolhip_jump_flood_3d", w, h, d, _addr(out), cw)
+        return out
+
+    def felzenszwalb_2d(self, w, h, dist):
+        out = np.ascontiguousarray(dist, dtype=np.float32).copy()
+        self._call("dolhip_felzenszwalb_2d", w, h, _addr(out))
+        return out
+
+    def edt_2d(self, w, h, mask, invert=False, want_d2=True, want_dist=True):
+        d2 = np.empty((h, w), dtype=np.int32) if want_d2 else None
+        fl = np.empty((h, w), dtype=np.float32) if want_dist else None
+        self._call("dolhip_edt_2d", w, h, _addr(np.ascontiguousarray(mask, np.uint32)), int(invert), _addr(d2), _addr(fl))
+        return d2, fl
+
+    def edt_3d(self, w, h, d, mask, invert=False, want_d2=True, want_dist=True):
+        d2 = np.empty((d, h, w), dtype=np.int32) if want_d2 else None
+        fl = np.empty((d, h, w), dtype=np.float32) if want_dist else None
+        self._call("dolhip_edt_3d", w, h, d, _addr(np.ascontiguousarray(mask, np.uint32)), int(invert), _addr(d2), _addr(fl))
+        return d2, fl
+
+    def signed_edt_2d(self, w, h, mask):
+        out = np.empty((h, w), dtype=np.float32)
+        self._call("dolhip_signed_edt_2d", w, h, _addr(np.ascontiguousarray(mask, np.uint32)), _addr(out))
+        return out
+
+    def signed_edt_3d(self, w, h, d, mask):
+        out = np.empty((d, h, w), dtype=np.float32)
+        self._call("dolhip_signed_edt_3d", w, h, d, _addr(np.ascontiguousarray(mask, np.uint32)), _addr(out))
+        return out
+
+    def signed_distance_2d(self, w, h, mask, delta):
+        out = np.empty((h, w), dtype=np.float32)
+        self._call("dolhip_signed_distance_2d", w, h, _addr(np.ascontiguousarray(mask, np.uint32)),
+                   _addr(np.ascontiguousarray(delta, np.int16)), _addr(out))
+        return out
+
+    def signed_distance_3d(self, w, h, d, mask, delta):
+        out = np.empty((d, h, w), dtype=np.float32)
+        self._call("dolhip_signed_distance_3d", w, h, d, _addr(np.ascontiguousarray(mask, np.uint32)),
+                   _addr(np.ascontiguousarray(delta, np.int16)), _addr(out))
+        return out
+
+    def encode_sdf_u8_2d(self, w, h, mask, delta, max_len):
+        out = np.empty((h, w), dtype=np.uint8)
+        self._call("dolhip_encode_sdf_u8_2d", w, h, _addr(np.ascontiguousarray(mask, np.uint32)),
+                   _addr(np.ascontiguousarray(delta, np.int16)), max_len, _addr(out))
+        return out
+
+    def encode_sdf_u8_3d(self, w, h, d, mask, delta, max_len):
+        out = np.empty((d, h, w), dtype=np.uint8)
+        self._call("dolhip_encode_sdf_u8_3d", w, h, d, _addr(np.ascontiguousarray(mask, np.uint32)),
+                   _addr(np.ascontiguousarray(delta, np.int16)), max_len, _addr(out))
+        return out
+
+    def encode_sdf_delta_rgba_2d(self, w, h, delta, max_len):
+        out = np.empty((h, w, 4), dtype=np.uint8)
+        self._call("dolhip_encode_sdf_delta_rgba_2d", w, h, _addr(np.ascontiguousarray(delta, np.int16)), max_len, _addr(out))
+        return out
+
+    def encode_sdf_rg_rgba_2d(self, w, h, mask, delta, max_len):
+        out = np.empty((h, w, 4), dtype=np.uint8)
+        self._call("dolhip_encode_sdf_rg_rgba_2d", w, h, _addr(np.ascontiguousarray(mask, np.uint32)),
+                   _addr(np.ascontiguousarray(delta, np.int16)), max_len, _addr(out))
+        return out
+
+    # ---- device entry points (torch tensors or raw addresses)
+    def workspace(self, name, *dims):
+        return getattr(self.L, f"dolhip_{name}_workspace")(*dims)
+
+    def call_dev(self, name, *args, stream=None):
+        """dolhip_<name>_dev with tensors/addresses converted; the stream is appended."""
+        conv = []
+        for a in args:
+            if a is None or isinstance(a, (np.ndarray,)) or hasattr(a, "data_ptr"):
+                conv.append(_addr(a))
+            else:
+                conv.append(a)
+        return self._call(f"dolhip_{name}_dev", *conv, _stream(stream))
+
+
+_LIB = None
+
+
+def load(path=LIB_PATH):
+    """The process-wide library instance."""
+    global _LIB
+    if _LIB is None:
+        _LIB = DolHip(path)
+    return _LIB

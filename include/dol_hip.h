@@ -1,0 +1,218 @@
+/*
+ * dol_hip.h -- C ABI of the MI355X (gfx950) distance-field library.
+ *
+ * It replaces the data-parallel distance-field path of DistanceOcclusionLib
+ * (DistanceOcclusionLib.h:61-98): initialisation from 1-bit masks, signed
+ * border initialisation, JumpFlood in 2D and 3D, Felzenszwalb, plus an exact
+ * integer squared EDT in 2D and 3D, the output encoders of the reference
+ * tool, batched ragged glyph SDFs and a continuous-seed distance with its
+ * gradient.
+ *
+ * Layouts are the reference's (DistanceOcclusionLib.h:16-57):
+ *   mask    uint32 words, rows padded to ceil(w/32) words, LSB first,
+ *           row r of slice z at word (z*h + r) * ceil(w/32)
+ *   delta2  int16 {x, y} per cell (4 bytes), index y*w + x
+ *   delta3  int16 {x, y, z} per cell (6 bytes, packed), index (z*h + y)*w + x
+ *   scalars row-major, same indexing as the deltas
+ *
+ * Two forms of every entry point:
+ *   dolhip_<name>(...)      host pointers; synchronous, like the reference's
+ *                           functions: copies in, runs, copies back, returns.
+ *   dolhip_<name>_dev(...)  device pointers and a hipStream_t (NULL = the
+ *                           default stream); asynchronous on that stream.
+ *                           Where a pass needs a temporary, `work` may point
+ *                           to device memory of at least the size the matching
+ *                           *_workspace() call returns, or be NULL, in which
+ *                           case the library uses a per-device buffer of its
+ *                           own that grows as needed (allocated synchronously).
+ *
+ * Every call returns a dolhip_status; dolhip_last_error() gives the message
+ * of the last failure on the calling thread.  The reference's functions
+ * return void and check nothing (DistanceOcclusionLib.cpp:14-18); here bad
+ * sizes and null pointers are rejected with DOLHIP_ERR_INVALID_ARG and device
+ * failures reported as DOLHIP_ERR_DEVICE.  There is no CPU fallback: without a
+ * usable gfx950 device every compute call fails with DOLHIP_ERR_NO_DEVICE.
+ */
+#ifndef DOL_HIP_H
+#define DOL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum dolhip_status {
+    DOLHIP_OK = 0,
+    DOLHIP_ERR_INVALID_ARG = 1,
+    DOLHIP_ERR_OUT_OF_MEMORY = 2,
+    DOLHIP_ERR_DEVICE = 3,
+    DOLHIP_ERR_NO_DEVICE = 4,
+} dolhip_status;
+
+#define DOLHIP_MAX_DELTA2 32767   /* kMaxDelta2, DistanceOcclusionLib.h:56 */
+#define DOLHIP_MAX_DELTA3 25000   /* kMaxDelta3, DistanceOcclusionLib.h:57 */
+#define DOLHIP_EDT_INF    0x7fffffff  /* squared distance of a cell no seed reaches */
+
+/* ---- runtime ------------------------------------------------------------ */
+const char* dolhip_last_error(void);
+const char* dolhip_version(void);
+int dolhip_device_count(void);            /* usable devices, 0 if none */
+/* Per-kernel timing with HIP events recorded on the launch stream, for
+ * measurement harnesses.  enable != 0 starts recording (clearing old records). */
+int dolhip_profile_enable(int enable);
+/* Average duration in ms and launch count of kernels whose name starts with
+ * `prefix`, over the records collected so far (synchronises the device). */
+int dolhip_profile_query(const char* prefix, double* avg_ms, int* count);
+
+/* ---- masks ---------------------------------------------------------------- */
+/* BitMaskFromImageFile's thresholding (DistanceOcclusionTool.cpp:646-665):
+ * bit set iff (byte >= threshold) XOR less_than; `rows` = h (2D) or h*d (3D).
+ * One wavefront ballot per 64 pixels. */
+int dolhip_mask_from_bytes(int w, int rows, const uint8_t* bytes, int threshold, int less_than,
+                           uint32_t* mask);
+int dolhip_mask_from_bytes_dev(int w, int rows, const uint8_t* bytes, int threshold,
+                               int less_than, uint32_t* mask, void* stream);
+
+/* ---- initialisation ------------------------------------------------------- */
+/* InitDeltasFromBitMask 2D (DistanceOcclusionLib.h:75, .cpp:224-254) */
+int dolhip_init_deltas_2d(int w, int h, const uint32_t* mask, int16_t* delta, int invert);
+int dolhip_init_deltas_2d_dev(int w, int h, const uint32_t* mask, int16_t* delta, int invert,
+                              void* stream);
+/* InitDeltasFromBitMask 3D (DistanceOcclusionLib.h:76, .cpp:1289-1321) */
+int dolhip_init_deltas_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta, int invert);
+int dolhip_init_deltas_3d_dev(int w, int h, int d, const uint32_t* mask, int16_t* delta,
+                              int invert, void* stream);
+/* InitBorderDeltasFromBitMask 2D (DistanceOcclusionLib.h:79, .cpp:504-619) */
+int dolhip_init_border_deltas_2d(int w, int h, const uint32_t* mask, int16_t* delta);
+int dolhip_init_border_deltas_2d_dev(int w, int h, const uint32_t* mask, int16_t* delta,
+                                     void* stream);
+/* InitBorderDeltasFromBitMask 3D (DistanceOcclusionLib.h:80, .cpp:1820-2173) */
+int dolhip_init_border_deltas_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta);
+int dolhip_init_border_deltas_3d_dev(int w, int h, int d, const uint32_t* mask, int16_t* delta,
+                                     void* stream);
+/* InitDistancesFromBitMask (DistanceOcclusionLib.h:61-63, .cpp:181-222) */
+int dolhip_init_distances_f32(int w, int h, const uint32_t* mask, float* dist, float max_d);
+int dolhip_init_distances_f32_dev(int w, int h, const uint32_t* mask, float* dist, float max_d,
+                                  void* stream);
+int dolhip_init_distances_i32(int w, int h, const uint32_t* mask, int32_t* dist, int32_t max_d);
+int dolhip_init_distances_i16(int w, int h, const uint32_t* mask, int16_t* dist, int16_t max_d);
+
+/* ---- propagation ---------------------------------------------------------- */
+/* JumpFlood 2D, in place (DistanceOcclusionLib.h:92, .cpp:780-908) */
+int dolhip_jump_flood_2d(int w, int h, int16_t* delta, int cw);
+size_t dolhip_jump_flood_2d_workspace(int w, int h);
+int dolhip_jump_flood_2d_dev(int w, int h, int16_t* delta, int cw, void* work, void* stream);
+/* JumpFlood 3D, in place (DistanceOcclusionLib.h:93, .cpp:2396-2472) */
+int dolhip_jump_flood_3d(int w, int h, int d, int16_t* delta, int cw);
+size_t dolhip_jump_flood_3d_workspace(int w, int h, int d);
+int dolhip_jump_flood_3d_dev(int w, int h, int d, int16_t* delta, int cw, void* work,
+                             void* stream);
+
+/* Felzenszwalb float squared distances in place, bit-identical to the
+ * reference's float arithmetic (DistanceOcclusionLib.h:71, .cpp:1018-1085) */
+int dolhip_felzenszwalb_2d(int w, int h, float* dist);
+size_t dolhip_felzenszwalb_2d_workspace(int w, int h);
+int dolhip_felzenszwalb_2d_dev(int w, int h, float* dist, void* work, void* stream);
+
+/* ---- exact squared EDT (extension of DistanceOcclusionLib.cpp:1018-1085) --- */
+/* Squared Euclidean distance from each cell to the nearest set bit (clear
+ * bit if invert), exact in integers; DOLHIP_EDT_INF where there is none.
+ * Either output may be NULL: d2 int32, dist = sqrtf((float)d2) (sqrtf(FLT_MAX)
+ * for DOLHIP_EDT_INF, as the reference's float path gives). */
+int dolhip_edt_2d(int w, int h, const uint32_t* mask, int invert, int32_t* d2, float* dist);
+size_t dolhip_edt_2d_workspace(int w, int h);
+int dolhip_edt_2d_dev(int w, int h, const uint32_t* mask, int invert, int32_t* d2, float* dist,
+                      void* work, void* stream);
+int dolhip_edt_3d(int w, int h, int d, const uint32_t* mask, int invert, int32_t* d2,
+                  float* dist);
+size_t dolhip_edt_3d_workspace(int w, int h, int d);
+int dolhip_edt_3d_dev(int w, int h, int d, const uint32_t* mask, int invert, int32_t* d2,
+                      float* dist, void* work, void* stream);
+/* Two-field signed distance (the -S semantics of DistanceOcclusionTool.cpp:1029-1036
+ * with exact fields): sdf = sqrt(EDT to set bits) - sqrt(EDT to clear bits),
+ * negative inside. */
+int dolhip_signed_edt_2d(int w, int h, const uint32_t* mask, float* sdf);
+int dolhip_signed_edt_2d_dev(int w, int h, const uint32_t* mask, float* sdf, void* work,
+                             void* stream);
+int dolhip_signed_edt_3d(int w, int h, int d, const uint32_t* mask, float* sdf);
+int dolhip_signed_edt_3d_dev(int w, int h, int d, const uint32_t* mask, float* sdf, void* work,
+                             void* stream);
+
+/* ---- distances and encoders (DistanceOcclusionTool.cpp:46-75) -------------- */
+/* Dist of each delta: sqrtf((float)(x*x + y*y)) (DistanceOcclusionTool.cpp:54-57) */
+int dolhip_dist_2d_dev(int n, const int16_t* delta, float* out, void* stream);
+/* Signed distance in cells from half-cell border deltas:
+ * (bit ? -0.5f : 0.5f) * sqrtf(|delta|^2) */
+int dolhip_signed_distance_2d(int w, int h, const uint32_t* mask, const int16_t* delta,
+                              float* out);
+int dolhip_signed_distance_2d_dev(int w, int h, const uint32_t* mask, const int16_t* delta,
+                                  float* out, void* stream);
+int dolhip_signed_distance_3d(int w, int h, int d, const uint32_t* mask, const int16_t* delta,
+                              float* out);
+int dolhip_signed_distance_3d_dev(int w, int h, int d, const uint32_t* mask,
+                                  const int16_t* delta, float* out, void* stream);
+/* uint8 signed image, zero crossing at 128 (DistanceOcclusionTool.cpp:1242-1271,
+ * 1369-1404); max_len is the divisor passed there (twice the tool's maxLen) */
+int dolhip_encode_sdf_u8_2d(int w, int h, const uint32_t* mask, const int16_t* delta,
+                            float max_len, uint8_t* out);
+int dolhip_encode_sdf_u8_2d_dev(int w, int h, const uint32_t* mask, const int16_t* delta,
+                                float max_len, uint8_t* out, void* stream);
+int dolhip_encode_sdf_u8_3d(int w, int h, int d, const uint32_t* mask, const int16_t* delta,
+                            float max_len, uint8_t* out);
+int dolhip_encode_sdf_u8_3d_dev(int w, int h, int d, const uint32_t* mask,
+                                const int16_t* delta, float max_len, uint8_t* out, void* stream);
+/* RGBA8 delta image (DistanceOcclusionTool.cpp:1281-1288) and red/green image
+ * (DistanceOcclusionTool.cpp:1289-1307) */
+int dolhip_encode_sdf_delta_rgba_2d(int w, int h, const int16_t* delta, float max_len,
+                                    uint8_t* out);
+int dolhip_encode_sdf_rg_rgba_2d(int w, int h, const uint32_t* mask, const int16_t* delta,
+                                 float max_len, uint8_t* out);
+
+/* ---- signed SDF pipelines (DistanceOcclusionTool.cpp:1333-1365, 1471-1498) --- */
+/* border init + JumpFlood(cw = 2), delta left in `delta` */
+int dolhip_signed_border_2d_dev(int w, int h, const uint32_t* mask, int16_t* delta, void* work,
+                                void* stream);
+int dolhip_signed_border_3d_dev(int w, int h, int d, const uint32_t* mask, int16_t* delta,
+                                void* work, void* stream);
+
+/* ---- batched ragged glyphs (extension) ------------------------------------- */
+/* n masks of uint8 coverage, mask i of size w[i] x h[i] starting at byte
+ * offset[i] of `bytes` (and of `out`).  Per mask: threshold (>= threshold XOR
+ * less_than), border init, JumpFlood(cw = 2), EncodeU8Signed with
+ * max_len = 0.5 * max(w, h) (the tool's 2 * maxLen with maxLen = 0.25 * max(w, h),
+ * DistanceOcclusionTool.cpp:1973-1974).  Every mask needs w, h in [1, 256]. */
+typedef struct dolhip_batch dolhip_batch;
+int dolhip_batch_create(int n, const int32_t* w, const int32_t* h, const int64_t* offset,
+                        int threshold, int less_than, dolhip_batch** out);
+int dolhip_batch_destroy(dolhip_batch* b);
+int64_t dolhip_batch_bytes(const dolhip_batch* b);
+/* bytes / out are device pointers; with graph != 0 the launch is captured in
+ * a hipGraph the first time and replayed afterwards for the same pointers. */
+int dolhip_batch_sdf_u8_dev(dolhip_batch* b, const uint8_t* bytes, uint8_t* out, int graph,
+                            void* stream);
+/* the same per-mask pipeline with one set of launches per mask (reference
+ * for the batched form) */
+int dolhip_glyph_sdf_u8_dev(int w, int h, const uint8_t* bytes, int threshold, int less_than,
+                            uint8_t* out, void* work, void* stream);
+/* tile configuration chosen by the tuner for this batch (for reports) */
+int dolhip_batch_config(const dolhip_batch* b, int* tile_cells, int* waves_per_group);
+int dolhip_batch_tune(dolhip_batch* b, const uint8_t* bytes, uint8_t* out, void* stream);
+
+/* ---- continuous seeds (extension) ---------------------------------------- */
+/* Nearest of n seeds (float x, y pairs) to each cell centre (x, y) of a w x h
+ * grid, exact under float64 comparison with ties to the lowest index; idx int32
+ * and dist float32 = |c - p|.  work: dolhip_seed_workspace(w, h, n) bytes. */
+size_t dolhip_seed_workspace(int w, int h, int n);
+int dolhip_seed_nearest_2d_dev(int w, int h, int n, const float* seeds, int32_t* idx,
+                               float* dist, void* work, void* stream);
+/* grad[k] += sum over cells of seed k of -gout * weight * (c - p_k) / d (zero at d = 0);
+ * grad is 2n floats and is overwritten */
+int dolhip_seed_grad_2d_dev(int w, int h, int n, const float* seeds, const int32_t* idx,
+                            const float* dist, const float* gout, float* grad, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,237 @@
+"""GPU parity: every entry point of libdolhip against the oracle (and the
+compiled reference where it was built), bit-exact, on seeded random masks.
+
+Sizes straddle the 32-bit word boundaries of the bitmask rows (31, 32, 33,
+63, 64, 65 ...) and include single rows/columns/slices, empty and full masks.
+"""
+import numpy as np
+import pytest
+
+from oracle_bind import empty_mask, pack_mask, unpack_mask
+
+pytestmark = pytest.mark.gpu
+
+SIZES_2D = [(1, 1), (1, 7), (7, 1), (2, 2), (5, 5), (31, 9), (32, 17), (33, 33), (63, 20),
+            (64, 64), (65, 3), (100, 71), (129, 130), (257, 64)]
+SIZES_3D = [(1, 1, 1), (2, 3, 4), (5, 5, 5), (31, 7, 6), (32, 9, 3), (33, 4, 9), (40, 17, 11),
+            (65, 6, 5), (70, 33, 2)]
+DENSITIES = [0.0, 0.01, 0.2, 0.5, 0.95, 1.0]
+
+
+def rand_mask(rng, dims, p):
+    bits = rng.random(tuple(reversed(dims))) < p
+    return pack_mask(bits)
+
+
+def blob_mask(rng, dims, n=6):
+    """Union of random balls: large uniform regions with curved borders."""
+    shape = tuple(reversed(dims))
+    grids = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
+    bits = np.zeros(shape, dtype=bool)
+    for _ in range(n):
+        c = [rng.uniform(0, s) for s in shape]
+        r = rng.uniform(1, max(2, max(shape) / 3))
+        d2 = sum((g - cc) ** 2 for g, cc in zip(grids, c))
+        bits |= d2 <= r * r
+    return pack_mask(bits)
+
+
+def masks_for(rng, dims):
+    out = [rand_mask(rng, dims, p) for p in DENSITIES]
+    out.append(blob_mask(rng, dims))
+    return out
+
+
+# ------------------------------------------------------------- init
+
+
+@pytest.mark.parametrize("dims", SIZES_2D)
+def test_init_deltas_2d(gpu, oracle, rng, dims):
+    w, h = dims
+    for m in masks_for(rng, dims):
+        for inv in (False, True):
+            np.testing.assert_array_equal(gpu.init_deltas_2d(w, h, m, inv), oracle.init_deltas_2d(w, h, m, inv))
+
+
+@pytest.mark.parametrize("dims", SIZES_3D)
+def test_init_deltas_3d(gpu, oracle, rng, dims):
+    w, h, d = dims
+    for m in masks_for(rng, dims):
+        for inv in (False, True):
+            np.testing.assert_array_equal(gpu.init_deltas_3d(w, h, d, m, inv), oracle.init_deltas_3d(w, h, d, m, inv))
+
+
+@pytest.mark.parametrize("dims", SIZES_2D)
+def test_border_2d(gpu, oracle, reference, rng, dims):
+    w, h = dims
+    for m in masks_for(rng, dims):
+        got = gpu.init_border_deltas_2d(w, h, m)
+        np.testing.assert_array_equal(got, oracle.border_2d(w, h, m))
+        if reference is not None:
+            np.testing.assert_array_equal(got, reference.border_2d(w, h, m))
+
+
+@pytest.mark.parametrize("dims", SIZES_3D)
+def test_border_3d(gpu, oracle, reference, rng, dims):
+    w, h, d = dims
+    for m in masks_for(rng, dims):
+        got = gpu.init_border_deltas_3d(w, h, d, m)
+        np.testing.assert_array_equal(got, oracle.border_3d(w, h, d, m))
+        if reference is not None:
+            np.testing.assert_array_equal(got, reference.border_3d(w, h, d, m))
+
+
+def test_init_distances(gpu, oracle, rng):
+    for w, h in [(1, 1), (33, 17), (100, 64)]:
+        m = rand_mask(rng, (w, h), 0.3)
+        np.testing.assert_array_equal(gpu.init_distances_f32(w, h, m), oracle.init_distances_f32(w, h, m))
+        exp = unpack_mask(m, w, h)
+        np.testing.assert_array_equal(gpu.init_distances_i32(w, h, m, 1234), np.where(exp, 0, 1234))
+        np.testing.assert_array_equal(gpu.init_distances_i16(w, h, m, 77), np.where(exp, 0, 77))
+
+
+@pytest.mark.parametrize("w,rows", [(1, 1), (31, 3), (32, 5), (33, 7), (100, 9), (256, 4), (257, 3)])
+def test_mask_from_bytes(gpu, oracle, rng, w, rows):
+    data = rng.integers(0, 256, size=w * rows, dtype=np.uint8)
+    for thr, lt in [(128, True), (128, False), (0, False), (255, True), (1, True)]:
+        np.testing.assert_array_equal(gpu.mask_from_bytes(w, rows, data, thr, lt),
+                                      oracle.mask_from_bytes(w, rows, data, thr, lt))
+
+
+# ------------------------------------------------------------- jump flood
+
+
+@pytest.mark.parametrize("dims", SIZES_2D)
+@pytest.mark.parametrize("cw", [1, 2])
+def test_jfa_2d(gpu, oracle, rng, dims, cw):
+    w, h = dims
+    for m in masks_for(rng, dims):
+        start = oracle.border_2d(w, h, m) if cw == 2 else oracle.init_deltas_2d(w, h, m)
+        np.testing.assert_array_equal(gpu.jump_flood_2d(w, h, start, cw), oracle.jfa_2d(w, h, start, cw))
+
+
+@pytest.mark.parametrize("dims", SIZES_3D)
+@pytest.mark.parametrize("cw", [1, 2])
+def test_jfa_3d(gpu, oracle, rng, dims, cw):
+    w, h, d = dims
+    for m in masks_for(rng, dims):
+        start = oracle.border_3d(w, h, d, m) if cw == 2 else oracle.init_deltas_3d(w, h, d, m)
+        np.testing.assert_array_equal(gpu.jump_flood_3d(w, h, d, start, cw), oracle.jfa_3d(w, h, d, start, cw))
+
+
+def test_jfa_2d_config1(gpu, oracle, reference):
+    """The tool's -p 100 case: 512x512, 100 points from seed 12345."""
+    w = h = 512
+    m = empty_mask(w, h)
+    oracle.mask_add_points_2d(w, h, 100, 12345, m)
+    start = oracle.init_deltas_2d(w, h, m)
+    got = gpu.jump_flood_2d(w, h, start, 1)
+    np.testing.assert_array_equal(got, oracle.jfa_2d(w, h, start, 1))
+    if reference is not None:
+        np.testing.assert_array_equal(got, reference.jfa_2d(w, h, start, 1))
+    # error against the exact transform stays below one cell (DistanceOcclusionTool.cpp:280-284)
+    exact = np.sqrt(oracle.edt_2d(w, h, m).astype(np.float64))
+    err = np.abs(oracle.dist_2d(got).astype(np.float64) - exact)
+    assert err.max() < 1.0
+    assert oracle.check_deltas_2d(w, h, got, 1) == 0
+
+
+# ------------------------------------------------------------- Felzenszwalb
+
+
+@pytest.mark.parametrize("dims", SIZES_2D + [(8192, 16), (16, 4100)])
+def test_felzenszwalb_float(gpu, oracle, reference, rng, dims):
+    w, h = dims
+    ms = masks_for(rng, dims) if w * h < 20000 else [rand_mask(rng, dims, 0.0005), rand_mask(rng, dims, 0.01)]
+    for m in ms:
+        f0 = oracle.init_distances_f32(w, h, m)
+        got = gpu.felzenszwalb_2d(w, h, f0)
+        exp = oracle.felzenszwalb_2d(w, h, f0)
+        np.testing.assert_array_equal(got.view(np.uint32), exp.view(np.uint32))
+        if reference is not None:
+            np.testing.assert_array_equal(got.view(np.uint32), reference.felzenszwalb_2d(w, h, f0).view(np.uint32))
+
+
+# ------------------------------------------------------------- exact EDT
+
+
+@pytest.mark.parametrize("dims", SIZES_2D + [(600, 3), (3, 600)])
+def test_edt_2d(gpu, oracle, rng, dims):
+    w, h = dims
+    for m in masks_for(rng, dims):
+        for inv in (False, True):
+            d2, fl = gpu.edt_2d(w, h, m, inv)
+            exp = oracle.edt_2d(w, h, m, inv)
+            np.testing.assert_array_equal(d2, exp)
+            expf = np.where(exp == 0x7FFFFFFF, np.sqrt(np.float32(np.finfo(np.float32).max)),
+                            np.sqrt(exp.astype(np.float32)))
+            np.testing.assert_array_equal(fl, expf.astype(np.float32))
+
+
+@pytest.mark.parametrize("dims", SIZES_3D + [(64, 64, 64)])
+def test_edt_3d(gpu, oracle, rng, dims):
+    w, h, d = dims
+    for m in masks_for(rng, dims):
+        for inv in (False, True):
+            d2, fl = gpu.edt_3d(w, h, d, m, inv)
+            exp = oracle.edt_3d(w, h, d, m, inv)
+            np.testing.assert_array_equal(d2, exp)
+            finite = exp != 0x7FFFFFFF
+            np.testing.assert_array_equal(fl[finite], np.sqrt(exp[finite].astype(np.float32)))
+
+
+def test_signed_edt(gpu, oracle, rng):
+    for dims in [(33, 17), (64, 64, 8)]:
+        m = blob_mask(rng, dims)
+        if len(dims) == 2:
+            got = gpu.signed_edt_2d(*dims, m)
+            a = oracle.edt_2d(*dims, m, False)
+            b = oracle.edt_2d(*dims, m, True)
+        else:
+            got = gpu.signed_edt_3d(*dims, m)
+            a = oracle.edt_3d(*dims, m, False)
+            b = oracle.edt_3d(*dims, m, True)
+        exp = np.sqrt(a.astype(np.float32)) - np.sqrt(b.astype(np.float32))
+        np.testing.assert_array_equal(got, exp.astype(np.float32))
+
+
+# ------------------------------------------------------------- encoders
+
+
+def test_encoders_2d(gpu, oracle, rng):
+    for w, h in [(33, 17), (128, 96)]:
+        m = blob_mask(rng, (w, h))
+        delta = oracle.jfa_2d(w, h, oracle.border_2d(w, h, m), 2)
+        max_len = 0.5 * max(w, h)
+        np.testing.assert_array_equal(gpu.signed_distance_2d(w, h, m, delta), oracle.signed_dist_2d(w, h, m, delta))
+        np.testing.assert_array_equal(gpu.encode_sdf_u8_2d(w, h, m, delta, max_len),
+                                      oracle.sdf_u8_2d(w, h, m, delta, max_len))
+        np.testing.assert_array_equal(gpu.encode_sdf_delta_rgba_2d(w, h, delta, max_len),
+                                      oracle.sdf_delta_rgba_2d(w, h, delta, max_len))
+        np.testing.assert_array_equal(gpu.encode_sdf_rg_rgba_2d(w, h, m, delta, max_len),
+                                      oracle.sdf_rg_rgba_2d(w, h, m, delta, max_len))
+
+
+def test_encoders_3d(gpu, oracle, rng):
+    w, h, d = 33, 20, 9
+    m = blob_mask(rng, (w, h, d))
+    delta = oracle.jfa_3d(w, h, d, oracle.border_3d(w, h, d, m), 2)
+    np.testing.assert_array_equal(gpu.signed_distance_3d(w, h, d, m, delta), oracle.signed_dist_3d(w, h, d, m, delta))
+    np.testing.assert_array_equal(gpu.encode_sdf_u8_3d(w, h, d, m, delta, 16.5),
+                                  oracle.sdf_u8_3d(w, h, d, m, delta, 16.5))
+
+
+# ------------------------------------------------------------- errors
+
+
+def test_invalid_arguments(gpu):
+    import dolhip
+
+    with pytest.raises(dolhip.DolHipError) as e:
+        gpu.jump_flood_2d(0, 5, np.zeros((5, 0, 2), np.int16))
+    assert e.value.code == dolhip.ERR_INVALID_ARG
+    with pytest.raises(dolhip.DolHipError) as e:
+        gpu.jump_flood_2d(4, 4, np.zeros((4, 4, 2), np.int16), cw=3)
+    assert e.value.code == dolhip.ERR_INVALID_ARG
+    with pytest.raises(dolhip.DolHipError):
+        gpu._call("dolhip_init_deltas_2d", 4, 4, None, None, 0)
