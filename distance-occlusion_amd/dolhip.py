@@ -100,6 +100,14 @@ _SIGS = {
     "dolhip_glyph_sdf_u8_dev": ([_I, _I, _P, _I, _I, _P, _P, _P], _I),
     "dolhip_batch_config": ([_P, _P, _P], _I),
     "dolhip_batch_tune": ([_P, _P, _P, _P], _I),
+    "dolhip_mask_add_points_2d": ([_I, _I, _I, ctypes.c_uint32, _P], None),
+    "dolhip_mask_add_points_3d": ([_I, _I, _I, _I, ctypes.c_uint32, _P], None),
+    "dolhip_mask_add_block_2d": ([_I, _I, _I, _P], None),
+    "dolhip_mask_add_block_3d": ([_I, _I, _I, _I, _P], None),
+    "dolhip_scene_shapes_2d": ([_I, _I, _I, ctypes.c_uint64, _P], None),
+    "dolhip_scene_shapes_3d": ([_I, _I, _I, _I, _I, ctypes.c_uint64, _P], None),
+    "dolhip_scene_glyphs": ([_I, _I, _I, ctypes.c_uint64, _P, _P, _P, _P], ctypes.c_int64),
+    "dolhip_scene_seeds": ([_I, _I, _I, ctypes.c_uint64, _P, _P], None),
     "dolhip_seed_workspace": ([_I, _I, _I], _SZ),
     "dolhip_seed_nearest_2d_dev": ([_I, _I, _I, _P, _P, _P, _P, _P], _I),
     "dolhip_seed_grad_2d_dev": ([_I, _I, _I, _P, _P, _P, _P, _P, _P], _I),
@@ -292,6 +300,45 @@ class DolHip:
         self._call("dolhip_encode_sdf_rg_rgba_2d", w, h, _addr(np.ascontiguousarray(mask, np.uint32)),
                    _addr(np.ascontiguousarray(delta, np.int16)), max_len, _addr(out))
         return out
+
+    # ---- host generators
+    def mask_add_points_2d(self, w, h, n, seed, mask):
+        self.L.dolhip_mask_add_points_2d(w, h, n, seed, _addr(mask))
+
+    def mask_add_points_3d(self, w, h, d, n, seed, mask):
+        self.L.dolhip_mask_add_points_3d(w, h, d, n, seed, _addr(mask))
+
+    def mask_add_block_2d(self, w, h, sides, mask):
+        self.L.dolhip_mask_add_block_2d(w, h, sides, _addr(mask))
+
+    def mask_add_block_3d(self, w, h, d, sides, mask):
+        self.L.dolhip_mask_add_block_3d(w, h, d, sides, _addr(mask))
+
+    def scene_shapes_2d(self, w, h, n=2000, seed=2):
+        mask = np.zeros(mask_stride(w) * h, dtype=np.uint32)
+        self.L.dolhip_scene_shapes_2d(w, h, n, seed, _addr(mask))
+        return mask
+
+    def scene_shapes_3d(self, w, h, d, n_ellipsoids=300, n_boxes=100, seed=3):
+        mask = np.zeros(mask_stride(w) * h * d, dtype=np.uint32)
+        self.L.dolhip_scene_shapes_3d(w, h, d, n_ellipsoids, n_boxes, seed, _addr(mask))
+        return mask
+
+    def scene_glyphs(self, n=20000, min_size=16, max_size=256, seed=4):
+        """(widths, heights, offsets, bytes) of a ragged glyph set."""
+        w = np.empty(n, dtype=np.int32)
+        h = np.empty(n, dtype=np.int32)
+        off = np.empty(n, dtype=np.int64)
+        total = self.L.dolhip_scene_glyphs(n, min_size, max_size, seed, _addr(w), _addr(h), _addr(off), None)
+        data = np.empty(total, dtype=np.uint8)
+        self.L.dolhip_scene_glyphs(n, min_size, max_size, seed, _addr(w), _addr(h), _addr(off), _addr(data))
+        return w, h, off, data
+
+    def scene_seeds(self, w, h, n=4096, seed=5, weights=True):
+        seeds = np.empty((n, 2), dtype=np.float32)
+        wt = np.empty((h, w), dtype=np.float32) if weights else None
+        self.L.dolhip_scene_seeds(w, h, n, seed, _addr(seeds), _addr(wt))
+        return seeds, wt
 
     # ---- device entry points (torch tensors or raw addresses)
     def workspace(self, name, *dims):

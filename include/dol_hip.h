@@ -75,6 +75,23 @@ int dolhip_mask_from_bytes(int w, int rows, const uint8_t* bytes, int threshold,
 int dolhip_mask_from_bytes_dev(int w, int rows, const uint8_t* bytes, int threshold,
                                int less_than, uint32_t* mask, void* stream);
 
+/* Host-side generators.  The reference's BitMaskAddPoints / BitMaskAddBlock
+ * (DistanceOcclusionLib.h:30-35, .cpp:95-179, 1224-1287; block sizes need
+ * w >= 2, h >= 3, d >= 3 there), and seeded scenes for larger workloads:
+ * unions of discs and boxes (2D), of rotated ellipsoids and boxes (3D), ragged
+ * glyph coverage masks (255 inside) and continuous seed points.  All OR into /
+ * fill host memory and touch no device. */
+void dolhip_mask_add_points_2d(int w, int h, int count, uint32_t seed, uint32_t* mask);
+void dolhip_mask_add_points_3d(int w, int h, int d, int count, uint32_t seed, uint32_t* mask);
+void dolhip_mask_add_block_2d(int w, int h, int sides, uint32_t* mask);
+void dolhip_mask_add_block_3d(int w, int h, int d, int sides, uint32_t* mask);
+void dolhip_scene_shapes_2d(int w, int h, int n, uint64_t seed, uint32_t* mask);
+void dolhip_scene_shapes_3d(int w, int h, int d, int n_ellipsoids, int n_boxes, uint64_t seed,
+                            uint32_t* mask);
+int64_t dolhip_scene_glyphs(int n, int min_size, int max_size, uint64_t seed, int32_t* w,
+                            int32_t* h, int64_t* offset, uint8_t* bytes);
+void dolhip_scene_seeds(int w, int h, int n, uint64_t seed, float* seeds, float* weights);
+
 /* ---- initialisation ------------------------------------------------------- */
 /* InitDeltasFromBitMask 2D (DistanceOcclusionLib.h:75, .cpp:224-254) */
 int dolhip_init_deltas_2d(int w, int h, const uint32_t* mask, int16_t* delta, int invert);
