@@ -23,6 +23,7 @@ SOURCES = [
     "dol_init.hip",
     "dol_jfa.hip",
     "dol_edt.hip",
+    "dol_edt_tile.hip",
     "dol_encode.hip",
     "dol_batch.hip",
     "dol_seeds.hip",
@@ -81,6 +82,29 @@ def build(verbose=False, jobs=None):
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
     return LIB
+
+
+
+TORCH_EXT_DIR = os.path.join(HERE, "torch_ext")
+
+
+def build_torch_ext(verbose=False):
+    """The libtorch autograd Function (csrc/dol_torch.cpp), linked to
+    libdolhip.so, built in-tree into torch_ext/dol_torch*.so."""
+    import torch.utils.cpp_extension as ce
+
+    os.makedirs(TORCH_EXT_DIR, exist_ok=True)
+    os.environ.setdefault("PYTORCH_ROCM_ARCH", ARCH)
+    return ce.load(
+        name="dol_torch",
+        sources=[os.path.join(CSRC, "dol_torch.cpp")],
+        extra_include_paths=[os.path.join(ROOT, "include")],
+        extra_cflags=["-O2"],
+        extra_ldflags=[f"-L{HERE}", "-ldolhip", f"-Wl,-rpath,{HERE}"],
+        build_directory=TORCH_EXT_DIR,
+        with_cuda=True,
+        verbose=verbose,
+    )
 
 
 if __name__ == "__main__":

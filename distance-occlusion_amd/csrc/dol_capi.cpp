@@ -34,7 +34,7 @@ void launch_jfa3d_pass(int step, int w, int h, int d, const int16_t* in, int16_t
                        hipStream_t s);
 size_t felz_workspace(int w, int h);
 void launch_felzenszwalb_2d(int w, int h, float* dist, void* work, hipStream_t s);
-size_t edt_workspace(long long cells, int dims);
+size_t edt_workspace(int w, int h, int d);
 void launch_edt_2d(int w, int h, const uint32_t* mask, int invert, int32_t* d2, float* fl,
                    void* work, hipStream_t s);
 void launch_edt_3d(int w, int h, int d, const uint32_t* mask, int invert, int32_t* d2, float* fl,
@@ -624,12 +624,12 @@ int dolhip_felzenszwalb_2d(int w, int h, float* dist)
 
 size_t dolhip_edt_2d_workspace(int w, int h)
 {
-    return (w < 1 || h < 1) ? 0 : edt_workspace((long long)w * h, 2);
+    return (w < 1 || h < 1) ? 0 : edt_workspace(w, h, 1);
 }
 
 size_t dolhip_edt_3d_workspace(int w, int h, int d)
 {
-    return (w < 1 || h < 1 || d < 1) ? 0 : edt_workspace((long long)w * h * d, 3);
+    return (w < 1 || h < 1 || d < 1) ? 0 : edt_workspace(w, h, d);
 }
 
 int dolhip_edt_2d_dev(int w, int h, const uint32_t* mask, int invert, int32_t* d2, float* dist,

@@ -1,0 +1,191 @@
+// dol_cells.hpp -- per-cell device functions shared by the per-image kernels
+// and the batched glyph kernel, so both compute identical bits.
+#pragma once
+
+#include "dol_common.hpp"
+
+namespace dolhip {
+
+// ------------------------------------------------------------ border init 2D
+//
+// Value of cell (x, y) after SetBorderCellsCombo (DistanceOcclusionLib.cpp:504-599),
+// obtained by replaying, in the reference's order, only the writes that land
+// on this cell.  For row pair p the window at bit b has a = P(b, p),
+// b' = P(b, p+1) and the same one column left (x = -1 replicating x = 0):
+//   vert = a ^ b', horz = a ^ a_left, vprev = a_left ^ b'_left,
+//   corner = vert ^ vprev; a corner goes to (b + vprev - 1, p + horz).
+// Within a 32-bit word the reference runs the corner loop, then the
+// horizontal loop, then the vertical loop over ascending bits, so when bits
+// x and x+1 share a word their writes interleave loop by loop; across a word
+// end all of bit x's writes come first.  `mask` may be global or LDS.
+__device__ __forceinline__ uint32_t border2_cell(const uint32_t* mask, int ws, int w, int h, int x,
+                                                 int y)
+{
+    const bool right = x + 1 < w;
+    const bool word_end = (x & 31) == 31;
+    int c[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const int yy = y - 1 + r;
+        const bool ok = yy >= 0 && yy < h;
+        const uint32_t* row = mask + (size_t)(ok ? yy : 0) * ws;
+        const int xl = x > 0 ? x - 1 : 0;
+        c[r][0] = ok ? (int)((row[xl >> 5] >> (xl & 31)) & 1u) : 0;
+        c[r][1] = ok ? (int)((row[x >> 5] >> (x & 31)) & 1u) : 0;
+        c[r][2] = (ok && right) ? (int)((row[(x + 1) >> 5] >> ((x + 1) & 31)) & 1u) : 0;
+    }
+    // rows outside the image never take part in a window; a uniform
+    // neighbourhood over the rows inside gets no write
+    {
+        const int lo = y >= 1 ? 0 : 1, hi = y + 1 < h ? 2 : 1;
+        const int ref = c[1][1];
+        bool uniform = true;
+        for (int r = lo; r <= hi; r++)
+            for (int k = 0; k < 3; k++)
+                if ((k < 2 || right) && c[r][k] != ref) uniform = false;
+        if (uniform) return pack2(kMaxDelta2, kMaxDelta2);
+    }
+    int sx = kMaxDelta2, sy = kMaxDelta2;
+    auto set = [&](int vx, int vy) { sx = vx; sy = vy; };
+    auto set_if_unset = [&](int vx, int vy) {
+        if (sx == kMaxDelta2) { sx = vx; sy = vy; }
+    };
+    if (y >= 1) {   // pair y-1: this cell is its lower row
+        const int vert0 = c[0][1] ^ c[1][1], horz0 = c[0][1] ^ c[0][0];
+        const int vprev0 = c[0][0] ^ c[1][0], corner0 = vert0 ^ vprev0;
+        const int vert1 = c[0][2] ^ c[1][2], horz1 = c[0][2] ^ c[0][1];
+        const int vprev1 = vert0, corner1 = vert1 ^ vprev1;
+        const bool e1 = corner0 && vprev0 && horz0;
+        const bool e2 = right && corner1 && !vprev1 && horz1;
+        const bool e3 = vert0;
+        if (!word_end) {
+            if (e1) set_if_unset(-1, -1);
+            if (e2) set_if_unset(+1, -1);
+            if (e3) set(0, -1);
+        } else {
+            if (e1) set_if_unset(-1, -1);
+            if (e3) set(0, -1);
+            if (e2) set_if_unset(+1, -1);
+        }
+    }
+    if (y + 1 < h) {   // pair y: this cell is its upper row
+        const int vert0 = c[1][1] ^ c[2][1], horz0 = c[1][1] ^ c[1][0];
+        const int vprev0 = c[1][0] ^ c[2][0], corner0 = vert0 ^ vprev0;
+        const int vert1 = c[1][2] ^ c[2][2], horz1 = c[1][2] ^ c[1][1];
+        const int vprev1 = vert0, corner1 = vert1 ^ vprev1;
+        const bool f1 = corner0 && vprev0 && !horz0;
+        const bool f2 = right && corner1 && !vprev1 && !horz1;
+        const bool f3 = horz0;
+        const bool f4 = right && horz1;
+        const bool f5 = vert0;
+        if (!word_end) {
+            if (f1) set_if_unset(-1, +1);
+            if (f2) set_if_unset(+1, +1);
+            if (f3) set(-1, 0);
+            if (f4) set(+1, 0);
+            if (f5) set(0, +1);
+        } else {
+            if (f1) set_if_unset(-1, +1);
+            if (f3) set(-1, 0);
+            if (f5) set(0, +1);
+            if (f2) set_if_unset(+1, +1);
+            if (f4) set(+1, 0);
+        }
+    } else {   // last row: horizontal changes only (DistanceOcclusionLib.cpp:571-598)
+        if (c[1][1] ^ c[1][0]) set(-1, 0);
+        if (right && (c[1][2] ^ c[1][1])) set(+1, 0);
+    }
+    return pack2(sx, sy);
+}
+
+// ------------------------------------------------------------ JumpFlood 2D
+
+__device__ __forceinline__ void jfa2_tap(int& bx, int& by, int& bl, uint32_t v, int ox, int oy)
+{
+    const int cx = wrap16(unpack2x(v) + ox), cy = wrap16(unpack2y(v) + oy);
+    const int cl = len2(cx, cy);
+    if (bl > cl) { bx = cx; by = cy; bl = cl; }
+}
+
+// One pass for cell (x, y) (DistanceOcclusionLib.cpp:782-873).  Tap order:
+// self; (-s,0), (-s,-s), (-s,+s) if x-s >= 0; (+s,0), (+s,-s), (+s,+s) if
+// x+s < w; (0,-s); (0,+s).  Candidates wrap to int16; strict ">".
+// `in` may be global or LDS.
+__device__ __forceinline__ uint32_t jfa2_cell(const uint32_t* in, int w, int h, int x, int y,
+                                              int step, int cw)
+{
+    const int off = cw * step;
+    const size_t row = (size_t)y * w;
+    const bool up = y - step >= 0, down = y + step < h;
+    const uint32_t self = in[row + x];
+    int bx = unpack2x(self), by = unpack2y(self);
+    int bl = len2(bx, by);
+    if (x - step >= 0) {
+        jfa2_tap(bx, by, bl, in[row + x - step], -off, 0);
+        if (up) jfa2_tap(bx, by, bl, in[row - (size_t)step * w + x - step], -off, -off);
+        if (down) jfa2_tap(bx, by, bl, in[row + (size_t)step * w + x - step], -off, +off);
+    }
+    if (x + step < w) {
+        jfa2_tap(bx, by, bl, in[row + x + step], +off, 0);
+        if (up) jfa2_tap(bx, by, bl, in[row - (size_t)step * w + x + step], +off, -off);
+        if (down) jfa2_tap(bx, by, bl, in[row + (size_t)step * w + x + step], +off, +off);
+    }
+    if (up) jfa2_tap(bx, by, bl, in[row - (size_t)step * w + x], 0, -off);
+    if (down) jfa2_tap(bx, by, bl, in[row + (size_t)step * w + x], 0, +off);
+    return pack2(bx, by);
+}
+
+// FloorPow2 of DistanceOcclusionLib.cpp:875-884
+__host__ __device__ inline int floor_pow2(int v)
+{
+    uint32_t u = (uint32_t)v;
+    u |= u >> 1; u |= u >> 2; u |= u >> 4; u |= u >> 8; u |= u >> 16;
+    return (int)((int32_t)(u + 1u) >> 1);
+}
+
+// number of passes of the driver (DistanceOcclusionLib.cpp:887-908): the
+// halving steps, plus one step-1 pass when their count is odd
+__host__ __device__ inline int jfa_pass_count(int maxdim)
+{
+    int n = 0;
+    for (int s = floor_pow2(maxdim - 1); s > 0; s /= 2) n++;
+    return n + (n & 1);
+}
+
+// step of pass k of that schedule
+__host__ __device__ inline int jfa_pass_step(int maxdim, int k)
+{
+    const int s0 = floor_pow2(maxdim - 1);
+    int s = s0;
+    for (int i = 0; i < k && s > 0; i++) s /= 2;
+    return s > 0 ? s : 1;
+}
+
+// ------------------------------------------------------------ encoders
+
+// EncodeU8Signed, DistanceOcclusionTool.cpp:71-75: clamp, 128 + v * 127 as a
+// separate multiply and add, round half to even
+__device__ __forceinline__ uint8_t encode_u8_signed(float v)
+{
+    v = (v < -1.0f) ? -1.0f : ((v > 1.0f) ? 1.0f : v);
+    return (uint8_t)(int)__builtin_rintf(__fadd_rn(128.0f, __fmul_rn(v, 127.0f)));
+}
+
+// EncodeU8, DistanceOcclusionLib.h:137-141
+__device__ __forceinline__ uint8_t encode_u8(float v)
+{
+    v = (v < 0.0f) ? 0.0f : ((v > 1.0f) ? 1.0f : v);
+    return (uint8_t)(int)__builtin_rintf(__fmul_rn(v, 255.0f));
+}
+
+// Dist of DistanceOcclusionTool.cpp:54-57
+__device__ __forceinline__ float dist_of(int l2) { return sqrt_rn((float)l2); }
+
+// standard signed image value (DistanceOcclusionTool.cpp:1259-1263)
+__device__ __forceinline__ uint8_t sdf_u8_cell(uint32_t delta, int bit, float max_len)
+{
+    const float q = __fdiv_rn(dist_of(len2(unpack2x(delta), unpack2y(delta))), max_len);
+    return encode_u8_signed(bit ? -q : q);
+}
+
+}  // namespace dolhip

@@ -239,10 +239,19 @@ __global__ void __launch_bounds__(256) k_edt_lines(int n, long long lines, long 
 
 static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-// 2D: row field g and the line stack; 3D: also the y-pass field
-size_t edt_workspace(long long cells, int dims)
+bool edt_tile_fits(int w, int h, int d);
+void launch_edt_tile_2d(int w, int h, const uint32_t* mask, int invert, int32_t* d2, float* fl,
+                        hipStream_t s);
+void launch_edt_tile_3d(int w, int h, int d, const uint32_t* mask, int invert, int32_t* d2,
+                        float* fl, void* work, hipStream_t s);
+
+// Tiled path (every dimension <= 512): 3D needs the y-pass field only.
+// Line path: row field g and the line stack; 3D also the y-pass field.
+size_t edt_workspace(int w, int h, int d)
 {
-    return align256((size_t)cells * 4) * (dims == 3 ? 3 : 2);
+    const size_t cells = (size_t)w * h * d;
+    if (edt_tile_fits(w, h, d)) return d == 1 ? 256 : align256(cells * 4);
+    return align256(cells * 4) * (d == 1 ? 2 : 3);
 }
 
 static void edt_lines(int n, long long lines, long long inner, long long outer, long long stride,
@@ -266,6 +275,10 @@ static void edt_rows(int w, long long rows, const uint32_t* mask, int invert, in
 void launch_edt_2d(int w, int h, const uint32_t* mask, int invert, int32_t* d2, float* fl,
                    void* work, hipStream_t s)
 {
+    if (edt_tile_fits(w, h, 1)) {
+        launch_edt_tile_2d(w, h, mask, invert, d2, fl, s);
+        return;
+    }
     const size_t part = align256((size_t)w * h * 4);
     int32_t* g = (int32_t*)work;
     int* stk = (int*)((char*)work + part);
@@ -276,6 +289,10 @@ void launch_edt_2d(int w, int h, const uint32_t* mask, int invert, int32_t* d2, 
 void launch_edt_3d(int w, int h, int d, const uint32_t* mask, int invert, int32_t* d2, float* fl,
                    void* work, hipStream_t s)
 {
+    if (edt_tile_fits(w, h, d)) {
+        launch_edt_tile_3d(w, h, d, mask, invert, d2, fl, work, s);
+        return;
+    }
     const long long sl = (long long)w * h, cells = sl * d;
     const size_t part = align256((size_t)cells * 4);
     int32_t* g = (int32_t*)work;

@@ -1,0 +1,401 @@
+// dol_edt_tile.hip -- LDS-tiled exact squared EDT for lines of up to 512
+// samples (the 512^3 volume and images up to 512 high).
+//
+// The separable transform needs, along each line, the lower envelope of the
+// parabolas  F(i) - 2 q i + q^2  with F(i) = f(i) + i^2 (f = squared distance
+// so far, or "none").  Its surviving sites are the vertices of the lower
+// convex hull of the points (i, F(i)); on realistic volumes most sites
+// survive, so a per-line stack would be as long as the line.  Instead a
+// workgroup holds a tile of 32 lines in LDS and splits each line into bands
+// of 32 sites:
+//   1. each thread builds the hull of one band, its stack being a 32-bit
+//      register whose set bits are the surviving sites;
+//   2. adjacent band hulls are merged in a tree of log2(bands) rounds by
+//      walking to their common bridge (x-separated convex chains), clearing
+//      the survivor bits between the bridge ends in an LDS bitmask;
+//   3. each thread finds the owning site of its band's first sample by a
+//      short walk along the survivors and then sweeps its band, advancing the
+//      owner monotonically, and writes the outputs.
+// All comparisons are exact integer cross-multiplications.  Values: with
+// lines <= 512 every F is below 2^20 and every product below 2^29, so int32
+// arithmetic is exact.
+//
+// edt3d_xy: one workgroup per (slice z, mask word j): the x pass is computed
+// straight from the bit rows into LDS (distance to the nearest set bit in
+// each row), then the y envelope; writes the int32 field f2 (or, for 2D, the
+// final int32 and float outputs).
+// edt3d_z: one workgroup per (row y, 32 columns): loads the f2 column tile
+// along z into LDS, z envelope, writes int32 and float distances.
+#include "dol_common.hpp"
+
+#include <cfloat>
+
+namespace dolhip {
+
+namespace tile {
+
+constexpr int kCols = 32;      // lines per tile (one mask word of x)
+constexpr int kBand = 32;      // sites per band
+constexpr int kMaxN = 512;     // longest line held in LDS
+constexpr int kBands = kMaxN / kBand;
+constexpr int kThreads = kCols * kBands;   // 512: thread = band * 32 + column
+constexpr int kNone = -1;
+
+__device__ __forceinline__ int hi_bit(uint32_t v) { return 31 - __clz(v); }
+__device__ __forceinline__ int lo_bit(uint32_t v) { return __ffs(v) - 1; }
+
+// survivor bitmask: word k of column c holds band k
+struct Bits {
+    uint32_t* w;   // [kBands][kCols]
+    __device__ uint32_t get(int k, int c) const { return w[k * kCols + c]; }
+    __device__ void put(int k, int c, uint32_t v) { w[k * kCols + c] = v; }
+
+    // highest survivor in [lo, pos), or -1
+    __device__ int prev(int c, int pos, int lo) const
+    {
+        if (pos <= lo) return -1;
+        int k = (pos - 1) >> 5;
+        uint32_t m = get(k, c) & (((pos - 1) & 31) == 31 ? 0xffffffffu : ((2u << ((pos - 1) & 31)) - 1u));
+        for (;;) {
+            if (m) {
+                int r = (k << 5) + hi_bit(m);
+                return r >= lo ? r : -1;
+            }
+            if (--k < (lo >> 5)) return -1;
+            m = get(k, c);
+        }
+    }
+    // lowest survivor in (pos, hi), or -1
+    __device__ int next(int c, int pos, int hi) const
+    {
+        int p = pos + 1;
+        if (p >= hi) return -1;
+        int k = p >> 5;
+        const int kend = (hi - 1) >> 5;
+        uint32_t m = get(k, c) & ~((1u << (p & 31)) - 1u);
+        for (;;) {
+            if (m) {
+                int r = (k << 5) + lo_bit(m);
+                return r < hi ? r : -1;
+            }
+            if (++k > kend) return -1;
+            m = get(k, c);
+        }
+    }
+    // clear survivors strictly between a and b
+    __device__ void clear_between(int c, int a, int b)
+    {
+        int p = a + 1, q = b - 1;
+        if (p > q) return;
+        int k0 = p >> 5, k1 = q >> 5;
+        for (int k = k0; k <= k1; k++) {
+            uint32_t lo = k == k0 ? ~((1u << (p & 31)) - 1u) : 0xffffffffu;
+            uint32_t hi = k == k1 ? ((q & 31) == 31 ? 0xffffffffu : ((2u << (q & 31)) - 1u)) : 0xffffffffu;
+            put(k, c, get(k, c) & ~(lo & hi));
+        }
+    }
+};
+
+// v lies on or above the chord u-w (its envelope interval is empty): the
+// intersection of u and v is not left of that of v and w.
+__device__ __forceinline__ bool removable(int u, int Fu, int v, int Fv, int w, int Fw)
+{
+    return (Fv - Fu) * (w - v) >= (Fw - Fv) * (v - u);
+}
+
+// Envelope of one tile.  Src gives f(c, q) >= 0 or kNone; Out(c, q, value)
+// stores a result (value kEdtInf where the line has no site).
+template <class Src, class Out>
+__device__ void envelope(int n, const Src& src, Out& out, Bits bits)
+{
+    const int tid = threadIdx.x;
+    const int c = tid & (kCols - 1);
+    const int b = tid >> 5;
+    const int nb = (n + kBand - 1) / kBand;
+    const int lo = b * kBand;
+    const int hi = min(n, lo + kBand);
+
+    // 1. band hull, register stack
+    if (b < nb) {
+        uint32_t st = 0;
+        int t = -1, s = -1, Ft = 0, Fs = 0;
+        for (int q = lo; q < hi; q++) {
+            const int f = src.f(c, q);
+            if (f < 0) continue;
+            const int Fq = f + q * q;
+            while (s >= 0 && removable(s, Fs, t, Ft, q, Fq)) {
+                st &= ~(1u << (t - lo));
+                t = s;
+                Ft = Fs;
+                const uint32_t below = st & ((1u << (t - lo)) - 1u);
+                if (below) {
+                    s = lo + hi_bit(below);
+                    Fs = src.f(c, s) + s * s;
+                } else {
+                    s = -1;
+                }
+            }
+            st |= 1u << (q - lo);
+            s = t;
+            Fs = Ft;
+            t = q;
+            Ft = Fq;
+        }
+        bits.put(b, c, st);
+    }
+    __syncthreads();
+
+    // 2. tree of bridge merges
+    for (int span = 1; span < nb; span <<= 1) {
+        if ((b % (2 * span)) == 0 && b + span < nb) {
+            const int mid = (b + span) * kBand;
+            const int end = min(n, (b + 2 * span) * kBand);
+            int a = bits.prev(c, mid, lo);
+            int r = bits.next(c, mid - 1, end);
+            if (a >= 0 && r >= 0) {
+                int Fa = src.f(c, a) + a * a, Fr = src.f(c, r) + r * r;
+                for (;;) {
+                    const int rn = bits.next(c, r, end);
+                    if (rn >= 0) {
+                        const int Frn = src.f(c, rn) + rn * rn;
+                        if (removable(a, Fa, r, Fr, rn, Frn)) {
+                            r = rn;
+                            Fr = Frn;
+                            continue;
+                        }
+                    }
+                    const int ap = bits.prev(c, a, lo);
+                    if (ap >= 0) {
+                        const int Fap = src.f(c, ap) + ap * ap;
+                        if (removable(ap, Fap, a, Fa, r, Fr)) {
+                            a = ap;
+                            Fa = Fap;
+                            continue;
+                        }
+                    }
+                    break;
+                }
+                bits.clear_between(c, a, r);
+            }
+        }
+        __syncthreads();
+    }
+
+    // 3. owners and outputs
+    if (b < nb) {
+        int a = bits.prev(c, lo + 1, 0);           // last survivor <= lo
+        if (a < 0) a = bits.next(c, lo, n);        // else the first one after
+        if (a < 0) {
+            for (int q = lo; q < hi; q++) out(c, q, kEdtInf);
+            return;
+        }
+        int fa = src.f(c, a);
+        // owner at lo: unimodal along the survivors, so walk left while
+        // strictly better, then right while not worse
+        for (;;) {
+            const int ap = bits.prev(c, a, 0);
+            if (ap < 0) break;
+            const int fp = src.f(c, ap);
+            if (fp + (lo - ap) * (lo - ap) < fa + (lo - a) * (lo - a)) {
+                a = ap;
+                fa = fp;
+            } else {
+                break;
+            }
+        }
+        int nx = bits.next(c, a, n);
+        int fn = nx >= 0 ? src.f(c, nx) : 0;
+        for (int q = lo; q < hi; q++) {
+            int va = fa + (q - a) * (q - a);
+            while (nx >= 0) {
+                const int vn = fn + (q - nx) * (q - nx);
+                if (vn > va) break;
+                a = nx;
+                fa = fn;
+                va = vn;
+                nx = bits.next(c, a, n);
+                fn = nx >= 0 ? src.f(c, nx) : 0;
+            }
+            out(c, q, va);
+        }
+    }
+}
+
+// ---- sources
+
+struct DxSrc {   // edt3d_xy: x distance per (row, column), 0xffff = none
+    const uint16_t* dx;
+    __device__ int f(int c, int q) const
+    {
+        const int v = dx[q * kCols + c];
+        return v == 0xffff ? kNone : v * v;
+    }
+};
+
+struct FieldSrc {   // edt3d_z: int32 field, kEdtInf = none
+    const int32_t* fs;
+    __device__ int f(int c, int q) const
+    {
+        const int v = fs[q * kCols + c];
+        return v == kEdtInf ? kNone : v;
+    }
+};
+
+// ---- outputs
+
+struct FieldOut {   // int32 field at (x0 + c, q, z) of a w x h x d volume
+    int32_t* f2;
+    int w, h, x0, z;
+    __device__ void operator()(int c, int q, int v) const
+    {
+        const int x = x0 + c;
+        if (x < w) f2[((size_t)z * h + q) * w + x] = v;
+    }
+};
+
+struct FinalOut {   // int32 + float at cell index base(q) + x
+    int32_t* d2;
+    float* fl;
+    size_t row_stride;   // elements between consecutive q
+    size_t base;         // element index of (x0, q = 0)
+    int valid;           // columns < valid are inside the volume
+    __device__ void operator()(int c, int q, int v) const
+    {
+        if (c >= valid) return;
+        const size_t i = base + (size_t)q * row_stride + c;
+        if (d2) d2[i] = v;
+        if (fl) fl[i] = v == kEdtInf ? sqrt_rn(FLT_MAX) : sqrt_rn((float)v);
+    }
+};
+
+}  // namespace tile
+
+using namespace tile;
+
+// x pass from the bit rows, then the y envelope.  Workgroup (z, j) with the
+// 16 tiles of one slice mapped to the same XCD (workgroups are dealt to the
+// 8 XCDs round-robin) so a slice's mask rows are fetched into one L2.
+template <bool FINAL>
+__global__ void __launch_bounds__(kThreads) k_edt3d_xy(int w, int h, int d, const uint32_t* __restrict__ mask,
+                                                       uint32_t invert, int32_t* __restrict__ f2,
+                                                       float* __restrict__ fl)
+{
+    __shared__ uint16_t dx[kMaxN * kCols];
+    __shared__ uint32_t bm[kBands * kCols];
+    const int ws = mask_stride(w);
+    // swizzle: blocks congruent mod 8 share an XCD; a group of (up to) 8
+    // slices takes 8*ws consecutive blocks, slice-minor
+    int z, j;
+    {
+        const int bid = blockIdx.x;
+        const int group = bid / (8 * ws);
+        const int gs = min(8, d - group * 8);
+        const int inner = bid - group * 8 * ws;
+        z = group * 8 + inner % gs;
+        j = inner / gs;
+    }
+    const int tail = w - 32 * (ws - 1);
+    const uint32_t last_valid = tail >= 32 ? 0xffffffffu : ((1u << tail) - 1u);
+
+    // x distances of the 32 columns of word j, one row per thread
+    for (int y = threadIdx.x; y < h; y += kThreads) {
+        const uint32_t* row = mask + ((size_t)z * h + y) * ws;
+        auto word = [&](int k) -> uint32_t {
+            const uint32_t m = row[k] ^ invert;
+            return k == ws - 1 ? (m & last_valid) : m;
+        };
+        const uint32_t wj = word(j);
+        int L = -1, R = -1;
+        for (int k = j - 1; k >= 0; k--) {
+            const uint32_t v = word(k);
+            if (v) { L = 32 * k + hi_bit(v); break; }
+        }
+        for (int k = j + 1; k < ws; k++) {
+            const uint32_t v = word(k);
+            if (v) { R = 32 * k + lo_bit(v); break; }
+        }
+        uint32_t* out = (uint32_t*)(dx + y * kCols);
+#pragma unroll
+        for (int i = 0; i < 32; i += 2) {
+            uint32_t pair = 0;
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const int ii = i + e;
+                const int x = 32 * j + ii;
+                const uint32_t lo = wj & (ii == 31 ? 0xffffffffu : ((2u << ii) - 1u));
+                const uint32_t hi = wj >> ii;
+                int dl = lo ? ii - hi_bit(lo) : (L >= 0 ? x - L : 0xffff);
+                int dr = hi ? lo_bit(hi) : (R >= 0 ? R - x : 0xffff);
+                const int dd = min(min(dl, dr), 0xffff);
+                pair |= (uint32_t)dd << (16 * e);
+            }
+            out[i >> 1] = pair;
+        }
+    }
+    __syncthreads();
+
+    DxSrc src{dx};
+    Bits bits{bm};
+    if (FINAL) {
+        // 2D image (d == 1): final outputs, rows of width w
+        FinalOut o{f2, fl, (size_t)w, (size_t)32 * j, min(32, w - 32 * j)};
+        envelope(h, src, o, bits);
+    } else {
+        FieldOut o{f2, w, h, 32 * j, z};
+        envelope(h, src, o, bits);
+    }
+}
+
+// z envelope over the f2 tile of row y, columns 32*jt .. 32*jt+31
+__global__ void __launch_bounds__(kThreads) k_edt3d_z(int w, int h, int d, const int32_t* __restrict__ f2,
+                                                      int32_t* __restrict__ d2, float* __restrict__ fl)
+{
+    __shared__ int32_t fs[kMaxN * kCols];
+    __shared__ uint32_t bm[kBands * kCols];
+    const int xt = (w + 31) >> 5;
+    const int y = blockIdx.x / xt;
+    const int x0 = (blockIdx.x - y * xt) * 32;
+    const int valid = min(32, w - x0);
+    for (int e = threadIdx.x; e < d * kCols; e += kThreads) {
+        const int z = e >> 5, c = e & 31;
+        fs[e] = c < valid ? f2[((size_t)z * h + y) * w + x0 + c] : kEdtInf;
+    }
+    __syncthreads();
+    FieldSrc src{fs};
+    Bits bits{bm};
+    FinalOut o{d2, fl, (size_t)w * h, (size_t)y * w + x0, valid};
+    envelope(d, src, o, bits);
+}
+
+// every dimension <= 512 keeps each F below 2^20 and each product in int32
+bool edt_tile_fits(int w, int h, int d) { return w <= kMaxN && h <= kMaxN && d <= kMaxN; }
+
+size_t edt_tile_workspace(long long cells) { return ((size_t)cells * 4 + 255) & ~(size_t)255; }
+
+void launch_edt_tile_2d(int w, int h, const uint32_t* mask, int invert, int32_t* d2, float* fl,
+                        hipStream_t s)
+{
+    const int ws = mask_stride(w);
+    ProfileScope p("edt2d_xy", s);
+    hipLaunchKernelGGL(k_edt3d_xy<true>, dim3((unsigned)ws), dim3(kThreads), 0, s, w, h, 1, mask,
+                       invert ? 0xffffffffu : 0u, d2, fl);
+}
+
+void launch_edt_tile_3d(int w, int h, int d, const uint32_t* mask, int invert, int32_t* d2,
+                        float* fl, void* work, hipStream_t s)
+{
+    const int ws = mask_stride(w);
+    int32_t* f2 = (int32_t*)work;
+    {
+        ProfileScope p("edt3d_xy", s);
+        hipLaunchKernelGGL(k_edt3d_xy<false>, dim3((unsigned)(ws * d)), dim3(kThreads), 0, s, w, h,
+                           d, mask, invert ? 0xffffffffu : 0u, f2, nullptr);
+    }
+    {
+        ProfileScope p("edt3d_z", s);
+        hipLaunchKernelGGL(k_edt3d_z, dim3((unsigned)(((w + 31) >> 5) * h)), dim3(kThreads), 0, s, w,
+                           h, d, f2, d2, fl);
+    }
+}
+
+}  // namespace dolhip

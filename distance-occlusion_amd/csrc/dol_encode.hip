@@ -6,27 +6,13 @@
 // int32, converted to float and passed to a correctly rounded sqrtf; divides
 // are correctly rounded; 128 + v * 127 is a separate multiply and add (no
 // FMA); lrintf is round-half-to-even.
-#include "dol_common.hpp"
+#include "dol_cells.hpp"
 
 #include <cfloat>
 
 namespace dolhip {
 
-// EncodeU8Signed, DistanceOcclusionTool.cpp:71-75
-__device__ __forceinline__ uint8_t encode_u8_signed(float v)
-{
-    v = (v < -1.0f) ? -1.0f : ((v > 1.0f) ? 1.0f : v);
-    return (uint8_t)(int)__builtin_rintf(__fadd_rn(128.0f, __fmul_rn(v, 127.0f)));
-}
-
-// EncodeU8, DistanceOcclusionLib.h:137-141
-__device__ __forceinline__ uint8_t encode_u8(float v)
-{
-    v = (v < 0.0f) ? 0.0f : ((v > 1.0f) ? 1.0f : v);
-    return (uint8_t)(int)__builtin_rintf(__fmul_rn(v, 255.0f));
-}
-
-__device__ __forceinline__ float dist_of(int l2) { return sqrt_rn((float)l2); }
+// encode_u8_signed, encode_u8, dist_of and sdf_u8_cell: dol_cells.hpp
 
 __global__ void __launch_bounds__(256) k_dist_2d(size_t n, const uint32_t* __restrict__ delta,
                                                  float* __restrict__ out)
@@ -63,8 +49,7 @@ __global__ void __launch_bounds__(256) k_signed_2d(int w, int xb, const uint32_t
         const float h = __fmul_rn(0.5f, dd);
         ((float*)out)[i] = bit ? -h : h;
     } else {
-        const float q = __fdiv_rn(dd, max_len);
-        ((uint8_t*)out)[i] = encode_u8_signed(bit ? -q : q);
+        ((uint8_t*)out)[i] = sdf_u8_cell(v, bit, max_len);
     }
 }
 

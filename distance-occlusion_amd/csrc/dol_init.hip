@@ -9,7 +9,7 @@
 // bit b in {x, x+1}, row pair q in {y-1, y} and slice pair p in {z-1, z}, so a
 // thread needs just the 3x3(x3) pixels around it; cells whose neighbourhood is
 // uniform receive no write and stay at kMaxDelta.
-#include "dol_common.hpp"
+#include "dol_cells.hpp"
 
 namespace dolhip {
 
@@ -122,113 +122,14 @@ template void launch_init_dist<int16_t>(int, int, const uint32_t*, int16_t*, int
 
 // ------------------------------------------------------------ border init 2D
 
-// pixel with the x = -1 column replicating x = 0 (the "lastBit" seed of
-// DistanceOcclusionLib.cpp:516-517)
-__device__ __forceinline__ int px2(const uint32_t* __restrict__ mask, int ws, int x, int y)
-{
-    return mask_bit(mask, ws, (size_t)y, x < 0 ? 0 : x);
-}
-
-struct Cell2 {
-    int x, y;
-    __device__ void set(int vx, int vy) { x = vx; y = vy; }
-    __device__ void set_if_unset(int vx, int vy)
-    {
-        if (x == kMaxDelta2) { x = vx; y = vy; }
-    }
-};
-
-// Writes to cell (x, y) in the reference's order.  For row pair p the window
-// at bit b has  a = P(b, p), b' = P(b, p+1), and the same one column left;
-//   vert = a ^ b', horz = a ^ a_left, vprev = a_left ^ b'_left,
-//   corner = vert ^ vprev; a corner goes to (b + vprev - 1, p + horz).
+// one thread per cell; see border2_cell (dol_cells.hpp)
 __global__ void __launch_bounds__(256) k_border_2d(int w, int h, int xb, const uint32_t* __restrict__ mask,
                                                    uint32_t* __restrict__ delta)
 {
     const int y = blockIdx.x / xb;
     const int x = (blockIdx.x - y * xb) * 256 + threadIdx.x;
     if (x >= w) return;
-    const int ws = mask_stride(w);
-    const bool right = x + 1 < w;
-    const bool word_end = (x & 31) == 31;
-
-    // 3x3 pixels: c[r][k] for row y-1+r, column x-1+k (right column 0 past w)
-    int c[3][3];
-#pragma unroll
-    for (int r = 0; r < 3; r++) {
-        int yy = y - 1 + r;
-        bool ok = yy >= 0 && yy < h;
-        c[r][0] = ok ? px2(mask, ws, x - 1, yy) : 0;
-        c[r][1] = ok ? px2(mask, ws, x, yy) : 0;
-        c[r][2] = (ok && right) ? px2(mask, ws, x + 1, yy) : 0;
-    }
-
-    Cell2 s{kMaxDelta2, kMaxDelta2};
-    const int any = (c[0][0] | c[0][1] | c[0][2] | c[1][0] | c[1][1] | c[1][2] | c[2][0] | c[2][1] | c[2][2]);
-    const int all = (c[0][0] & c[0][1] & c[1][0] & c[1][1] & c[2][0] & c[2][1]);
-    // rows out of range read as 0 and never take part in a window below, so
-    // a uniform check over the rows in range is enough
-    bool uniform;
-    {
-        int lo = y >= 1 ? 0 : 1, hi = y + 1 < h ? 2 : 1;
-        int ref = c[1][1];
-        uniform = true;
-        for (int r = lo; r <= hi; r++)
-            for (int k = 0; k < 3; k++)
-                if ((k < 2 || right) && c[r][k] != ref) uniform = false;
-        (void)any; (void)all;
-    }
-    if (!uniform) {
-        // pair p = y - 1: this cell is its lower row (r = 1), window rows 0/1
-        if (y >= 1) {
-            const int vert0 = c[0][1] ^ c[1][1], horz0 = c[0][1] ^ c[0][0];
-            const int vprev0 = c[0][0] ^ c[1][0], corner0 = vert0 ^ vprev0;
-            const int vert1 = c[0][2] ^ c[1][2], horz1 = c[0][2] ^ c[0][1];
-            const int vprev1 = vert0, corner1 = vert1 ^ vprev1;
-            const bool e1 = corner0 && vprev0 && horz0;
-            const bool e2 = right && corner1 && !vprev1 && horz1;
-            const bool e3 = vert0;
-            if (!word_end) {
-                if (e1) s.set_if_unset(-1, -1);
-                if (e2) s.set_if_unset(+1, -1);
-                if (e3) s.set(0, -1);
-            } else {
-                if (e1) s.set_if_unset(-1, -1);
-                if (e3) s.set(0, -1);
-                if (e2) s.set_if_unset(+1, -1);
-            }
-        }
-        if (y + 1 < h) {
-            // pair p = y: this cell is its upper row, window rows 1/2
-            const int vert0 = c[1][1] ^ c[2][1], horz0 = c[1][1] ^ c[1][0];
-            const int vprev0 = c[1][0] ^ c[2][0], corner0 = vert0 ^ vprev0;
-            const int vert1 = c[1][2] ^ c[2][2], horz1 = c[1][2] ^ c[1][1];
-            const int vprev1 = vert0, corner1 = vert1 ^ vprev1;
-            const bool f1 = corner0 && vprev0 && !horz0;
-            const bool f2 = right && corner1 && !vprev1 && !horz1;
-            const bool f3 = horz0;
-            const bool f4 = right && horz1;
-            const bool f5 = vert0;
-            if (!word_end) {
-                if (f1) s.set_if_unset(-1, +1);
-                if (f2) s.set_if_unset(+1, +1);
-                if (f3) s.set(-1, 0);
-                if (f4) s.set(+1, 0);
-                if (f5) s.set(0, +1);
-            } else {
-                if (f1) s.set_if_unset(-1, +1);
-                if (f3) s.set(-1, 0);
-                if (f5) s.set(0, +1);
-                if (f2) s.set_if_unset(+1, +1);
-                if (f4) s.set(+1, 0);
-            }
-        } else {
-            // last row: horizontal changes only (DistanceOcclusionLib.cpp:571-598)
-            if (c[1][1] ^ c[1][0]) s.set(-1, 0);
-            if (right && (c[1][2] ^ c[1][1])) s.set(+1, 0);
-        }
-    }
-    delta[(size_t)y * w + x] = pack2(s.x, s.y);
+    delta[(size_t)y * w + x] = border2_cell(mask, mask_stride(w), w, h, x, y);
 }
 
 void launch_border_2d(int w, int h, const uint32_t* mask, int16_t* delta, hipStream_t s)

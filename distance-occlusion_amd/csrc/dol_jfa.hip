@@ -7,22 +7,13 @@
 // candidates in the reference's tap order wins.  The pass schedule (steps
 // FloorPow2(max dim - 1) down to 1, plus one step-1 pass when the count is
 // odd so the result ends in the caller's buffer) is the host driver's.
-#include "dol_common.hpp"
+#include "dol_cells.hpp"
 
 namespace dolhip {
 
 // ------------------------------------------------------------------ 2D
 
-__device__ __forceinline__ void tap2(int& bx, int& by, int& bl, uint32_t v, int ox, int oy)
-{
-    const int cx = wrap16(unpack2x(v) + ox), cy = wrap16(unpack2y(v) + oy);
-    const int cl = len2(cx, cy);
-    if (bl > cl) { bx = cx; by = cy; bl = cl; }
-}
-
-// Tap order (DistanceOcclusionLib.cpp:792-868): self; (-s,0), (-s,-s),
-// (-s,+s) if x-s >= 0; (+s,0), (+s,-s), (+s,+s) if x+s < w; (0,-s); (0,+s).
-// Rows outside [0, h) are not read.
+// one thread per cell; see jfa2_cell (dol_cells.hpp)
 __global__ void __launch_bounds__(256) k_jfa2d(int step, int w, int h, int xb,
                                                const uint32_t* __restrict__ in,
                                                uint32_t* __restrict__ out, int cw)
@@ -30,26 +21,7 @@ __global__ void __launch_bounds__(256) k_jfa2d(int step, int w, int h, int xb,
     const int y = blockIdx.x / xb;
     const int x = (blockIdx.x - y * xb) * 256 + threadIdx.x;
     if (x >= w) return;
-    const int off = cw * step;
-    const size_t row = (size_t)y * w;
-    const bool up = y - step >= 0, down = y + step < h;
-    const bool left = x - step >= 0, rightok = x + step < w;
-    const uint32_t self = in[row + x];
-    int bx = unpack2x(self), by = unpack2y(self);
-    int bl = len2(bx, by);
-    if (left) {
-        tap2(bx, by, bl, in[row + x - step], -off, 0);
-        if (up) tap2(bx, by, bl, in[row - (size_t)step * w + x - step], -off, -off);
-        if (down) tap2(bx, by, bl, in[row + (size_t)step * w + x - step], -off, +off);
-    }
-    if (rightok) {
-        tap2(bx, by, bl, in[row + x + step], +off, 0);
-        if (up) tap2(bx, by, bl, in[row - (size_t)step * w + x + step], +off, -off);
-        if (down) tap2(bx, by, bl, in[row + (size_t)step * w + x + step], +off, +off);
-    }
-    if (up) tap2(bx, by, bl, in[row - (size_t)step * w + x], 0, -off);
-    if (down) tap2(bx, by, bl, in[row + (size_t)step * w + x], 0, +off);
-    out[row + x] = pack2(bx, by);
+    out[(size_t)y * w + x] = jfa2_cell(in, w, h, x, y, step, cw);
 }
 
 void launch_jfa2d_pass(int step, int w, int h, const int16_t* in, int16_t* out, int cw,

@@ -98,7 +98,9 @@ _SIGS = {
     "dolhip_batch_bytes": ([_P], ctypes.c_int64),
     "dolhip_batch_sdf_u8_dev": ([_P, _P, _P, _I, _P], _I),
     "dolhip_glyph_sdf_u8_dev": ([_I, _I, _P, _I, _I, _P, _P, _P], _I),
+    "dolhip_batch_per_image_dev": ([_P, _P, _P, _P], _I),
     "dolhip_batch_config": ([_P, _P, _P], _I),
+    "dolhip_batch_set_variant": ([_P, _I], _I),
     "dolhip_batch_tune": ([_P, _P, _P, _P], _I),
     "dolhip_mask_add_points_2d": ([_I, _I, _I, ctypes.c_uint32, _P], None),
     "dolhip_mask_add_points_3d": ([_I, _I, _I, _I, ctypes.c_uint32, _P], None),
@@ -153,6 +155,14 @@ class DolHip:
                 f"{path} is not built: run python distance-occlusion_amd/build.py "
                 "(there is no CPU fallback)")
         self.path = path
+        # PyTorch ships its own HIP runtime under the same soname as ROCm's
+        # (libamdhip64.so.7); whichever is loaded first serves the whole
+        # process.  Load PyTorch's first so both share the runtime PyTorch
+        # initialises the device with.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         self.L = ctypes.CDLL(path)
         self.missing = []
         for name, (args, res) in _SIGS.items():
@@ -353,6 +363,55 @@ class DolHip:
             else:
                 conv.append(a)
         return self._call(f"dolhip_{name}_dev", *conv, _stream(stream))
+
+
+class Batch:
+    """A ragged set of glyph masks (dolhip_batch_*); arrays are numpy."""
+
+    def __init__(self, lib, widths, heights, offsets, threshold=128, less_than=False):
+        self.lib = lib
+        self.w = np.ascontiguousarray(widths, dtype=np.int32)
+        self.h = np.ascontiguousarray(heights, dtype=np.int32)
+        self.off = np.ascontiguousarray(offsets, dtype=np.int64)
+        self.ptr = ctypes.c_void_p()
+        lib._call("dolhip_batch_create", len(self.w), _addr(self.w), _addr(self.h), _addr(self.off),
+                  threshold, int(less_than), ctypes.byref(self.ptr))
+
+    def close(self):
+        if self.ptr:
+            self.lib.L.dolhip_batch_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def nbytes(self):
+        return self.lib.L.dolhip_batch_bytes(self.ptr)
+
+    def variants(self):
+        return self.lib.L.dolhip_batch_set_variant(self.ptr, -1)
+
+    def set_variant(self, v):
+        self.lib._call("dolhip_batch_set_variant", self.ptr, v)
+
+    def config(self):
+        cells = ctypes.c_int(0)
+        waves = ctypes.c_int(0)
+        self.lib._call("dolhip_batch_config", self.ptr, ctypes.byref(cells), ctypes.byref(waves))
+        return cells.value, waves.value
+
+    def run(self, data, out, graph=False, stream=None):
+        self.lib._call("dolhip_batch_sdf_u8_dev", self.ptr, _addr(data), _addr(out), int(graph), _stream(stream))
+
+    def run_per_image(self, data, out, stream=None):
+        self.lib._call("dolhip_batch_per_image_dev", self.ptr, _addr(data), _addr(out), _stream(stream))
+
+    def tune(self, data, out, stream=None):
+        self.lib._call("dolhip_batch_tune", self.ptr, _addr(data), _addr(out), _stream(stream))
 
 
 _LIB = None

@@ -209,12 +209,20 @@ int64_t dolhip_batch_bytes(const dolhip_batch* b);
  * a hipGraph the first time and replayed afterwards for the same pointers. */
 int dolhip_batch_sdf_u8_dev(dolhip_batch* b, const uint8_t* bytes, uint8_t* out, int graph,
                             void* stream);
-/* the same per-mask pipeline with one set of launches per mask (reference
- * for the batched form) */
+/* the same per-mask pipeline with one kernel launch per step (the per-image
+ * form); work: at least (ceil(w/32)*h + 2*w*h)*4 + 512 bytes of device memory */
 int dolhip_glyph_sdf_u8_dev(int w, int h, const uint8_t* bytes, int threshold, int less_than,
                             uint8_t* out, void* work, void* stream);
+/* every mask of the batch through dolhip_glyph_sdf_u8_dev, one after another
+ * on `stream` (the per-image form of the same work) */
+int dolhip_batch_per_image_dev(dolhip_batch* b, const uint8_t* bytes, uint8_t* out, void* stream);
 /* tile configuration chosen by the tuner for this batch (for reports) */
 int dolhip_batch_config(const dolhip_batch* b, int* tile_cells, int* waves_per_group);
+/* select kernel variant `variant` (0 <= variant < count) for later calls;
+ * with variant < 0 returns the number of variants instead */
+int dolhip_batch_set_variant(dolhip_batch* b, int variant);
+/* times every (threads, cells per thread) variant on this batch and keeps the
+ * fastest for later dolhip_batch_sdf_u8_dev calls; overwrites `out` */
 int dolhip_batch_tune(dolhip_batch* b, const uint8_t* bytes, uint8_t* out, void* stream);
 
 /* ---- continuous seeds (extension) ---------------------------------------- */
@@ -224,8 +232,9 @@ int dolhip_batch_tune(dolhip_batch* b, const uint8_t* bytes, uint8_t* out, void*
 size_t dolhip_seed_workspace(int w, int h, int n);
 int dolhip_seed_nearest_2d_dev(int w, int h, int n, const float* seeds, int32_t* idx,
                                float* dist, void* work, void* stream);
-/* grad[k] += sum over cells of seed k of -gout * weight * (c - p_k) / d (zero at d = 0);
- * grad is 2n floats and is overwritten */
+/* grad[k] = sum over cells c of seed k of -gout(c) (c - p_k) / d(c), zero where
+ * d = 0; grad is 2n floats and is overwritten (float atomics: the summation
+ * order, hence the last bits, vary between runs) */
 int dolhip_seed_grad_2d_dev(int w, int h, int n, const float* seeds, const int32_t* idx,
                             const float* dist, const float* gout, float* grad, void* stream);
 

@@ -235,3 +235,58 @@ def test_invalid_arguments(gpu):
     assert e.value.code == dolhip.ERR_INVALID_ARG
     with pytest.raises(dolhip.DolHipError):
         gpu._call("dolhip_init_deltas_2d", 4, 4, None, None, 0)
+
+
+def _pattern_masks(rng, dims):
+    """Adversarial patterns for the envelope: walls (every site survives),
+    checkerboards, single voxels, diagonal lines, plus random and blobs."""
+    shape = tuple(reversed(dims))
+    out = [np.zeros(shape, bool), np.ones(shape, bool)]
+    one = np.zeros(shape, bool)
+    one[tuple(int(rng.integers(0, s)) for s in shape)] = True
+    out.append(one)
+    wall = np.zeros(shape, bool)
+    wall[..., shape[-1] // 3] = True          # plane x = const
+    out.append(wall)
+    wall2 = np.zeros(shape, bool)
+    wall2[..., shape[-2] // 2, :] = True      # plane y = const
+    out.append(wall2)
+    idx = np.indices(shape).sum(axis=0)
+    out.append(idx % 2 == 0)
+    out.append(idx % 7 == 0)
+    out.append(rng.random(shape) < 0.001)
+    out.append(rng.random(shape) < 0.3)
+    return [pack_mask(b) for b in out] + [blob_mask(rng, dims, 12)]
+
+
+@pytest.mark.parametrize("dims", [(512, 64, 24), (130, 257, 33), (33, 512, 5), (5, 7, 512),
+                                  (512, 512, 3), (31, 1, 512), (1, 40, 1)])
+def test_edt_3d_tiled_patterns(gpu, oracle, rng, dims):
+    w, h, d = dims
+    for m in _pattern_masks(rng, dims):
+        for inv in (False, True):
+            d2, fl = gpu.edt_3d(w, h, d, m, inv)
+            exp = oracle.edt_3d(w, h, d, m, inv)
+            np.testing.assert_array_equal(d2, exp)
+            finite = exp != 0x7FFFFFFF
+            np.testing.assert_array_equal(fl[finite], np.sqrt(exp[finite].astype(np.float32)))
+
+
+@pytest.mark.parametrize("dims", [(512, 512), (300, 511), (512, 7), (7, 512), (513, 40), (40, 513)])
+def test_edt_2d_patterns(gpu, oracle, rng, dims):
+    w, h = dims
+    for m in _pattern_masks(rng, dims):
+        for inv in (False, True):
+            d2, _ = gpu.edt_2d(w, h, m, inv)
+            np.testing.assert_array_equal(d2, oracle.edt_2d(w, h, m, inv))
+
+
+def test_edt_3d_scene_slab(gpu, oracle):
+    """A 512 x 512 x 48 slab of the seeded ellipsoid/box scene."""
+    import dolhip
+
+    w = h = 512
+    d = 48
+    m = dolhip.load().scene_shapes_3d(w, h, d, 40, 14, seed=11)
+    d2, _ = gpu.edt_3d(w, h, d, m)
+    np.testing.assert_array_equal(d2, oracle.edt_3d(w, h, d, m))
