@@ -245,24 +245,43 @@ int64_t dolhip_scene_glyphs(int n, int min_size, int max_size, uint64_t seed, in
         }
         if (bytes) {
             uint8_t* o = bytes + off;
-            for (int y = 0; y < h; y++)
-                for (int x = 0; x < w; x++) {
-                    bool in = false;
-                    for (int k = 0, j = nv - 1; k < nv; j = k++) {
-                        if (((py[k] > y) != (py[j] > y)) &&
-                            (x < (px[j] - px[k]) * (y - py[k]) / (py[j] - py[k]) + px[k]))
-                            in = !in;
-                    }
-                    for (int k = 0; k < ns && !in; k++) {
-                        double vx = sx1[k] - sx0[k], vy = sy1[k] - sy0[k];
-                        double l2 = vx * vx + vy * vy;
+            std::memset(o, 0, (size_t)w * h);
+            // polygon, even-odd rule at pixel centres, one scanline at a time:
+            // pixel x is inside when an odd number of edge crossings lie right
+            // of it (crossing test (py[k] > y) != (py[j] > y), x < crossing)
+            std::vector<double> xs;
+            for (int y = 0; y < h; y++) {
+                xs.clear();
+                for (int k = 0, j = nv - 1; k < nv; j = k++)
+                    if ((py[k] > y) != (py[j] > y))
+                        xs.push_back((px[j] - px[k]) * (y - py[k]) / (py[j] - py[k]) + px[k]);
+                std::sort(xs.begin(), xs.end());
+                uint8_t* row = o + (size_t)y * w;
+                for (size_t e = 0; e + 1 < xs.size(); e += 2) {
+                    // x < xs[e+1] and not x < xs[e]: xs[e] <= x < xs[e+1]
+                    int x0 = (int)std::ceil(xs[e]), x1 = (int)std::ceil(xs[e + 1]) - 1;
+                    x0 = std::max(x0, 0);
+                    x1 = std::min(x1, w - 1);
+                    for (int x = x0; x <= x1; x++) row[x] = 255;
+                }
+            }
+            // strokes: capsules of diameter sw around each segment
+            for (int k = 0; k < ns; k++) {
+                const double rad = 0.5 * sw[k];
+                const int xa = std::max(0, (int)std::floor(std::min(sx0[k], sx1[k]) - rad));
+                const int xb = std::min(w - 1, (int)std::ceil(std::max(sx0[k], sx1[k]) + rad));
+                const int ya = std::max(0, (int)std::floor(std::min(sy0[k], sy1[k]) - rad));
+                const int yb = std::min(h - 1, (int)std::ceil(std::max(sy0[k], sy1[k]) + rad));
+                const double vx = sx1[k] - sx0[k], vy = sy1[k] - sy0[k];
+                const double l2 = vx * vx + vy * vy;
+                for (int y = ya; y <= yb; y++)
+                    for (int x = xa; x <= xb; x++) {
                         double t = l2 > 0 ? ((x - sx0[k]) * vx + (y - sy0[k]) * vy) / l2 : 0.0;
                         t = t < 0 ? 0 : (t > 1 ? 1 : t);
-                        double ex = x - (sx0[k] + t * vx), ey = y - (sy0[k] + t * vy);
-                        if (ex * ex + ey * ey <= 0.25 * sw[k] * sw[k]) in = true;
+                        const double ex = x - (sx0[k] + t * vx), ey = y - (sy0[k] + t * vy);
+                        if (ex * ex + ey * ey <= rad * rad) o[(size_t)y * w + x] = 255;
                     }
-                    o[(size_t)y * w + x] = in ? 255 : 0;
-                }
+            }
         }
         off += (int64_t)w * h;
     }
