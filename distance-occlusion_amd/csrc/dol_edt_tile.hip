@@ -297,23 +297,36 @@ __global__ void __launch_bounds__(kThreads) k_edt3d_xy(int w, int h, int d, cons
     const int tail = w - 32 * (ws - 1);
     const uint32_t last_valid = tail >= 32 ? 0xffffffffu : ((1u << tail) - 1u);
 
-    // x distances of the 32 columns of word j, one row per thread
+    // x distances of the 32 columns of word j, one row per thread; the whole
+    // row (<= 16 words) is loaded at once so the loads overlap
     for (int y = threadIdx.x; y < h; y += kThreads) {
         const uint32_t* row = mask + ((size_t)z * h + y) * ws;
-        auto word = [&](int k) -> uint32_t {
-            const uint32_t m = row[k] ^ invert;
-            return k == ws - 1 ? (m & last_valid) : m;
-        };
-        const uint32_t wj = word(j);
+        uint32_t words[16];
+        if (ws == 16) {
+            const uint4* r4 = (const uint4*)row;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint4 v = r4[k];
+                words[4 * k] = v.x; words[4 * k + 1] = v.y; words[4 * k + 2] = v.z; words[4 * k + 3] = v.w;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 16; k++) words[k] = k < ws ? row[k] : 0u;
+        }
+        uint32_t wj = 0;
         int L = -1, R = -1;
-        for (int k = j - 1; k >= 0; k--) {
-            const uint32_t v = word(k);
-            if (v) { L = 32 * k + hi_bit(v); break; }
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            uint32_t m = words[k] ^ invert;
+            if (k == ws - 1) m &= last_valid;
+            if (k >= ws) m = 0u;
+            words[k] = m;
+            if (k == j) wj = m;
+            if (k < j && m) L = 32 * k + hi_bit(m);
         }
-        for (int k = j + 1; k < ws; k++) {
-            const uint32_t v = word(k);
-            if (v) { R = 32 * k + lo_bit(v); break; }
-        }
+#pragma unroll
+        for (int k = 15; k >= 0; k--)
+            if (k > j && words[k]) R = 32 * k + lo_bit(words[k]);
         uint32_t* out = (uint32_t*)(dx + y * kCols);
 #pragma unroll
         for (int i = 0; i < 32; i += 2) {
@@ -356,9 +369,25 @@ __global__ void __launch_bounds__(kThreads) k_edt3d_z(int w, int h, int d, const
     const int y = blockIdx.x / xt;
     const int x0 = (blockIdx.x - y * xt) * 32;
     const int valid = min(32, w - x0);
-    for (int e = threadIdx.x; e < d * kCols; e += kThreads) {
-        const int z = e >> 5, c = e & 31;
-        fs[e] = c < valid ? f2[((size_t)z * h + y) * w + x0 + c] : kEdtInf;
+    if (valid == kCols && d == kMaxN && (w & 3) == 0) {
+        // full tile: 16-byte loads, all in flight before the first LDS store
+        constexpr int kIter = kMaxN * (kCols / 4) / kThreads;   // 8
+        uint4 v[kIter];
+#pragma unroll
+        for (int k = 0; k < kIter; k++) {
+            const int e = threadIdx.x + k * kThreads;
+            v[k] = *(const uint4*)(f2 + ((size_t)(e >> 3) * h + y) * w + x0 + (e & 7) * 4);
+        }
+#pragma unroll
+        for (int k = 0; k < kIter; k++) {
+            const int e = threadIdx.x + k * kThreads;
+            *(uint4*)(fs + (e >> 3) * kCols + (e & 7) * 4) = v[k];
+        }
+    } else {
+        for (int e = threadIdx.x; e < d * kCols; e += kThreads) {
+            const int z = e >> 5, c = e & 31;
+            fs[e] = c < valid ? f2[((size_t)z * h + y) * w + x0 + c] : kEdtInf;
+        }
     }
     __syncthreads();
     FieldSrc src{fs};
