@@ -81,7 +81,25 @@ def build(verbose=False, jobs=None):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cli(verbose)
     return LIB
+
+
+CLI_SRC = os.path.join(HERE, "cli", "distocc_hip.cpp")
+CLI_BIN = os.path.join(HERE, "cli", "distocc_hip")
+
+
+def build_cli(verbose=False):
+    """The distocc_hip command (DOL:: interface over libdolhip.so)."""
+    cxx = shutil.which("g++") or "c++"
+    cmd = [cxx, "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), CLI_SRC,
+           "-L" + HERE, "-ldolhip", "-Wl,-rpath," + HERE, "-o", CLI_BIN]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"distocc_hip build failed:\n{r.stdout}\n{r.stderr}")
+    return CLI_BIN
 
 
 

@@ -1,0 +1,100 @@
+// dol_shim.cpp -- the DOL:: drop-in interface (include/DistanceOcclusionLib.h)
+// over the C ABI (include/dol_hip.h).
+//
+// Each function keeps the reference's signature, its in-place host-memory
+// contract and its synchronous behaviour (DistanceOcclusionLib.h:21-98).  The
+// reference reports nothing, so a failure here -- a bad argument the reference
+// would have run into undefined behaviour with, or no usable device -- prints
+// the library's message and aborts.
+#include "../../include/DistanceOcclusionLib.h"
+#include "../../include/dol_hip.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+static_assert(sizeof(DOL::cCellDelta2) == 4, "cCellDelta2 must be 2 x int16");
+static_assert(sizeof(DOL::cCellDelta3) == 6, "cCellDelta3 must be 3 x int16, packed");
+
+namespace {
+
+void require(int rc, const char* what)
+{
+    if (rc != DOLHIP_OK) {
+        std::fprintf(stderr, "DOL::%s failed: %s\n", what, dolhip_last_error());
+        std::abort();
+    }
+}
+
+int16_t* raw(DOL::cCellDelta2* p) { return reinterpret_cast<int16_t*>(p); }
+int16_t* raw(DOL::cCellDelta3* p) { return reinterpret_cast<int16_t*>(p); }
+
+}  // namespace
+
+// DistanceOcclusionLib.cpp:79-93, 1214-1222: new[]-allocated, zeroed, so
+// callers may delete[] them directly
+uint32_t* DOL::CreateBitMask(int w, int h)
+{
+    size_t n = (size_t)MaskSize(w, h);
+    uint32_t* m = new uint32_t[n];
+    std::memset(m, 0, n * sizeof(uint32_t));
+    return m;
+}
+
+uint32_t* DOL::CreateBitMask(int w, int h, int d)
+{
+    size_t n = (size_t)MaskSize(w, h, d);
+    uint32_t* m = new uint32_t[n];
+    std::memset(m, 0, n * sizeof(uint32_t));
+    return m;
+}
+
+void DOL::DestroyBitMask(uint32_t*& mask)
+{
+    delete[] mask;
+    mask = nullptr;
+}
+
+void DOL::BitMaskAddPoints(int w, int h, int n, uint32_t seed, uint32_t mask[])
+{ dolhip_mask_add_points_2d(w, h, n, seed, mask); }
+void DOL::BitMaskAddPoints(int w, int h, int d, int n, uint32_t seed, uint32_t mask[])
+{ dolhip_mask_add_points_3d(w, h, d, n, seed, mask); }
+void DOL::BitMaskAddBlock(int w, int h, int sides, uint32_t mask[])
+{ dolhip_mask_add_block_2d(w, h, sides, mask); }
+void DOL::BitMaskAddBlock(int w, int h, int d, int sides, uint32_t mask[])
+{ dolhip_mask_add_block_3d(w, h, d, sides, mask); }
+
+void DOL::InitDistancesFromBitMask(int w, int h, const uint32_t mask[], int16_t distances[], int16_t maxD)
+{ require(dolhip_init_distances_i16(w, h, mask, distances, maxD), "InitDistancesFromBitMask"); }
+void DOL::InitDistancesFromBitMask(int w, int h, const uint32_t mask[], int32_t distances[], int32_t maxD)
+{ require(dolhip_init_distances_i32(w, h, mask, distances, maxD), "InitDistancesFromBitMask"); }
+void DOL::InitDistancesFromBitMask(int w, int h, const uint32_t mask[], float distances[], float maxD)
+{ require(dolhip_init_distances_f32(w, h, mask, distances, maxD), "InitDistancesFromBitMask"); }
+
+void DOL::Felzenszwalb(int w, int h, float distances[])
+{ require(dolhip_felzenszwalb_2d(w, h, distances), "Felzenszwalb"); }
+
+void DOL::InitDeltasFromBitMask(int w, int h, const uint32_t mask[], cCellDelta2 delta[], bool invert)
+{ require(dolhip_init_deltas_2d(w, h, mask, raw(delta), invert), "InitDeltasFromBitMask"); }
+void DOL::InitDeltasFromBitMask(int w, int h, int d, const uint32_t mask[], cCellDelta3 delta[], bool invert)
+{ require(dolhip_init_deltas_3d(w, h, d, mask, raw(delta), invert), "InitDeltasFromBitMask"); }
+
+void DOL::InitBorderDeltasFromBitMask(int w, int h, const uint32_t mask[], cCellDelta2 delta[])
+{ require(dolhip_init_border_deltas_2d(w, h, mask, raw(delta)), "InitBorderDeltasFromBitMask"); }
+void DOL::InitBorderDeltasFromBitMask(int w, int h, int d, const uint32_t mask[], cCellDelta3 delta[])
+{ require(dolhip_init_border_deltas_3d(w, h, d, mask, raw(delta)), "InitBorderDeltasFromBitMask"); }
+
+void DOL::JumpFlood(int w, int h, cCellDelta2 delta[], int cw)
+{ require(dolhip_jump_flood_2d(w, h, raw(delta), cw), "JumpFlood"); }
+void DOL::JumpFlood(int w, int h, int d, cCellDelta3 delta[], int cw)
+{ require(dolhip_jump_flood_3d(w, h, d, raw(delta), cw), "JumpFlood"); }
+
+void DOL::EDT(int w, int h, const uint32_t mask[], int32_t d2[], float dist[], bool invert)
+{ require(dolhip_edt_2d(w, h, mask, invert, d2, dist), "EDT"); }
+void DOL::EDT(int w, int h, int d, const uint32_t mask[], int32_t d2[], float dist[], bool invert)
+{ require(dolhip_edt_3d(w, h, d, mask, invert, d2, dist), "EDT"); }
+
+void DOL::SignedEDT(int w, int h, const uint32_t mask[], float sdf[])
+{ require(dolhip_signed_edt_2d(w, h, mask, sdf), "SignedEDT"); }
+void DOL::SignedEDT(int w, int h, int d, const uint32_t mask[], float sdf[])
+{ require(dolhip_signed_edt_3d(w, h, d, mask, sdf), "SignedEDT"); }
