@@ -115,31 +115,38 @@ __device__ void envelope(int n, const Src& src, Out& out, Bits bits)
     const int lo = b * kBand;
     const int hi = min(n, lo + kBand);
 
-    // 1. band hull, register stack
+    // 1. band hull, register stack; the band's 32 samples are read from LDS
+    //    up front (independent loads) and the loop is unrolled over them
     if (b < nb) {
+        int fr[kBand];
+#pragma unroll
+        for (int i = 0; i < kBand; i++) fr[i] = lo + i < hi ? src.f(c, lo + i) : kNone;
         uint32_t st = 0;
         int t = -1, s = -1, Ft = 0, Fs = 0;
-        for (int q = lo; q < hi; q++) {
-            const int f = src.f(c, q);
-            if (f < 0) continue;
-            const int Fq = f + q * q;
-            while (s >= 0 && removable(s, Fs, t, Ft, q, Fq)) {
-                st &= ~(1u << (t - lo));
-                t = s;
-                Ft = Fs;
-                const uint32_t below = st & ((1u << (t - lo)) - 1u);
-                if (below) {
-                    s = lo + hi_bit(below);
-                    Fs = src.f(c, s) + s * s;
-                } else {
-                    s = -1;
+#pragma unroll
+        for (int i = 0; i < kBand; i++) {
+            const int f = fr[i];
+            if (f >= 0) {
+                const int q = lo + i;
+                const int Fq = f + q * q;
+                while (s >= 0 && removable(s, Fs, t, Ft, q, Fq)) {
+                    st &= ~(1u << (t - lo));
+                    t = s;
+                    Ft = Fs;
+                    const uint32_t below = st & ((1u << (t - lo)) - 1u);
+                    if (below) {
+                        s = lo + hi_bit(below);
+                        Fs = src.f(c, s) + s * s;
+                    } else {
+                        s = -1;
+                    }
                 }
+                st |= 1u << i;
+                s = t;
+                Fs = Ft;
+                t = q;
+                Ft = Fq;
             }
-            st |= 1u << (q - lo);
-            s = t;
-            Fs = Ft;
-            t = q;
-            Ft = Fq;
         }
         bits.put(b, c, st);
     }
@@ -181,10 +188,14 @@ __device__ void envelope(int n, const Src& src, Out& out, Bits bits)
         __syncthreads();
     }
 
-    // 3. owners and outputs
+    // 3. owners and outputs.  The band's survivor word stays in a register,
+    //    so the next survivor inside the band is found with bit operations;
+    //    LDS is scanned only across band ends.
     if (b < nb) {
-        int a = bits.prev(c, lo + 1, 0);           // last survivor <= lo
-        if (a < 0) a = bits.next(c, lo, n);        // else the first one after
+        const uint32_t mine = bits.get(b, c);
+        const int after = bits.next(c, hi - 1, n);        // first survivor past the band
+        int a = (mine & 1u) ? lo : bits.prev(c, lo, 0);   // last survivor <= lo
+        if (a < 0) a = mine ? lo + lo_bit(mine) : after;  // else the first one after
         if (a < 0) {
             for (int q = lo; q < hi; q++) out(c, q, kEdtInf);
             return;
@@ -203,7 +214,15 @@ __device__ void envelope(int n, const Src& src, Out& out, Bits bits)
                 break;
             }
         }
-        int nx = bits.next(c, a, n);
+        auto next_of = [&](int v) -> int {
+            if (v >= lo && v < hi) {
+                const int k = v - lo;
+                const uint32_t m = k == 31 ? 0u : (mine & (0xfffffffeu << k));
+                return m ? lo + lo_bit(m) : after;
+            }
+            return bits.next(c, v, n);
+        };
+        int nx = next_of(a);
         int fn = nx >= 0 ? src.f(c, nx) : 0;
         for (int q = lo; q < hi; q++) {
             int va = fa + (q - a) * (q - a);
@@ -213,7 +232,7 @@ __device__ void envelope(int n, const Src& src, Out& out, Bits bits)
                 a = nx;
                 fa = fn;
                 va = vn;
-                nx = bits.next(c, a, n);
+                nx = next_of(a);
                 fn = nx >= 0 ? src.f(c, nx) : 0;
             }
             out(c, q, va);
@@ -223,11 +242,18 @@ __device__ void envelope(int n, const Src& src, Out& out, Bits bits)
 
 // ---- sources
 
+// x distances, row q at q * 32 + (q / 32) * 8: each band's rows are skewed by
+// 16 more bytes than the previous band's, so the two bands one wavefront
+// reads fall in different LDS banks (the skew grows monotonically, so rows
+// never overlap)
+__device__ __forceinline__ int dx_index(int q) { return q * kCols + (q >> 5) * 8; }
+constexpr int kDxWords = kMaxN * kCols + kBands * 8;
+
 struct DxSrc {   // edt3d_xy: x distance per (row, column), 0xffff = none
     const uint16_t* dx;
     __device__ int f(int c, int q) const
     {
-        const int v = dx[q * kCols + c];
+        const int v = dx[dx_index(q) + c];
         return v == 0xffff ? kNone : v * v;
     }
 };
@@ -249,20 +275,20 @@ struct FieldOut {   // int32 field at (x0 + c, q, z) of a w x h x d volume
     __device__ void operator()(int c, int q, int v) const
     {
         const int x = x0 + c;
-        if (x < w) f2[((size_t)z * h + q) * w + x] = v;
+        if (x < w) f2[((uint32_t)z * h + q) * (uint32_t)w + x] = v;
     }
 };
 
-struct FinalOut {   // int32 + float at cell index base(q) + x
+struct FinalOut {   // int32 + float at cell index base(q) + x (< 2^31 on this path)
     int32_t* d2;
     float* fl;
-    size_t row_stride;   // elements between consecutive q
-    size_t base;         // element index of (x0, q = 0)
-    int valid;           // columns < valid are inside the volume
+    uint32_t row_stride;   // elements between consecutive q
+    uint32_t base;         // element index of (x0, q = 0)
+    int valid;             // columns < valid are inside the volume
     __device__ void operator()(int c, int q, int v) const
     {
         if (c >= valid) return;
-        const size_t i = base + (size_t)q * row_stride + c;
+        const uint32_t i = base + (uint32_t)q * row_stride + (uint32_t)c;
         if (d2) d2[i] = v;
         if (fl) fl[i] = v == kEdtInf ? sqrt_rn(FLT_MAX) : sqrt_rn((float)v);
     }
@@ -280,7 +306,7 @@ __global__ void __launch_bounds__(kThreads) k_edt3d_xy(int w, int h, int d, cons
                                                        uint32_t invert, int32_t* __restrict__ f2,
                                                        float* __restrict__ fl)
 {
-    __shared__ uint16_t dx[kMaxN * kCols];
+    __shared__ __align__(16) uint16_t dx[kDxWords];
     __shared__ uint32_t bm[kBands * kCols];
     const int ws = mask_stride(w);
     // swizzle: blocks congruent mod 8 share an XCD; a group of (up to) 8
@@ -327,7 +353,7 @@ __global__ void __launch_bounds__(kThreads) k_edt3d_xy(int w, int h, int d, cons
 #pragma unroll
         for (int k = 15; k >= 0; k--)
             if (k > j && words[k]) R = 32 * k + lo_bit(words[k]);
-        uint32_t* out = (uint32_t*)(dx + y * kCols);
+        uint32_t* out = (uint32_t*)(dx + dx_index(y));
 #pragma unroll
         for (int i = 0; i < 32; i += 2) {
             uint32_t pair = 0;
@@ -351,7 +377,7 @@ __global__ void __launch_bounds__(kThreads) k_edt3d_xy(int w, int h, int d, cons
     Bits bits{bm};
     if (FINAL) {
         // 2D image (d == 1): final outputs, rows of width w
-        FinalOut o{f2, fl, (size_t)w, (size_t)32 * j, min(32, w - 32 * j)};
+        FinalOut o{f2, fl, (uint32_t)w, (uint32_t)(32 * j), min(32, w - 32 * j)};
         envelope(h, src, o, bits);
     } else {
         FieldOut o{f2, w, h, 32 * j, z};
@@ -392,9 +418,10 @@ __global__ void __launch_bounds__(kThreads) k_edt3d_z(int w, int h, int d, const
     __syncthreads();
     FieldSrc src{fs};
     Bits bits{bm};
-    FinalOut o{d2, fl, (size_t)w * h, (size_t)y * w + x0, valid};
+    FinalOut o{d2, fl, (uint32_t)(w * h), (uint32_t)(y * w + x0), valid};
     envelope(d, src, o, bits);
 }
+
 
 // every dimension <= 512 keeps each F below 2^20 and each product in int32
 bool edt_tile_fits(int w, int h, int d) { return w <= kMaxN && h <= kMaxN && d <= kMaxN; }
