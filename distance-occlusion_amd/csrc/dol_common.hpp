@@ -54,6 +54,23 @@ __device__ __forceinline__ float sqrt_rn(float x)
     return r;
 }
 
+// sqrt_rn of a squared distance given as an int32 >= 0, with kEdtInf ("no
+// site") giving sqrt_rn(FLT_MAX).  The input is 0 or >= 1, so the residual
+// step above needs no tiny-input branch (for 0 the estimate is 0 and neither
+// neighbour test fires, the lower neighbour being NaN).
+__device__ __forceinline__ float sqrt_d2(int v)
+{
+    const float x = v == kEdtInf ? 3.40282347e+38f : (float)v;
+    float r = __builtin_amdgcn_sqrtf(x);
+    const float rm = __uint_as_float(__float_as_uint(r) - 1u);
+    const float rp = __uint_as_float(__float_as_uint(r) + 1u);
+    const float em = __builtin_fmaf(-rm, r, x);
+    const float ep = __builtin_fmaf(-rp, r, x);
+    r = em <= 0.0f ? rm : r;
+    r = ep > 0.0f ? rp : r;
+    return r;
+}
+
 // delta2 packed in a uint32: x in the low half, y in the high half
 __device__ __forceinline__ uint32_t pack2(int x, int y)
 {

@@ -233,7 +233,7 @@ __global__ void __launch_bounds__(256) k_edt_lines(int n, long long lines, long 
         }
         const int32_t r = c0 > 0x7ffffffeLL ? kEdtInf : (int32_t)c0;
         if (dp) dp[stride * q] = r;
-        if (lp) lp[stride * q] = r == kEdtInf ? sqrt_rn(FLT_MAX) : sqrt_rn((float)r);
+        if (lp) lp[stride * q] = sqrt_d2(r);
     }
 }
 

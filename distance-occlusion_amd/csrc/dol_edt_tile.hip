@@ -13,9 +13,12 @@
 //   2. adjacent band hulls are merged in a tree of log2(bands) rounds by
 //      walking to their common bridge (x-separated convex chains), clearing
 //      the survivor bits between the bridge ends in an LDS bitmask;
-//   3. each thread finds the owning site of its band's first sample by a
-//      short walk along the survivors and then sweeps its band, advancing the
-//      owner monotonically, and writes the outputs.
+//   3. every surviving site v gets the start of its interval of ownership,
+//      l_v = ceil((F(v) - F(u)) / (2 (v - u))) against its hull predecessor
+//      u, and, if that interval is not empty, marks its first sample (so no
+//      sample is marked twice); the owner of q is then the last mark at or
+//      before q, a running maximum along the line, and each output is
+//      computed without data-dependent loops.
 // All comparisons are exact integer cross-multiplications.  Values: with
 // lines <= 512 every F is below 2^20 and every product below 2^29, so int32
 // arithmetic is exact.
@@ -96,6 +99,35 @@ struct Bits {
     }
 };
 
+// ---- LDS words.  Sample q of column c is word cell(q, c): bits 0..19 hold
+// f (kNoSite: none; every f is below 2^20 on this path), bits 20..29 the mark
+// of phase 3 (index + 1 of the site whose interval starts at q).  Each band's
+// rows are skewed by 32 words more than the previous band's, so the two
+// bands one wavefront reads fall in different LDS banks.
+
+constexpr uint32_t kValueMask = 0xfffffu;
+constexpr uint32_t kNoSite = 0xfffffu;
+constexpr int kSkew = 32;
+constexpr int kWords = kMaxN * kCols + kBands * kSkew;
+
+__device__ __forceinline__ int cell(int q, int c) { return q * kCols + (q >> 5) * kSkew + c; }
+
+__device__ __forceinline__ int fval(const uint32_t* fs, int c, int q)
+{
+    const uint32_t v = fs[cell(q, c)] & kValueMask;
+    return v == kNoSite ? kNone : (int)v;
+}
+
+// smallest k with k * b >= a, for b in [2, 1022] and |a| < 2^20: the float
+// quotient is within 1/8 of a / b, so one integer correction makes it exact
+__device__ __forceinline__ int ceil_div(int a, int b)
+{
+    int k = (int)__builtin_ceilf((float)a * __builtin_amdgcn_rcpf((float)b));
+    if (k * b < a) k++;
+    else if ((k - 1) * b >= a) k--;
+    return k;
+}
+
 // v lies on or above the chord u-w (its envelope interval is empty): the
 // intersection of u and v is not left of that of v and w.
 __device__ __forceinline__ bool removable(int u, int Fu, int v, int Fv, int w, int Fw)
@@ -103,10 +135,11 @@ __device__ __forceinline__ bool removable(int u, int Fu, int v, int Fv, int w, i
     return (Fv - Fu) * (w - v) >= (Fw - Fv) * (v - u);
 }
 
-// Envelope of one tile.  Src gives f(c, q) >= 0 or kNone; Out(c, q, value)
-// stores a result (value kEdtInf where the line has no site).
-template <class Src, class Out>
-__device__ void envelope(int n, const Src& src, Out& out, Bits bits)
+// Envelope of one tile of n-sample lines held in fs; band_max is [kBands][32]
+// scratch.  Out(c, q, value) stores a result (kEdtInf where the line has no
+// site).
+template <class Out>
+__device__ void envelope(int n, uint32_t* fs, uint32_t* band_max, Out& out, Bits bits)
 {
     const int tid = threadIdx.x;
     const int c = tid & (kCols - 1);
@@ -120,7 +153,7 @@ __device__ void envelope(int n, const Src& src, Out& out, Bits bits)
     if (b < nb) {
         int fr[kBand];
 #pragma unroll
-        for (int i = 0; i < kBand; i++) fr[i] = lo + i < hi ? src.f(c, lo + i) : kNone;
+        for (int i = 0; i < kBand; i++) fr[i] = lo + i < hi ? fval(fs, c, lo + i) : kNone;
         uint32_t st = 0;
         int t = -1, s = -1, Ft = 0, Fs = 0;
 #pragma unroll
@@ -136,7 +169,7 @@ __device__ void envelope(int n, const Src& src, Out& out, Bits bits)
                     const uint32_t below = st & ((1u << (t - lo)) - 1u);
                     if (below) {
                         s = lo + hi_bit(below);
-                        Fs = src.f(c, s) + s * s;
+                        Fs = fval(fs, c, s) + s * s;
                     } else {
                         s = -1;
                     }
@@ -160,11 +193,11 @@ __device__ void envelope(int n, const Src& src, Out& out, Bits bits)
             int a = bits.prev(c, mid, lo);
             int r = bits.next(c, mid - 1, end);
             if (a >= 0 && r >= 0) {
-                int Fa = src.f(c, a) + a * a, Fr = src.f(c, r) + r * r;
+                int Fa = fval(fs, c, a) + a * a, Fr = fval(fs, c, r) + r * r;
                 for (;;) {
                     const int rn = bits.next(c, r, end);
                     if (rn >= 0) {
-                        const int Frn = src.f(c, rn) + rn * rn;
+                        const int Frn = fval(fs, c, rn) + rn * rn;
                         if (removable(a, Fa, r, Fr, rn, Frn)) {
                             r = rn;
                             Fr = Frn;
@@ -173,7 +206,7 @@ __device__ void envelope(int n, const Src& src, Out& out, Bits bits)
                     }
                     const int ap = bits.prev(c, a, lo);
                     if (ap >= 0) {
-                        const int Fap = src.f(c, ap) + ap * ap;
+                        const int Fap = fval(fs, c, ap) + ap * ap;
                         if (removable(ap, Fap, a, Fa, r, Fr)) {
                             a = ap;
                             Fa = Fap;
@@ -188,84 +221,64 @@ __device__ void envelope(int n, const Src& src, Out& out, Bits bits)
         __syncthreads();
     }
 
-    // 3. owners and outputs.  The band's survivor word stays in a register,
-    //    so the next survivor inside the band is found with bit operations;
-    //    LDS is scanned only across band ends.
+    // 3a. interval starts and marks
     if (b < nb) {
         const uint32_t mine = bits.get(b, c);
-        const int after = bits.next(c, hi - 1, n);        // first survivor past the band
-        int a = (mine & 1u) ? lo : bits.prev(c, lo, 0);   // last survivor <= lo
-        if (a < 0) a = mine ? lo + lo_bit(mine) : after;  // else the first one after
-        if (a < 0) {
-            for (int q = lo; q < hi; q++) out(c, q, kEdtInf);
-            return;
-        }
-        int fa = src.f(c, a);
-        // owner at lo: unimodal along the survivors, so walk left while
-        // strictly better, then right while not worse
-        for (;;) {
-            const int ap = bits.prev(c, a, 0);
-            if (ap < 0) break;
-            const int fp = src.f(c, ap);
-            if (fp + (lo - ap) * (lo - ap) < fa + (lo - a) * (lo - a)) {
-                a = ap;
-                fa = fp;
-            } else {
-                break;
-            }
-        }
-        auto next_of = [&](int v) -> int {
-            if (v >= lo && v < hi) {
-                const int k = v - lo;
-                const uint32_t m = k == 31 ? 0u : (mine & (0xfffffffeu << k));
-                return m ? lo + lo_bit(m) : after;
-            }
-            return bits.next(c, v, n);
+        int pv = bits.prev(c, lo, 0);                  // hull predecessor of the band
+        int pF = pv >= 0 ? fval(fs, c, pv) + pv * pv : 0;
+        int cur = -1, cur_l = 0;
+        auto mark = [&](int v, int from, int to) {     // v owns [from, to) if not empty
+            from = max(from, 0);
+            if (from < min(to, n)) atomicOr(&fs[cell(from, c)], (uint32_t)(v + 1) << 20);
         };
-        int nx = next_of(a);
-        int fn = nx >= 0 ? src.f(c, nx) : 0;
-        for (int q = lo; q < hi; q++) {
-            int va = fa + (q - a) * (q - a);
-            while (nx >= 0) {
-                const int vn = fn + (q - nx) * (q - nx);
-                if (vn > va) break;
-                a = nx;
-                fa = fn;
-                va = vn;
-                nx = next_of(a);
-                fn = nx >= 0 ? src.f(c, nx) : 0;
+#pragma unroll
+        for (int i = 0; i < kBand; i++) {
+            if ((mine >> i) & 1u) {
+                const int v = lo + i;
+                const int Fv = fval(fs, c, v) + v * v;
+                const int l = pv < 0 ? -kMaxN : ceil_div(Fv - pF, 2 * (v - pv));
+                if (cur >= 0) mark(cur, cur_l, l);
+                cur = v;
+                cur_l = l;
+                pv = v;
+                pF = Fv;
             }
-            out(c, q, va);
+        }
+        if (cur >= 0) {
+            const int sv = bits.next(c, hi - 1, n);    // hull successor of the band
+            const int l = sv < 0 ? n : ceil_div(fval(fs, c, sv) + sv * sv - pF, 2 * (sv - pv));
+            mark(cur, cur_l, l);
+        }
+    }
+    __syncthreads();
+
+    // 3b. owners: running maximum of the marks, carried across bands
+    constexpr uint32_t kMarkMask = ~kValueMask;
+    if (b < nb) {
+        uint32_t m = 0;
+#pragma unroll
+        for (int i = 0; i < kBand; i++)
+            if (lo + i < hi) m = max(m, fs[cell(lo + i, c)] & kMarkMask);
+        band_max[b * kCols + c] = m;
+    }
+    __syncthreads();
+    if (b < nb) {
+        uint32_t run = 0;
+#pragma unroll
+        for (int k = 0; k < kBands - 1; k++)
+            if (k < b) run = max(run, band_max[k * kCols + c]);
+#pragma unroll
+        for (int i = 0; i < kBand; i++) {
+            const int q = lo + i;
+            if (q < hi) {
+                run = max(run, fs[cell(q, c)] & kMarkMask);
+                const int a = (int)(run >> 20) - 1;
+                const int fa = (int)(fs[cell(max(a, 0), c)] & kValueMask);
+                out(c, q, a < 0 ? kEdtInf : fa + (q - a) * (q - a));
+            }
         }
     }
 }
-
-// ---- sources
-
-// x distances, row q at q * 32 + (q / 32) * 8: each band's rows are skewed by
-// 16 more bytes than the previous band's, so the two bands one wavefront
-// reads fall in different LDS banks (the skew grows monotonically, so rows
-// never overlap)
-__device__ __forceinline__ int dx_index(int q) { return q * kCols + (q >> 5) * 8; }
-constexpr int kDxWords = kMaxN * kCols + kBands * 8;
-
-struct DxSrc {   // edt3d_xy: x distance per (row, column), 0xffff = none
-    const uint16_t* dx;
-    __device__ int f(int c, int q) const
-    {
-        const int v = dx[dx_index(q) + c];
-        return v == 0xffff ? kNone : v * v;
-    }
-};
-
-struct FieldSrc {   // edt3d_z: int32 field, kEdtInf = none
-    const int32_t* fs;
-    __device__ int f(int c, int q) const
-    {
-        const int v = fs[q * kCols + c];
-        return v == kEdtInf ? kNone : v;
-    }
-};
 
 // ---- outputs
 
@@ -290,7 +303,7 @@ struct FinalOut {   // int32 + float at cell index base(q) + x (< 2^31 on this p
         if (c >= valid) return;
         const uint32_t i = base + (uint32_t)q * row_stride + (uint32_t)c;
         if (d2) d2[i] = v;
-        if (fl) fl[i] = v == kEdtInf ? sqrt_rn(FLT_MAX) : sqrt_rn((float)v);
+        if (fl) fl[i] = sqrt_d2(v);
     }
 };
 
@@ -302,12 +315,13 @@ using namespace tile;
 // 16 tiles of one slice mapped to the same XCD (workgroups are dealt to the
 // 8 XCDs round-robin) so a slice's mask rows are fetched into one L2.
 template <bool FINAL>
-__global__ void __launch_bounds__(kThreads) k_edt3d_xy(int w, int h, int d, const uint32_t* __restrict__ mask,
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) k_edt3d_xy(int w, int h, int d, const uint32_t* __restrict__ mask,
                                                        uint32_t invert, int32_t* __restrict__ f2,
                                                        float* __restrict__ fl)
 {
-    __shared__ __align__(16) uint16_t dx[kDxWords];
+    __shared__ __align__(16) uint32_t fs[kWords];
     __shared__ uint32_t bm[kBands * kCols];
+    __shared__ uint32_t band_max[kBands * kCols];
     const int ws = mask_stride(w);
     // swizzle: blocks congruent mod 8 share an XCD; a group of (up to) 8
     // slices takes 8*ws consecutive blocks, slice-minor
@@ -353,44 +367,47 @@ __global__ void __launch_bounds__(kThreads) k_edt3d_xy(int w, int h, int d, cons
 #pragma unroll
         for (int k = 15; k >= 0; k--)
             if (k > j && words[k]) R = 32 * k + lo_bit(words[k]);
-        uint32_t* out = (uint32_t*)(dx + dx_index(y));
+        uint4* out = (uint4*)(fs + cell(y, 0));
 #pragma unroll
-        for (int i = 0; i < 32; i += 2) {
-            uint32_t pair = 0;
+        for (int i = 0; i < 32; i += 4) {
+            uint32_t quad[4];
 #pragma unroll
-            for (int e = 0; e < 2; e++) {
+            for (int e = 0; e < 4; e++) {
                 const int ii = i + e;
                 const int x = 32 * j + ii;
                 const uint32_t lo = wj & (ii == 31 ? 0xffffffffu : ((2u << ii) - 1u));
                 const uint32_t hi = wj >> ii;
-                int dl = lo ? ii - hi_bit(lo) : (L >= 0 ? x - L : 0xffff);
-                int dr = hi ? lo_bit(hi) : (R >= 0 ? R - x : 0xffff);
-                const int dd = min(min(dl, dr), 0xffff);
-                pair |= (uint32_t)dd << (16 * e);
+                const int dl = lo ? ii - hi_bit(lo) : (L >= 0 ? x - L : kMaxN);
+                const int dr = hi ? lo_bit(hi) : (R >= 0 ? R - x : kMaxN);
+                const int dd = min(dl, dr);
+                quad[e] = dd >= kMaxN ? kNoSite : (uint32_t)(dd * dd);
             }
-            out[i >> 1] = pair;
+            out[i >> 2] = make_uint4(quad[0], quad[1], quad[2], quad[3]);
         }
     }
     __syncthreads();
 
-    DxSrc src{dx};
     Bits bits{bm};
     if (FINAL) {
         // 2D image (d == 1): final outputs, rows of width w
         FinalOut o{f2, fl, (uint32_t)w, (uint32_t)(32 * j), min(32, w - 32 * j)};
-        envelope(h, src, o, bits);
+        envelope(h, fs, band_max, o, bits);
     } else {
         FieldOut o{f2, w, h, 32 * j, z};
-        envelope(h, src, o, bits);
+        envelope(h, fs, band_max, o, bits);
     }
 }
 
+// an f2 sample (< 2^20, or kEdtInf) as an LDS word
+__device__ __forceinline__ uint32_t packed(int v) { return v == kEdtInf ? kNoSite : (uint32_t)v; }
+
 // z envelope over the f2 tile of row y, columns 32*jt .. 32*jt+31
-__global__ void __launch_bounds__(kThreads) k_edt3d_z(int w, int h, int d, const int32_t* __restrict__ f2,
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) k_edt3d_z(int w, int h, int d, const int32_t* __restrict__ f2,
                                                       int32_t* __restrict__ d2, float* __restrict__ fl)
 {
-    __shared__ int32_t fs[kMaxN * kCols];
+    __shared__ __align__(16) uint32_t fs[kWords];
     __shared__ uint32_t bm[kBands * kCols];
+    __shared__ uint32_t band_max[kBands * kCols];
     const int xt = (w + 31) >> 5;
     const int y = blockIdx.x / xt;
     const int x0 = (blockIdx.x - y * xt) * 32;
@@ -398,28 +415,28 @@ __global__ void __launch_bounds__(kThreads) k_edt3d_z(int w, int h, int d, const
     if (valid == kCols && d == kMaxN && (w & 3) == 0) {
         // full tile: 16-byte loads, all in flight before the first LDS store
         constexpr int kIter = kMaxN * (kCols / 4) / kThreads;   // 8
-        uint4 v[kIter];
+        int4 v[kIter];
 #pragma unroll
         for (int k = 0; k < kIter; k++) {
             const int e = threadIdx.x + k * kThreads;
-            v[k] = *(const uint4*)(f2 + ((size_t)(e >> 3) * h + y) * w + x0 + (e & 7) * 4);
+            v[k] = *(const int4*)(f2 + ((size_t)(e >> 3) * h + y) * w + x0 + (e & 7) * 4);
         }
 #pragma unroll
         for (int k = 0; k < kIter; k++) {
             const int e = threadIdx.x + k * kThreads;
-            *(uint4*)(fs + (e >> 3) * kCols + (e & 7) * 4) = v[k];
+            *(uint4*)(fs + cell(e >> 3, (e & 7) * 4)) =
+                make_uint4(packed(v[k].x), packed(v[k].y), packed(v[k].z), packed(v[k].w));
         }
     } else {
         for (int e = threadIdx.x; e < d * kCols; e += kThreads) {
             const int z = e >> 5, c = e & 31;
-            fs[e] = c < valid ? f2[((size_t)z * h + y) * w + x0 + c] : kEdtInf;
+            fs[cell(z, c)] = c < valid ? packed(f2[((size_t)z * h + y) * w + x0 + c]) : kNoSite;
         }
     }
     __syncthreads();
-    FieldSrc src{fs};
     Bits bits{bm};
     FinalOut o{d2, fl, (uint32_t)(w * h), (uint32_t)(y * w + x0), valid};
-    envelope(d, src, o, bits);
+    envelope(d, fs, band_max, o, bits);
 }
 
 

@@ -157,8 +157,8 @@ __global__ void __launch_bounds__(256) k_signed_edt(size_t n, const int32_t* __r
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int32_t a = dout[i], b = din[i];
-    const float fa = a == kEdtInf ? sqrt_rn(FLT_MAX) : sqrt_rn((float)a);
-    const float fb = b == kEdtInf ? sqrt_rn(FLT_MAX) : sqrt_rn((float)b);
+    const float fa = sqrt_d2(a);
+    const float fb = sqrt_d2(b);
     sdf[i] = __fsub_rn(fa, fb);
 }
 
