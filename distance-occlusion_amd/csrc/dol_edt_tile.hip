@@ -38,16 +38,18 @@ namespace dolhip {
 namespace tile {
 
 constexpr int kCols = 32;      // lines per tile (one mask word of x)
-constexpr int kBand = 32;      // sites per band
+constexpr int kBand = 16;      // sites per band
 constexpr int kMaxN = 512;     // longest line held in LDS
 constexpr int kBands = kMaxN / kBand;
-constexpr int kThreads = kCols * kBands;   // 512: thread = band * 32 + column
+constexpr int kThreads = kCols * kBands;   // 1024: thread = band * 32 + column
 constexpr int kNone = -1;
 
 __device__ __forceinline__ int hi_bit(uint32_t v) { return 31 - __clz(v); }
 __device__ __forceinline__ int lo_bit(uint32_t v) { return __ffs(v) - 1; }
 
-// survivor bitmask: word k of column c holds band k
+// survivor bitmask: word k of column c holds band k in its low kBand bits
+__device__ __forceinline__ uint32_t bits_upto(int i) { return i >= 31 ? 0xffffffffu : ((2u << i) - 1u); }
+
 struct Bits {
     uint32_t* w;   // [kBands][kCols]
     __device__ uint32_t get(int k, int c) const { return w[k * kCols + c]; }
@@ -57,28 +59,28 @@ struct Bits {
     __device__ int prev(int c, int pos, int lo) const
     {
         if (pos <= lo) return -1;
-        int k = (pos - 1) >> 5;
-        uint32_t m = get(k, c) & (((pos - 1) & 31) == 31 ? 0xffffffffu : ((2u << ((pos - 1) & 31)) - 1u));
+        int k = (pos - 1) / kBand;
+        uint32_t m = get(k, c) & bits_upto((pos - 1) % kBand);
         for (;;) {
             if (m) {
-                int r = (k << 5) + hi_bit(m);
+                const int r = k * kBand + hi_bit(m);
                 return r >= lo ? r : -1;
             }
-            if (--k < (lo >> 5)) return -1;
+            if (--k < lo / kBand) return -1;
             m = get(k, c);
         }
     }
     // lowest survivor in (pos, hi), or -1
     __device__ int next(int c, int pos, int hi) const
     {
-        int p = pos + 1;
+        const int p = pos + 1;
         if (p >= hi) return -1;
-        int k = p >> 5;
-        const int kend = (hi - 1) >> 5;
-        uint32_t m = get(k, c) & ~((1u << (p & 31)) - 1u);
+        int k = p / kBand;
+        const int kend = (hi - 1) / kBand;
+        uint32_t m = get(k, c) & ~((1u << (p % kBand)) - 1u);
         for (;;) {
             if (m) {
-                int r = (k << 5) + lo_bit(m);
+                const int r = k * kBand + lo_bit(m);
                 return r < hi ? r : -1;
             }
             if (++k > kend) return -1;
@@ -88,12 +90,12 @@ struct Bits {
     // clear survivors strictly between a and b
     __device__ void clear_between(int c, int a, int b)
     {
-        int p = a + 1, q = b - 1;
+        const int p = a + 1, q = b - 1;
         if (p > q) return;
-        int k0 = p >> 5, k1 = q >> 5;
+        const int k0 = p / kBand, k1 = q / kBand;
         for (int k = k0; k <= k1; k++) {
-            uint32_t lo = k == k0 ? ~((1u << (p & 31)) - 1u) : 0xffffffffu;
-            uint32_t hi = k == k1 ? ((q & 31) == 31 ? 0xffffffffu : ((2u << (q & 31)) - 1u)) : 0xffffffffu;
+            const uint32_t lo = k == k0 ? ~((1u << (p % kBand)) - 1u) : 0xffffffffu;
+            const uint32_t hi = k == k1 ? bits_upto(q % kBand) : 0xffffffffu;
             put(k, c, get(k, c) & ~(lo & hi));
         }
     }
@@ -103,14 +105,15 @@ struct Bits {
 // f (kNoSite: none; every f is below 2^20 on this path), bits 20..29 the mark
 // of phase 3 (index + 1 of the site whose interval starts at q).  Each band's
 // rows are skewed by 32 words more than the previous band's, so the two
-// bands one wavefront reads fall in different LDS banks.
+// bands one wavefront reads fall in different LDS banks; rows stay 16-byte
+// aligned.
 
 constexpr uint32_t kValueMask = 0xfffffu;
 constexpr uint32_t kNoSite = 0xfffffu;
 constexpr int kSkew = 32;
 constexpr int kWords = kMaxN * kCols + kBands * kSkew;
 
-__device__ __forceinline__ int cell(int q, int c) { return q * kCols + (q >> 5) * kSkew + c; }
+__device__ __forceinline__ int cell(int q, int c) { return q * kCols + (q / kBand) * kSkew + c; }
 
 __device__ __forceinline__ int fval(const uint32_t* fs, int c, int q)
 {
@@ -135,11 +138,11 @@ __device__ __forceinline__ bool removable(int u, int Fu, int v, int Fv, int w, i
     return (Fv - Fu) * (w - v) >= (Fw - Fv) * (v - u);
 }
 
-// Envelope of one tile of n-sample lines held in fs; band_max is [kBands][32]
-// scratch.  Out(c, q, value) stores a result (kEdtInf where the line has no
-// site).
+// Envelope of one tile of n-sample lines held in fs.  Out(c, q, value) stores
+// a result (kEdtInf where the line has no site).  The survivor bitmask's
+// storage is reused for the band maxima of phase 3b.
 template <class Out>
-__device__ void envelope(int n, uint32_t* fs, uint32_t* band_max, Out& out, Bits bits)
+__device__ void envelope(int n, uint32_t* fs, Out& out, Bits bits)
 {
     const int tid = threadIdx.x;
     const int c = tid & (kCols - 1);
@@ -185,7 +188,9 @@ __device__ void envelope(int n, uint32_t* fs, uint32_t* band_max, Out& out, Bits
     }
     __syncthreads();
 
-    // 2. tree of bridge merges
+    // 2. tree of bridge merges.  Each walk keeps the bitmask words of its two
+    //    ends in registers and the sample of each end's next candidate
+    //    loaded ahead, so a step waits for one LDS read.
     for (int span = 1; span < nb; span <<= 1) {
         if ((b % (2 * span)) == 0 && b + span < nb) {
             const int mid = (b + span) * kBand;
@@ -193,25 +198,43 @@ __device__ void envelope(int n, uint32_t* fs, uint32_t* band_max, Out& out, Bits
             int a = bits.prev(c, mid, lo);
             int r = bits.next(c, mid - 1, end);
             if (a >= 0 && r >= 0) {
-                int Fa = fval(fs, c, a) + a * a, Fr = fval(fs, c, r) + r * r;
-                for (;;) {
-                    const int rn = bits.next(c, r, end);
-                    if (rn >= 0) {
-                        const int Frn = fval(fs, c, rn) + rn * rn;
-                        if (removable(a, Fa, r, Fr, rn, Frn)) {
-                            r = rn;
-                            Fr = Frn;
-                            continue;
-                        }
+                const int kfirst = lo / kBand, klast = (end - 1) / kBand;
+                int ka = a / kBand, kr = r / kBand;
+                uint32_t wa = bits.get(ka, c), wr = bits.get(kr, c);
+                auto right_of = [&](int v) -> int {   // next survivor after v (in word kr)
+                    uint32_t m = wr & ~bits_upto(v % kBand);
+                    while (!m) {
+                        if (++kr > klast) return -1;
+                        m = wr = bits.get(kr, c);
                     }
-                    const int ap = bits.prev(c, a, lo);
-                    if (ap >= 0) {
-                        const int Fap = fval(fs, c, ap) + ap * ap;
-                        if (removable(ap, Fap, a, Fa, r, Fr)) {
-                            a = ap;
-                            Fa = Fap;
-                            continue;
-                        }
+                    return kr * kBand + lo_bit(m);
+                };
+                auto left_of = [&](int v) -> int {    // survivor before v (in word ka)
+                    uint32_t m = wa & ((1u << (v % kBand)) - 1u);
+                    while (!m) {
+                        if (--ka < kfirst) return -1;
+                        m = wa = bits.get(ka, c);
+                    }
+                    return ka * kBand + hi_bit(m);
+                };
+                auto F = [&](int v) { return v >= 0 ? fval(fs, c, v) + v * v : 0; };
+                int Fa = F(a), Fr = F(r);
+                int rn = right_of(r), ap = left_of(a);
+                int Frn = F(rn), Fap = F(ap);
+                for (;;) {
+                    if (rn >= 0 && removable(a, Fa, r, Fr, rn, Frn)) {
+                        r = rn;
+                        Fr = Frn;
+                        rn = right_of(r);
+                        Frn = F(rn);
+                        continue;
+                    }
+                    if (ap >= 0 && removable(ap, Fap, a, Fa, r, Fr)) {
+                        a = ap;
+                        Fa = Fap;
+                        ap = left_of(a);
+                        Fap = F(ap);
+                        continue;
                     }
                     break;
                 }
@@ -254,6 +277,7 @@ __device__ void envelope(int n, uint32_t* fs, uint32_t* band_max, Out& out, Bits
 
     // 3b. owners: running maximum of the marks, carried across bands
     constexpr uint32_t kMarkMask = ~kValueMask;
+    uint32_t* band_max = bits.w;
     if (b < nb) {
         uint32_t m = 0;
 #pragma unroll
@@ -315,13 +339,12 @@ using namespace tile;
 // 16 tiles of one slice mapped to the same XCD (workgroups are dealt to the
 // 8 XCDs round-robin) so a slice's mask rows are fetched into one L2.
 template <bool FINAL>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) k_edt3d_xy(int w, int h, int d, const uint32_t* __restrict__ mask,
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) k_edt3d_xy(int w, int h, int d, const uint32_t* __restrict__ mask,
                                                        uint32_t invert, int32_t* __restrict__ f2,
                                                        float* __restrict__ fl)
 {
     __shared__ __align__(16) uint32_t fs[kWords];
     __shared__ uint32_t bm[kBands * kCols];
-    __shared__ uint32_t band_max[kBands * kCols];
     const int ws = mask_stride(w);
     // swizzle: blocks congruent mod 8 share an XCD; a group of (up to) 8
     // slices takes 8*ws consecutive blocks, slice-minor
@@ -337,9 +360,11 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
     const int tail = w - 32 * (ws - 1);
     const uint32_t last_valid = tail >= 32 ? 0xffffffffu : ((1u << tail) - 1u);
 
-    // x distances of the 32 columns of word j, one row per thread; the whole
-    // row (<= 16 words) is loaded at once so the loads overlap
-    for (int y = threadIdx.x; y < h; y += kThreads) {
+    // x distances of the 32 columns of word j, half a row (16 columns) per
+    // thread; the whole row (<= 16 words) is loaded at once so the loads
+    // overlap
+    for (int e = threadIdx.x; e < 2 * h; e += kThreads) {
+        const int y = e >> 1, half = e & 1;
         const uint32_t* row = mask + ((size_t)z * h + y) * ws;
         uint32_t words[16];
         if (ws == 16) {
@@ -367,20 +392,20 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
 #pragma unroll
         for (int k = 15; k >= 0; k--)
             if (k > j && words[k]) R = 32 * k + lo_bit(words[k]);
-        uint4* out = (uint4*)(fs + cell(y, 0));
+        uint4* out = (uint4*)(fs + cell(y, 16 * half));
 #pragma unroll
-        for (int i = 0; i < 32; i += 4) {
+        for (int i = 0; i < 16; i += 4) {
             uint32_t quad[4];
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int ii = i + e;
+            for (int t = 0; t < 4; t++) {
+                const int ii = 16 * half + i + t;
                 const int x = 32 * j + ii;
-                const uint32_t lo = wj & (ii == 31 ? 0xffffffffu : ((2u << ii) - 1u));
+                const uint32_t lo = wj & bits_upto(ii);
                 const uint32_t hi = wj >> ii;
                 const int dl = lo ? ii - hi_bit(lo) : (L >= 0 ? x - L : kMaxN);
                 const int dr = hi ? lo_bit(hi) : (R >= 0 ? R - x : kMaxN);
                 const int dd = min(dl, dr);
-                quad[e] = dd >= kMaxN ? kNoSite : (uint32_t)(dd * dd);
+                quad[t] = dd >= kMaxN ? kNoSite : (uint32_t)(dd * dd);
             }
             out[i >> 2] = make_uint4(quad[0], quad[1], quad[2], quad[3]);
         }
@@ -391,10 +416,10 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
     if (FINAL) {
         // 2D image (d == 1): final outputs, rows of width w
         FinalOut o{f2, fl, (uint32_t)w, (uint32_t)(32 * j), min(32, w - 32 * j)};
-        envelope(h, fs, band_max, o, bits);
+        envelope(h, fs, o, bits);
     } else {
         FieldOut o{f2, w, h, 32 * j, z};
-        envelope(h, fs, band_max, o, bits);
+        envelope(h, fs, o, bits);
     }
 }
 
@@ -402,19 +427,18 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
 __device__ __forceinline__ uint32_t packed(int v) { return v == kEdtInf ? kNoSite : (uint32_t)v; }
 
 // z envelope over the f2 tile of row y, columns 32*jt .. 32*jt+31
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) k_edt3d_z(int w, int h, int d, const int32_t* __restrict__ f2,
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) k_edt3d_z(int w, int h, int d, const int32_t* __restrict__ f2,
                                                       int32_t* __restrict__ d2, float* __restrict__ fl)
 {
     __shared__ __align__(16) uint32_t fs[kWords];
     __shared__ uint32_t bm[kBands * kCols];
-    __shared__ uint32_t band_max[kBands * kCols];
     const int xt = (w + 31) >> 5;
     const int y = blockIdx.x / xt;
     const int x0 = (blockIdx.x - y * xt) * 32;
     const int valid = min(32, w - x0);
     if (valid == kCols && d == kMaxN && (w & 3) == 0) {
         // full tile: 16-byte loads, all in flight before the first LDS store
-        constexpr int kIter = kMaxN * (kCols / 4) / kThreads;   // 8
+        constexpr int kIter = kMaxN * (kCols / 4) / kThreads;
         int4 v[kIter];
 #pragma unroll
         for (int k = 0; k < kIter; k++) {
@@ -436,7 +460,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
     __syncthreads();
     Bits bits{bm};
     FinalOut o{d2, fl, (uint32_t)(w * h), (uint32_t)(y * w + x0), valid};
-    envelope(d, fs, band_max, o, bits);
+    envelope(d, fs, o, bits);
 }
 
 
