@@ -111,6 +111,10 @@ struct Bits {
 constexpr uint32_t kValueMask = 0xfffffu;
 constexpr uint32_t kNoSite = 0xfffffu;
 constexpr int kSkew = 32;
+// band maxima of phase 3b: one 16-byte aligned row per column, padded so that
+// 16 lanes' rows start in different bank quads
+constexpr int kBmStride = kBands + 4;
+constexpr int kBmWords = kCols * kBmStride > kBands * kCols ? kCols * kBmStride : kBands * kCols;
 constexpr int kWords = kMaxN * kCols + kBands * kSkew;
 
 __device__ __forceinline__ int cell(int q, int c) { return q * kCols + (q / kBand) * kSkew + c; }
@@ -151,38 +155,38 @@ __device__ void envelope(int n, uint32_t* fs, Out& out, Bits bits)
     const int lo = b * kBand;
     const int hi = min(n, lo + kBand);
 
-    // 1. band hull, register stack; the band's 32 samples are read from LDS
-    //    up front (independent loads) and the loop is unrolled over them
+    // 1. band hull, register stack.  The band's samples are read up front
+    //    (rows past n hold stale words and are masked off), so the loads
+    //    overlap, and the loop is unrolled over them.
     if (b < nb) {
-        int fr[kBand];
+        uint32_t raw[kBand];
 #pragma unroll
-        for (int i = 0; i < kBand; i++) fr[i] = lo + i < hi ? fval(fs, c, lo + i) : kNone;
+        for (int i = 0; i < kBand; i++) raw[i] = fs[cell(lo + i, c)];
         uint32_t st = 0;
         int t = -1, s = -1, Ft = 0, Fs = 0;
 #pragma unroll
         for (int i = 0; i < kBand; i++) {
-            const int f = fr[i];
-            if (f >= 0) {
-                const int q = lo + i;
-                const int Fq = f + q * q;
-                while (s >= 0 && removable(s, Fs, t, Ft, q, Fq)) {
-                    st &= ~(1u << (t - lo));
-                    t = s;
-                    Ft = Fs;
-                    const uint32_t below = st & ((1u << (t - lo)) - 1u);
-                    if (below) {
-                        s = lo + hi_bit(below);
-                        Fs = fval(fs, c, s) + s * s;
-                    } else {
-                        s = -1;
-                    }
+            const int q = lo + i;
+            const uint32_t v = raw[i] & kValueMask;
+            const bool site = q < hi && v != kNoSite;
+            const int Fq = (int)v + q * q;
+            while (site && s >= 0 && removable(s, Fs, t, Ft, q, Fq)) {
+                st &= ~(1u << (t - lo));
+                t = s;
+                Ft = Fs;
+                const uint32_t below = st & ((1u << (t - lo)) - 1u);
+                if (below) {
+                    s = lo + hi_bit(below);
+                    Fs = fval(fs, c, s) + s * s;
+                } else {
+                    s = -1;
                 }
-                st |= 1u << i;
-                s = t;
-                Fs = Ft;
-                t = q;
-                Ft = Fq;
             }
+            st |= (uint32_t)site << i;
+            s = site ? t : s;
+            Fs = site ? Ft : Fs;
+            t = site ? q : t;
+            Ft = site ? Fq : Ft;
         }
         bits.put(b, c, st);
     }
@@ -244,62 +248,80 @@ __device__ void envelope(int n, uint32_t* fs, Out& out, Bits bits)
         __syncthreads();
     }
 
-    // 3a. interval starts and marks
+    // 3a. interval starts and marks, without branches: a mark that does not
+    //    apply ORs zero into a word in range
     if (b < nb) {
         const uint32_t mine = bits.get(b, c);
         int pv = bits.prev(c, lo, 0);                  // hull predecessor of the band
         int pF = pv >= 0 ? fval(fs, c, pv) + pv * pv : 0;
+        const int sv = bits.next(c, hi - 1, n);        // hull successor of the band
+        const int sF = sv >= 0 ? fval(fs, c, sv) + sv * sv : 0;
+        uint32_t raw[kBand];
+#pragma unroll
+        for (int i = 0; i < kBand; i++) raw[i] = fs[cell(lo + i, c)];
         int cur = -1, cur_l = 0;
-        auto mark = [&](int v, int from, int to) {     // v owns [from, to) if not empty
+        auto mark = [&](bool on, int v, int from, int to) {   // v owns [from, to)
             from = max(from, 0);
-            if (from < min(to, n)) atomicOr(&fs[cell(from, c)], (uint32_t)(v + 1) << 20);
+            on = on && from < min(to, n);
+            atomicOr(&fs[cell(min(from, kMaxN - 1), c)], on ? (uint32_t)(v + 1) << 20 : 0u);
         };
 #pragma unroll
         for (int i = 0; i < kBand; i++) {
-            if ((mine >> i) & 1u) {
-                const int v = lo + i;
-                const int Fv = fval(fs, c, v) + v * v;
-                const int l = pv < 0 ? -kMaxN : ceil_div(Fv - pF, 2 * (v - pv));
-                if (cur >= 0) mark(cur, cur_l, l);
-                cur = v;
-                cur_l = l;
-                pv = v;
-                pF = Fv;
-            }
+            const bool sur = (mine >> i) & 1u;
+            const int v = lo + i;
+            const int Fv = (int)(raw[i] & kValueMask) + v * v;
+            const int l = pv < 0 ? -kMaxN : ceil_div(Fv - pF, 2 * (v - pv));
+            mark(sur && cur >= 0, cur, cur_l, l);
+            cur = sur ? v : cur;
+            cur_l = sur ? l : cur_l;
+            pv = sur ? v : pv;
+            pF = sur ? Fv : pF;
         }
-        if (cur >= 0) {
-            const int sv = bits.next(c, hi - 1, n);    // hull successor of the band
-            const int l = sv < 0 ? n : ceil_div(fval(fs, c, sv) + sv * sv - pF, 2 * (sv - pv));
-            mark(cur, cur_l, l);
-        }
+        const int l = sv < 0 ? n : ceil_div(sF - pF, max(2 * (sv - pv), 2));
+        mark(cur >= 0, cur, cur_l, l);
     }
     __syncthreads();
 
-    // 3b. owners: running maximum of the marks, carried across bands
+    // 3b. owners: running maximum of the marks, carried across bands (the
+    //    band maxima are kept column-major, kBmStride words a column, in the
+    //    bitmask's storage)
     constexpr uint32_t kMarkMask = ~kValueMask;
     uint32_t* band_max = bits.w;
+    uint32_t raw[kBand];
     if (b < nb) {
         uint32_t m = 0;
 #pragma unroll
-        for (int i = 0; i < kBand; i++)
-            if (lo + i < hi) m = max(m, fs[cell(lo + i, c)] & kMarkMask);
-        band_max[b * kCols + c] = m;
+        for (int i = 0; i < kBand; i++) {
+            raw[i] = lo + i < hi ? fs[cell(lo + i, c)] & kMarkMask : 0u;
+            m = max(m, raw[i]);
+        }
+        band_max[c * kBmStride + b] = m;
     }
     __syncthreads();
-    if (b < nb) {
+    if (b < nb && out.active(c)) {
         uint32_t run = 0;
+        const uint4* bm4 = (const uint4*)(band_max + c * kBmStride);
 #pragma unroll
-        for (int k = 0; k < kBands - 1; k++)
-            if (k < b) run = max(run, band_max[k * kCols + c]);
+        for (int k4 = 0; k4 < kBands / 4; k4++) {
+            const uint4 v = bm4[k4];
+            run = max(run, 4 * k4 + 0 < b ? v.x : 0u);
+            run = max(run, 4 * k4 + 1 < b ? v.y : 0u);
+            run = max(run, 4 * k4 + 2 < b ? v.z : 0u);
+            run = max(run, 4 * k4 + 3 < b ? v.w : 0u);
+        }
+        int own[kBand];
+        uint32_t fa[kBand];
+#pragma unroll
+        for (int i = 0; i < kBand; i++) {
+            run = max(run, raw[i]);
+            own[i] = (int)(run >> 20) - 1;
+            fa[i] = fs[cell(max(own[i], 0), c)] & kValueMask;
+        }
 #pragma unroll
         for (int i = 0; i < kBand; i++) {
             const int q = lo + i;
-            if (q < hi) {
-                run = max(run, fs[cell(q, c)] & kMarkMask);
-                const int a = (int)(run >> 20) - 1;
-                const int fa = (int)(fs[cell(max(a, 0), c)] & kValueMask);
-                out(c, q, a < 0 ? kEdtInf : fa + (q - a) * (q - a));
-            }
+            const int a = own[i];
+            if (q < hi) out(c, q, a < 0 ? kEdtInf : (int)fa[i] + (q - a) * (q - a));
         }
     }
 }
@@ -309,10 +331,10 @@ __device__ void envelope(int n, uint32_t* fs, Out& out, Bits bits)
 struct FieldOut {   // int32 field at (x0 + c, q, z) of a w x h x d volume
     int32_t* f2;
     int w, h, x0, z;
+    __device__ bool active(int c) const { return x0 + c < w; }
     __device__ void operator()(int c, int q, int v) const
     {
-        const int x = x0 + c;
-        if (x < w) f2[((uint32_t)z * h + q) * (uint32_t)w + x] = v;
+        f2[((uint32_t)z * h + q) * (uint32_t)w + x0 + c] = v;
     }
 };
 
@@ -322,9 +344,9 @@ struct FinalOut {   // int32 + float at cell index base(q) + x (< 2^31 on this p
     uint32_t row_stride;   // elements between consecutive q
     uint32_t base;         // element index of (x0, q = 0)
     int valid;             // columns < valid are inside the volume
+    __device__ bool active(int c) const { return c < valid; }
     __device__ void operator()(int c, int q, int v) const
     {
-        if (c >= valid) return;
         const uint32_t i = base + (uint32_t)q * row_stride + (uint32_t)c;
         if (d2) d2[i] = v;
         if (fl) fl[i] = sqrt_d2(v);
@@ -344,7 +366,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
                                                        float* __restrict__ fl)
 {
     __shared__ __align__(16) uint32_t fs[kWords];
-    __shared__ uint32_t bm[kBands * kCols];
+    __shared__ __align__(16) uint32_t bm[kBmWords];
     const int ws = mask_stride(w);
     // swizzle: blocks congruent mod 8 share an XCD; a group of (up to) 8
     // slices takes 8*ws consecutive blocks, slice-minor
@@ -431,7 +453,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
                                                       int32_t* __restrict__ d2, float* __restrict__ fl)
 {
     __shared__ __align__(16) uint32_t fs[kWords];
-    __shared__ uint32_t bm[kBands * kCols];
+    __shared__ __align__(16) uint32_t bm[kBmWords];
     const int xt = (w + 31) >> 5;
     const int y = blockIdx.x / xt;
     const int x0 = (blockIdx.x - y * xt) * 32;
