@@ -37,11 +37,11 @@ namespace dolhip {
 
 namespace tile {
 
-constexpr int kCols = 32;      // lines per tile (one mask word of x)
+constexpr int kCols = 16;      // lines per tile (half a mask word of x)
 constexpr int kBand = 16;      // sites per band
 constexpr int kMaxN = 512;     // longest line held in LDS
 constexpr int kBands = kMaxN / kBand;
-constexpr int kThreads = kCols * kBands;   // 1024: thread = band * 32 + column
+constexpr int kThreads = kCols * kBands;   // 512: thread = band * kCols + column
 constexpr int kNone = -1;
 
 __device__ __forceinline__ int hi_bit(uint32_t v) { return 31 - __clz(v); }
@@ -104,13 +104,12 @@ struct Bits {
 // ---- LDS words.  Sample q of column c is word cell(q, c): bits 0..19 hold
 // f (kNoSite: none; every f is below 2^20 on this path), bits 20..29 the mark
 // of phase 3 (index + 1 of the site whose interval starts at q).  Each band's
-// rows are skewed by 32 words more than the previous band's, so the two
-// bands one wavefront reads fall in different LDS banks; rows stay 16-byte
-// aligned.
+// rows are skewed by kCols words more than the previous band's, so the bands
+// one wavefront reads fall in different LDS banks; rows stay 16-byte aligned.
 
 constexpr uint32_t kValueMask = 0xfffffu;
 constexpr uint32_t kNoSite = 0xfffffu;
-constexpr int kSkew = 32;
+constexpr int kSkew = kCols;
 // band maxima of phase 3b: one 16-byte aligned row per column, padded so that
 // 16 lanes' rows start in different bank quads
 constexpr int kBmStride = kBands + 4;
@@ -150,7 +149,7 @@ __device__ void envelope(int n, uint32_t* fs, Out& out, Bits bits)
 {
     const int tid = threadIdx.x;
     const int c = tid & (kCols - 1);
-    const int b = tid >> 5;
+    const int b = tid / kCols;
     const int nb = (n + kBand - 1) / kBand;
     const int lo = b * kBand;
     const int hi = min(n, lo + kBand);
@@ -357,9 +356,10 @@ struct FinalOut {   // int32 + float at cell index base(q) + x (< 2^31 on this p
 
 using namespace tile;
 
-// x pass from the bit rows, then the y envelope.  Workgroup (z, j) with the
-// 16 tiles of one slice mapped to the same XCD (workgroups are dealt to the
-// 8 XCDs round-robin) so a slice's mask rows are fetched into one L2.
+// x pass from the bit rows, then the y envelope.  Workgroup (z, jt) covers
+// columns kCols*jt .. of slice z; the tiles of one slice map to the same XCD
+// (workgroups are dealt to the 8 XCDs round-robin) so a slice's mask rows are
+// fetched into one L2.
 template <bool FINAL>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) k_edt3d_xy(int w, int h, int d, const uint32_t* __restrict__ mask,
                                                        uint32_t invert, int32_t* __restrict__ f2,
@@ -368,25 +368,28 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
     __shared__ __align__(16) uint32_t fs[kWords];
     __shared__ __align__(16) uint32_t bm[kBmWords];
     const int ws = mask_stride(w);
+    const int xt = (w + kCols - 1) / kCols;
     // swizzle: blocks congruent mod 8 share an XCD; a group of (up to) 8
-    // slices takes 8*ws consecutive blocks, slice-minor
-    int z, j;
+    // slices takes 8*xt consecutive blocks, slice-minor
+    int z, jt;
     {
         const int bid = blockIdx.x;
-        const int group = bid / (8 * ws);
+        const int group = bid / (8 * xt);
         const int gs = min(8, d - group * 8);
-        const int inner = bid - group * 8 * ws;
+        const int inner = bid - group * 8 * xt;
         z = group * 8 + inner % gs;
-        j = inner / gs;
+        jt = inner / gs;
     }
+    const int x0 = jt * kCols, j = x0 >> 5, boff = x0 & 31;
     const int tail = w - 32 * (ws - 1);
     const uint32_t last_valid = tail >= 32 ? 0xffffffffu : ((1u << tail) - 1u);
 
-    // x distances of the 32 columns of word j, half a row (16 columns) per
-    // thread; the whole row (<= 16 words) is loaded at once so the loads
-    // overlap
-    for (int e = threadIdx.x; e < 2 * h; e += kThreads) {
-        const int y = e >> 1, half = e & 1;
+    // x distances of the tile's columns (bits boff.. of mask word j), 16
+    // columns per thread; the whole row (<= 16 words) is loaded at once so
+    // the loads overlap
+    constexpr int kParts = kCols / 16;
+    for (int e = threadIdx.x; e < kParts * h; e += kThreads) {
+        const int y = e / kParts, part = e % kParts;
         const uint32_t* row = mask + ((size_t)z * h + y) * ws;
         uint32_t words[16];
         if (ws == 16) {
@@ -414,13 +417,13 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
 #pragma unroll
         for (int k = 15; k >= 0; k--)
             if (k > j && words[k]) R = 32 * k + lo_bit(words[k]);
-        uint4* out = (uint4*)(fs + cell(y, 16 * half));
+        uint4* out = (uint4*)(fs + cell(y, 16 * part));
 #pragma unroll
         for (int i = 0; i < 16; i += 4) {
             uint32_t quad[4];
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                const int ii = 16 * half + i + t;
+                const int ii = boff + 16 * part + i + t;   // bit in word j
                 const int x = 32 * j + ii;
                 const uint32_t lo = wj & bits_upto(ii);
                 const uint32_t hi = wj >> ii;
@@ -437,10 +440,10 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
     Bits bits{bm};
     if (FINAL) {
         // 2D image (d == 1): final outputs, rows of width w
-        FinalOut o{f2, fl, (uint32_t)w, (uint32_t)(32 * j), min(32, w - 32 * j)};
+        FinalOut o{f2, fl, (uint32_t)w, (uint32_t)x0, min(kCols, w - x0)};
         envelope(h, fs, o, bits);
     } else {
-        FieldOut o{f2, w, h, 32 * j, z};
+        FieldOut o{f2, w, h, x0, z};
         envelope(h, fs, o, bits);
     }
 }
@@ -448,16 +451,16 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
 // an f2 sample (< 2^20, or kEdtInf) as an LDS word
 __device__ __forceinline__ uint32_t packed(int v) { return v == kEdtInf ? kNoSite : (uint32_t)v; }
 
-// z envelope over the f2 tile of row y, columns 32*jt .. 32*jt+31
+// z envelope over the f2 tile of row y, columns kCols*jt .. kCols*jt+kCols-1
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) k_edt3d_z(int w, int h, int d, const int32_t* __restrict__ f2,
                                                       int32_t* __restrict__ d2, float* __restrict__ fl)
 {
     __shared__ __align__(16) uint32_t fs[kWords];
     __shared__ __align__(16) uint32_t bm[kBmWords];
-    const int xt = (w + 31) >> 5;
+    const int xt = (w + kCols - 1) / kCols;
     const int y = blockIdx.x / xt;
-    const int x0 = (blockIdx.x - y * xt) * 32;
-    const int valid = min(32, w - x0);
+    const int x0 = (blockIdx.x - y * xt) * kCols;
+    const int valid = min(kCols, w - x0);
     if (valid == kCols && d == kMaxN && (w & 3) == 0) {
         // full tile: 16-byte loads, all in flight before the first LDS store
         constexpr int kIter = kMaxN * (kCols / 4) / kThreads;
@@ -465,17 +468,17 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
 #pragma unroll
         for (int k = 0; k < kIter; k++) {
             const int e = threadIdx.x + k * kThreads;
-            v[k] = *(const int4*)(f2 + ((size_t)(e >> 3) * h + y) * w + x0 + (e & 7) * 4);
+            v[k] = *(const int4*)(f2 + ((size_t)(e / (kCols / 4)) * h + y) * w + x0 + (e % (kCols / 4)) * 4);
         }
 #pragma unroll
         for (int k = 0; k < kIter; k++) {
             const int e = threadIdx.x + k * kThreads;
-            *(uint4*)(fs + cell(e >> 3, (e & 7) * 4)) =
+            *(uint4*)(fs + cell(e / (kCols / 4), (e % (kCols / 4)) * 4)) =
                 make_uint4(packed(v[k].x), packed(v[k].y), packed(v[k].z), packed(v[k].w));
         }
     } else {
         for (int e = threadIdx.x; e < d * kCols; e += kThreads) {
-            const int z = e >> 5, c = e & 31;
+            const int z = e / kCols, c = e % kCols;
             fs[cell(z, c)] = c < valid ? packed(f2[((size_t)z * h + y) * w + x0 + c]) : kNoSite;
         }
     }
@@ -494,25 +497,25 @@ size_t edt_tile_workspace(long long cells) { return ((size_t)cells * 4 + 255) & 
 void launch_edt_tile_2d(int w, int h, const uint32_t* mask, int invert, int32_t* d2, float* fl,
                         hipStream_t s)
 {
-    const int ws = mask_stride(w);
+    const int xt = (w + kCols - 1) / kCols;
     ProfileScope p("edt2d_xy", s);
-    hipLaunchKernelGGL(k_edt3d_xy<true>, dim3((unsigned)ws), dim3(kThreads), 0, s, w, h, 1, mask,
+    hipLaunchKernelGGL(k_edt3d_xy<true>, dim3((unsigned)xt), dim3(kThreads), 0, s, w, h, 1, mask,
                        invert ? 0xffffffffu : 0u, d2, fl);
 }
 
 void launch_edt_tile_3d(int w, int h, int d, const uint32_t* mask, int invert, int32_t* d2,
                         float* fl, void* work, hipStream_t s)
 {
-    const int ws = mask_stride(w);
+    const int xt = (w + kCols - 1) / kCols;
     int32_t* f2 = (int32_t*)work;
     {
         ProfileScope p("edt3d_xy", s);
-        hipLaunchKernelGGL(k_edt3d_xy<false>, dim3((unsigned)(ws * d)), dim3(kThreads), 0, s, w, h,
+        hipLaunchKernelGGL(k_edt3d_xy<false>, dim3((unsigned)(xt * d)), dim3(kThreads), 0, s, w, h,
                            d, mask, invert ? 0xffffffffu : 0u, f2, nullptr);
     }
     {
         ProfileScope p("edt3d_z", s);
-        hipLaunchKernelGGL(k_edt3d_z, dim3((unsigned)(((w + 31) >> 5) * h)), dim3(kThreads), 0, s, w,
+        hipLaunchKernelGGL(k_edt3d_z, dim3((unsigned)(xt * h)), dim3(kThreads), 0, s, w,
                            h, d, f2, d2, fl);
     }
 }
