@@ -37,7 +37,7 @@ namespace dolhip {
 
 namespace tile {
 
-constexpr int kCols = 16;      // lines per tile (half a mask word of x)
+constexpr int kCols = 8;       // lines per tile (a quarter of a mask word of x)
 constexpr int kBand = 16;      // sites per band
 constexpr int kMaxN = 512;     // longest line held in LDS
 constexpr int kBands = kMaxN / kBand;
@@ -384,10 +384,11 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
     const int tail = w - 32 * (ws - 1);
     const uint32_t last_valid = tail >= 32 ? 0xffffffffu : ((1u << tail) - 1u);
 
-    // x distances of the tile's columns (bits boff.. of mask word j), 16
-    // columns per thread; the whole row (<= 16 words) is loaded at once so
+    // x distances of the tile's columns (bits boff.. of mask word j), up to
+    // 16 columns per thread; the whole row (<= 16 words) is loaded at once so
     // the loads overlap
-    constexpr int kParts = kCols / 16;
+    constexpr int kXCols = kCols < 16 ? kCols : 16;
+    constexpr int kParts = kCols / kXCols;
     for (int e = threadIdx.x; e < kParts * h; e += kThreads) {
         const int y = e / kParts, part = e % kParts;
         const uint32_t* row = mask + ((size_t)z * h + y) * ws;
@@ -417,13 +418,13 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
 #pragma unroll
         for (int k = 15; k >= 0; k--)
             if (k > j && words[k]) R = 32 * k + lo_bit(words[k]);
-        uint4* out = (uint4*)(fs + cell(y, 16 * part));
+        uint4* out = (uint4*)(fs + cell(y, kXCols * part));
 #pragma unroll
-        for (int i = 0; i < 16; i += 4) {
+        for (int i = 0; i < kXCols; i += 4) {
             uint32_t quad[4];
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                const int ii = boff + 16 * part + i + t;   // bit in word j
+                const int ii = boff + kXCols * part + i + t;   // bit in word j
                 const int x = 32 * j + ii;
                 const uint32_t lo = wj & bits_upto(ii);
                 const uint32_t hi = wj >> ii;
@@ -458,8 +459,13 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
     __shared__ __align__(16) uint32_t fs[kWords];
     __shared__ __align__(16) uint32_t bm[kBmWords];
     const int xt = (w + kCols - 1) / kCols;
-    const int y = blockIdx.x / xt;
-    const int x0 = (blockIdx.x - y * xt) * kCols;
+    // swizzle: blocks congruent mod 8 share an XCD; give each XCD a run of
+    // consecutive tiles, so the partial rows that neighbouring tiles write
+    // meet in one L2
+    int bid = blockIdx.x;
+    if ((gridDim.x & 7) == 0) bid = (bid & 7) * (gridDim.x >> 3) + (bid >> 3);
+    const int y = bid / xt;
+    const int x0 = (bid - y * xt) * kCols;
     const int valid = min(kCols, w - x0);
     if (valid == kCols && d == kMaxN && (w & 3) == 0) {
         // full tile: 16-byte loads, all in flight before the first LDS store
