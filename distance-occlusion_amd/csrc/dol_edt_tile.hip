@@ -191,24 +191,30 @@ __device__ void envelope(int n, uint32_t* fs, Out& out, Bits bits)
     }
     __syncthreads();
 
-    // 2. tree of bridge merges.  Each walk keeps the bitmask words of its two
-    //    ends in registers and the sample of each end's next candidate
-    //    loaded ahead, so a step waits for one LDS read.
+    // 2. tree of bridge merges.  A round's merges are dealt to the first
+    //    threads of the workgroup (kCols consecutive threads per group of
+    //    bands), so the walks fill whole wavefronts.  Each walk keeps the
+    //    bitmask words of its two ends in registers and the sample of each
+    //    end's next candidate loaded ahead, so a step waits for one LDS read.
     for (int span = 1; span < nb; span <<= 1) {
-        if ((b % (2 * span)) == 0 && b + span < nb) {
-            const int mid = (b + span) * kBand;
-            const int end = min(n, (b + 2 * span) * kBand);
-            int a = bits.prev(c, mid, lo);
-            int r = bits.next(c, mid - 1, end);
+        const int groups = (nb - span + 2 * span - 1) / (2 * span);
+        if (tid < groups * kCols) {
+            const int mc = tid % kCols;
+            const int gb = (tid / kCols) * 2 * span;   // first band of the group
+            const int glo = gb * kBand;
+            const int mid = (gb + span) * kBand;
+            const int end = min(n, (gb + 2 * span) * kBand);
+            int a = bits.prev(mc, mid, glo);
+            int r = bits.next(mc, mid - 1, end);
             if (a >= 0 && r >= 0) {
-                const int kfirst = lo / kBand, klast = (end - 1) / kBand;
+                const int kfirst = gb, klast = (end - 1) / kBand;
                 int ka = a / kBand, kr = r / kBand;
-                uint32_t wa = bits.get(ka, c), wr = bits.get(kr, c);
+                uint32_t wa = bits.get(ka, mc), wr = bits.get(kr, mc);
                 auto right_of = [&](int v) -> int {   // next survivor after v (in word kr)
                     uint32_t m = wr & ~bits_upto(v % kBand);
                     while (!m) {
                         if (++kr > klast) return -1;
-                        m = wr = bits.get(kr, c);
+                        m = wr = bits.get(kr, mc);
                     }
                     return kr * kBand + lo_bit(m);
                 };
@@ -216,11 +222,11 @@ __device__ void envelope(int n, uint32_t* fs, Out& out, Bits bits)
                     uint32_t m = wa & ((1u << (v % kBand)) - 1u);
                     while (!m) {
                         if (--ka < kfirst) return -1;
-                        m = wa = bits.get(ka, c);
+                        m = wa = bits.get(ka, mc);
                     }
                     return ka * kBand + hi_bit(m);
                 };
-                auto F = [&](int v) { return v >= 0 ? fval(fs, c, v) + v * v : 0; };
+                auto F = [&](int v) { return v >= 0 ? fval(fs, mc, v) + v * v : 0; };
                 int Fa = F(a), Fr = F(r);
                 int rn = right_of(r), ap = left_of(a);
                 int Frn = F(rn), Fap = F(ap);
@@ -241,7 +247,7 @@ __device__ void envelope(int n, uint32_t* fs, Out& out, Bits bits)
                     }
                     break;
                 }
-                bits.clear_between(c, a, r);
+                bits.clear_between(mc, a, r);
             }
         }
         __syncthreads();
