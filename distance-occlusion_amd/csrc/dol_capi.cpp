@@ -30,6 +30,7 @@ void launch_border_2d(int w, int h, const uint32_t* mask, int16_t* delta, hipStr
 void launch_border_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta, hipStream_t s);
 void launch_jfa2d_pass(int step, int w, int h, const int16_t* in, int16_t* out, int cw,
                        hipStream_t s);
+void jfa3d_run(int w, int h, int d, int16_t* delta, int cw, int16_t* tmp, hipStream_t s);
 void launch_jfa3d_pass(int step, int w, int h, int d, const int16_t* in, int16_t* out, int cw,
                        hipStream_t s);
 size_t felz_workspace(int w, int h);
@@ -299,19 +300,6 @@ static void jfa2d_schedule(int w, int h, int16_t* delta, int cw, int16_t* tmp, h
     if (a != delta) launch_jfa2d_pass(1, w, h, a, delta, cw, s);
 }
 
-static void jfa3d_schedule(int w, int h, int d, int16_t* delta, int cw, int16_t* tmp, hipStream_t s)
-{
-    int16_t* a = delta;
-    int16_t* b = tmp;
-    int m = w >= h ? w : h;
-    m = m >= d ? m : d;
-    for (int step = floor_pow2(m - 1); step > 0; step /= 2) {
-        launch_jfa3d_pass(step, w, h, d, a, b, cw, s);
-        int16_t* t = a; a = b; b = t;
-    }
-    if (a != delta) launch_jfa3d_pass(1, w, h, d, a, delta, cw, s);
-}
-
 template <class T>
 int init_dist_host(int w, int h, const uint32_t* mask, T* dist, T max_d)
 {
@@ -571,7 +559,7 @@ int dolhip_jump_flood_3d_dev(int w, int h, int d, int16_t* delta, int cw, void* 
     TRY(require_device());
     void* tmp = nullptr;
     TRY(resolve_work(work, dolhip_jump_flood_3d_workspace(w, h, d), &tmp));
-    jfa3d_schedule(w, h, d, delta, cw, (int16_t*)tmp, as_stream(stream));
+    jfa3d_run(w, h, d, delta, cw, (int16_t*)tmp, as_stream(stream));
     return finish(as_stream(stream), "jump_flood_3d");
 }
 
@@ -585,7 +573,7 @@ int dolhip_jump_flood_3d(int w, int h, int d, int16_t* delta, int cw)
     DevBuf a, t;
     TRY(a.upload(delta, bytes));
     TRY(t.alloc(bytes));
-    jfa3d_schedule(w, h, d, a.as<int16_t>(), cw, t.as<int16_t>(), 0);
+    jfa3d_run(w, h, d, a.as<int16_t>(), cw, t.as<int16_t>(), 0);
     TRY(finish_sync("jump_flood_3d"));
     return a.download(delta, bytes);
 }
@@ -921,7 +909,7 @@ int dolhip_signed_border_3d_dev(int w, int h, int d, const uint32_t* mask, int16
     void* tmp = nullptr;
     TRY(resolve_work(work, dolhip_jump_flood_3d_workspace(w, h, d), &tmp));
     launch_border_3d(w, h, d, mask, delta, as_stream(stream));
-    jfa3d_schedule(w, h, d, delta, 2, (int16_t*)tmp, as_stream(stream));
+    jfa3d_run(w, h, d, delta, 2, (int16_t*)tmp, as_stream(stream));
     return finish(as_stream(stream), "signed_border_3d");
 }
 

@@ -72,13 +72,196 @@ __global__ void __launch_bounds__(256) k_jfa3d(int step, int w, int h, int d, in
     o[0] = (int16_t)bx; o[1] = (int16_t)by; o[2] = (int16_t)bz;
 }
 
+// Row-staged form, for widths w % 8 == 0 up to kRowMax: one workgroup per
+// output row (y, z) copies the (up to) 9 input rows (y + jc*step,
+// z + kc*step) whole into LDS with 16-byte loads, then each thread reads its
+// taps from LDS.  Same taps, order, skips and arithmetic as k_jfa3d.
+//
+// Between passes the field is kept in the same 6 bytes a cell, split into
+// two planes: packed (x, y) int16 pairs for all cells, then z for all cells.
+// A tap then costs one 32-bit and one 16-bit LDS read, x and y are offset
+// with one packed 16-bit add (wrapping as the reference's int16 store does),
+// and the best (x, y) moves as one register.  The first pass reads the
+// caller's interleaved cCellDelta3 layout and the last writes it back.
+constexpr int kRowMax = 1024;
+
+typedef unsigned short ushort2v __attribute__((ext_vector_type(2)));
+typedef short short2v __attribute__((ext_vector_type(2)));
+
+// squared length of (x, y) packed as int16 pairs plus z*z, wrapping in 32
+// bits as len2 does
+__device__ __forceinline__ int len2_packed(uint32_t xy, int z)
+{
+    const short2v v = __builtin_bit_cast(short2v, xy);
+    return __builtin_amdgcn_sdot2(v, v, z * z, false);
+}
+
+// W > 0: the row width as a constant (LDS offsets become immediates)
+template <bool IN_AOS, bool OUT_AOS, int W>
+__global__ void __launch_bounds__(kRowMax) k_jfa3d_rows(int step, int w_arg, int h, int d,
+                                                        size_t cells,
+                                                        const int16_t* __restrict__ in,
+                                                        int16_t* __restrict__ out, int cw)
+{
+    const int w = W > 0 ? W : w_arg;
+    extern __shared__ __align__(16) uint32_t lds[];
+    // IN_AOS: [9][3w] int16; else [9][w] packed (x, y), then [9][w] int16 z
+    const int16_t* lds16 = (const int16_t*)lds;
+    const int row = blockIdx.x;   // z*h + y
+    const int z = row / h, y = row - z * h;
+    auto in_row = [&](int r, int& src_row) {
+        const int zz = z + (r / 3 - 1) * step, yy = y + (r % 3 - 1) * step;
+        src_row = zz * h + yy;
+        return zz >= 0 && zz < d && yy >= 0 && yy < h;
+    };
+    if (IN_AOS) {
+        const int chunks = (3 * w) / 8;   // 16-byte chunks in a row
+        for (int e = threadIdx.x; e < 9 * chunks; e += blockDim.x) {
+            const int r = e / chunks, k = e - r * chunks;
+            int sr;
+            if (!in_row(r, sr)) continue;
+            ((uint4*)(lds + r * (3 * w / 2)))[k] = ((const uint4*)(in + (size_t)3 * w * sr))[k];
+        }
+    } else {
+        const int cxy = w / 4, cz = w / 8;   // 16-byte chunks in an xy row and a z row
+        const uint32_t* pxy = (const uint32_t*)in;
+        const int16_t* pz = in + 2 * cells;
+        for (int e = threadIdx.x; e < 9 * (cxy + cz); e += blockDim.x) {
+            const int r = e / (cxy + cz), k = e - r * (cxy + cz);
+            int sr;
+            if (!in_row(r, sr)) continue;
+            if (k < cxy)
+                ((uint4*)(lds + r * w))[k] = ((const uint4*)(pxy + (size_t)w * sr))[k];
+            else
+                ((uint4*)(lds + 9 * w + r * (w / 2)))[k - cxy] = ((const uint4*)(pz + (size_t)w * sr))[k - cxy];
+        }
+    }
+    __syncthreads();
+    const int x = threadIdx.x;
+    if (x >= w) return;
+    // tap ic of staged row r; the per-ic bases are formed once so that row
+    // offsets fold into the LDS instructions' immediates when W is known
+    const int xs[3] = {x - step, x, x + step};
+    auto tap = [&](int r, int ic, uint32_t& xy, int& tz) {
+        if (IN_AOS) {
+            const int16_t* c = lds16 + 3 * xs[ic + 1] + r * 3 * w;
+            xy = (uint32_t)(uint16_t)c[0] | ((uint32_t)(uint16_t)c[1] << 16);
+            tz = c[2];
+        } else {
+            xy = (lds + xs[ic + 1])[r * w];
+            tz = (lds16 + 2 * 9 * w + xs[ic + 1])[r * w];
+        }
+    };
+    uint32_t bxy;
+    int bz;
+    tap(4, 0, bxy, bz);
+    int bl = len2_packed(bxy, bz);
+    const int off = cw * step;
+    const bool xl = x - step >= 0, xr = x + step < w;
+#pragma unroll
+    for (int kc = -1; kc <= 1; kc++) {
+        const int zz = z + kc * step;
+        if (zz < 0 || zz >= d) continue;
+#pragma unroll
+        for (int jc = -1; jc <= 1; jc++) {
+            const int yy = y + jc * step;
+            if (yy < 0 || yy >= h) continue;
+            const int r = (kc + 1) * 3 + (jc + 1);
+#pragma unroll
+            for (int ic = -1; ic <= 1; ic++) {
+                if ((ic < 0 && !xl) || (ic > 0 && !xr)) continue;
+                uint32_t sxy;
+                int sz;
+                tap(r, ic, sxy, sz);
+                ushort2v v = __builtin_bit_cast(ushort2v, sxy);
+                v += ushort2v{(unsigned short)(ic * off), (unsigned short)(jc * off)};
+                const int cz = wrap16(sz + kc * off);
+                const int cl = len2_packed(__builtin_bit_cast(uint32_t, v), cz);
+                if ((int16_t)sxy != kMaxDelta3 && bl > cl) {
+                    bxy = __builtin_bit_cast(uint32_t, v);
+                    bz = cz;
+                    bl = cl;
+                }
+            }
+        }
+    }
+    const size_t i = (size_t)row * w + x;
+    if (OUT_AOS) {
+        int16_t* o = out + 3 * i;
+        o[0] = (int16_t)bxy; o[1] = (int16_t)(bxy >> 16); o[2] = (int16_t)bz;
+    } else {
+        ((uint32_t*)out)[i] = bxy;
+        out[2 * cells + i] = (int16_t)bz;
+    }
+}
+
 void launch_jfa3d_pass(int step, int w, int h, int d, const int16_t* in, int16_t* out, int cw,
                        hipStream_t s)
 {
-    int xb = (w + 255) / 256;
     ProfileScope p("jfa3d_pass", s);
+    int xb = (w + 255) / 256;
     hipLaunchKernelGGL(k_jfa3d, dim3((unsigned)((size_t)xb * h * d)), dim3(256), 0, s, step, w, h,
                        d, xb, in, out, cw);
+}
+
+static void launch_jfa3d_rows(bool in_aos, bool out_aos, int step, int w, int h, int d,
+                              const int16_t* in, int16_t* out, int cw, hipStream_t s)
+{
+    ProfileScope p("jfa3d_pass", s);
+    const int threads = (w + 63) / 64 * 64;
+    const size_t lds = (size_t)9 * w * 6;
+    const size_t cells = (size_t)w * h * d;
+    const dim3 grid((unsigned)((size_t)h * d));
+#define DOL_JFA3D_ROWS(W)                                                                          \
+    if (in_aos && out_aos)                                                                         \
+        hipLaunchKernelGGL((k_jfa3d_rows<true, true, W>), grid, dim3(threads), lds, s, step, w, h, \
+                           d, cells, in, out, cw);                                                 \
+    else if (in_aos)                                                                               \
+        hipLaunchKernelGGL((k_jfa3d_rows<true, false, W>), grid, dim3(threads), lds, s, step, w,   \
+                           h, d, cells, in, out, cw);                                              \
+    else if (out_aos)                                                                              \
+        hipLaunchKernelGGL((k_jfa3d_rows<false, true, W>), grid, dim3(threads), lds, s, step, w,   \
+                           h, d, cells, in, out, cw);                                              \
+    else                                                                                           \
+        hipLaunchKernelGGL((k_jfa3d_rows<false, false, W>), grid, dim3(threads), lds, s, step, w,  \
+                           h, d, cells, in, out, cw);
+    if (w == 512) {
+        DOL_JFA3D_ROWS(512)
+    } else {
+        DOL_JFA3D_ROWS(0)
+    }
+#undef DOL_JFA3D_ROWS
+}
+
+// FloorPow2 of DistanceOcclusionLib.cpp:875-884
+static int floor_pow2_host(int v)
+{
+    uint32_t u = (uint32_t)v;
+    u |= u >> 1; u |= u >> 2; u |= u >> 4; u |= u >> 8; u |= u >> 16;
+    return (int)((int32_t)(u + 1u) >> 1);
+}
+
+// The pass schedule of DistanceOcclusionLib.cpp:2450-2472: steps
+// FloorPow2(max dim - 1) down to 1, ping-ponging between delta and tmp, plus
+// one step-1 pass when the count is odd so the result ends in delta.
+void jfa3d_run(int w, int h, int d, int16_t* delta, int cw, int16_t* tmp, hipStream_t s)
+{
+    int m = w >= h ? w : h;
+    m = m >= d ? m : d;
+    int steps[40], n = 0;
+    for (int step = floor_pow2_host(m - 1); step > 0; step /= 2) steps[n++] = step;
+    if (n & 1) steps[n++] = 1;
+    const bool rows = w % 8 == 0 && w <= kRowMax && ((uintptr_t)delta & 15) == 0 &&
+                      ((uintptr_t)tmp & 15) == 0;
+    int16_t* a = delta;
+    int16_t* b = tmp;
+    for (int k = 0; k < n; k++) {
+        if (rows)
+            launch_jfa3d_rows(k == 0, k == n - 1, steps[k], w, h, d, a, b, cw, s);
+        else
+            launch_jfa3d_pass(steps[k], w, h, d, a, b, cw, s);
+        int16_t* t = a; a = b; b = t;
+    }
 }
 
 }  // namespace dolhip

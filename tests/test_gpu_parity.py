@@ -119,6 +119,18 @@ def test_jfa_3d(gpu, oracle, rng, dims, cw):
         np.testing.assert_array_equal(gpu.jump_flood_3d(w, h, d, start, cw), oracle.jfa_3d(w, h, d, start, cw))
 
 
+@pytest.mark.parametrize("dims", [(8, 8, 8), (64, 20, 12), (128, 9, 40), (512, 16, 16), (1024, 4, 3),
+                                  (1032, 2, 3)])
+@pytest.mark.parametrize("cw", [1, 2])
+def test_jfa_3d_staged_rows(gpu, oracle, rng, dims, cw):
+    """Widths that are multiples of 8 (rows staged whole in LDS up to 1024
+    cells; 1032 takes the per-cell kernel)."""
+    w, h, d = dims
+    for m in masks_for(rng, dims):
+        start = oracle.border_3d(w, h, d, m) if cw == 2 else oracle.init_deltas_3d(w, h, d, m)
+        np.testing.assert_array_equal(gpu.jump_flood_3d(w, h, d, start, cw), oracle.jfa_3d(w, h, d, start, cw))
+
+
 def test_jfa_2d_config1(gpu, oracle, reference):
     """The tool's -p 100 case: 512x512, 100 points from seed 12345."""
     w = h = 512
