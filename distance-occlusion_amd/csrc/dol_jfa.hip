@@ -139,25 +139,48 @@ __global__ void __launch_bounds__(kRowMax) k_jfa3d_rows(int step, int w_arg, int
     __syncthreads();
     const int x = threadIdx.x;
     if (x >= w) return;
-    // tap ic of staged row r; the per-ic bases are formed once so that row
-    // offsets fold into the LDS instructions' immediates when W is known
-    const int xs[3] = {x - step, x, x + step};
+    // tap ic of staged row r.  The x of a tap is clamped into the row so
+    // every read is in range (an out-of-row tap is masked off instead of
+    // branched around), and the per-ic bases are formed once so that row
+    // offsets fold into the LDS instructions' immediates when W is known.
+    const int xs[3] = {max(x - step, 0), x, min(x + step, w - 1)};
+    const bool okx[3] = {x - step >= 0, true, x + step < w};
+    // byte offsets of the three tap columns, held in registers (the empty
+    // asm keeps the compiler from re-forming them in every row block)
+    uint32_t oxy[3], oz[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        oxy[k] = 4u * (uint32_t)xs[k];
+        oz[k] = 2u * (uint32_t)xs[k];
+        asm volatile("" : "+v"(oxy[k]), "+v"(oz[k]));
+    }
     auto tap = [&](int r, int ic, uint32_t& xy, int& tz) {
         if (IN_AOS) {
             const int16_t* c = lds16 + 3 * xs[ic + 1] + r * 3 * w;
             xy = (uint32_t)(uint16_t)c[0] | ((uint32_t)(uint16_t)c[1] << 16);
             tz = c[2];
         } else {
-            xy = (lds + xs[ic + 1])[r * w];
-            tz = (lds16 + 2 * 9 * w + xs[ic + 1])[r * w];
+            xy = ((const uint32_t*)((const char*)lds + oxy[ic + 1]))[r * w];
+            tz = ((const int16_t*)((const char*)(lds16 + 2 * 9 * w) + oz[ic + 1]))[r * w];
         }
     };
-    uint32_t bxy;
-    int bz;
-    tap(4, 0, bxy, bz);
-    int bl = len2_packed(bxy, bz);
     const int off = cw * step;
-    const bool xl = x - step >= 0, xr = x + step < w;
+    // candidate of tap t = 9 (kc+1) + 3 (jc+1) + (ic+1): the source offset by
+    // (ic, jc, kc) * off, each component wrapped to int16
+    auto candidate = [&](int t, uint32_t sxy, int sz, uint32_t& cxy, int& cz) {
+        const int ic = t % 3 - 1, jc = (t / 3) % 3 - 1, kc = t / 9 - 1;
+        ushort2v v = __builtin_bit_cast(ushort2v, sxy);
+        v += ushort2v{(unsigned short)(ic * off), (unsigned short)(jc * off)};
+        cxy = __builtin_bit_cast(uint32_t, v);
+        cz = wrap16(sz + kc * off);
+    };
+    // the loop keeps only the best length and its tap; the winner's value is
+    // formed again at the end
+    uint32_t sxy;
+    int sz;
+    tap(4, 0, sxy, sz);
+    int bl = len2_packed(sxy, sz);
+    int bt = 13;
 #pragma unroll
     for (int kc = -1; kc <= 1; kc++) {
         const int zz = z + kc * step;
@@ -169,21 +192,32 @@ __global__ void __launch_bounds__(kRowMax) k_jfa3d_rows(int step, int w_arg, int
             const int r = (kc + 1) * 3 + (jc + 1);
 #pragma unroll
             for (int ic = -1; ic <= 1; ic++) {
-                if ((ic < 0 && !xl) || (ic > 0 && !xr)) continue;
-                uint32_t sxy;
-                int sz;
+                const int t = 3 * r + ic + 1;
                 tap(r, ic, sxy, sz);
-                ushort2v v = __builtin_bit_cast(ushort2v, sxy);
-                v += ushort2v{(unsigned short)(ic * off), (unsigned short)(jc * off)};
-                const int cz = wrap16(sz + kc * off);
-                const int cl = len2_packed(__builtin_bit_cast(uint32_t, v), cz);
-                if ((int16_t)sxy != kMaxDelta3 && bl > cl) {
-                    bxy = __builtin_bit_cast(uint32_t, v);
-                    bz = cz;
-                    bl = cl;
-                }
+                uint32_t cxy;
+                int cz;
+                candidate(t, sxy, sz, cxy, cz);
+                const int cl = len2_packed(cxy, cz);
+                const bool take = okx[ic + 1] && (int16_t)sxy != kMaxDelta3 && bl > cl;
+                bl = take ? cl : bl;
+                bt = take ? t : bt;
             }
         }
+    }
+    uint32_t bxy;
+    int bz;
+    {
+        const int r = bt / 3, ic = bt % 3 - 1;
+        const int xx = xs[ic + 1];
+        if (IN_AOS) {
+            const int16_t* c = lds16 + 3 * xx + r * 3 * w;
+            sxy = (uint32_t)(uint16_t)c[0] | ((uint32_t)(uint16_t)c[1] << 16);
+            sz = c[2];
+        } else {
+            sxy = lds[r * w + xx];
+            sz = lds16[2 * 9 * w + r * w + xx];
+        }
+        candidate(bt, sxy, sz, bxy, bz);
     }
     const size_t i = (size_t)row * w + x;
     if (OUT_AOS) {
