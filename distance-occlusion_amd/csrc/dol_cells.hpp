@@ -100,11 +100,27 @@ __device__ __forceinline__ uint32_t border2_cell(const uint32_t* mask, int ws, i
 
 // ------------------------------------------------------------ JumpFlood 2D
 
-__device__ __forceinline__ void jfa2_tap(int& bx, int& by, int& bl, uint32_t v, int ox, int oy)
+// The cell is kept packed: x and y as the int16 halves of one 32-bit word.
+// A candidate is the source plus (ox, oy) with one packed 16-bit add, which
+// wraps as the reference's int16 store does, and its squared length is one
+// dot product of the int16 pair with itself, summed in 32 bits with wrap as
+// len2 does.
+typedef unsigned short jfa_u16x2 __attribute__((ext_vector_type(2)));
+typedef short jfa_i16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ int len2_packed(uint32_t xy)
 {
-    const int cx = wrap16(unpack2x(v) + ox), cy = wrap16(unpack2y(v) + oy);
-    const int cl = len2(cx, cy);
-    if (bl > cl) { bx = cx; by = cy; bl = cl; }
+    const jfa_i16x2 v = __builtin_bit_cast(jfa_i16x2, xy);
+    return __builtin_amdgcn_sdot2(v, v, 0, false);
+}
+
+__device__ __forceinline__ void jfa2_tap(uint32_t& best, int& bl, uint32_t v, int ox, int oy)
+{
+    jfa_u16x2 c = __builtin_bit_cast(jfa_u16x2, v);
+    c += jfa_u16x2{(unsigned short)ox, (unsigned short)oy};
+    const uint32_t cv = __builtin_bit_cast(uint32_t, c);
+    const int cl = len2_packed(cv);
+    if (bl > cl) { best = cv; bl = cl; }
 }
 
 // One pass for cell (x, y) (DistanceOcclusionLib.cpp:782-873).  Tap order:
@@ -117,22 +133,21 @@ __device__ __forceinline__ uint32_t jfa2_cell(const uint32_t* in, int w, int h, 
     const int off = cw * step;
     const size_t row = (size_t)y * w;
     const bool up = y - step >= 0, down = y + step < h;
-    const uint32_t self = in[row + x];
-    int bx = unpack2x(self), by = unpack2y(self);
-    int bl = len2(bx, by);
+    uint32_t best = in[row + x];
+    int bl = len2_packed(best);
     if (x - step >= 0) {
-        jfa2_tap(bx, by, bl, in[row + x - step], -off, 0);
-        if (up) jfa2_tap(bx, by, bl, in[row - (size_t)step * w + x - step], -off, -off);
-        if (down) jfa2_tap(bx, by, bl, in[row + (size_t)step * w + x - step], -off, +off);
+        jfa2_tap(best, bl, in[row + x - step], -off, 0);
+        if (up) jfa2_tap(best, bl, in[row - (size_t)step * w + x - step], -off, -off);
+        if (down) jfa2_tap(best, bl, in[row + (size_t)step * w + x - step], -off, +off);
     }
     if (x + step < w) {
-        jfa2_tap(bx, by, bl, in[row + x + step], +off, 0);
-        if (up) jfa2_tap(bx, by, bl, in[row - (size_t)step * w + x + step], +off, -off);
-        if (down) jfa2_tap(bx, by, bl, in[row + (size_t)step * w + x + step], +off, +off);
+        jfa2_tap(best, bl, in[row + x + step], +off, 0);
+        if (up) jfa2_tap(best, bl, in[row - (size_t)step * w + x + step], +off, -off);
+        if (down) jfa2_tap(best, bl, in[row + (size_t)step * w + x + step], +off, +off);
     }
-    if (up) jfa2_tap(bx, by, bl, in[row - (size_t)step * w + x], 0, -off);
-    if (down) jfa2_tap(bx, by, bl, in[row + (size_t)step * w + x], 0, +off);
-    return pack2(bx, by);
+    if (up) jfa2_tap(best, bl, in[row - (size_t)step * w + x], 0, -off);
+    if (down) jfa2_tap(best, bl, in[row + (size_t)step * w + x], 0, +off);
+    return best;
 }
 
 // FloorPow2 of DistanceOcclusionLib.cpp:875-884
