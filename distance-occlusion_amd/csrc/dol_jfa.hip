@@ -24,11 +24,77 @@ __global__ void __launch_bounds__(256) k_jfa2d(int step, int w, int h, int xb,
     out[(size_t)y * w + x] = jfa2_cell(in, w, h, x, y, step, cw);
 }
 
+// Four cells a thread, for w % 4 == 0 and 16-byte aligned fields: every tap
+// of the four cells comes from one 16-byte load (the cells x..x+3 of a tap
+// row), so each lane has 144 bytes in flight.  SM is the step when it is 1
+// or 2 (the shifted taps then come from the windows at x - 4, x, x + 4) and
+// 4 for every step >= 4 (x +- step stays 4-aligned).  Taps, order and
+// arithmetic as jfa2_cell.
+template <int SM>
+__global__ void __launch_bounds__(256) k_jfa2d_x4(int step, int w, int h, int xb,
+                                                  const uint32_t* __restrict__ in,
+                                                  uint32_t* __restrict__ out, int cw)
+{
+    const int y = blockIdx.x / xb;
+    const int x = ((blockIdx.x - y * xb) * 256 + threadIdx.x) * 4;
+    if (x >= w) return;
+    const int off = cw * step;
+    const bool up = y - step >= 0, down = y + step < h;
+    const bool rok[3] = {up, true, down};
+    const int lsh = SM < 4 ? 4 : step;   // distance of the left and right windows
+    uint4 L[3], C[3], R[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        L[r] = C[r] = R[r] = make_uint4(0, 0, 0, 0);
+        if (!rok[r]) continue;
+        const uint32_t* row = in + (size_t)(y + (r - 1) * step) * w;
+        C[r] = *(const uint4*)(row + x);
+        if (x - lsh >= 0) L[r] = *(const uint4*)(row + x - lsh);
+        if (x + lsh < w) R[r] = *(const uint4*)(row + x + lsh);
+    }
+    auto el = [](const uint4& v, int i) { return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w; };
+    uint32_t res[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        auto left = [&](int r) { return SM < 4 ? (i >= SM ? el(C[r], i - SM) : el(L[r], 4 + i - SM)) : el(L[r], i); };
+        auto right = [&](int r) { return SM < 4 ? (i + SM < 4 ? el(C[r], i + SM) : el(R[r], i + SM - 4)) : el(R[r], i); };
+        uint32_t best = el(C[1], i);
+        int bl = len2_packed(best);
+        if (x + i - step >= 0) {
+            jfa2_tap(best, bl, left(1), -off, 0);
+            if (up) jfa2_tap(best, bl, left(0), -off, -off);
+            if (down) jfa2_tap(best, bl, left(2), -off, +off);
+        }
+        if (x + i + step < w) {
+            jfa2_tap(best, bl, right(1), +off, 0);
+            if (up) jfa2_tap(best, bl, right(0), +off, -off);
+            if (down) jfa2_tap(best, bl, right(2), +off, +off);
+        }
+        if (up) jfa2_tap(best, bl, el(C[0], i), 0, -off);
+        if (down) jfa2_tap(best, bl, el(C[2], i), 0, +off);
+        res[i] = best;
+    }
+    *(uint4*)(out + (size_t)y * w + x) = make_uint4(res[0], res[1], res[2], res[3]);
+}
+
 void launch_jfa2d_pass(int step, int w, int h, const int16_t* in, int16_t* out, int cw,
                        hipStream_t s)
 {
-    int xb = (w + 255) / 256;
     ProfileScope p("jfa2d_pass", s);
+    if (w % 4 == 0 && ((uintptr_t)in & 15) == 0 && ((uintptr_t)out & 15) == 0) {
+        const int xb = (w / 4 + 255) / 256;
+        const dim3 grid((unsigned)(xb * h));
+        const uint32_t* i32 = (const uint32_t*)in;
+        uint32_t* o32 = (uint32_t*)out;
+        if (step == 1)
+            hipLaunchKernelGGL(k_jfa2d_x4<1>, grid, dim3(256), 0, s, step, w, h, xb, i32, o32, cw);
+        else if (step == 2)
+            hipLaunchKernelGGL(k_jfa2d_x4<2>, grid, dim3(256), 0, s, step, w, h, xb, i32, o32, cw);
+        else
+            hipLaunchKernelGGL(k_jfa2d_x4<4>, grid, dim3(256), 0, s, step, w, h, xb, i32, o32, cw);
+        return;
+    }
+    int xb = (w + 255) / 256;
     hipLaunchKernelGGL(k_jfa2d, dim3((unsigned)(xb * h)), dim3(256), 0, s, step, w, h, xb,
                        (const uint32_t*)in, (uint32_t*)out, cw);
 }
