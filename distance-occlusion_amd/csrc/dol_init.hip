@@ -122,22 +122,72 @@ template void launch_init_dist<int16_t>(int, int, const uint32_t*, int16_t*, int
 
 // ------------------------------------------------------------ border init 2D
 
-// one thread per cell; see border2_cell (dol_cells.hpp)
-__global__ void __launch_bounds__(256) k_border_2d(int w, int h, int xb, const uint32_t* __restrict__ mask,
-                                                   uint32_t* __restrict__ delta)
+// Pixel x-1, x and x+1 of the 32 cells of mask word j of a row, as three
+// words (bit i = cell 32j + i): x = -1 replicates x = 0, and the right
+// neighbour of the row's last cell is made equal to the cell so that it
+// never counts.
+struct RowBits {
+    uint32_t l, m, r;
+};
+
+__device__ __forceinline__ RowBits row_bits(const uint32_t* row, int j, int ws, int w)
 {
-    const int y = blockIdx.x / xb;
-    const int x = (blockIdx.x - y * xb) * 256 + threadIdx.x;
-    if (x >= w) return;
-    delta[(size_t)y * w + x] = border2_cell(mask, mask_stride(w), w, h, x, y);
+    RowBits b;
+    b.m = row[j];
+    const uint32_t prev = j > 0 ? row[j - 1] : 0u, next = j + 1 < ws ? row[j + 1] : 0u;
+    b.l = (b.m << 1) | (j > 0 ? prev >> 31 : (b.m & 1u));
+    b.r = (b.m >> 1) | (next << 31);
+    if (j == ws - 1) {
+        const uint32_t last = 1u << ((w - 1) & 31);
+        b.r = (b.r & ~last) | (b.m & last);
+    }
+    return b;
+}
+
+// One thread per 32 cells (one mask word of a row).  A cell whose 3x3
+// neighbourhood inside the image is uniform gets no write; such cells are
+// found for the whole word with bit operations, their value is stored in
+// bulk, and only the others run the per-cell replay.
+__global__ void __launch_bounds__(256) k_border_2d_words(int w, int h,
+                                                         const uint32_t* __restrict__ mask,
+                                                         uint32_t* __restrict__ delta)
+{
+    const int ws = mask_stride(w);
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    if (gid >= ws * h) return;
+    const int y = gid / ws, j = gid - y * ws;
+    const int x0 = 32 * j, n = min(32, w - x0);
+    const uint32_t centre = mask[(size_t)y * ws + j];
+    uint32_t diff = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int yy = y - 1 + k;
+        if (yy < 0 || yy >= h) continue;
+        const RowBits b = row_bits(mask + (size_t)yy * ws, j, ws, w);
+        diff |= (centre ^ b.l) | (centre ^ b.m) | (centre ^ b.r);
+    }
+    if (n < 32) diff &= (1u << n) - 1u;
+    uint32_t* out = delta + (size_t)y * w + x0;
+    const uint32_t fill = pack2(kMaxDelta2, kMaxDelta2);
+    if (n == 32 && ((uintptr_t)out & 15) == 0) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) ((uint4*)out)[k] = make_uint4(fill, fill, fill, fill);
+    } else {
+        for (int k = 0; k < n; k++) out[k] = fill;
+    }
+    while (diff) {
+        const int i = __ffs(diff) - 1;
+        diff &= diff - 1u;
+        out[i] = border2_cell(mask, ws, w, h, x0 + i, y);
+    }
 }
 
 void launch_border_2d(int w, int h, const uint32_t* mask, int16_t* delta, hipStream_t s)
 {
-    int xb = (w + 255) / 256;
     ProfileScope p("border_2d", s);
-    hipLaunchKernelGGL(k_border_2d, dim3((unsigned)(xb * h)), dim3(256), 0, s, w, h, xb, mask,
-                       (uint32_t*)delta);
+    const int words = mask_stride(w) * h;
+    hipLaunchKernelGGL(k_border_2d_words, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, w,
+                       h, mask, (uint32_t*)delta);
 }
 
 // ------------------------------------------------------------ border init 3D
@@ -264,36 +314,12 @@ __device__ __forceinline__ void row_step3(Cell3& s, uint32_t cube, int kinds, bo
     }
 }
 
-__global__ void __launch_bounds__(256) k_border_3d(int w, int h, int d, int xb,
-                                                   const uint32_t* __restrict__ mask,
-                                                   int16_t* __restrict__ delta)
+// The cell's value given its pixel cube and the cube bits inside the volume:
+// uniform neighbourhoods get no write; otherwise the reference's writes that
+// land on the cell are replayed in order (DistanceOcclusionLib.cpp:1899-1973).
+__device__ __forceinline__ Cell3 border3_from_cube(uint32_t cube, uint32_t inside, bool right,
+                                                   bool word_end, int y, int z, int h, int d)
 {
-    const int row = blockIdx.x / xb;           // z*h + y
-    const int x = (blockIdx.x - row * xb) * 256 + threadIdx.x;
-    if (x >= w) return;
-    const int z = row / h, y = row - z * h;
-    const int ws = mask_stride(w);
-    const bool right = x + 1 < w;
-    const bool word_end = (x & 31) == 31;
-
-    // gather the 3x3x3 pixel cube and the bits that lie inside the volume
-    uint32_t cube = 0, inside = 0;
-    for (int dz = 0; dz < 3; dz++) {
-        const int zz = z - 1 + dz;
-        if (zz < 0 || zz >= d) continue;
-        for (int dy = 0; dy < 3; dy++) {
-            const int yy = y - 1 + dy;
-            if (yy < 0 || yy >= h) continue;
-            const uint32_t* r = mask + ((size_t)zz * h + yy) * ws;
-            const int xl = x > 0 ? x - 1 : 0;
-            const int b0 = (int)((r[xl >> 5] >> (xl & 31)) & 1u);
-            const int b1 = (int)((r[x >> 5] >> (x & 31)) & 1u);
-            const int b2 = right ? (int)((r[(x + 1) >> 5] >> ((x + 1) & 31)) & 1u) : 0;
-            const int base = dz * 9 + dy * 3;
-            cube |= (uint32_t)(b0 | (b1 << 1) | (b2 << 2)) << base;
-            inside |= (right ? 7u : 3u) << base;
-        }
-    }
     Cell3 s{kMaxDelta3, kMaxDelta3, kMaxDelta3};
     const uint32_t in_set = cube & inside;
     if (in_set != 0 && in_set != inside) {
@@ -320,16 +346,73 @@ __global__ void __launch_bounds__(256) k_border_3d(int w, int h, int d, int xb,
             if (y == h - 1) row_step3(s, cube, L_EX, right, word_end, 1, 1, 0, 0);
         }
     }
-    int16_t* c = delta + 3 * ((size_t)row * w + x);
-    c[0] = (int16_t)s.x; c[1] = (int16_t)s.y; c[2] = (int16_t)s.z;
+    return s;
+}
+
+// One thread per 32 cells (one mask word of a row).  A cell whose
+// neighbourhood inside the volume is uniform gets no write, and such cells are
+// found for the whole word with bit operations; their value is stored in bulk
+// and only the others are replayed cell by cell.
+__global__ void __launch_bounds__(256) k_border_3d_words(int w, int h, int d,
+                                                         const uint32_t* __restrict__ mask,
+                                                         int16_t* __restrict__ delta)
+{
+    const int ws = mask_stride(w);
+    const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= (size_t)ws * h * d) return;
+    const int row = (int)(gid / ws), j = (int)(gid - (size_t)row * ws);
+    const int z = row / h, y = row - z * h;
+    const int x0 = 32 * j, n = min(32, w - x0);
+    RowBits rb[9];
+    bool rin[9];
+    uint32_t diff = 0;
+    const uint32_t centre = mask[(size_t)row * ws + j];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const int zz = z - 1 + k / 3, yy = y - 1 + k % 3;
+        rin[k] = zz >= 0 && zz < d && yy >= 0 && yy < h;
+        rb[k] = RowBits{0u, 0u, 0u};
+        if (rin[k]) {
+            rb[k] = row_bits(mask + ((size_t)zz * h + yy) * ws, j, ws, w);
+            diff |= (centre ^ rb[k].l) | (centre ^ rb[k].m) | (centre ^ rb[k].r);
+        }
+    }
+    if (n < 32) diff &= (1u << n) - 1u;
+    int16_t* out = delta + 3 * ((size_t)row * w + x0);
+    // every cell first gets the no-write value (kMaxDelta3 in all three
+    // components, 16-byte stores when aligned), then the others are replaced
+    const uint32_t fill = (uint32_t)(uint16_t)kMaxDelta3 * 0x10001u;
+    if (n == 32 && ((uintptr_t)out & 15) == 0) {
+#pragma unroll
+        for (int k = 0; k < 12; k++) ((uint4*)out)[k] = make_uint4(fill, fill, fill, fill);
+    } else {
+        for (int k = 0; k < 3 * n; k++) out[k] = (int16_t)kMaxDelta3;
+    }
+    while (diff) {
+        const int i = __ffs(diff) - 1;
+        diff &= diff - 1u;
+        const int x = x0 + i;
+        const bool right = x + 1 < w;
+        uint32_t cube = 0, inside = 0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            if (!rin[k]) continue;
+            const uint32_t b = ((rb[k].l >> i) & 1u) | (((rb[k].m >> i) & 1u) << 1) |
+                               (right ? ((rb[k].r >> i) & 1u) << 2 : 0u);
+            cube |= b << (3 * k);
+            inside |= (right ? 7u : 3u) << (3 * k);
+        }
+        const Cell3 s = border3_from_cube(cube, inside, right, (x & 31) == 31, y, z, h, d);
+        out[3 * i] = (int16_t)s.x; out[3 * i + 1] = (int16_t)s.y; out[3 * i + 2] = (int16_t)s.z;
+    }
 }
 
 void launch_border_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta, hipStream_t s)
 {
-    int xb = (w + 255) / 256;
     ProfileScope p("border_3d", s);
-    hipLaunchKernelGGL(k_border_3d, dim3((unsigned)((size_t)xb * h * d)), dim3(256), 0, s, w, h, d,
-                       xb, mask, delta);
+    const size_t words = (size_t)mask_stride(w) * h * d;
+    hipLaunchKernelGGL(k_border_3d_words, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, w,
+                       h, d, mask, delta);
 }
 
 }  // namespace dolhip
