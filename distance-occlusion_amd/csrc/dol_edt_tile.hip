@@ -101,22 +101,21 @@ struct Bits {
     }
 };
 
-// ---- LDS words.  Sample q of column c is word cell(q, c): bits 0..19 hold
-// f (kNoSite: none; every f is below 2^20 on this path), bits 20..29 the mark
-// of phase 3 (index + 1 of the site whose interval starts at q).  Each band's
-// rows are skewed by kCols words more than the previous band's, so the bands
-// one wavefront reads fall in different LDS banks; rows stay 16-byte aligned.
+// ---- LDS words.  Sample q of column c is word cell(q, c) = q * kCols + c
+// (rows at a constant stride, so the scan's reads take immediate offsets):
+// bits 0..19 hold f (kNoSite: none; every f is below 2^20 on this path),
+// bits 20..29 the mark of phase 3 (index + 1 of the site whose interval
+// starts at q).
 
 constexpr uint32_t kValueMask = 0xfffffu;
 constexpr uint32_t kNoSite = 0xfffffu;
-constexpr int kSkew = kCols;
 // band maxima of phase 3b: one 16-byte aligned row per column, padded so that
 // 16 lanes' rows start in different bank quads
 constexpr int kBmStride = kBands + 4;
 constexpr int kBmWords = kCols * kBmStride > kBands * kCols ? kCols * kBmStride : kBands * kCols;
-constexpr int kWords = kMaxN * kCols + kBands * kSkew;
+constexpr int kWords = kMaxN * kCols;
 
-__device__ __forceinline__ int cell(int q, int c) { return q * kCols + (q / kBand) * kSkew + c; }
+__device__ __forceinline__ int cell(int q, int c) { return q * kCols + c; }
 
 __device__ __forceinline__ int fval(const uint32_t* fs, int c, int q)
 {
@@ -358,6 +357,86 @@ struct FinalOut {   // int32 + float at cell index base(q) + x (< 2^31 on this p
     }
 };
 
+// ---- outward scan, tried first on every tile
+//
+// d(p) = min_q f(q) + (p - q)^2 is found by reading the samples p -/+ k for
+// k = 1, 2, ... and stopping once k^2 >= the best value so far (every sample
+// further out costs at least k^2) or every sample of the line has been seen.
+// The steps are about the distance computed, so on volumes with sites near
+// every sample this takes far fewer instructions than the envelope.  A
+// wavefront computes kRowsPerWave consecutive samples of all the tile's lines
+// at once: k is the same for all its active lanes (the squares are scalars)
+// and each read covers 64 consecutive LDS words.  Steps go kStepGroup at a
+// time; while the wavefront's rows p -/+ (k + kStepGroup - 1) stay inside
+// the line the reads take immediate offsets, and near the line's ends the
+// row is clamped (a clamped read repeats an end sample at a larger distance
+// than it was already counted at, so it never lowers the minimum).  If some
+// output would need more than kScanCap steps the tile is left to the
+// envelope (returns false).
+constexpr int kScanCap = 160;
+constexpr int kStepGroup = 4;
+constexpr int kRowsPerWave = 64 / kCols;
+constexpr int kRowsPerPass = kThreads / kCols;
+
+template <class Out>
+__device__ bool scan(int n, const uint32_t* fs, Out& out, int* abort_flag)
+{
+    const int tid = threadIdx.x;
+    const int c = tid % kCols;
+    const int r0 = tid / kCols;
+    const int wr0 = (tid / 64) * kRowsPerWave;   // the wavefront's first row
+    const bool active = out.active(c);
+    for (int base = 0; base < n; base += kRowsPerPass) {
+        if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+        const int pw = base + wr0;
+        if (pw >= n) break;
+        const int p = base + r0;
+        const bool live = active && p < n;
+        const int reach = max(p, n - 1 - p);   // farthest sample of the line
+        int best = live ? (int)(fs[cell(p, c)] & kValueMask) : 0;
+        int k = 1;
+        bool fail = false;
+        for (;;) {
+            if (k * k >= best || k > reach) break;
+            if (k > kScanCap) {
+                fail = true;
+                break;
+            }
+            if (pw - (k + kStepGroup - 1) >= 0 && pw + kRowsPerWave - 1 + k + kStepGroup - 1 < n) {
+                const uint32_t* lo = fs + cell(p - k - (kStepGroup - 1), c);
+                const uint32_t* hi = fs + cell(p + k, c);
+#pragma unroll
+                for (int t = 0; t < kStepGroup; t++) {
+                    const int a = (int)lo[(kStepGroup - 1 - t) * kCols], b = (int)hi[t * kCols];
+                    best = min(best, min(a, b) + (k + t) * (k + t));
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < kStepGroup; t++) {
+                    const int kk = k + t;
+                    const int a = (int)fs[cell(max(p - kk, 0), c)];
+                    const int b = (int)fs[cell(min(p + kk, n - 1), c)];
+                    best = min(best, min(a, b) + kk * kk);
+                }
+            }
+            k += kStepGroup;
+        }
+        if (fail) __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else if (live) out(c, p, best >= (int)kNoSite ? kEdtInf : best);
+    }
+    __syncthreads();
+    return *abort_flag == 0;
+}
+
+// every output of a tile whose lines have no site
+template <class Out>
+__device__ void no_sites(int n, Out& out)
+{
+    const int c = threadIdx.x % kCols;
+    if (!out.active(c)) return;
+    for (int p = threadIdx.x / kCols; p < n; p += kRowsPerPass) out(c, p, kEdtInf);
+}
+
 }  // namespace tile
 
 using namespace tile;
@@ -373,6 +452,8 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
 {
     __shared__ __align__(16) uint32_t fs[kWords];
     __shared__ __align__(16) uint32_t bm[kBmWords];
+    __shared__ int abort_flag;
+    if (threadIdx.x == 0) abort_flag = 0;
     const int ws = mask_stride(w);
     const int xt = (w + kCols - 1) / kCols;
     // swizzle: blocks congruent mod 8 share an XCD; a group of (up to) 8
@@ -395,6 +476,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
     // the loads overlap
     constexpr int kXCols = kCols < 16 ? kCols : 16;
     constexpr int kParts = kCols / kXCols;
+    int has_site = 0;   // a row with a set bit gives every column a site
     for (int e = threadIdx.x; e < kParts * h; e += kThreads) {
         const int y = e / kParts, part = e % kParts;
         const uint32_t* row = mask + ((size_t)z * h + y) * ws;
@@ -424,6 +506,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
 #pragma unroll
         for (int k = 15; k >= 0; k--)
             if (k > j && words[k]) R = 32 * k + lo_bit(words[k]);
+        has_site |= (wj != 0u || L >= 0 || R >= 0);
         uint4* out = (uint4*)(fs + cell(y, kXCols * part));
 #pragma unroll
         for (int i = 0; i < kXCols; i += 4) {
@@ -442,16 +525,18 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
             out[i >> 2] = make_uint4(quad[0], quad[1], quad[2], quad[3]);
         }
     }
-    __syncthreads();
+    const bool any = __syncthreads_or(has_site);
 
     Bits bits{bm};
     if (FINAL) {
         // 2D image (d == 1): final outputs, rows of width w
         FinalOut o{f2, fl, (uint32_t)w, (uint32_t)x0, min(kCols, w - x0)};
-        envelope(h, fs, o, bits);
+        if (!any) no_sites(h, o);
+        else if (!scan(h, fs, o, &abort_flag)) envelope(h, fs, o, bits);
     } else {
         FieldOut o{f2, w, h, x0, z};
-        envelope(h, fs, o, bits);
+        if (!any) no_sites(h, o);
+        else if (!scan(h, fs, o, &abort_flag)) envelope(h, fs, o, bits);
     }
 }
 
@@ -464,6 +549,8 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
 {
     __shared__ __align__(16) uint32_t fs[kWords];
     __shared__ __align__(16) uint32_t bm[kBmWords];
+    __shared__ int abort_flag;
+    if (threadIdx.x == 0) abort_flag = 0;
     const int xt = (w + kCols - 1) / kCols;
     // swizzle: blocks congruent mod 8 share an XCD; give each XCD a run of
     // consecutive tiles, so the partial rows that neighbouring tiles write
@@ -473,6 +560,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
     const int y = bid / xt;
     const int x0 = (bid - y * xt) * kCols;
     const int valid = min(kCols, w - x0);
+    int has_site = 0;
     if (valid == kCols && d == kMaxN && (w & 3) == 0) {
         // full tile: 16-byte loads, all in flight before the first LDS store
         constexpr int kIter = kMaxN * (kCols / 4) / kThreads;
@@ -485,19 +573,23 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
 #pragma unroll
         for (int k = 0; k < kIter; k++) {
             const int e = threadIdx.x + k * kThreads;
-            *(uint4*)(fs + cell(e / (kCols / 4), (e % (kCols / 4)) * 4)) =
-                make_uint4(packed(v[k].x), packed(v[k].y), packed(v[k].z), packed(v[k].w));
+            const uint4 q = make_uint4(packed(v[k].x), packed(v[k].y), packed(v[k].z), packed(v[k].w));
+            has_site |= (q.x & q.y & q.z & q.w) != kNoSite;
+            *(uint4*)(fs + cell(e / (kCols / 4), (e % (kCols / 4)) * 4)) = q;
         }
     } else {
         for (int e = threadIdx.x; e < d * kCols; e += kThreads) {
             const int z = e / kCols, c = e % kCols;
-            fs[cell(z, c)] = c < valid ? packed(f2[((size_t)z * h + y) * w + x0 + c]) : kNoSite;
+            const uint32_t q = c < valid ? packed(f2[((size_t)z * h + y) * w + x0 + c]) : kNoSite;
+            has_site |= q != kNoSite;
+            fs[cell(z, c)] = q;
         }
     }
-    __syncthreads();
+    const bool any = __syncthreads_or(has_site);
     Bits bits{bm};
     FinalOut o{d2, fl, (uint32_t)(w * h), (uint32_t)(y * w + x0), valid};
-    envelope(d, fs, o, bits);
+    if (!any) no_sites(d, o);
+    else if (!scan(d, fs, o, &abort_flag)) envelope(d, fs, o, bits);
 }
 
 
