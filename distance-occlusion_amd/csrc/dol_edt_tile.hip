@@ -374,7 +374,7 @@ struct FinalOut {   // int32 + float at cell index base(q) + x (< 2^31 on this p
 // output would need more than kScanCap steps the tile is left to the
 // envelope (returns false).
 constexpr int kScanCap = 160;
-constexpr int kStepGroup = 4;
+constexpr int kStepGroup = 8;
 constexpr int kRowsPerWave = 64 / kCols;
 constexpr int kRowsPerPass = kThreads / kCols;
 
