@@ -248,6 +248,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only copy 512 MiB device to device 5 times (calibrates HBM byte counters)")
+    ap.add_argument("--only-extra", choices=["jfa3d", "glyphs", "jfa2d", "config1", "seeds"],
+                    help="run only this secondary workload and print its figures (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -276,9 +278,19 @@ def main():
     w = h = d = N
     cells = w * h * d
 
+    if args.only_extra and args.only_extra != "jfa3d":
+        run = {"glyphs": lambda: bench_glyphs(L, torch, dev, args), "jfa2d": lambda: bench_jfa2d(L, torch, dev),
+               "config1": lambda: bench_config1(L, torch, dev), "seeds": lambda: bench_seeds(L, torch, dev)}
+        print(json.dumps({args.only_extra: run[args.only_extra]()}), flush=True)
+        return
+
     print(f"# generating {N}^3 scene", file=sys.stderr, flush=True)
     mask = L.scene_shapes_3d(w, h, d, 300, 100, seed=3)
     mask_t = torch.from_numpy(mask.view(np.int32)).to(dev)
+    if args.only_extra == "jfa3d":
+        fl3 = torch.empty(cells, dtype=torch.float32, device=dev)
+        print(json.dumps({"jfa3d_signed": bench_jfa3d(L, torch, dev, mask_t, w, h, d, fl3)}), flush=True)
+        return
     d2 = torch.empty(cells, dtype=torch.int32, device=dev)
     fl = torch.empty(cells, dtype=torch.float32, device=dev)
     work = torch.empty(L.workspace("edt_3d", w, h, d), dtype=torch.uint8, device=dev)
