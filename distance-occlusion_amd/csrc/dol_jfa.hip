@@ -146,9 +146,9 @@ __global__ void __launch_bounds__(256) k_jfa3d(int step, int w, int h, int d, in
 // Between passes the field is kept in the same 6 bytes a cell, split into
 // two planes: packed (x, y) int16 pairs for all cells, then z for all cells.
 // A tap then costs one 32-bit and one 16-bit LDS read, x and y are offset
-// with one packed 16-bit add (wrapping as the reference's int16 store does),
-// and the best (x, y) moves as one register.  The first pass reads the
-// caller's interleaved cCellDelta3 layout and the last writes it back.
+// with one packed 16-bit add (wrapping as the reference's int16 store does).
+// The first pass reads the caller's interleaved cCellDelta3 layout (and
+// converts the staged rows in LDS) and the last writes it back.
 constexpr int kRowMax = 1024;
 
 typedef unsigned short ushort2v __attribute__((ext_vector_type(2)));
@@ -171,7 +171,8 @@ __global__ void __launch_bounds__(kRowMax) k_jfa3d_rows(int step, int w_arg, int
 {
     const int w = W > 0 ? W : w_arg;
     extern __shared__ __align__(16) uint32_t lds[];
-    // IN_AOS: [9][3w] int16; else [9][w] packed (x, y), then [9][w] int16 z
+    // [9][w] packed (x, y), then [9][w] int16 z (IN_AOS: staged first as
+    // [9][3w] int16 and converted in place)
     const int16_t* lds16 = (const int16_t*)lds;
     const int row = blockIdx.x;   // z*h + y
     const int z = row / h, y = row - z * h;
@@ -204,6 +205,31 @@ __global__ void __launch_bounds__(kRowMax) k_jfa3d_rows(int step, int w_arg, int
     }
     __syncthreads();
     const int x = threadIdx.x;
+    if (IN_AOS) {
+        // turn the staged rows into the planar form in place: each thread
+        // takes its own column of the 9 rows into registers, then writes it
+        // back (rows that were not staged carry garbage and are never read)
+        uint32_t cxy[9];
+        int16_t cz[9];
+        if (x < w) {
+#pragma unroll
+            for (int r = 0; r < 9; r++) {
+                const int16_t* c = lds16 + r * 3 * w + 3 * x;
+                cxy[r] = (uint32_t)(uint16_t)c[0] | ((uint32_t)(uint16_t)c[1] << 16);
+                cz[r] = c[2];
+            }
+        }
+        __syncthreads();
+        if (x < w) {
+            int16_t* z16 = (int16_t*)lds + 2 * 9 * w;
+#pragma unroll
+            for (int r = 0; r < 9; r++) {
+                lds[r * w + x] = cxy[r];
+                z16[r * w + x] = cz[r];
+            }
+        }
+        __syncthreads();
+    }
     if (x >= w) return;
     // tap ic of staged row r.  The x of a tap is clamped into the row so
     // every read is in range (an out-of-row tap is masked off instead of
@@ -221,14 +247,8 @@ __global__ void __launch_bounds__(kRowMax) k_jfa3d_rows(int step, int w_arg, int
         asm volatile("" : "+v"(oxy[k]), "+v"(oz[k]));
     }
     auto tap = [&](int r, int ic, uint32_t& xy, int& tz) {
-        if (IN_AOS) {
-            const int16_t* c = lds16 + 3 * xs[ic + 1] + r * 3 * w;
-            xy = (uint32_t)(uint16_t)c[0] | ((uint32_t)(uint16_t)c[1] << 16);
-            tz = c[2];
-        } else {
-            xy = ((const uint32_t*)((const char*)lds + oxy[ic + 1]))[r * w];
-            tz = ((const int16_t*)((const char*)(lds16 + 2 * 9 * w) + oz[ic + 1]))[r * w];
-        }
+        xy = ((const uint32_t*)((const char*)lds + oxy[ic + 1]))[r * w];
+        tz = ((const int16_t*)((const char*)(lds16 + 2 * 9 * w) + oz[ic + 1]))[r * w];
     };
     const int off = cw * step;
     // candidate of tap t = 9 (kc+1) + 3 (jc+1) + (ic+1): the source offset by
@@ -259,6 +279,7 @@ __global__ void __launch_bounds__(kRowMax) k_jfa3d_rows(int step, int w_arg, int
 #pragma unroll
             for (int ic = -1; ic <= 1; ic++) {
                 const int t = 3 * r + ic + 1;
+                if (t == 13) continue;   // the cell itself never beats itself (strict >)
                 tap(r, ic, sxy, sz);
                 uint32_t cxy;
                 int cz;
@@ -275,14 +296,8 @@ __global__ void __launch_bounds__(kRowMax) k_jfa3d_rows(int step, int w_arg, int
     {
         const int r = bt / 3, ic = bt % 3 - 1;
         const int xx = xs[ic + 1];
-        if (IN_AOS) {
-            const int16_t* c = lds16 + 3 * xx + r * 3 * w;
-            sxy = (uint32_t)(uint16_t)c[0] | ((uint32_t)(uint16_t)c[1] << 16);
-            sz = c[2];
-        } else {
-            sxy = lds[r * w + xx];
-            sz = lds16[2 * 9 * w + r * w + xx];
-        }
+        sxy = lds[r * w + xx];
+        sz = lds16[2 * 9 * w + r * w + xx];
         candidate(bt, sxy, sz, bxy, bz);
     }
     const size_t i = (size_t)row * w + x;
