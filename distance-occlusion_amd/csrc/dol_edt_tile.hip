@@ -37,7 +37,7 @@ namespace dolhip {
 
 namespace tile {
 
-constexpr int kCols = 8;       // lines per tile (a quarter of a mask word of x)
+constexpr int kCols = 16;      // lines per tile (half a mask word of x)
 constexpr int kBand = 16;      // sites per band
 constexpr int kMaxN = 512;     // longest line held in LDS
 constexpr int kBands = kMaxN / kBand;
