@@ -209,14 +209,20 @@ __global__ void __launch_bounds__(kRowMax) k_jfa3d_rows(int step, int w_arg, int
         // turn the staged rows into the planar form in place: each thread
         // takes its own column of the 9 rows into registers, then writes it
         // back (rows that were not staged carry garbage and are never read)
+        // cell x's 6 bytes start at byte 6x, which is 4-aligned only for
+        // even x: read the two aligned words that hold them and take the
+        // halves (an unaligned LDS access would be split and serialised)
         uint32_t cxy[9];
         int16_t cz[9];
         if (x < w) {
+            const int k0 = (3 * x) >> 1;   // word holding the cell's first byte
+            const bool odd = x & 1;
 #pragma unroll
             for (int r = 0; r < 9; r++) {
-                const int16_t* c = lds16 + r * 3 * w + 3 * x;
-                cxy[r] = (uint32_t)(uint16_t)c[0] | ((uint32_t)(uint16_t)c[1] << 16);
-                cz[r] = c[2];
+                const uint32_t* wrow = lds + r * (3 * w / 2);
+                const uint32_t a = wrow[k0], b = wrow[k0 + 1];
+                cxy[r] = odd ? __builtin_amdgcn_alignbit(b, a, 16) : a;
+                cz[r] = (int16_t)(odd ? (b >> 16) : (b & 0xffffu));
             }
         }
         __syncthreads();

@@ -224,8 +224,9 @@ def test_encoders_2d(gpu, oracle, rng):
                                       oracle.sdf_rg_rgba_2d(w, h, m, delta, max_len))
 
 
-def test_encoders_3d(gpu, oracle, rng):
-    w, h, d = 33, 20, 9
+@pytest.mark.parametrize("dims", [(33, 20, 9), (32, 12, 7), (64, 5, 3)])
+def test_encoders_3d(gpu, oracle, rng, dims):
+    w, h, d = dims
     m = blob_mask(rng, (w, h, d))
     delta = oracle.jfa_3d(w, h, d, oracle.border_3d(w, h, d, m), 2)
     np.testing.assert_array_equal(gpu.signed_distance_3d(w, h, d, m, delta), oracle.signed_dist_3d(w, h, d, m, delta))
