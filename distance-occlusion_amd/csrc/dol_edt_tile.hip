@@ -144,9 +144,8 @@ __device__ __forceinline__ bool removable(int u, int Fu, int v, int Fv, int w, i
 // a result (kEdtInf where the line has no site).  The survivor bitmask's
 // storage is reused for the band maxima of phase 3b.
 template <class Out>
-__device__ void envelope(int n, uint32_t* fs, Out& out, Bits bits)
+__device__ __forceinline__ void envelope(int n, uint32_t* fs, Out& out, Bits bits, int tid)
 {
-    const int tid = threadIdx.x;
     const int c = tid & (kCols - 1);
     const int b = tid / kCols;
     const int nb = (n + kBand - 1) / kBand;
@@ -340,6 +339,13 @@ struct FieldOut {   // int32 field at (x0 + c, q, z) of a w x h x d volume
     {
         f2[((uint32_t)z * h + q) * (uint32_t)w + x0 + c] = v;
     }
+    // columns c, c + 1 (c even; w, x0 even and f2 8-byte aligned)
+    __device__ void pair(int c, int q, int v0, int v1) const
+    {
+        int32_t* p = f2 + ((uint32_t)z * h + q) * (uint32_t)w + x0 + c;
+        if (x0 + c + 1 < w) *(int2*)p = make_int2(v0, v1);
+        else if (x0 + c < w) *p = v0;
+    }
 };
 
 struct FinalOut {   // int32 + float at cell index base(q) + x (< 2^31 on this path)
@@ -354,6 +360,17 @@ struct FinalOut {   // int32 + float at cell index base(q) + x (< 2^31 on this p
         const uint32_t i = base + (uint32_t)q * row_stride + (uint32_t)c;
         if (d2) d2[i] = v;
         if (fl) fl[i] = sqrt_d2(v);
+    }
+    // columns c, c + 1 (c even; base and row_stride even, outputs 8-byte aligned)
+    __device__ void pair(int c, int q, int v0, int v1) const
+    {
+        const uint32_t i = base + (uint32_t)q * row_stride + (uint32_t)c;
+        if (c + 1 < valid) {
+            if (d2) *(int2*)(d2 + i) = make_int2(v0, v1);
+            if (fl) *(float2*)(fl + i) = make_float2(sqrt_d2(v0), sqrt_d2(v1));
+        } else if (c < valid) {
+            (*this)(c, q, v0);
+        }
     }
 };
 
@@ -379,7 +396,7 @@ constexpr int kRowsPerWave = 64 / kCols;
 constexpr int kRowsPerPass = kThreads / kCols;
 
 template <class Out>
-__device__ bool scan(int n, const uint32_t* fs, Out& out, int* abort_flag)
+__device__ __forceinline__ bool scan(int n, const uint32_t* fs, Out& out, int* abort_flag)
 {
     const int tid = threadIdx.x;
     const int c = tid % kCols;
@@ -402,6 +419,7 @@ __device__ bool scan(int n, const uint32_t* fs, Out& out, int* abort_flag)
                 fail = true;
                 break;
             }
+            k = __builtin_amdgcn_readfirstlane(k);   // the same in every active lane
             if (pw - (k + kStepGroup - 1) >= 0 && pw + kRowsPerWave - 1 + k + kStepGroup - 1 < n) {
                 const uint32_t* lo = fs + cell(p - k - (kStepGroup - 1), c);
                 const uint32_t* hi = fs + cell(p + k, c);
@@ -437,48 +455,166 @@ __device__ void no_sites(int n, Out& out)
     for (int p = threadIdx.x / kCols; p < n; p += kRowsPerPass) out(c, p, kEdtInf);
 }
 
+// ---- the same scan over wide tiles of 2 * kPairs lines held as 16-bit
+// values, two adjacent lines in each LDS word (word q * kPairs + i holds
+// lines 2i and 2i + 1 at sample q), so one read and three packed operations
+// serve two lines.  Values are clamped to kClamp = 2^16 - 1 and the sums
+// saturate there.  A result below kClamp is exact: its minimising sample was
+// not clamped and its sum did not saturate, and every clamped or saturated
+// candidate is at least kClamp.  An output that stays at kClamp (a line with
+// no site within reach), or that needs more than kPairScanCap steps, fails
+// the tile, which is then redone exactly; the cap keeps every square k^2
+// below 2^16.
+constexpr int kPairs = 16;
+constexpr int kWideCols = 2 * kPairs;                 // lines per wide tile
+constexpr int kPairRowsPerWave = 64 / kPairs;
+constexpr int kPairRowsPerPass = kThreads / kPairs;
+constexpr uint32_t kClamp = 0xffff;
+static_assert(kMaxN * kPairs <= kWords, "a wide tile fits the LDS words");
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pk_min(uint32_t a, uint32_t b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, a),
+                                                                  __builtin_bit_cast(u16x2, b)));
+}
+// saturating (v_pk_add_u16 with clamp)
+__device__ __forceinline__ uint32_t pk_add_sat(uint32_t a, uint32_t b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_add_sat(__builtin_bit_cast(u16x2, a),
+                                                                      __builtin_bit_cast(u16x2, b)));
+}
+__device__ __forceinline__ uint32_t clamp16(uint32_t v) { return min(v, kClamp); }
+
+// A lane computes kLaneRows consecutive samples of its two lines: the rows
+// its samples read at steps k .. k + kStepGroup - 1 overlap, so each group
+// reads kStepGroup + kLaneRows - 1 rows on either side instead of
+// kStepGroup * kLaneRows.
+#ifndef DOL_LANE_ROWS
+#define DOL_LANE_ROWS 4
+#endif
+#ifndef DOL_PAIR_GROUP
+#define DOL_PAIR_GROUP 8
+#endif
+constexpr int kLaneRows = DOL_LANE_ROWS;
+constexpr int kPairStepGroup = DOL_PAIR_GROUP;
+#ifndef DOL_PAIR_CAP
+#define DOL_PAIR_CAP (256 - kPairStepGroup)
+#endif
+constexpr int kPairScanCap = DOL_PAIR_CAP;
+static_assert((kPairScanCap + kPairStepGroup - 1) * (kPairScanCap + kPairStepGroup - 1) <= 0xffff,
+              "16-bit squares");
+constexpr int kPairWaveRows = kPairRowsPerWave * kLaneRows;
+constexpr int kPairPassRows = (kThreads / 64) * kPairWaveRows;
+
+template <class Out>
+__device__ __forceinline__ bool scan_pairs(int n, const uint32_t* hs, Out& out, int* abort_flag)
+{
+    constexpr int R = kLaneRows, G = kPairStepGroup;
+    const int tid = threadIdx.x;
+    const int i = tid % kPairs, c = 2 * i;
+    const int wr0 = (tid / 64) * kPairWaveRows;          // the wavefront's first row
+    const int rr = ((tid % 64) / kPairs) * R;            // the lane's first row in it
+    const bool active = out.active(c);
+    for (int base = 0; base < n; base += kPairPassRows) {
+        if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+        const int pw = base + wr0;
+        if (pw >= n) break;
+        const int p = pw + rr;
+        const int reach = max(p + R - 1, n - 1 - p);   // farthest sample from any of the rows
+        uint32_t best[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) best[r] = active && p + r < n ? hs[(p + r) * kPairs + i] : 0u;
+        int k = 1;
+        bool fail = false;
+        for (;;) {
+            uint32_t top = 0;
+#pragma unroll
+            for (int r = 0; r < R; r++) top = max(top, max(best[r] & 0xffffu, best[r] >> 16));
+            if (k * k >= (int)top || k > reach) break;
+            if (k > kPairScanCap) {
+                fail = true;
+                break;
+            }
+            // k is the same in every active lane: keep it (and the squares) scalar
+            k = __builtin_amdgcn_readfirstlane(k);
+            if (pw - (k + G - 1) >= 0 && pw + kPairWaveRows - 1 + k + G - 1 < n) {
+                // row p + r - k - t is lo[G - 1 - t + r], row p + r + k + t is hi[t + r]
+                const uint32_t* lo = hs + (p - k - (G - 1)) * kPairs + i;
+                const uint32_t* hi = hs + (p + k) * kPairs + i;
+                uint32_t a[G + R - 1], b[G + R - 1];
+#pragma unroll
+                for (int u = 0; u < G + R - 1; u++) {
+                    a[u] = lo[u * kPairs];
+                    b[u] = hi[u * kPairs];
+                }
+#pragma unroll
+                for (int t = 0; t < G; t++) {
+                    const uint32_t sq = (uint32_t)((k + t) * (k + t)) * 0x10001u;
+#pragma unroll
+                    for (int r = 0; r < R; r++)
+                        best[r] = pk_min(best[r], pk_add_sat(pk_min(a[G - 1 - t + r], b[t + r]), sq));
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < G; t++) {
+                    const int kk = k + t;
+                    const uint32_t sq = (uint32_t)(kk * kk) * 0x10001u;
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        const uint32_t a = hs[max(p + r - kk, 0) * kPairs + i];
+                        const uint32_t b = hs[min(p + r + kk, n - 1) * kPairs + i];
+                        best[r] = pk_min(best[r], pk_add_sat(pk_min(a, b), sq));
+                    }
+                }
+            }
+            k += G;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            if (max(best[r] & 0xffffu, best[r] >> 16) >= kClamp) fail = true;
+        if (fail) {
+            __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else if (active) {
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                if (p + r < n) out.pair(c, p + r, (int)(best[r] & 0xffffu), (int)(best[r] >> 16));
+        }
+    }
+    __syncthreads();
+    return *abort_flag == 0;
+}
+
+template <class Out>
+__device__ void no_sites_pairs(int n, Out& out)
+{
+    const int c = 2 * (threadIdx.x % kPairs);
+    if (!out.active(c)) return;
+    for (int p = threadIdx.x / kPairs; p < n; p += kPairRowsPerPass) out.pair(c, p, kEdtInf, kEdtInf);
+}
+
 }  // namespace tile
 
 using namespace tile;
 
-// x pass from the bit rows, then the y envelope.  Workgroup (z, jt) covers
-// columns kCols*jt .. of slice z; the tiles of one slice map to the same XCD
-// (workgroups are dealt to the 8 XCDs round-robin) so a slice's mask rows are
-// fetched into one L2.
-template <bool FINAL>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) k_edt3d_xy(int w, int h, int d, const uint32_t* __restrict__ mask,
-                                                       uint32_t invert, int32_t* __restrict__ f2,
-                                                       float* __restrict__ fl)
+// x distances into LDS of the columns x0 .. of slice z (16 columns, x0 a
+// multiple of 16; or, PACKED, the kWideCols columns of mask word x0 / 32 as
+// clamped 16-bit values two to a word, 0 past the volume's edge), one thread
+// per row; the whole row (<= 16 words) is loaded at once so the loads
+// overlap.  Without PACKED a column with no site in its row holds kNoSite.
+// Returns whether the slice has a site: a row with a set bit gives every
+// column one.
+template <bool PACKED>
+__device__ __forceinline__ bool xpass(int w, int h, int z, int x0, const uint32_t* __restrict__ mask,
+                      uint32_t invert, uint32_t* fs, int tid)
 {
-    __shared__ __align__(16) uint32_t fs[kWords];
-    __shared__ __align__(16) uint32_t bm[kBmWords];
-    __shared__ int abort_flag;
-    if (threadIdx.x == 0) abort_flag = 0;
+    constexpr int kN = PACKED ? kWideCols : kCols;
     const int ws = mask_stride(w);
-    const int xt = (w + kCols - 1) / kCols;
-    // swizzle: blocks congruent mod 8 share an XCD; a group of (up to) 8
-    // slices takes 8*xt consecutive blocks, slice-minor
-    int z, jt;
-    {
-        const int bid = blockIdx.x;
-        const int group = bid / (8 * xt);
-        const int gs = min(8, d - group * 8);
-        const int inner = bid - group * 8 * xt;
-        z = group * 8 + inner % gs;
-        jt = inner / gs;
-    }
-    const int x0 = jt * kCols, j = x0 >> 5, boff = x0 & 31;
+    const int j = x0 >> 5, boff = PACKED ? 0 : (x0 & 31);
     const int tail = w - 32 * (ws - 1);
     const uint32_t last_valid = tail >= 32 ? 0xffffffffu : ((1u << tail) - 1u);
-
-    // x distances of the tile's columns (bits boff.. of mask word j), up to
-    // 16 columns per thread; the whole row (<= 16 words) is loaded at once so
-    // the loads overlap
-    constexpr int kXCols = kCols < 16 ? kCols : 16;
-    constexpr int kParts = kCols / kXCols;
-    int has_site = 0;   // a row with a set bit gives every column a site
-    for (int e = threadIdx.x; e < kParts * h; e += kThreads) {
-        const int y = e / kParts, part = e % kParts;
+    int has_site = 0;
+    for (int y = tid; y < h; y += kThreads) {
         const uint32_t* row = mask + ((size_t)z * h + y) * ws;
         uint32_t words[16];
         if (ws == 16) {
@@ -507,43 +643,169 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
         for (int k = 15; k >= 0; k--)
             if (k > j && words[k]) R = 32 * k + lo_bit(words[k]);
         has_site |= (wj != 0u || L >= 0 || R >= 0);
-        uint4* out = (uint4*)(fs + cell(y, kXCols * part));
+        uint32_t v[kN];
 #pragma unroll
-        for (int i = 0; i < kXCols; i += 4) {
-            uint32_t quad[4];
+        for (int t = 0; t < kN; t++) {
+            const int ii = boff + t;   // bit in word j
+            const int x = 32 * j + ii;
+            const uint32_t lo = wj & bits_upto(ii);
+            const uint32_t hi = wj >> ii;
+            const int dl = lo ? ii - hi_bit(lo) : (L >= 0 ? x - L : kMaxN);
+            const int dr = hi ? lo_bit(hi) : (R >= 0 ? R - x : kMaxN);
+            const int dd = min(dl, dr);
+            if (PACKED) v[t] = x < w ? clamp16((uint32_t)(dd * dd)) : 0u;
+            else v[t] = dd >= kMaxN ? kNoSite : (uint32_t)(dd * dd);
+        }
+        uint4* out = (uint4*)(fs + y * (PACKED ? kPairs : kCols));
+        if (PACKED) {
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const int ii = boff + kXCols * part + i + t;   // bit in word j
-                const int x = 32 * j + ii;
-                const uint32_t lo = wj & bits_upto(ii);
-                const uint32_t hi = wj >> ii;
-                const int dl = lo ? ii - hi_bit(lo) : (L >= 0 ? x - L : kMaxN);
-                const int dr = hi ? lo_bit(hi) : (R >= 0 ? R - x : kMaxN);
-                const int dd = min(dl, dr);
-                quad[t] = dd >= kMaxN ? kNoSite : (uint32_t)(dd * dd);
-            }
-            out[i >> 2] = make_uint4(quad[0], quad[1], quad[2], quad[3]);
+            for (int t = 0; t < kN; t += 8)
+                out[t >> 3] = make_uint4(v[t] | v[t + 1] << 16, v[t + 2] | v[t + 3] << 16,
+                                         v[t + 4] | v[t + 5] << 16, v[t + 6] | v[t + 7] << 16);
+        } else {
+#pragma unroll
+            for (int t = 0; t < kN; t += 4) out[t >> 2] = make_uint4(v[t], v[t + 1], v[t + 2], v[t + 3]);
         }
     }
-    const bool any = __syncthreads_or(has_site);
+    return __syncthreads_or(has_site);
+}
 
+// slice z of a tile sequence whose workgroups are dealt to the 8 XCDs
+// round-robin: a group of (up to) 8 slices takes 8 * xt consecutive blocks,
+// slice-minor, so the tiles of one slice share an XCD and its mask rows are
+// fetched into one L2
+__device__ __forceinline__ void xy_tile(int d, int xt, int& z, int& jt)
+{
+    const int bid = blockIdx.x;
+    const int group = bid / (8 * xt);
+    const int gs = min(8, d - group * 8);
+    const int inner = bid - group * 8 * xt;
+    z = group * 8 + inner % gs;
+    jt = inner / gs;
+}
+
+// x pass from the bit rows, then the y scan (or envelope) of kCols lines of
+// slice z; writes the int32 field f2 (or, for 2D, the final int32 and float
+// outputs)
+template <bool FINAL>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) k_edt3d_xy(int w, int h, int d, const uint32_t* __restrict__ mask,
+                                                       uint32_t invert, int32_t* __restrict__ f2,
+                                                       float* __restrict__ fl)
+{
+    __shared__ __align__(16) uint32_t fs[kWords];
+    __shared__ __align__(16) uint32_t bm[kBmWords];
+    __shared__ int abort_flag;
+    if (threadIdx.x == 0) abort_flag = 0;
+    int z, jt;
+    xy_tile(d, (w + kCols - 1) / kCols, z, jt);
+    const int x0 = jt * kCols;
+    const bool any = xpass<false>(w, h, z, x0, mask, invert, fs, threadIdx.x);
     Bits bits{bm};
     if (FINAL) {
         // 2D image (d == 1): final outputs, rows of width w
         FinalOut o{f2, fl, (uint32_t)w, (uint32_t)x0, min(kCols, w - x0)};
         if (!any) no_sites(h, o);
-        else if (!scan(h, fs, o, &abort_flag)) envelope(h, fs, o, bits);
+        else if (!scan(h, fs, o, &abort_flag)) envelope(h, fs, o, bits, threadIdx.x);
     } else {
         FieldOut o{f2, w, h, x0, z};
         if (!any) no_sites(h, o);
-        else if (!scan(h, fs, o, &abort_flag)) envelope(h, fs, o, bits);
+        else if (!scan(h, fs, o, &abort_flag)) envelope(h, fs, o, bits, threadIdx.x);
+    }
+}
+
+// the same over wide tiles (kWideCols lines, one mask word): packed scan,
+// and if that fails, each half again exactly (x pass into 32-bit words, then
+// the envelope).  w even, f2 8-byte aligned.
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) k_edt3d_xy_wide(int w, int h, int d, const uint32_t* __restrict__ mask,
+                                                            uint32_t invert, int32_t* __restrict__ f2)
+{
+    __shared__ __align__(16) uint32_t fs[kWords];
+    __shared__ __align__(16) uint32_t bm[kBmWords];
+    __shared__ int abort_flag;
+    if (threadIdx.x == 0) abort_flag = 0;
+    int z, jt;
+    xy_tile(d, (w + kWideCols - 1) / kWideCols, z, jt);
+    const int x0 = jt * kWideCols;
+    const bool any = xpass<true>(w, h, z, x0, mask, invert, fs, threadIdx.x);
+    FieldOut o{f2, w, h, x0, z};
+    if (!any) {
+        no_sites_pairs(h, o);
+    } else if (!scan_pairs(h, fs, o, &abort_flag)) {
+        Bits bits{bm};
+        for (int half = 0; half < 2 && x0 + kCols * half < w; half++) {
+            // the thread index is opaque to the compiler here, so nothing of
+            // the half's work is hoisted out of the loop (register pressure)
+            int tid = threadIdx.x;
+            asm volatile("" : "+v"(tid));
+            __syncthreads();
+            FieldOut oh{f2, w, h, x0 + kCols * half, z};
+            xpass<false>(w, h, z, x0 + kCols * half, mask, invert, fs, tid);
+            envelope(h, fs, oh, bits, tid);
+        }
     }
 }
 
 // an f2 sample (< 2^20, or kEdtInf) as an LDS word
 __device__ __forceinline__ uint32_t packed(int v) { return v == kEdtInf ? kNoSite : (uint32_t)v; }
 
-// z envelope over the f2 tile of row y, columns kCols*jt .. kCols*jt+kCols-1
+// loads the f2 tile of row y, columns x0 .. x0 + valid - 1 (of kCols, or of
+// kWideCols two to a word as clamped 16-bit values when PACKED), along z
+// into LDS; columns past valid hold kNoSite (0 when PACKED).  Returns
+// whether the tile has a site.
+template <bool PACKED>
+__device__ __forceinline__ bool zload(int w, int h, int d, int y, int x0, int valid, const int32_t* __restrict__ f2,
+                      uint32_t* fs, int tid)
+{
+    constexpr int kTileCols = PACKED ? kWideCols : kCols;
+    int has_site = 0;
+    if (valid == kTileCols && d == kMaxN && (w & 3) == 0) {
+        // full tile: 16-byte loads, all in flight before the first LDS store
+        constexpr int kQuads = kTileCols / 4;
+        constexpr int kIter = kMaxN * kQuads / kThreads;
+        int4 v[kIter];
+#pragma unroll
+        for (int k = 0; k < kIter; k++) {
+            const int e = tid + k * kThreads;
+            v[k] = *(const int4*)(f2 + ((size_t)(e / kQuads) * h + y) * w + x0 + (e % kQuads) * 4);
+        }
+#pragma unroll
+        for (int k = 0; k < kIter; k++) {
+            const int e = tid + k * kThreads;
+            has_site |= (v[k].x & v[k].y & v[k].z & v[k].w) != kEdtInf;
+            if (PACKED) {
+                const uint2 q = make_uint2(clamp16(v[k].x) | clamp16(v[k].y) << 16,
+                                           clamp16(v[k].z) | clamp16(v[k].w) << 16);
+                *(uint2*)(fs + (e / kQuads) * kPairs + (e % kQuads) * 2) = q;
+            } else {
+                const uint4 q = make_uint4(packed(v[k].x), packed(v[k].y), packed(v[k].z), packed(v[k].w));
+                *(uint4*)(fs + cell(e / kQuads, (e % kQuads) * 4)) = q;
+            }
+        }
+    } else {
+        for (int e = tid; e < d * kTileCols; e += kThreads) {
+            const int z = e / kTileCols, c = e % kTileCols;
+            const int v = c < valid ? f2[((size_t)z * h + y) * w + x0 + c] : kEdtInf;
+            has_site |= v != kEdtInf;
+            if (PACKED) ((uint16_t*)fs)[z * kTileCols + c] = c < valid ? (uint16_t)clamp16(v) : 0;
+            else fs[cell(z, c)] = packed(v);
+        }
+    }
+    return __syncthreads_or(has_site);
+}
+
+// tile (y, x0) of the z pass; blocks congruent mod 8 share an XCD, and each
+// XCD gets a run of consecutive tiles, so the partial rows that neighbouring
+// tiles write meet in one L2
+__device__ __forceinline__ void z_tile(int xt, int cols, int& y, int& x0)
+{
+    int bid = blockIdx.x;
+    if ((gridDim.x & 7) == 0) bid = (bid & 7) * (gridDim.x >> 3) + (bid >> 3);
+    y = bid / xt;
+    x0 = (bid - y * xt) * cols;
+}
+
+// z scan (or envelope) over the f2 tile of row y, columns x0 .. x0 + kCols - 1;
+// writes int32 and float distances
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) k_edt3d_z(int w, int h, int d, const int32_t* __restrict__ f2,
                                                       int32_t* __restrict__ d2, float* __restrict__ fl)
 {
@@ -551,45 +813,44 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
     __shared__ __align__(16) uint32_t bm[kBmWords];
     __shared__ int abort_flag;
     if (threadIdx.x == 0) abort_flag = 0;
-    const int xt = (w + kCols - 1) / kCols;
-    // swizzle: blocks congruent mod 8 share an XCD; give each XCD a run of
-    // consecutive tiles, so the partial rows that neighbouring tiles write
-    // meet in one L2
-    int bid = blockIdx.x;
-    if ((gridDim.x & 7) == 0) bid = (bid & 7) * (gridDim.x >> 3) + (bid >> 3);
-    const int y = bid / xt;
-    const int x0 = (bid - y * xt) * kCols;
+    int y, x0;
+    z_tile((w + kCols - 1) / kCols, kCols, y, x0);
     const int valid = min(kCols, w - x0);
-    int has_site = 0;
-    if (valid == kCols && d == kMaxN && (w & 3) == 0) {
-        // full tile: 16-byte loads, all in flight before the first LDS store
-        constexpr int kIter = kMaxN * (kCols / 4) / kThreads;
-        int4 v[kIter];
-#pragma unroll
-        for (int k = 0; k < kIter; k++) {
-            const int e = threadIdx.x + k * kThreads;
-            v[k] = *(const int4*)(f2 + ((size_t)(e / (kCols / 4)) * h + y) * w + x0 + (e % (kCols / 4)) * 4);
-        }
-#pragma unroll
-        for (int k = 0; k < kIter; k++) {
-            const int e = threadIdx.x + k * kThreads;
-            const uint4 q = make_uint4(packed(v[k].x), packed(v[k].y), packed(v[k].z), packed(v[k].w));
-            has_site |= (q.x & q.y & q.z & q.w) != kNoSite;
-            *(uint4*)(fs + cell(e / (kCols / 4), (e % (kCols / 4)) * 4)) = q;
-        }
-    } else {
-        for (int e = threadIdx.x; e < d * kCols; e += kThreads) {
-            const int z = e / kCols, c = e % kCols;
-            const uint32_t q = c < valid ? packed(f2[((size_t)z * h + y) * w + x0 + c]) : kNoSite;
-            has_site |= q != kNoSite;
-            fs[cell(z, c)] = q;
-        }
-    }
-    const bool any = __syncthreads_or(has_site);
+    const bool any = zload<false>(w, h, d, y, x0, valid, f2, fs, threadIdx.x);
     Bits bits{bm};
     FinalOut o{d2, fl, (uint32_t)(w * h), (uint32_t)(y * w + x0), valid};
     if (!any) no_sites(d, o);
-    else if (!scan(d, fs, o, &abort_flag)) envelope(d, fs, o, bits);
+    else if (!scan(d, fs, o, &abort_flag)) envelope(d, fs, o, bits, threadIdx.x);
+}
+
+// the same over wide tiles: packed scan, and if that fails, each half again
+// exactly.  w even, outputs 8-byte aligned.
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) k_edt3d_z_wide(int w, int h, int d, const int32_t* __restrict__ f2,
+                                                           int32_t* __restrict__ d2, float* __restrict__ fl)
+{
+    __shared__ __align__(16) uint32_t fs[kWords];
+    __shared__ __align__(16) uint32_t bm[kBmWords];
+    __shared__ int abort_flag;
+    if (threadIdx.x == 0) abort_flag = 0;
+    int y, x0;
+    z_tile((w + kWideCols - 1) / kWideCols, kWideCols, y, x0);
+    const int valid = min(kWideCols, w - x0);
+    const bool any = zload<true>(w, h, d, y, x0, valid, f2, fs, threadIdx.x);
+    FinalOut o{d2, fl, (uint32_t)(w * h), (uint32_t)(y * w + x0), valid};
+    if (!any) {
+        no_sites_pairs(d, o);
+    } else if (!scan_pairs(d, fs, o, &abort_flag)) {
+        Bits bits{bm};
+        for (int half = 0; half < 2 && kCols * half < valid; half++) {
+            int tid = threadIdx.x;   // opaque, as in k_edt3d_xy_wide
+            asm volatile("" : "+v"(tid));
+            __syncthreads();
+            const int xh = x0 + kCols * half, vh = min(kCols, valid - kCols * half);
+            FinalOut oh{d2, fl, (uint32_t)(w * h), (uint32_t)(y * w + xh), vh};
+            zload<false>(w, h, d, y, xh, vh, f2, fs, tid);
+            envelope(d, fs, oh, bits, tid);
+        }
+    }
 }
 
 
@@ -610,8 +871,23 @@ void launch_edt_tile_2d(int w, int h, const uint32_t* mask, int invert, int32_t*
 void launch_edt_tile_3d(int w, int h, int d, const uint32_t* mask, int invert, int32_t* d2,
                         float* fl, void* work, hipStream_t s)
 {
-    const int xt = (w + kCols - 1) / kCols;
     int32_t* f2 = (int32_t*)work;
+    // wide tiles store column pairs with 8-byte stores
+    if ((w & 1) == 0 && (((uintptr_t)d2 | (uintptr_t)fl | (uintptr_t)f2) & 7) == 0) {
+        const int xt = (w + kWideCols - 1) / kWideCols;
+        {
+            ProfileScope p("edt3d_xy", s);
+            hipLaunchKernelGGL(k_edt3d_xy_wide, dim3((unsigned)(xt * d)), dim3(kThreads), 0, s, w,
+                               h, d, mask, invert ? 0xffffffffu : 0u, f2);
+        }
+        {
+            ProfileScope p("edt3d_z", s);
+            hipLaunchKernelGGL(k_edt3d_z_wide, dim3((unsigned)(xt * h)), dim3(kThreads), 0, s, w, h,
+                               d, f2, d2, fl);
+        }
+        return;
+    }
+    const int xt = (w + kCols - 1) / kCols;
     {
         ProfileScope p("edt3d_xy", s);
         hipLaunchKernelGGL(k_edt3d_xy<false>, dim3((unsigned)(xt * d)), dim3(kThreads), 0, s, w, h,

@@ -273,7 +273,8 @@ def _pattern_masks(rng, dims):
 
 
 @pytest.mark.parametrize("dims", [(512, 64, 24), (130, 257, 33), (33, 512, 5), (5, 7, 512),
-                                  (512, 512, 3), (31, 1, 512), (1, 40, 1)])
+                                  (512, 512, 3), (31, 1, 512), (1, 40, 1), (64, 3, 512),
+                                  (96, 2, 512)])
 def test_edt_3d_tiled_patterns(gpu, oracle, rng, dims):
     w, h, d = dims
     for m in _pattern_masks(rng, dims):
@@ -283,6 +284,55 @@ def test_edt_3d_tiled_patterns(gpu, oracle, rng, dims):
             np.testing.assert_array_equal(d2, exp)
             finite = exp != 0x7FFFFFFF
             np.testing.assert_array_equal(fl[finite], np.sqrt(exp[finite].astype(np.float32)))
+
+
+@pytest.mark.parametrize("dims", [(400, 8, 3), (402, 3, 8), (300, 300, 2), (12, 400, 6)])
+def test_edt_3d_far_sites(gpu, oracle, rng, dims):
+    """Sites farther than 181 cells along x or y from short lines: the 16-bit
+    scan of wide tiles cannot give these distances exactly and must hand the
+    tile to the exact path."""
+    w, h, d = dims
+    shape = (d, h, w)
+    cases = []
+    for corner in [(0, 0, 0), (d - 1, h - 1, w - 1), (d // 2, 0, w - 1)]:
+        b = np.zeros(shape, bool)
+        b[corner] = True
+        cases.append(b)
+    b = np.zeros(shape, bool)
+    b[:, :, 0] = True                              # plane x = 0
+    cases.append(b)
+    b = np.zeros(shape, bool)
+    b[0, :, :] = rng.random((h, w)) < 0.0005      # a few sites in slice 0 only
+    cases.append(b)
+    for bits in cases:
+        m = pack_mask(bits)
+        d2, fl = gpu.edt_3d(w, h, d, m)
+        exp = oracle.edt_3d(w, h, d, m)
+        np.testing.assert_array_equal(d2, exp)
+        finite = exp != 0x7FFFFFFF
+        np.testing.assert_array_equal(fl[finite], np.sqrt(exp[finite].astype(np.float32)))
+
+
+@pytest.mark.parametrize("dims", [(64, 40, 24), (33, 20, 9), (64, 64)])
+def test_edt_one_output(gpu, oracle, rng, dims):
+    """Either output may be omitted (a null pointer)."""
+    m = blob_mask(rng, dims)
+    if len(dims) == 2:
+        exp = oracle.edt_2d(*dims, m, False)
+        d2, fl = gpu.edt_2d(*dims, m, want_dist=False)
+        assert fl is None
+        np.testing.assert_array_equal(d2, exp)
+        d2, fl = gpu.edt_2d(*dims, m, want_d2=False)
+    else:
+        exp = oracle.edt_3d(*dims, m, False)
+        d2, fl = gpu.edt_3d(*dims, m, want_dist=False)
+        assert fl is None
+        np.testing.assert_array_equal(d2, exp)
+        d2, fl = gpu.edt_3d(*dims, m, want_d2=False)
+    assert d2 is None
+    finite = exp != 0x7FFFFFFF
+    np.testing.assert_array_equal(fl[finite], np.sqrt(exp[finite].astype(np.float32)))
+    assert np.all(np.isinf(fl[~finite]) | (fl[~finite] > 1e19))
 
 
 @pytest.mark.parametrize("dims", [(512, 512), (300, 511), (512, 7), (7, 512), (513, 40), (40, 513)])
