@@ -40,13 +40,15 @@ EDT3D_BYTES_PER_VOXEL = {
     "edt_rows": 0.125 + 4.0,          # mask bits in, row field out
     "edt_cols": 4.0 + 4.0,            # row field in, y field out
     "edt_slabs": 4.0 + 4.0 + 4.0,     # y field in, int32 + float32 out
-    "edt3d_xy": 0.125 + 4.0,          # fused x+y pass: mask in, y field out
-    "edt3d_z": 4.0 + 4.0 + 4.0,       # z pass: y field in, int32 + float32 out
+    "edt3d_xy": 0.125 + 2.0,          # fused x+y pass: mask in, 16-bit y field out
+    "edt3d_z": 2.0 + 4.0 + 4.0,       # z pass: 16-bit y field in, int32 + float32 out
 }
-# the three-pass form (x, y, z passes each through HBM) moves 24.125 B/voxel;
-# the tiled kernels fuse x into y and move 16.125
+# the three-pass form (x, y, z passes each through HBM as int32) moves
+# 24.125 B/voxel; the tiled kernels fuse x into y and hand the y field to the
+# z pass as 16-bit values (exact below 0xffff; the rare tiles with larger
+# values also write, and are read from, an int32 copy): 12.125
 EDT3D_3PASS_BYTES_PER_VOXEL = 24.125
-EDT3D_FUSED_BYTES_PER_VOXEL = 16.125
+EDT3D_FUSED_BYTES_PER_VOXEL = 12.125
 
 
 def timed(fn, steps, torch):

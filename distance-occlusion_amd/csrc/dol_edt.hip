@@ -245,12 +245,14 @@ void launch_edt_tile_2d(int w, int h, const uint32_t* mask, int invert, int32_t*
 void launch_edt_tile_3d(int w, int h, int d, const uint32_t* mask, int invert, int32_t* d2,
                         float* fl, void* work, hipStream_t s);
 
-// Tiled path (every dimension <= 512): 3D needs the y-pass field only.
+size_t edt_tile_workspace_3d(int w, int h, int d);
+
+// Tiled path (every dimension <= 512): 3D needs the y-pass fields only.
 // Line path: row field g and the line stack; 3D also the y-pass field.
 size_t edt_workspace(int w, int h, int d)
 {
     const size_t cells = (size_t)w * h * d;
-    if (edt_tile_fits(w, h, d)) return d == 1 ? 256 : align256(cells * 4);
+    if (edt_tile_fits(w, h, d)) return d == 1 ? 256 : edt_tile_workspace_3d(w, h, d);
     return align256(cells * 4) * (d == 1 ? 2 : 3);
 }
 

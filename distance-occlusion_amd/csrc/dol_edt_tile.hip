@@ -348,6 +348,34 @@ struct FieldOut {   // int32 field at (x0 + c, q, z) of a w x h x d volume
     }
 };
 
+// the field of the wide tiles as 16-bit values clamped to 0xffff ("0xffff or
+// more"; f16) and, written only by tiles whose field reaches that, the exact
+// int32 field (f32)
+struct Field16Out {   // pairs of exact values below 0xffff, or 0xffff
+    uint16_t* f16;
+    int w, h, x0, z;
+    __device__ bool active(int c) const { return x0 + c < w; }
+    // columns c, c + 1 (c even; w, x0 even and f16 4-byte aligned)
+    __device__ void pair(int c, int q, int v0, int v1) const
+    {
+        const uint32_t i = ((uint32_t)z * h + q) * (uint32_t)w + x0 + c;
+        *(uint32_t*)(f16 + i) = min((uint32_t)v0, 0xffffu) | min((uint32_t)v1, 0xffffu) << 16;
+    }
+};
+
+struct FieldBothOut {   // exact value into f32, clamped into f16
+    int32_t* f32;
+    uint16_t* f16;
+    int w, h, x0, z;
+    __device__ bool active(int c) const { return x0 + c < w; }
+    __device__ void operator()(int c, int q, int v) const
+    {
+        const uint32_t i = ((uint32_t)z * h + q) * (uint32_t)w + x0 + c;
+        f32[i] = v;
+        f16[i] = (uint16_t)min((uint32_t)v, 0xffffu);
+    }
+};
+
 struct FinalOut {   // int32 + float at cell index base(q) + x (< 2^31 on this path)
     int32_t* d2;
     float* fl;
@@ -713,23 +741,31 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
     }
 }
 
-// the same over wide tiles (kWideCols lines, one mask word): packed scan,
-// and if that fails, each half again exactly (x pass into 32-bit words, then
-// the envelope).  w even, f2 8-byte aligned.
+// the same over wide tiles (kWideCols lines, one mask word): packed scan
+// into the 16-bit field, and if that fails, each half again exactly (x pass
+// into 32-bit words, then the envelope) into both fields.  flags[z * xt +
+// jt] records which field the z pass reads for this tile: 0 the 16-bit one
+// (every value below 0xffff), 1 the exact one, 2 none (no site: every value
+// is kEdtInf).  w even, f16 4-byte aligned.
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) k_edt3d_xy_wide(int w, int h, int d, const uint32_t* __restrict__ mask,
-                                                            uint32_t invert, int32_t* __restrict__ f2)
+                                                            uint32_t invert, uint16_t* __restrict__ f16,
+                                                            int32_t* __restrict__ f32,
+                                                            uint8_t* __restrict__ flags)
 {
     __shared__ __align__(16) uint32_t fs[kWords];
     __shared__ __align__(16) uint32_t bm[kBmWords];
     __shared__ int abort_flag;
     if (threadIdx.x == 0) abort_flag = 0;
+    const int xt = (w + kWideCols - 1) / kWideCols;
     int z, jt;
-    xy_tile(d, (w + kWideCols - 1) / kWideCols, z, jt);
+    xy_tile(d, xt, z, jt);
     const int x0 = jt * kWideCols;
     const bool any = xpass<true>(w, h, z, x0, mask, invert, fs, threadIdx.x);
-    FieldOut o{f2, w, h, x0, z};
+    Field16Out o{f16, w, h, x0, z};
+    uint8_t flag = 0;
     if (!any) {
         no_sites_pairs(h, o);
+        flag = 2;
     } else if (!scan_pairs(h, fs, o, &abort_flag)) {
         Bits bits{bm};
         for (int half = 0; half < 2 && x0 + kCols * half < w; half++) {
@@ -738,29 +774,28 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
             int tid = threadIdx.x;
             asm volatile("" : "+v"(tid));
             __syncthreads();
-            FieldOut oh{f2, w, h, x0 + kCols * half, z};
+            FieldBothOut oh{f32, f16, w, h, x0 + kCols * half, z};
             xpass<false>(w, h, z, x0 + kCols * half, mask, invert, fs, tid);
             envelope(h, fs, oh, bits, tid);
         }
+        flag = 1;
     }
+    if (threadIdx.x == 0) flags[z * xt + jt] = flag;
 }
 
 // an f2 sample (< 2^20, or kEdtInf) as an LDS word
 __device__ __forceinline__ uint32_t packed(int v) { return v == kEdtInf ? kNoSite : (uint32_t)v; }
 
-// loads the f2 tile of row y, columns x0 .. x0 + valid - 1 (of kCols, or of
-// kWideCols two to a word as clamped 16-bit values when PACKED), along z
-// into LDS; columns past valid hold kNoSite (0 when PACKED).  Returns
-// whether the tile has a site.
-template <bool PACKED>
-__device__ __forceinline__ bool zload(int w, int h, int d, int y, int x0, int valid, const int32_t* __restrict__ f2,
-                      uint32_t* fs, int tid)
+// loads the int32 f2 tile of row y, columns x0 .. x0 + valid - 1 of kCols,
+// along z into LDS; columns past valid hold kNoSite.  Returns whether the
+// tile has a site.
+__device__ __forceinline__ bool zload(int w, int h, int d, int y, int x0, int valid,
+                                      const int32_t* __restrict__ f2, uint32_t* fs, int tid)
 {
-    constexpr int kTileCols = PACKED ? kWideCols : kCols;
     int has_site = 0;
-    if (valid == kTileCols && d == kMaxN && (w & 3) == 0) {
+    if (valid == kCols && d == kMaxN && (w & 3) == 0) {
         // full tile: 16-byte loads, all in flight before the first LDS store
-        constexpr int kQuads = kTileCols / 4;
+        constexpr int kQuads = kCols / 4;
         constexpr int kIter = kMaxN * kQuads / kThreads;
         int4 v[kIter];
 #pragma unroll
@@ -772,25 +807,74 @@ __device__ __forceinline__ bool zload(int w, int h, int d, int y, int x0, int va
         for (int k = 0; k < kIter; k++) {
             const int e = tid + k * kThreads;
             has_site |= (v[k].x & v[k].y & v[k].z & v[k].w) != kEdtInf;
-            if (PACKED) {
-                const uint2 q = make_uint2(clamp16(v[k].x) | clamp16(v[k].y) << 16,
-                                           clamp16(v[k].z) | clamp16(v[k].w) << 16);
-                *(uint2*)(fs + (e / kQuads) * kPairs + (e % kQuads) * 2) = q;
-            } else {
-                const uint4 q = make_uint4(packed(v[k].x), packed(v[k].y), packed(v[k].z), packed(v[k].w));
-                *(uint4*)(fs + cell(e / kQuads, (e % kQuads) * 4)) = q;
-            }
+            const uint4 q = make_uint4(packed(v[k].x), packed(v[k].y), packed(v[k].z), packed(v[k].w));
+            *(uint4*)(fs + cell(e / kQuads, (e % kQuads) * 4)) = q;
         }
     } else {
-        for (int e = tid; e < d * kTileCols; e += kThreads) {
-            const int z = e / kTileCols, c = e % kTileCols;
+        for (int e = tid; e < d * kCols; e += kThreads) {
+            const int z = e / kCols, c = e % kCols;
             const int v = c < valid ? f2[((size_t)z * h + y) * w + x0 + c] : kEdtInf;
             has_site |= v != kEdtInf;
-            if (PACKED) ((uint16_t*)fs)[z * kTileCols + c] = c < valid ? (uint16_t)clamp16(v) : 0;
-            else fs[cell(z, c)] = packed(v);
+            fs[cell(z, c)] = packed(v);
         }
     }
     return __syncthreads_or(has_site);
+}
+
+// loads the 16-bit field of row y, columns x0 .. x0 + valid - 1 of a wide
+// tile, along z into LDS, two columns a word (0 past valid).  Returns whether
+// some value is below 0xffff.
+__device__ __forceinline__ bool zload16(int w, int h, int d, int y, int x0, int valid,
+                                        const uint16_t* __restrict__ f16, uint32_t* fs, int tid)
+{
+    int known = 0;
+    if (valid == kWideCols && d == kMaxN && (w & 7) == 0) {
+        // full tile: 16-byte loads (8 columns), all in flight before the first LDS store
+        constexpr int kOcts = kWideCols / 8;
+        constexpr int kIter = kMaxN * kOcts / kThreads;
+        uint4 v[kIter];
+#pragma unroll
+        for (int k = 0; k < kIter; k++) {
+            const int e = tid + k * kThreads;
+            v[k] = *(const uint4*)(f16 + ((size_t)(e / kOcts) * h + y) * w + x0 + (e % kOcts) * 8);
+        }
+#pragma unroll
+        for (int k = 0; k < kIter; k++) {
+            const int e = tid + k * kThreads;
+            known |= (v[k].x & v[k].y & v[k].z & v[k].w) != 0xffffffffu;
+            *(uint4*)(fs + (e / kOcts) * kPairs + (e % kOcts) * 4) = v[k];
+        }
+    } else {
+        for (int e = tid; e < d * kWideCols; e += kThreads) {
+            const int z = e / kWideCols, c = e % kWideCols;
+            const uint16_t v = c < valid ? f16[((size_t)z * h + y) * w + x0 + c] : (uint16_t)0;
+            known |= c < valid && v != 0xffff;
+            ((uint16_t*)fs)[z * kWideCols + c] = v;
+        }
+    }
+    return __syncthreads_or(known);
+}
+
+// the exact field of row y, columns x0 .. x0 + valid - 1 of kCols, along z
+// into LDS, each slice's values from the field its flag names (x tile jt)
+__device__ __forceinline__ void zload_exact(int w, int h, int d, int y, int x0, int valid, int jt,
+                                            const uint16_t* __restrict__ f16,
+                                            const int32_t* __restrict__ f32,
+                                            const uint8_t* __restrict__ flags, uint32_t* fs, int tid)
+{
+    const int xt = (w + kWideCols - 1) / kWideCols;
+    for (int e = tid; e < d * kCols; e += kThreads) {
+        const int z = e / kCols, c = e % kCols;
+        int v = kEdtInf;
+        if (c < valid) {
+            const size_t i = ((size_t)z * h + y) * w + x0 + c;
+            const uint8_t flag = flags[z * xt + jt];
+            if (flag == 0) v = f16[i];
+            else if (flag == 1) v = f32[i];
+        }
+        fs[cell(z, c)] = packed(v);
+    }
+    __syncthreads();
 }
 
 // tile (y, x0) of the z pass; blocks congruent mod 8 share an XCD, and each
@@ -816,16 +900,19 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
     int y, x0;
     z_tile((w + kCols - 1) / kCols, kCols, y, x0);
     const int valid = min(kCols, w - x0);
-    const bool any = zload<false>(w, h, d, y, x0, valid, f2, fs, threadIdx.x);
+    const bool any = zload(w, h, d, y, x0, valid, f2, fs, threadIdx.x);
     Bits bits{bm};
     FinalOut o{d2, fl, (uint32_t)(w * h), (uint32_t)(y * w + x0), valid};
     if (!any) no_sites(d, o);
     else if (!scan(d, fs, o, &abort_flag)) envelope(d, fs, o, bits, threadIdx.x);
 }
 
-// the same over wide tiles: packed scan, and if that fails, each half again
-// exactly.  w even, outputs 8-byte aligned.
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) k_edt3d_z_wide(int w, int h, int d, const int32_t* __restrict__ f2,
+// the same over wide tiles: packed scan of the 16-bit field, and if that
+// fails (or no value of the tile is known below 0xffff), each half again
+// exactly from the fields the flags name.  w even, outputs 8-byte aligned.
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) k_edt3d_z_wide(int w, int h, int d, const uint16_t* __restrict__ f16,
+                                                           const int32_t* __restrict__ f32,
+                                                           const uint8_t* __restrict__ flags,
                                                            int32_t* __restrict__ d2, float* __restrict__ fl)
 {
     __shared__ __align__(16) uint32_t fs[kWords];
@@ -835,11 +922,9 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
     int y, x0;
     z_tile((w + kWideCols - 1) / kWideCols, kWideCols, y, x0);
     const int valid = min(kWideCols, w - x0);
-    const bool any = zload<true>(w, h, d, y, x0, valid, f2, fs, threadIdx.x);
+    const bool known = zload16(w, h, d, y, x0, valid, f16, fs, threadIdx.x);
     FinalOut o{d2, fl, (uint32_t)(w * h), (uint32_t)(y * w + x0), valid};
-    if (!any) {
-        no_sites_pairs(d, o);
-    } else if (!scan_pairs(d, fs, o, &abort_flag)) {
+    if (!known || !scan_pairs(d, fs, o, &abort_flag)) {
         Bits bits{bm};
         for (int half = 0; half < 2 && kCols * half < valid; half++) {
             int tid = threadIdx.x;   // opaque, as in k_edt3d_xy_wide
@@ -847,7 +932,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
             __syncthreads();
             const int xh = x0 + kCols * half, vh = min(kCols, valid - kCols * half);
             FinalOut oh{d2, fl, (uint32_t)(w * h), (uint32_t)(y * w + xh), vh};
-            zload<false>(w, h, d, y, xh, vh, f2, fs, tid);
+            zload_exact(w, h, d, y, xh, vh, x0 / kWideCols, f16, f32, flags, fs, tid);
             envelope(d, fs, oh, bits, tid);
         }
     }
@@ -857,7 +942,14 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
 // every dimension <= 512 keeps each F below 2^20 and each product in int32
 bool edt_tile_fits(int w, int h, int d) { return w <= kMaxN && h <= kMaxN && d <= kMaxN; }
 
-size_t edt_tile_workspace(long long cells) { return ((size_t)cells * 4 + 255) & ~(size_t)255; }
+static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// 3D: the exact int32 field, the 16-bit field and the wide tiles' flags
+size_t edt_tile_workspace_3d(int w, int h, int d)
+{
+    const size_t cells = (size_t)w * h * d;
+    return align256(cells * 4) + align256(cells * 2) + align256((size_t)d * ((w + kWideCols - 1) / kWideCols));
+}
 
 void launch_edt_tile_2d(int w, int h, const uint32_t* mask, int invert, int32_t* d2, float* fl,
                         hipStream_t s)
@@ -872,18 +964,22 @@ void launch_edt_tile_3d(int w, int h, int d, const uint32_t* mask, int invert, i
                         float* fl, void* work, hipStream_t s)
 {
     int32_t* f2 = (int32_t*)work;
-    // wide tiles store column pairs with 8-byte stores
-    if ((w & 1) == 0 && (((uintptr_t)d2 | (uintptr_t)fl | (uintptr_t)f2) & 7) == 0) {
+    // wide tiles store column pairs with 8-byte stores and read the 16-bit
+    // field with 16-byte loads
+    if ((w & 1) == 0 && (((uintptr_t)d2 | (uintptr_t)fl) & 7) == 0 && ((uintptr_t)work & 15) == 0) {
+        const size_t cells = (size_t)w * h * d;
         const int xt = (w + kWideCols - 1) / kWideCols;
+        uint16_t* f16 = (uint16_t*)((char*)work + align256(cells * 4));
+        uint8_t* flags = (uint8_t*)((char*)f16 + align256(cells * 2));
         {
             ProfileScope p("edt3d_xy", s);
             hipLaunchKernelGGL(k_edt3d_xy_wide, dim3((unsigned)(xt * d)), dim3(kThreads), 0, s, w,
-                               h, d, mask, invert ? 0xffffffffu : 0u, f2);
+                               h, d, mask, invert ? 0xffffffffu : 0u, f16, f2, flags);
         }
         {
             ProfileScope p("edt3d_z", s);
             hipLaunchKernelGGL(k_edt3d_z_wide, dim3((unsigned)(xt * h)), dim3(kThreads), 0, s, w, h,
-                               d, f2, d2, fl);
+                               d, f16, f2, flags, d2, fl);
         }
         return;
     }
