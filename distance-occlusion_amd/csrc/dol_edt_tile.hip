@@ -625,18 +625,41 @@ __device__ void no_sites_pairs(int n, Out& out)
 
 using namespace tile;
 
+// leading / trailing zero counts that give 0xffffffff for 0: what
+// v_ffbh_u32 / v_ffbl_b32 return (written out, so that no select for the
+// zero case is added around them)
+__device__ __forceinline__ uint32_t clz_or_max(uint32_t x)
+{
+    uint32_t r;
+    asm("v_ffbh_u32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+__device__ __forceinline__ uint32_t ctz_or_max(uint32_t x)
+{
+    uint32_t r;
+    asm("v_ffbl_b32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
 // x distances into LDS of the columns x0 .. of slice z (16 columns, x0 a
 // multiple of 16; or, PACKED, the kWideCols columns of mask word x0 / 32 as
 // clamped 16-bit values two to a word, 0 past the volume's edge), one thread
-// per row; the whole row (<= 16 words) is loaded at once so the loads
-// overlap.  Without PACKED a column with no site in its row holds kNoSite.
-// Returns whether the slice has a site: a row with a set bit gives every
-// column one.
+// per row.  The row's mask words pass through its own LDS row first (<= 16
+// words, the size of its distance row), so the nearest set bits left and
+// right of word j are found from a bitmask of the non-empty words and two
+// reads.  A column's distance is then the least of: the bits of word j at or
+// left of it (leading zeros of the word shifted up), at or right of it
+// (trailing zeros shifted down), the set bit before word j, the one after it.
+// Without PACKED a column with no site in its row holds kNoSite.  Returns
+// whether the slice has a site: a row with a set bit gives every column one.
 template <bool PACKED>
 __device__ __forceinline__ bool xpass(int w, int h, int z, int x0, const uint32_t* __restrict__ mask,
                       uint32_t invert, uint32_t* fs, int tid)
 {
     constexpr int kN = PACKED ? kWideCols : kCols;
+    constexpr uint32_t kFar = 0x10000u;                 // no set bit on that side
+    constexpr uint32_t kCap = PACKED ? 256u : (uint32_t)kMaxN;
+    static_assert(kPairs == 16 && kCols == 16, "a row of LDS words holds the row's mask words");
     const int ws = mask_stride(w);
     const int j = x0 >> 5, boff = PACKED ? 0 : (x0 & 31);
     const int tail = w - 32 * (ws - 1);
@@ -644,6 +667,7 @@ __device__ __forceinline__ bool xpass(int w, int h, int z, int x0, const uint32_
     int has_site = 0;
     for (int y = tid; y < h; y += kThreads) {
         const uint32_t* row = mask + ((size_t)z * h + y) * ws;
+        uint32_t* srow = fs + y * 16;
         uint32_t words[16];
         if (ws == 16) {
             const uint4* r4 = (const uint4*)row;
@@ -656,35 +680,46 @@ __device__ __forceinline__ bool xpass(int w, int h, int z, int x0, const uint32_
 #pragma unroll
             for (int k = 0; k < 16; k++) words[k] = k < ws ? row[k] : 0u;
         }
-        uint32_t wj = 0;
-        int L = -1, R = -1;
+        uint32_t nz = 0;   // bit k: word k has a set bit
 #pragma unroll
         for (int k = 0; k < 16; k++) {
-            uint32_t m = words[k] ^ invert;
+            uint32_t m = k < ws ? words[k] ^ invert : 0u;
             if (k == ws - 1) m &= last_valid;
-            if (k >= ws) m = 0u;
             words[k] = m;
-            if (k == j) wj = m;
-            if (k < j && m) L = 32 * k + hi_bit(m);
+            nz |= (m != 0u ? 1u : 0u) << k;
         }
 #pragma unroll
-        for (int k = 15; k >= 0; k--)
-            if (k > j && words[k]) R = 32 * k + lo_bit(words[k]);
-        has_site |= (wj != 0u || L >= 0 || R >= 0);
+        for (int k = 0; k < 16; k += 4)
+            *(uint4*)(srow + k) = make_uint4(words[k], words[k + 1], words[k + 2], words[k + 3]);
+        const uint32_t wj = srow[j];
+        const uint32_t below = nz & ((1u << j) - 1u), above = nz & ~((2u << j) - 1u);
+        uint32_t dL0 = kFar, dR0 = kFar;   // from bit 0 of word j to the set bit before / after it
+        if (below) {
+            const int kl = hi_bit(below);
+            dL0 = (uint32_t)(32 * (j - kl) - hi_bit(srow[kl]));
+        }
+        if (above) {
+            const int kr = lo_bit(above);
+            dR0 = (uint32_t)(32 * (kr - j) + lo_bit(srow[kr]));
+        }
+        has_site |= nz != 0u;
         uint32_t v[kN];
 #pragma unroll
         for (int t = 0; t < kN; t++) {
             const int ii = boff + t;   // bit in word j
-            const int x = 32 * j + ii;
-            const uint32_t lo = wj & bits_upto(ii);
-            const uint32_t hi = wj >> ii;
-            const int dl = lo ? ii - hi_bit(lo) : (L >= 0 ? x - L : kMaxN);
-            const int dr = hi ? lo_bit(hi) : (R >= 0 ? R - x : kMaxN);
-            const int dd = min(dl, dr);
-            if (PACKED) v[t] = x < w ? clamp16((uint32_t)(dd * dd)) : 0u;
-            else v[t] = dd >= kMaxN ? kNoSite : (uint32_t)(dd * dd);
+            const uint32_t a = clz_or_max(wj << (31 - ii));
+            const uint32_t b = ctz_or_max(wj >> ii);
+            const uint32_t e = min(min(dL0 + ii, dR0 - ii), kCap);
+            const uint32_t dd = min(a, min(b, e));
+            if (PACKED) v[t] = min(dd * dd, 0xffffu);
+            else v[t] = dd >= kCap ? kNoSite : dd * dd;
         }
-        uint4* out = (uint4*)(fs + y * (PACKED ? kPairs : kCols));
+        if (PACKED && 32 * j + kN > w) {   // the last tile of a row: columns past w hold 0
+#pragma unroll
+            for (int t = 0; t < kN; t++)
+                if (32 * j + t >= w) v[t] = 0u;
+        }
+        uint4* out = (uint4*)srow;
         if (PACKED) {
 #pragma unroll
             for (int t = 0; t < kN; t += 8)
