@@ -541,7 +541,7 @@ __device__ __forceinline__ bool scan_pairs(int n, const uint32_t* hs, Out& out, 
     constexpr int R = kLaneRows, G = kPairStepGroup;
     const int tid = threadIdx.x;
     const int i = tid % kPairs, c = 2 * i;
-    const int wr0 = (tid / 64) * kPairWaveRows;          // the wavefront's first row
+    const int wr0 = __builtin_amdgcn_readfirstlane((tid / 64) * kPairWaveRows);   // the wavefront's first row
     const int rr = ((tid % 64) / kPairs) * R;            // the lane's first row in it
     const bool active = out.active(c);
     for (int base = 0; base < n; base += kPairPassRows) {
@@ -553,19 +553,21 @@ __device__ __forceinline__ bool scan_pairs(int n, const uint32_t* hs, Out& out, 
         uint32_t best[R];
 #pragma unroll
         for (int r = 0; r < R; r++) best[r] = active && p + r < n ? hs[(p + r) * kPairs + i] : 0u;
+        // the loop is the wavefront's: it steps while any lane needs more
+        // (further steps never change a finished lane's result), so k and the
+        // squares are scalars and the loop needs no per-lane exits
         int k = 1;
         bool fail = false;
         for (;;) {
             uint32_t top = 0;
 #pragma unroll
             for (int r = 0; r < R; r++) top = max(top, max(best[r] & 0xffffu, best[r] >> 16));
-            if (k * k >= (int)top || k > reach) break;
+            const bool more = k * k < (int)top && k <= reach;
+            if (__builtin_amdgcn_ballot_w64(more) == 0) break;
             if (k > kPairScanCap) {
                 fail = true;
                 break;
             }
-            // k is the same in every active lane: keep it (and the squares) scalar
-            k = __builtin_amdgcn_readfirstlane(k);
             if (pw - (k + G - 1) >= 0 && pw + kPairWaveRows - 1 + k + G - 1 < n) {
                 // row p + r - k - t is lo[G - 1 - t + r], row p + r + k + t is hi[t + r]
                 const uint32_t* lo = hs + (p - k - (G - 1)) * kPairs + i;
@@ -740,9 +742,15 @@ __device__ __forceinline__ bool xpass(int w, int h, int z, int x0, const uint32_
 __device__ __forceinline__ void xy_tile(int d, int xt, int& z, int& jt)
 {
     const int bid = blockIdx.x;
-    const int group = bid / (8 * xt);
+    int group = bid / (8 * xt);
     const int gs = min(8, d - group * 8);
     const int inner = bid - group * 8 * xt;
+    // groups from both ends of the volume first: the slices near its faces
+    // tend to have the most empty space, the longest scans
+    if ((d & 7) == 0) {
+        const int ng = d >> 3;
+        group = (group & 1) ? ng - 1 - (group >> 1) : (group >> 1);
+    }
     z = group * 8 + inner % gs;
     jt = inner / gs;
 }
