@@ -535,6 +535,26 @@ static_assert((kPairScanCap + kPairStepGroup - 1) * (kPairScanCap + kPairStepGro
 constexpr int kPairWaveRows = kPairRowsPerWave * kLaneRows;
 constexpr int kPairPassRows = (kThreads / 64) * kPairWaveRows;
 
+// M consecutive rows row0 .. of column pair i; rows outside [0, n) read as
+// 0xffff (no candidate).  inside: all of them are inside for every lane of
+// the wavefront, so the reads take immediate offsets.
+template <int M>
+__device__ __forceinline__ void pair_rows(const uint32_t* hs, int row0, int n, int i, bool inside, uint32_t* v)
+{
+    if (inside) {
+        const uint32_t* src = hs + row0 * kPairs + i;
+#pragma unroll
+        for (int u = 0; u < M; u++) v[u] = src[u * kPairs];
+    } else {
+#pragma unroll
+        for (int u = 0; u < M; u++) {
+            const int q = row0 + u;
+            const uint32_t x = hs[min(max(q, 0), n - 1) * kPairs + i];
+            v[u] = q >= 0 && q < n ? x : 0xffffffffu;
+        }
+    }
+}
+
 template <class Out>
 __device__ __forceinline__ bool scan_pairs(int n, const uint32_t* hs, Out& out, int* abort_flag)
 {
@@ -568,34 +588,40 @@ __device__ __forceinline__ bool scan_pairs(int n, const uint32_t* hs, Out& out, 
                 fail = true;
                 break;
             }
-            if (pw - (k + G - 1) >= 0 && pw + kPairWaveRows - 1 + k + G - 1 < n) {
-                // row p + r - k - t is lo[G - 1 - t + r], row p + r + k + t is hi[t + r]
-                const uint32_t* lo = hs + (p - k - (G - 1)) * kPairs + i;
-                const uint32_t* hi = hs + (p + k) * kPairs + i;
-                uint32_t a[G + R - 1], b[G + R - 1];
+            // rows p + r - k - t (below) are a[G - 1 - t + r], rows p + r + k + t
+            // (above) b[t + r].  Near the ends of the line a side may lie
+            // partly or wholly outside it: rows outside give no candidate.
+            constexpr int M = G + R - 1;
+            const bool lo_none = pw + kPairWaveRows - 1 - k < 0;
+            const bool hi_none = pw + k >= n;
+            const bool lo_all = pw - k - (G - 1) >= 0;
+            const bool hi_all = pw + kPairWaveRows - 1 + k + G - 1 < n;
+            uint32_t a[M], b[M];
+            if (lo_none) {
+                pair_rows<M>(hs, p + k, n, i, hi_all, b);
 #pragma unroll
-                for (int u = 0; u < G + R - 1; u++) {
-                    a[u] = lo[u * kPairs];
-                    b[u] = hi[u * kPairs];
+                for (int t = 0; t < G; t++) {
+                    const uint32_t sq = (uint32_t)((k + t) * (k + t)) * 0x10001u;
+#pragma unroll
+                    for (int r = 0; r < R; r++) best[r] = pk_min(best[r], pk_add_sat(b[t + r], sq));
                 }
+            } else if (hi_none) {
+                pair_rows<M>(hs, p - k - (G - 1), n, i, lo_all, a);
+#pragma unroll
+                for (int t = 0; t < G; t++) {
+                    const uint32_t sq = (uint32_t)((k + t) * (k + t)) * 0x10001u;
+#pragma unroll
+                    for (int r = 0; r < R; r++) best[r] = pk_min(best[r], pk_add_sat(a[G - 1 - t + r], sq));
+                }
+            } else {
+                pair_rows<M>(hs, p - k - (G - 1), n, i, lo_all, a);
+                pair_rows<M>(hs, p + k, n, i, hi_all, b);
 #pragma unroll
                 for (int t = 0; t < G; t++) {
                     const uint32_t sq = (uint32_t)((k + t) * (k + t)) * 0x10001u;
 #pragma unroll
                     for (int r = 0; r < R; r++)
                         best[r] = pk_min(best[r], pk_add_sat(pk_min(a[G - 1 - t + r], b[t + r]), sq));
-                }
-            } else {
-#pragma unroll
-                for (int t = 0; t < G; t++) {
-                    const int kk = k + t;
-                    const uint32_t sq = (uint32_t)(kk * kk) * 0x10001u;
-#pragma unroll
-                    for (int r = 0; r < R; r++) {
-                        const uint32_t a = hs[max(p + r - kk, 0) * kPairs + i];
-                        const uint32_t b = hs[min(p + r + kk, n - 1) * kPairs + i];
-                        best[r] = pk_min(best[r], pk_add_sat(pk_min(a, b), sq));
-                    }
                 }
             }
             k += G;
