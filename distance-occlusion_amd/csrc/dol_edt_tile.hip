@@ -389,13 +389,17 @@ struct FinalOut {   // int32 + float at cell index base(q) + x (< 2^31 on this p
         if (d2) d2[i] = v;
         if (fl) fl[i] = sqrt_d2(v);
     }
-    // columns c, c + 1 (c even; base and row_stride even, outputs 8-byte aligned)
+    // columns c, c + 1 (c even; base and row_stride even, outputs 8-byte
+    // aligned).  Nontemporal: the outputs are not read again here, and
+    // these stores leave the caches to the z pass's input.
     __device__ void pair(int c, int q, int v0, int v1) const
     {
+        typedef int i32x2 __attribute__((ext_vector_type(2)));
+        typedef float f32x2 __attribute__((ext_vector_type(2)));
         const uint32_t i = base + (uint32_t)q * row_stride + (uint32_t)c;
         if (c + 1 < valid) {
-            if (d2) *(int2*)(d2 + i) = make_int2(v0, v1);
-            if (fl) *(float2*)(fl + i) = make_float2(sqrt_d2(v0), sqrt_d2(v1));
+            if (d2) __builtin_nontemporal_store((i32x2){v0, v1}, (i32x2*)(d2 + i));
+            if (fl) __builtin_nontemporal_store((f32x2){sqrt_d2(v0), sqrt_d2(v1)}, (f32x2*)(fl + i));
         } else if (c < valid) {
             (*this)(c, q, v0);
         }
@@ -519,7 +523,7 @@ __device__ __forceinline__ uint32_t clamp16(uint32_t v) { return min(v, kClamp);
 // reads kStepGroup + kLaneRows - 1 rows on either side instead of
 // kStepGroup * kLaneRows.
 #ifndef DOL_LANE_ROWS
-#define DOL_LANE_ROWS 4
+#define DOL_LANE_ROWS 3
 #endif
 #ifndef DOL_PAIR_GROUP
 #define DOL_PAIR_GROUP 8
