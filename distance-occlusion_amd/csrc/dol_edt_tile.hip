@@ -1,34 +1,31 @@
 // dol_edt_tile.hip -- LDS-tiled exact squared EDT for lines of up to 512
-// samples (the 512^3 volume and images up to 512 high).
+// samples (volumes up to 512^3, images up to 512 high).
 //
-// The separable transform needs, along each line, the lower envelope of the
-// parabolas  F(i) - 2 q i + q^2  with F(i) = f(i) + i^2 (f = squared distance
-// so far, or "none").  Its surviving sites are the vertices of the lower
-// convex hull of the points (i, F(i)); on realistic volumes most sites
-// survive, so a per-line stack would be as long as the line.  Instead a
-// workgroup holds a tile of 32 lines in LDS and splits each line into bands
-// of 32 sites:
-//   1. each thread builds the hull of one band, its stack being a 32-bit
-//      register whose set bits are the surviving sites;
-//   2. adjacent band hulls are merged in a tree of log2(bands) rounds by
-//      walking to their common bridge (x-separated convex chains), clearing
-//      the survivor bits between the bridge ends in an LDS bitmask;
-//   3. every surviving site v gets the start of its interval of ownership,
-//      l_v = ceil((F(v) - F(u)) / (2 (v - u))) against its hull predecessor
-//      u, and, if that interval is not empty, marks its first sample (so no
-//      sample is marked twice); the owner of q is then the last mark at or
-//      before q, a running maximum along the line, and each output is
-//      computed without data-dependent loops.
-// All comparisons are exact integer cross-multiplications.  Values: with
-// lines <= 512 every F is below 2^20 and every product below 2^29, so int32
-// arithmetic is exact.
+// Separable form (Felzenszwalb): per line, d(p) = min_q f(q) + (p - q)^2,
+// with f the squared distance so far (x distance of the row's bits first).
+// A workgroup holds a tile of lines in LDS and computes the transform of
+// each line in one of two exact ways:
+//   scan      read f at p -/+ k for k = 1, 2, ... and stop once k^2 reaches
+//             the best value so far; the steps are about the distance
+//             computed, so on volumes with sites near every sample this is
+//             far cheaper than building an envelope.  Tried first.
+//   envelope  the lower envelope of the parabolas F(i) - 2 q i + q^2
+//             (F = f + i^2): band hulls in register stacks, a merge tree of
+//             bridge walks over an LDS survivor bitmask, then interval-start
+//             marks and a running maximum for the owners.  Used for a tile
+//             whose scan would need more than a capped number of steps.
+// Wide tiles (32 lines, even widths) scan two lines at once as clamped
+// 16-bit values packed in one word; a result below 0xffff is exact, and a
+// tile with a larger or capped result is redone exactly, half a tile at a
+// time, by the 16-line envelope.  Other widths use 16-line tiles throughout.
 //
-// edt3d_xy: one workgroup per (slice z, mask word j): the x pass is computed
-// straight from the bit rows into LDS (distance to the nearest set bit in
-// each row), then the y envelope; writes the int32 field f2 (or, for 2D, the
-// final int32 and float outputs).
-// edt3d_z: one workgroup per (row y, 32 columns): loads the f2 column tile
-// along z into LDS, z envelope, writes int32 and float distances.
+// edt3d_xy: one workgroup per (slice z, tile of x): the x pass straight from
+// the bit rows into LDS, then the y transform; writes the y field (for 2D,
+// the final int32 and float outputs).  The wide kernel hands the field to
+// the z pass as 16-bit values, plus an exact int32 copy for the rare tiles
+// whose values reach 0xffff and a per-tile flag naming the field to read.
+// edt3d_z: one workgroup per (row y, tile of x): the field's column tile
+// along z into LDS, the z transform, int32 and float distances out.
 #include "dol_common.hpp"
 
 #include <cfloat>
