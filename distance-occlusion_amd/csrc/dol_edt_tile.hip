@@ -994,6 +994,17 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
     const int valid = min(kWideCols, w - x0);
     const bool known = zload16(w, h, d, y, x0, valid, f16, fs, threadIdx.x);
     FinalOut o{d2, fl, (uint32_t)(w * h), (uint32_t)(y * w + x0), valid};
+    if (!known) {
+        // every value reads 0xffff: if no slice of the tile has a site, no
+        // line has one
+        const int xt = (w + kWideCols - 1) / kWideCols, jt = x0 / kWideCols;
+        int exact = 0;
+        for (int z = threadIdx.x; z < d; z += kThreads) exact |= flags[z * xt + jt] != 2;
+        if (!__syncthreads_or(exact)) {
+            no_sites_pairs(d, o);
+            return;
+        }
+    }
     if (!known || !scan_pairs(d, fs, o, &abort_flag)) {
         Bits bits{bm};
         for (int half = 0; half < 2 && kCols * half < valid; half++) {
