@@ -535,6 +535,43 @@ static_assert((kPairScanCap + kPairStepGroup - 1) * (kPairScanCap + kPairStepGro
               "16-bit squares");
 constexpr int kPairWaveRows = kPairRowsPerWave * kLaneRows;
 constexpr int kPairPassRows = (kThreads / 64) * kPairWaveRows;
+static_assert(DOL_PAIR_GROUP + kPairWaveRows - 1 + 7 <= 32, "a side's rows lie in one 32-row window");
+
+// Window minima for skipping: word b * kPairs + i holds the packed minima of
+// column pair i over rows 8b .. 8b + 31 (rows past n count as no
+// candidate).  A group's rows on one side span at most kPairStepGroup +
+// kPairWaveRows - 1 rows from a start within 7 rows of some 8b, so one word
+// bounds them all from below.
+constexpr int kWinBlocks = kMaxN / 8;
+constexpr int kWinWords = kWinBlocks * kPairs;
+static_assert(kWinWords % kThreads == 0, "window minima split evenly");
+__device__ __forceinline__ void window_minima(int n, const uint32_t* hs, uint32_t* mwin, int tid)
+{
+    constexpr int kPer = kWinWords / kThreads;
+    for (int e = tid; e < kWinWords; e += kThreads) {
+        const int b = e / kPairs, i = e % kPairs;
+        uint32_t m = 0xffffffffu;
+#pragma unroll
+        for (int r = 0; r < 8; r++)
+            if (8 * b + r < n) m = pk_min(m, hs[(8 * b + r) * kPairs + i]);
+        mwin[e] = m;
+    }
+    __syncthreads();
+    uint32_t v[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+        const int e = tid + j * kThreads, b = e / kPairs, i = e % kPairs;
+        uint32_t m = mwin[e];
+#pragma unroll
+        for (int s = 1; s < 4; s++)
+            if (b + s < kWinBlocks) m = pk_min(m, mwin[(b + s) * kPairs + i]);
+        v[j] = m;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPer; j++) mwin[tid + j * kThreads] = v[j];
+    __syncthreads();
+}
 
 // M consecutive rows row0 .. of column pair i; rows outside [0, n) read as
 // 0xffff (no candidate).  inside: all of them are inside for every lane of
@@ -557,7 +594,8 @@ __device__ __forceinline__ void pair_rows(const uint32_t* hs, int row0, int n, i
 }
 
 template <class Out>
-__device__ __forceinline__ bool scan_pairs(int n, const uint32_t* hs, Out& out, int* abort_flag)
+__device__ __forceinline__ bool scan_pairs(int n, const uint32_t* hs, const uint32_t* mwin, Out& out,
+                                           int* abort_flag)
 {
     constexpr int R = kLaneRows, G = kPairStepGroup;
     const int tid = threadIdx.x;
@@ -593,12 +631,27 @@ __device__ __forceinline__ bool scan_pairs(int n, const uint32_t* hs, Out& out, 
             // (above) b[t + r].  Near the ends of the line a side may lie
             // partly or wholly outside it: rows outside give no candidate.
             constexpr int M = G + R - 1;
-            const bool lo_none = pw + kPairWaveRows - 1 - k < 0;
-            const bool hi_none = pw + k >= n;
             const bool lo_all = pw - k - (G - 1) >= 0;
             const bool hi_all = pw + kPairWaveRows - 1 + k + G - 1 < n;
+            // a side is skipped when it lies outside the line, or when the
+            // least value in its rows plus k^2 reaches every lane's largest
+            // best so far (no candidate there can improve an output)
+            const uint32_t kk = (uint32_t)(k * k);
+            const uint32_t room = top > kk ? top - kk : 0u;
+            bool lo_skip = pw + kPairWaveRows - 1 - k < 0;
+            bool hi_skip = pw + k >= n;
+            if (!lo_skip) {
+                const uint32_t m = mwin[(max(pw - k - (G - 1), 0) >> 3) * kPairs + i];
+                lo_skip = __builtin_amdgcn_ballot_w64(min(m & 0xffffu, m >> 16) < room) == 0;
+            }
+            if (!hi_skip) {
+                const uint32_t m = mwin[((pw + k) >> 3) * kPairs + i];
+                hi_skip = __builtin_amdgcn_ballot_w64(min(m & 0xffffu, m >> 16) < room) == 0;
+            }
             uint32_t a[M], b[M];
-            if (lo_none) {
+            if (lo_skip && hi_skip) {
+                // nothing in this group
+            } else if (lo_skip) {
                 pair_rows<M>(hs, p + k, n, i, hi_all, b);
 #pragma unroll
                 for (int t = 0; t < G; t++) {
@@ -606,7 +659,7 @@ __device__ __forceinline__ bool scan_pairs(int n, const uint32_t* hs, Out& out, 
 #pragma unroll
                     for (int r = 0; r < R; r++) best[r] = pk_min(best[r], pk_add_sat(b[t + r], sq));
                 }
-            } else if (hi_none) {
+            } else if (hi_skip) {
                 pair_rows<M>(hs, p - k - (G - 1), n, i, lo_all, a);
 #pragma unroll
                 for (int t = 0; t < G; t++) {
@@ -823,7 +876,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
                                                             uint8_t* __restrict__ flags)
 {
     __shared__ __align__(16) uint32_t fs[kWords];
-    __shared__ __align__(16) uint32_t bm[kBmWords];
+    __shared__ __align__(16) uint32_t bm[kBmWords > kWinWords ? kBmWords : kWinWords];   // window minima, then the envelope's bitmask
     __shared__ int abort_flag;
     if (threadIdx.x == 0) abort_flag = 0;
     const int xt = (w + kWideCols - 1) / kWideCols;
@@ -836,7 +889,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
     if (!any) {
         no_sites_pairs(h, o);
         flag = 2;
-    } else if (!scan_pairs(h, fs, o, &abort_flag)) {
+    } else if (window_minima(h, fs, bm, threadIdx.x), !scan_pairs(h, fs, bm, o, &abort_flag)) {
         Bits bits{bm};
         for (int half = 0; half < 2 && x0 + kCols * half < w; half++) {
             // the thread index is opaque to the compiler here, so nothing of
@@ -986,7 +1039,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
                                                            int32_t* __restrict__ d2, float* __restrict__ fl)
 {
     __shared__ __align__(16) uint32_t fs[kWords];
-    __shared__ __align__(16) uint32_t bm[kBmWords];
+    __shared__ __align__(16) uint32_t bm[kBmWords > kWinWords ? kBmWords : kWinWords];   // window minima, then the envelope's bitmask
     __shared__ int abort_flag;
     if (threadIdx.x == 0) abort_flag = 0;
     int y, x0;
@@ -1005,7 +1058,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
             return;
         }
     }
-    if (!known || !scan_pairs(d, fs, o, &abort_flag)) {
+    if (!known || (window_minima(d, fs, bm, threadIdx.x), !scan_pairs(d, fs, bm, o, &abort_flag))) {
         Bits bits{bm};
         for (int half = 0; half < 2 && kCols * half < valid; half++) {
             int tid = threadIdx.x;   // opaque, as in k_edt3d_xy_wide
