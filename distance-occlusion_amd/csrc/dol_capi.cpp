@@ -51,6 +51,10 @@ void launch_sdf_rg_rgba(int w, int h, const uint32_t* mask, const int16_t* delta
                         uint8_t* out, hipStream_t s);
 void launch_signed_edt(size_t n, const int32_t* dout, const int32_t* din, float* sdf,
                        hipStream_t s);
+void launch_two_field_2d(int w, int h, const int16_t* d0, const int16_t* d1, float max_len,
+                         int type, uint8_t* out, hipStream_t s);
+void launch_two_field_3d(int w, int h, int d, const int16_t* d0, const int16_t* d1, float max_len,
+                         int type, uint8_t* out, hipStream_t s);
 
 // ------------------------------------------------------------------ errors
 
@@ -911,6 +915,169 @@ int dolhip_signed_border_3d_dev(int w, int h, int d, const uint32_t* mask, int16
     launch_border_3d(w, h, d, mask, delta, as_stream(stream));
     jfa3d_run(w, h, d, delta, 2, (int16_t*)tmp, as_stream(stream));
     return finish(as_stream(stream), "signed_border_3d");
+}
+
+
+// ---- two-field signed images (the tool's -S mode,
+// DistanceOcclusionTool.cpp:1082-1124, 1197-1232)
+
+static int check_image_type(int type, float max_len)
+{
+    if (type < 0 || type > 3)
+        return fail(DOLHIP_ERR_INVALID_ARG, "image type must be 0 (sdf), 1 (sdf-delta), 2 (sdf-rg) or 3 (sdf-angles), got %d", type);
+    if (!(max_len > 0.0f)) return fail(DOLHIP_ERR_INVALID_ARG, "max_len must be > 0");
+    return DOLHIP_OK;
+}
+
+size_t dolhip_two_field_2d_workspace(int w, int h) { return dolhip_jump_flood_2d_workspace(w, h); }
+
+size_t dolhip_two_field_3d_workspace(int w, int h, int d) { return dolhip_jump_flood_3d_workspace(w, h, d); }
+
+int dolhip_two_field_deltas_2d_dev(int w, int h, const uint32_t* mask, int16_t* delta0,
+                                   int16_t* delta1, int cw, void* work, void* stream)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(delta0, "delta0"));
+    TRY(check_ptr(delta1, "delta1"));
+    TRY(check_cw(cw));
+    TRY(require_device());
+    void* tmp = nullptr;
+    TRY(resolve_work(work, dolhip_two_field_2d_workspace(w, h), &tmp));
+    const hipStream_t s = as_stream(stream);
+    launch_init_deltas_2d(w, h, mask, delta0, 0, s);
+    launch_init_deltas_2d(w, h, mask, delta1, 1, s);
+    jfa2d_schedule(w, h, delta0, cw, (int16_t*)tmp, s);
+    jfa2d_schedule(w, h, delta1, cw, (int16_t*)tmp, s);
+    return finish(s, "two_field_deltas_2d");
+}
+
+int dolhip_two_field_deltas_2d(int w, int h, const uint32_t* mask, int16_t* delta0,
+                               int16_t* delta1, int cw)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(delta0, "delta0"));
+    TRY(check_ptr(delta1, "delta1"));
+    TRY(check_cw(cw));
+    TRY(require_device());
+    const size_t bytes = (size_t)w * h * 4;
+    DevBuf m, a, b, t;
+    TRY(m.upload(mask, mask_bytes(w, h)));
+    TRY(a.alloc(bytes));
+    TRY(b.alloc(bytes));
+    TRY(t.alloc(dolhip_two_field_2d_workspace(w, h)));
+    TRY(dolhip_two_field_deltas_2d_dev(w, h, m.as<uint32_t>(), a.as<int16_t>(), b.as<int16_t>(), cw,
+                                       t.p, nullptr));
+    TRY(finish_sync("two_field_deltas_2d"));
+    TRY(a.download(delta0, bytes));
+    return b.download(delta1, bytes);
+}
+
+int dolhip_two_field_deltas_3d_dev(int w, int h, int d, const uint32_t* mask, int16_t* delta0,
+                                   int16_t* delta1, int cw, void* work, void* stream)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(delta0, "delta0"));
+    TRY(check_ptr(delta1, "delta1"));
+    TRY(check_cw(cw));
+    TRY(require_device());
+    void* tmp = nullptr;
+    TRY(resolve_work(work, dolhip_two_field_3d_workspace(w, h, d), &tmp));
+    const hipStream_t s = as_stream(stream);
+    launch_init_deltas_3d(w, h, d, mask, delta0, 0, s);
+    launch_init_deltas_3d(w, h, d, mask, delta1, 1, s);
+    jfa3d_run(w, h, d, delta0, cw, (int16_t*)tmp, s);
+    jfa3d_run(w, h, d, delta1, cw, (int16_t*)tmp, s);
+    return finish(s, "two_field_deltas_3d");
+}
+
+int dolhip_two_field_deltas_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta0,
+                               int16_t* delta1, int cw)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(delta0, "delta0"));
+    TRY(check_ptr(delta1, "delta1"));
+    TRY(check_cw(cw));
+    TRY(require_device());
+    const size_t bytes = (size_t)w * h * d * 6;
+    DevBuf m, a, b, t;
+    TRY(m.upload(mask, mask_bytes(w, (long long)h * d)));
+    TRY(a.alloc(bytes));
+    TRY(b.alloc(bytes));
+    TRY(t.alloc(dolhip_two_field_3d_workspace(w, h, d)));
+    TRY(dolhip_two_field_deltas_3d_dev(w, h, d, m.as<uint32_t>(), a.as<int16_t>(), b.as<int16_t>(),
+                                       cw, t.p, nullptr));
+    TRY(finish_sync("two_field_deltas_3d"));
+    TRY(a.download(delta0, bytes));
+    return b.download(delta1, bytes);
+}
+
+int dolhip_encode_two_field_2d_dev(int w, int h, const int16_t* delta0, const int16_t* delta1,
+                                   float max_len, int type, uint8_t* out, void* stream)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(delta0, "delta0"));
+    TRY(check_ptr(delta1, "delta1"));
+    TRY(check_ptr(out, "out"));
+    TRY(check_image_type(type, max_len));
+    TRY(require_device());
+    launch_two_field_2d(w, h, delta0, delta1, max_len, type, out, as_stream(stream));
+    return finish(as_stream(stream), "encode_two_field_2d");
+}
+
+int dolhip_encode_two_field_2d(int w, int h, const int16_t* delta0, const int16_t* delta1,
+                               float max_len, int type, uint8_t* out)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(delta0, "delta0"));
+    TRY(check_ptr(delta1, "delta1"));
+    TRY(check_ptr(out, "out"));
+    TRY(check_image_type(type, max_len));
+    TRY(require_device());
+    const size_t cells = (size_t)w * h, out_bytes = cells * (type == 0 ? 1 : 4);
+    DevBuf a, b, o;
+    TRY(a.upload(delta0, cells * 4));
+    TRY(b.upload(delta1, cells * 4));
+    TRY(o.alloc(out_bytes));
+    launch_two_field_2d(w, h, a.as<int16_t>(), b.as<int16_t>(), max_len, type, o.as<uint8_t>(), 0);
+    TRY(finish_sync("encode_two_field_2d"));
+    return o.download(out, out_bytes);
+}
+
+int dolhip_encode_two_field_3d_dev(int w, int h, int d, const int16_t* delta0,
+                                   const int16_t* delta1, float max_len, int type, uint8_t* out,
+                                   void* stream)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(delta0, "delta0"));
+    TRY(check_ptr(delta1, "delta1"));
+    TRY(check_ptr(out, "out"));
+    TRY(check_image_type(type, max_len));
+    TRY(require_device());
+    launch_two_field_3d(w, h, d, delta0, delta1, max_len, type, out, as_stream(stream));
+    return finish(as_stream(stream), "encode_two_field_3d");
+}
+
+int dolhip_encode_two_field_3d(int w, int h, int d, const int16_t* delta0, const int16_t* delta1,
+                               float max_len, int type, uint8_t* out)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(delta0, "delta0"));
+    TRY(check_ptr(delta1, "delta1"));
+    TRY(check_ptr(out, "out"));
+    TRY(check_image_type(type, max_len));
+    TRY(require_device());
+    const size_t cells = (size_t)w * h * d, out_bytes = cells * (type == 0 ? 1 : 4);
+    DevBuf a, b, o;
+    TRY(a.upload(delta0, cells * 6));
+    TRY(b.upload(delta1, cells * 6));
+    TRY(o.alloc(out_bytes));
+    launch_two_field_3d(w, h, d, a.as<int16_t>(), b.as<int16_t>(), max_len, type, o.as<uint8_t>(), 0);
+    TRY(finish_sync("encode_two_field_3d"));
+    return o.download(out, out_bytes);
 }
 
 }  // extern "C"

@@ -202,6 +202,103 @@ void launch_sdf_rg_rgba(int w, int h, const uint32_t* mask, const int16_t* delta
                        (const uint32_t*)delta, max_len, (uint32_t*)out);
 }
 
+// ---- two-field signed images: the tool's -S mode, exterior field d0 (seeds
+// at set cells) and interior field d1 (seeds at clear cells), written as
+// DistanceOcclusionTool.cpp:1029-1080 (2D) and 1126-1195 (3D).  type 0:
+// EncodeU8Signed((Dist(d0) - Dist(d1)) / maxLen), one byte a cell; types 1-3
+// are RGBA: 1 the summed deltas over maxLen, 2 EncodeU8 of each distance over
+// maxLen, 3 the summed deltas over Dist(d0) - Dist(d1).  Every quotient is a
+// single rounded division, as there.
+template <int TYPE>
+__global__ void __launch_bounds__(256) k_two_field_2d(size_t n, const uint32_t* __restrict__ d0,
+                                                      const uint32_t* __restrict__ d1, float max_len,
+                                                      void* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t a = d0[i], b = d1[i];
+    const float da = dist_of(len2(unpack2x(a), unpack2y(a)));
+    const float db = dist_of(len2(unpack2x(b), unpack2y(b)));
+    if (TYPE == 0) {
+        ((uint8_t*)out)[i] = encode_u8_signed(__fdiv_rn(__fsub_rn(da, db), max_len));
+        return;
+    }
+    const float sx = (float)(unpack2x(a) + unpack2x(b)), sy = (float)(unpack2y(a) + unpack2y(b));
+    uint32_t r, g;
+    if (TYPE == 1) {
+        r = encode_u8_signed(__fdiv_rn(sx, max_len));
+        g = encode_u8_signed(__fdiv_rn(sy, max_len));
+    } else if (TYPE == 2) {
+        r = encode_u8(__fdiv_rn(da, max_len));
+        g = encode_u8(__fdiv_rn(db, max_len));
+    } else {
+        const float dd = __fsub_rn(da, db);
+        r = encode_u8_signed(__fdiv_rn(sx, dd));
+        g = encode_u8_signed(__fdiv_rn(sy, dd));
+    }
+    ((uint32_t*)out)[i] = r | (g << 8) | (255u << 24);
+}
+
+// 3D: the same per slice, blue the z channel of type 1 and 3.  Type 3 reads
+// the first slice's deltas for every slice, as the tool does
+// (DistanceOcclusionTool.cpp:1178-1186 index deltas0/deltas1, not the slice's).
+template <int TYPE>
+__global__ void __launch_bounds__(256) k_two_field_3d(size_t n, size_t slice, const int16_t* __restrict__ d0,
+                                                      const int16_t* __restrict__ d1, float max_len,
+                                                      void* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t j = TYPE == 3 ? i % slice : i;
+    const int ax = d0[3 * j], ay = d0[3 * j + 1], az = d0[3 * j + 2];
+    const int bx = d1[3 * j], by = d1[3 * j + 1], bz = d1[3 * j + 2];
+    const float da = dist_of(len2(ax, ay, az)), db = dist_of(len2(bx, by, bz));
+    if (TYPE == 0) {
+        ((uint8_t*)out)[i] = encode_u8_signed(__fdiv_rn(__fsub_rn(da, db), max_len));
+        return;
+    }
+    uint32_t r, g, bl = 0;
+    if (TYPE == 2) {
+        r = encode_u8(__fdiv_rn(da, max_len));
+        g = encode_u8(__fdiv_rn(db, max_len));
+    } else {
+        const float q = TYPE == 1 ? max_len : __fsub_rn(da, db);
+        r = encode_u8_signed(__fdiv_rn((float)(ax + bx), q));
+        g = encode_u8_signed(__fdiv_rn((float)(ay + by), q));
+        bl = encode_u8_signed(__fdiv_rn((float)(az + bz), q));
+    }
+    ((uint32_t*)out)[i] = r | (g << 8) | (bl << 16) | (255u << 24);
+}
+
+void launch_two_field_2d(int w, int h, const int16_t* d0, const int16_t* d1, float max_len,
+                         int type, uint8_t* out, hipStream_t s)
+{
+    const size_t n = (size_t)w * h;
+    const dim3 grid(grid1d(n, 256)), block(256);
+    const uint32_t *a = (const uint32_t*)d0, *b = (const uint32_t*)d1;
+    ProfileScope p("encode_two_field_2d", s);
+    switch (type) {
+    case 0: hipLaunchKernelGGL(k_two_field_2d<0>, grid, block, 0, s, n, a, b, max_len, out); break;
+    case 1: hipLaunchKernelGGL(k_two_field_2d<1>, grid, block, 0, s, n, a, b, max_len, out); break;
+    case 2: hipLaunchKernelGGL(k_two_field_2d<2>, grid, block, 0, s, n, a, b, max_len, out); break;
+    default: hipLaunchKernelGGL(k_two_field_2d<3>, grid, block, 0, s, n, a, b, max_len, out); break;
+    }
+}
+
+void launch_two_field_3d(int w, int h, int d, const int16_t* d0, const int16_t* d1, float max_len,
+                         int type, uint8_t* out, hipStream_t s)
+{
+    const size_t slice = (size_t)w * h, n = slice * d;
+    const dim3 grid(grid1d(n, 256)), block(256);
+    ProfileScope p("encode_two_field_3d", s);
+    switch (type) {
+    case 0: hipLaunchKernelGGL(k_two_field_3d<0>, grid, block, 0, s, n, slice, d0, d1, max_len, out); break;
+    case 1: hipLaunchKernelGGL(k_two_field_3d<1>, grid, block, 0, s, n, slice, d0, d1, max_len, out); break;
+    case 2: hipLaunchKernelGGL(k_two_field_3d<2>, grid, block, 0, s, n, slice, d0, d1, max_len, out); break;
+    default: hipLaunchKernelGGL(k_two_field_3d<3>, grid, block, 0, s, n, slice, d0, d1, max_len, out); break;
+    }
+}
+
 // two exact fields -> signed distance sqrt(out) - sqrt(in)
 __global__ void __launch_bounds__(256) k_signed_edt(size_t n, const int32_t* __restrict__ dout,
                                                     const int32_t* __restrict__ din,

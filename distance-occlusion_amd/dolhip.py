@@ -93,6 +93,16 @@ _SIGS = {
     "dolhip_encode_sdf_rg_rgba_2d": ([_I, _I, _P, _P, _F, _P], _I),
     "dolhip_signed_border_2d_dev": ([_I, _I, _P, _P, _P, _P], _I),
     "dolhip_signed_border_3d_dev": ([_I, _I, _I, _P, _P, _P, _P], _I),
+    "dolhip_two_field_2d_workspace": ([_I, _I], _SZ),
+    "dolhip_two_field_3d_workspace": ([_I, _I, _I], _SZ),
+    "dolhip_two_field_deltas_2d": ([_I, _I, _P, _P, _P, _I], _I),
+    "dolhip_two_field_deltas_2d_dev": ([_I, _I, _P, _P, _P, _I, _P, _P], _I),
+    "dolhip_two_field_deltas_3d": ([_I, _I, _I, _P, _P, _P, _I], _I),
+    "dolhip_two_field_deltas_3d_dev": ([_I, _I, _I, _P, _P, _P, _I, _P, _P], _I),
+    "dolhip_encode_two_field_2d": ([_I, _I, _P, _P, _F, _I, _P], _I),
+    "dolhip_encode_two_field_2d_dev": ([_I, _I, _P, _P, _F, _I, _P, _P], _I),
+    "dolhip_encode_two_field_3d": ([_I, _I, _I, _P, _P, _F, _I, _P], _I),
+    "dolhip_encode_two_field_3d_dev": ([_I, _I, _I, _P, _P, _F, _I, _P, _P], _I),
     "dolhip_batch_create": ([_I, _P, _P, _P, _I, _I, _P], _I),
     "dolhip_batch_destroy": ([_P], _I),
     "dolhip_batch_bytes": ([_P], ctypes.c_int64),
@@ -309,6 +319,33 @@ class DolHip:
         out = np.empty((h, w, 4), dtype=np.uint8)
         self._call("dolhip_encode_sdf_rg_rgba_2d", w, h, _addr(np.ascontiguousarray(mask, np.uint32)),
                    _addr(np.ascontiguousarray(delta, np.int16)), max_len, _addr(out))
+        return out
+
+    def two_field_deltas_2d(self, w, h, mask, cw=1):
+        """exterior and interior JumpFlood fields of the tool's -S mode"""
+        d0 = np.empty((h, w, 2), dtype=np.int16)
+        d1 = np.empty((h, w, 2), dtype=np.int16)
+        self._call("dolhip_two_field_deltas_2d", w, h, _addr(np.ascontiguousarray(mask, np.uint32)),
+                   _addr(d0), _addr(d1), cw)
+        return d0, d1
+
+    def two_field_deltas_3d(self, w, h, d, mask, cw=1):
+        d0 = np.empty((d, h, w, 3), dtype=np.int16)
+        d1 = np.empty((d, h, w, 3), dtype=np.int16)
+        self._call("dolhip_two_field_deltas_3d", w, h, d, _addr(np.ascontiguousarray(mask, np.uint32)),
+                   _addr(d0), _addr(d1), cw)
+        return d0, d1
+
+    def encode_two_field_2d(self, w, h, d0, d1, max_len, image_type):
+        out = np.empty((h, w) if image_type == 0 else (h, w, 4), dtype=np.uint8)
+        self._call("dolhip_encode_two_field_2d", w, h, _addr(np.ascontiguousarray(d0, np.int16)),
+                   _addr(np.ascontiguousarray(d1, np.int16)), max_len, image_type, _addr(out))
+        return out
+
+    def encode_two_field_3d(self, w, h, d, d0, d1, max_len, image_type):
+        out = np.empty((d, h, w) if image_type == 0 else (d, h, w, 4), dtype=np.uint8)
+        self._call("dolhip_encode_two_field_3d", w, h, d, _addr(np.ascontiguousarray(d0, np.int16)),
+                   _addr(np.ascontiguousarray(d1, np.int16)), max_len, image_type, _addr(out))
         return out
 
     # ---- host generators

@@ -194,6 +194,39 @@ int dolhip_signed_border_2d_dev(int w, int h, const uint32_t* mask, int16_t* del
 int dolhip_signed_border_3d_dev(int w, int h, int d, const uint32_t* mask, int16_t* delta,
                                 void* work, void* stream);
 
+/* ---- two-field signed images: the tool's -S mode (GenerateSignedDistance,
+ * DistanceOcclusionTool.cpp:1082-1124 (2D), 1197-1232 (3D)) ---------------- */
+/* exterior field delta0 = InitDeltasFromBitMask(mask, invert = false) and
+ * interior field delta1 = (invert = true), each then JumpFlood(cw) (the tool
+ * uses cw = 1).  work: dolhip_two_field_*_workspace bytes, or NULL. */
+size_t dolhip_two_field_2d_workspace(int w, int h);
+size_t dolhip_two_field_3d_workspace(int w, int h, int d);
+int dolhip_two_field_deltas_2d(int w, int h, const uint32_t* mask, int16_t* delta0,
+                               int16_t* delta1, int cw);
+int dolhip_two_field_deltas_2d_dev(int w, int h, const uint32_t* mask, int16_t* delta0,
+                                   int16_t* delta1, int cw, void* work, void* stream);
+int dolhip_two_field_deltas_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta0,
+                               int16_t* delta1, int cw);
+int dolhip_two_field_deltas_3d_dev(int w, int h, int d, const uint32_t* mask, int16_t* delta0,
+                                   int16_t* delta1, int cw, void* work, void* stream);
+/* the images WriteDistances writes from the two fields
+ * (DistanceOcclusionTool.cpp:1029-1080 (2D), 1126-1195 (3D)); type:
+ *   0 sdf:        EncodeU8Signed((Dist(d0) - Dist(d1)) / max_len), 1 byte a cell
+ *   1 sdf-delta:  RGBA, channels EncodeU8Signed((d0.c + d1.c) / max_len) for
+ *                 x, y (and z in 3D; blue 0 in 2D), alpha 255
+ *   2 sdf-rg:     RGBA, EncodeU8(Dist(d0) / max_len), EncodeU8(Dist(d1) / max_len), 0, 255
+ *   3 sdf-angles: as 1 over Dist(d0) - Dist(d1) instead of max_len; in 3D every
+ *                 slice shows the first slice's values, as the tool writes them */
+int dolhip_encode_two_field_2d(int w, int h, const int16_t* delta0, const int16_t* delta1,
+                               float max_len, int type, uint8_t* out);
+int dolhip_encode_two_field_2d_dev(int w, int h, const int16_t* delta0, const int16_t* delta1,
+                                   float max_len, int type, uint8_t* out, void* stream);
+int dolhip_encode_two_field_3d(int w, int h, int d, const int16_t* delta0, const int16_t* delta1,
+                               float max_len, int type, uint8_t* out);
+int dolhip_encode_two_field_3d_dev(int w, int h, int d, const int16_t* delta0,
+                                   const int16_t* delta1, float max_len, int type, uint8_t* out,
+                                   void* stream);
+
 /* ---- batched ragged glyphs (extension) ------------------------------------- */
 /* n masks of uint8 coverage, mask i of size w[i] x h[i] starting at byte
  * offset[i] of `bytes` (and of `out`).  Per mask: threshold (>= threshold XOR

@@ -984,13 +984,71 @@ void oc_sdf_rg_rgba_2d(int w, int h, const uint32_t* mask, const int16_t* delta,
         }
 }
 
-/* two-field signed image, DistanceOcclusionTool.cpp:1035-1036 */
-void oc_sdf_two_field_u8_2d(int w, int h, const int16_t* d0, const int16_t* d1, float max_len,
-                            uint8_t* out)
+/* Two-field signed images of the tool's -S mode: WriteDistances,
+ * DistanceOcclusionTool.cpp:1029-1080 (2D).  d0 is the exterior field, d1 the
+ * interior one; type 0 sdf (1 byte a cell), 1 sdf-delta, 2 sdf-rg,
+ * 3 sdf-angles (RGBA).  Dist is sqrtf of the int32 squared length
+ * (DistanceOcclusionTool.cpp:46-57); sums of deltas are int, as there. */
+void oc_two_field_image_2d(int w, int h, const int16_t* d0, const int16_t* d1, float max_len,
+                           int type, uint8_t* out)
 {
-    for (size_t i = 0; i < (size_t)w * h; i++)
-        out[i] = oc_encode_u8_signed((sqrtf((float)oc_len2_2(d0 + 2 * i))
-                                      - sqrtf((float)oc_len2_2(d1 + 2 * i))) / max_len);
+    for (size_t i = 0; i < (size_t)w * h; i++) {
+        const int16_t* a = d0 + 2 * i;
+        const int16_t* b = d1 + 2 * i;
+        const float da = sqrtf((float)oc_len2_2(a)), db = sqrtf((float)oc_len2_2(b));
+        if (type == 0) {
+            out[i] = oc_encode_u8_signed((da - db) / max_len);
+            continue;
+        }
+        uint8_t* o = out + 4 * i;
+        if (type == 1) {
+            o[0] = oc_encode_u8_signed((a[0] + b[0]) / max_len);
+            o[1] = oc_encode_u8_signed((a[1] + b[1]) / max_len);
+        } else if (type == 2) {
+            o[0] = oc_encode_u8(da / max_len);
+            o[1] = oc_encode_u8(db / max_len);
+        } else {
+            const float d = da - db;
+            o[0] = oc_encode_u8_signed((a[0] + b[0]) / d);
+            o[1] = oc_encode_u8_signed((a[1] + b[1]) / d);
+        }
+        o[2] = 0;
+        o[3] = 255;
+    }
+}
+
+/* The same in 3D, slice by slice: DistanceOcclusionTool.cpp:1126-1195.
+ * Type 1 and 3 put z in blue.  Type 3 indexes deltas0/deltas1 with the
+ * in-slice index (DistanceOcclusionTool.cpp:1178-1186), so every slice shows
+ * the first slice's values. */
+void oc_two_field_image_3d(int w, int h, int d, const int16_t* d0, const int16_t* d1,
+                           float max_len, int type, uint8_t* out)
+{
+    const size_t n = (size_t)w * h;
+    for (int slice = 0; slice < d; slice++)
+        for (size_t i = 0; i < n; i++) {
+            const size_t c = (size_t)slice * n + i;
+            const size_t j = type == 3 ? i : c;
+            const int16_t* a = d0 + 3 * j;
+            const int16_t* b = d1 + 3 * j;
+            const float da = sqrtf((float)oc_len2_3(a)), db = sqrtf((float)oc_len2_3(b));
+            if (type == 0) {
+                out[c] = oc_encode_u8_signed((da - db) / max_len);
+                continue;
+            }
+            uint8_t* o = out + 4 * c;
+            if (type == 2) {
+                o[0] = oc_encode_u8(da / max_len);
+                o[1] = oc_encode_u8(db / max_len);
+                o[2] = 0;
+            } else {
+                const float q = type == 1 ? max_len : da - db;
+                o[0] = oc_encode_u8_signed((a[0] + b[0]) / q);
+                o[1] = oc_encode_u8_signed((a[1] + b[1]) / q);
+                o[2] = oc_encode_u8_signed((a[2] + b[2]) / q);
+            }
+            o[3] = 255;
+        }
 }
 
 /* ------------------------------------------------------------- invariants */

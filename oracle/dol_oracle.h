@@ -81,8 +81,10 @@ void oc_sdf_u8_3d(int w, int h, int d, const uint32_t* mask, const int16_t* delt
 void oc_sdf_delta_rgba_2d(int w, int h, const int16_t* delta, float max_len, uint8_t* out);
 void oc_sdf_rg_rgba_2d(int w, int h, const uint32_t* mask, const int16_t* delta, float max_len,
                        uint8_t* out);
-void oc_sdf_two_field_u8_2d(int w, int h, const int16_t* d0, const int16_t* d1, float max_len,
-                            uint8_t* out);
+void oc_two_field_image_2d(int w, int h, const int16_t* d0, const int16_t* d1, float max_len,
+                           int type, uint8_t* out);
+void oc_two_field_image_3d(int w, int h, int d, const int16_t* d0, const int16_t* d1,
+                           float max_len, int type, uint8_t* out);
 
 /* invariants of Tool:371-439 (CheckDeltas); returns number of failing cells */
 long oc_check_deltas_2d(int w, int h, const int16_t* delta, int cw);

@@ -215,6 +215,18 @@ class Oracle:
         self.L.oc_sdf_rg_rgba_2d(_I(w), _I(h), _ptr(mask), _ptr(np.ascontiguousarray(delta)), _F(max_len), _ptr(out))
         return out
 
+    def two_field_image_2d(self, w, h, d0, d1, max_len, image_type):
+        out = np.empty((h, w) if image_type == 0 else (h, w, 4), dtype=np.uint8)
+        self.L.oc_two_field_image_2d(_I(w), _I(h), _ptr(np.ascontiguousarray(d0, np.int16)),
+                                     _ptr(np.ascontiguousarray(d1, np.int16)), _F(max_len), _I(image_type), _ptr(out))
+        return out
+
+    def two_field_image_3d(self, w, h, d, d0, d1, max_len, image_type):
+        out = np.empty((d, h, w) if image_type == 0 else (d, h, w, 4), dtype=np.uint8)
+        self.L.oc_two_field_image_3d(_I(w), _I(h), _I(d), _ptr(np.ascontiguousarray(d0, np.int16)),
+                                     _ptr(np.ascontiguousarray(d1, np.int16)), _F(max_len), _I(image_type), _ptr(out))
+        return out
+
     def dist_2d(self, delta):
         delta = np.ascontiguousarray(delta, dtype=np.int16)
         n = delta.size // 2

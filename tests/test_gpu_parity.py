@@ -237,6 +237,44 @@ def test_encoders_3d(gpu, oracle, rng, dims):
 # ------------------------------------------------------------- errors
 
 
+# ------------------------------------------------------------- two-field (-S) images
+
+
+@pytest.mark.parametrize("dims", [(1, 1), (5, 1), (1, 7), (33, 20), (64, 64), (100, 71)])
+@pytest.mark.parametrize("cw", [1, 2])
+def test_two_field_2d(gpu, oracle, rng, dims, cw):
+    w, h = dims
+    for m in [rand_mask(rng, dims, p) for p in (0.0, 0.05, 0.5, 1.0)] + [blob_mask(rng, dims)]:
+        d0, d1 = gpu.two_field_deltas_2d(w, h, m, cw)
+        np.testing.assert_array_equal(d0, oracle.jfa_2d(w, h, oracle.init_deltas_2d(w, h, m, False), cw))
+        np.testing.assert_array_equal(d1, oracle.jfa_2d(w, h, oracle.init_deltas_2d(w, h, m, True), cw))
+        for max_len in (0.25 * max(w, h), 3.0):
+            for t in range(4):
+                np.testing.assert_array_equal(gpu.encode_two_field_2d(w, h, d0, d1, max_len, t),
+                                              oracle.two_field_image_2d(w, h, d0, d1, max_len, t))
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (5, 5, 5), (33, 7, 6), (64, 12, 9)])
+@pytest.mark.parametrize("cw", [1, 2])
+def test_two_field_3d(gpu, oracle, rng, dims, cw):
+    w, h, d = dims
+    for m in [rand_mask(rng, dims, p) for p in (0.0, 0.05, 0.5)] + [blob_mask(rng, dims)]:
+        d0, d1 = gpu.two_field_deltas_3d(w, h, d, m, cw)
+        np.testing.assert_array_equal(d0, oracle.jfa_3d(w, h, d, oracle.init_deltas_3d(w, h, d, m, False), cw))
+        np.testing.assert_array_equal(d1, oracle.jfa_3d(w, h, d, oracle.init_deltas_3d(w, h, d, m, True), cw))
+        for t in range(4):
+            np.testing.assert_array_equal(gpu.encode_two_field_3d(w, h, d, d0, d1, 0.25 * max(w, h, d), t),
+                                          oracle.two_field_image_3d(w, h, d, d0, d1, 0.25 * max(w, h, d), t))
+
+
+def test_two_field_rejects_bad_image_type(gpu):
+    d0 = np.zeros((2, 2, 2), np.int16)
+    with pytest.raises(Exception):
+        gpu.encode_two_field_2d(2, 2, d0, d0, 1.0, 4)
+    with pytest.raises(Exception):
+        gpu.encode_two_field_2d(2, 2, d0, d0, 0.0, 0)
+
+
 def test_invalid_arguments(gpu):
     import dolhip
 

@@ -234,6 +234,35 @@ def test_encoders_closed_form(oracle):
     np.testing.assert_array_equal(got, exp)
 
 
+def test_two_field_images_known_answers(oracle):
+    """The -S images of a 3 x 1 image whose middle cell is set, worked by hand
+    from WriteDistances (DistanceOcclusionTool.cpp:1029-1080): exterior deltas
+    (+1,0) (0,0) (-1,0), interior deltas (0,0) (-1,0) (0,0), maxLen 2."""
+    d0 = np.array([[[1, 0], [0, 0], [-1, 0]]], np.int16)
+    d1 = np.array([[[0, 0], [-1, 0], [0, 0]]], np.int16)
+    # sdf: +-0.5 -> 128 +- 63.5 -> 191.5 / 64.5, rounded half to even
+    np.testing.assert_array_equal(oracle.two_field_image_2d(3, 1, d0, d1, 2.0, 0), [[192, 64, 192]])
+    # sdf-delta: summed deltas over maxLen
+    np.testing.assert_array_equal(oracle.two_field_image_2d(3, 1, d0, d1, 2.0, 1),
+                                  [[[192, 128, 0, 255], [64, 128, 0, 255], [64, 128, 0, 255]]])
+    # sdf-rg: EncodeU8(0.5) = lrintf(127.5) = 128 in the channel of the field
+    np.testing.assert_array_equal(oracle.two_field_image_2d(3, 1, d0, d1, 2.0, 2),
+                                  [[[128, 0, 0, 255], [0, 128, 0, 255], [128, 0, 0, 255]]])
+    # sdf-angles: summed deltas over Dist(d0) - Dist(d1) = 1, -1, 1
+    np.testing.assert_array_equal(oracle.two_field_image_2d(3, 1, d0, d1, 2.0, 3),
+                                  [[[255, 128, 0, 255], [255, 128, 0, 255], [1, 128, 0, 255]]])
+    # 3D, one cell a slice: slice 0 exterior (1,0,0), slice 1 exterior (0,0,-2);
+    # the angles image repeats slice 0 in slice 1, the others do not
+    e0 = np.array([[[[1, 0, 0]]], [[[0, 0, -2]]]], np.int16)
+    e1 = np.zeros_like(e0)
+    ang = oracle.two_field_image_3d(1, 1, 2, e0, e1, 4.0, 3)
+    np.testing.assert_array_equal(ang.reshape(2, 4), [[255, 128, 128, 255], [255, 128, 128, 255]])
+    dl = oracle.two_field_image_3d(1, 1, 2, e0, e1, 4.0, 1)
+    # 1/4 -> 128 + 31.75 -> 160; -2/4 -> 128 - 63.5 = 64.5 -> 64
+    np.testing.assert_array_equal(dl.reshape(2, 4), [[160, 128, 128, 255], [128, 128, 64, 255]])
+    np.testing.assert_array_equal(oracle.two_field_image_3d(1, 1, 2, e0, e1, 4.0, 0).ravel(), [160, 192])
+
+
 def test_mask_bytes_roundtrip(oracle, rng):
     w, h = 77, 13
     data = rng.integers(0, 256, size=w * h, dtype=np.uint8)
