@@ -1,14 +1,16 @@
 // distocc_hip -- the distance modes of the reference tool (distocc) through
 // the DOL:: drop-in interface, for checking the C++ boundary end to end.
 //
-//   distocc_hip (-d | -s) [-m method] [-w size] [-W size] [-p count [seed]]
+//   distocc_hip (-d | -s | -S) [-m method] [-w size] [-W size] [-p count [seed]]
 //               [-b sides] [-l maxLen] [-c] [-o file]
 //
 // Flags as the reference's (DistanceOcclusionTool.cpp:1782-1883): -d distance,
-// -s signed distance with border initialisation, -m 2 JumpFlood (default here)
+// -s signed distance with border initialisation, -S signed distance from
+// exterior and interior fields, -m 2 JumpFlood (default here)
 // or 5 Felzenszwalb (-d only), -w image size, -W volume size, -p random points
 // (default seed 12345), -b block sides, -l maxLen, -c check.  Flows follow
-// GenerateDistance (:832-883), GenerateDistanceFelzenszwalb (:741-782) and
+// GenerateDistance (:832-883), GenerateDistanceFelzenszwalb (:741-782),
+// GenerateSignedDistance (:1082-1124, 1197-1232) and
 // GenerateSignedDistanceBorder (:1333-1365, 1471-1498).  Instead of PNG
 // images the encoded bytes are written raw to -o.  -c compares distances
 // with the exact EDT (max error < 1 cell, :265-311) and exits with -1 on
@@ -56,6 +58,7 @@ int main(int argc, char** argv)
         const char* a = argv[i];
         if (!std::strcmp(a, "-d")) mode = 'd';
         else if (!std::strcmp(a, "-s")) mode = 's';
+        else if (!std::strcmp(a, "-S")) mode = 'S';
         else if (!std::strcmp(a, "-c")) check = true;
         else if (!std::strcmp(a, "-m") && i + 1 < argc) method = std::atoi(argv[++i]);
         else if (!std::strcmp(a, "-w") && i + 1 < argc) dim = std::atoi(argv[++i]);
@@ -72,7 +75,7 @@ int main(int argc, char** argv)
         }
     }
     if (!mode) {
-        std::fprintf(stderr, "Please specify -d or -s\n");
+        std::fprintf(stderr, "Please specify -d, -s or -S\n");
         return -1;
     }
     const int w = dim ? dim : (volume ? 32 : 256), h = w, d = volume ? w : 1;
@@ -132,6 +135,30 @@ int main(int argc, char** argv)
                 const float v = Dist(delta[i].x, delta[i].y) / (maxLen * 2);
                 sdf[i] = EncodeU8Signed(inside ? -v : v);
             }
+        }
+        if (!write_raw(out, sdf.data(), n)) ok = false;
+        delete[] mask;
+        return ok ? 0 : -1;
+    } else if (mode == 'S' && method == 2) {
+        // GenerateSignedDistance (:1082-1124, 1197-1232): exterior and
+        // interior fields, then (Dist(d0) - Dist(d1)) / maxLen (:1035-1036)
+        std::vector<uint8_t> sdf(n);
+        if (volume) {
+            std::vector<cCellDelta3> d0(n), d1(n);
+            InitDeltasFromBitMask(w, h, d, mask, d0.data(), false);
+            InitDeltasFromBitMask(w, h, d, mask, d1.data(), true);
+            JumpFlood(w, h, d, d0.data());
+            JumpFlood(w, h, d, d1.data());
+            for (size_t i = 0; i < n; i++)
+                sdf[i] = EncodeU8Signed((Dist(d0[i].x, d0[i].y, d0[i].z) - Dist(d1[i].x, d1[i].y, d1[i].z)) / maxLen);
+        } else {
+            std::vector<cCellDelta2> d0(n), d1(n);
+            InitDeltasFromBitMask(w, h, mask, d0.data(), false);
+            InitDeltasFromBitMask(w, h, mask, d1.data(), true);
+            JumpFlood(w, h, d0.data());
+            JumpFlood(w, h, d1.data());
+            for (size_t i = 0; i < n; i++)
+                sdf[i] = EncodeU8Signed((Dist(d0[i].x, d0[i].y) - Dist(d1[i].x, d1[i].y)) / maxLen);
         }
         if (!write_raw(out, sdf.data(), n)) ok = false;
         delete[] mask;

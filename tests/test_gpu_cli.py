@@ -80,6 +80,25 @@ def test_cli_volume(gpu, oracle, tmp_path):
                                   oracle.sdf_u8_3d(w, w, w, m, d, 2 * 0.25 * w))
 
 
+def test_cli_two_field(gpu, oracle, tmp_path):
+    """-S: exterior and interior JumpFlood fields, (Dist(d0) - Dist(d1)) / maxLen"""
+    r, out = run(tmp_path, "-S", "-w", "200", "-b", "4", "-p", "30")
+    assert r.returncode == 0, r.stderr
+    m = mask2(oracle, 200, 30, 4)
+    d0 = oracle.jfa_2d(200, 200, oracle.init_deltas_2d(200, 200, m, False), 1)
+    d1 = oracle.jfa_2d(200, 200, oracle.init_deltas_2d(200, 200, m, True), 1)
+    np.testing.assert_array_equal(np.fromfile(out, np.uint8).reshape(200, 200),
+                                  oracle.two_field_image_2d(200, 200, d0, d1, 0.25 * 200, 0))
+    w = 24
+    r, out = run(tmp_path, "-S", "-W", str(w), "-b", "5", "-p", "10")
+    assert r.returncode == 0, r.stderr
+    m = mask3(oracle, w, 10, 5)
+    d0 = oracle.jfa_3d(w, w, w, oracle.init_deltas_3d(w, w, w, m, False), 1)
+    d1 = oracle.jfa_3d(w, w, w, oracle.init_deltas_3d(w, w, w, m, True), 1)
+    np.testing.assert_array_equal(np.fromfile(out, np.uint8).reshape(w, w, w),
+                                  oracle.two_field_image_3d(w, w, w, d0, d1, 0.25 * w, 0))
+
+
 def test_cli_rejects_unknown(gpu, tmp_path):
     r, _ = run(tmp_path, "-s", "-m", "5")
     assert r.returncode != 0
