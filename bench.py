@@ -191,6 +191,44 @@ def bench_jfa2d(L, torch, dev):
     }
 
 
+def bench_two_field(L, torch, dev):
+    """The tool's -S mode on the 8192 x 8192 disc/box scene: exterior and
+    interior fields (unsigned init + JumpFlood cw = 1 each) and the sdf byte
+    image."""
+    w = h = 8192
+    mask = L.scene_shapes_2d(w, h, 2000, seed=2)
+    m_t = torch.from_numpy(mask.view(np.int32)).to(dev)
+    d0 = torch.empty(w * h * 2, dtype=torch.int16, device=dev)
+    d1 = torch.empty(w * h * 2, dtype=torch.int16, device=dev)
+    out = torch.empty(w * h, dtype=torch.uint8, device=dev)
+    wk = torch.empty(L.workspace("two_field_2d", w, h), dtype=torch.uint8, device=dev)
+
+    def step():
+        L.call_dev("two_field_deltas_2d", w, h, m_t, d0, d1, 1, wk)
+        L.call_dev("encode_two_field_2d", w, h, d0, d1, 0.25 * w, 0, out)
+
+    step()
+    L.profile_enable(True)
+    ms = timed(step, 3, torch)
+    ks = {}
+    for k in ("init_deltas_2d", "jfa2d_pass", "encode_two_field_2d"):
+        avg, n = L.profile_query(k)
+        ks[k] = {"avg_ms": round(avg, 4), "launches": n}
+    L.profile_enable(False)
+    passes = ks["jfa2d_pass"]["launches"] // 6
+    # per field: mask in, init out, the passes' reads and writes; then two
+    # fields in and one byte out
+    bpp = 2 * (0.125 + 4 + passes * 8) + 8 + 1
+    return {
+        "ms": round(ms, 3),
+        "gpixels_per_s": round(w * h / (ms * 1e-3) / 1e9, 3),
+        "kernels": ks,
+        "passes_per_field": passes,
+        "algorithmic_bytes_per_pixel": bpp,
+        "hbm_roofline_pct": round(100 * bpp * w * h / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 2),
+    }
+
+
 def bench_config1(L, torch, dev):
     """The tool's -p 100 case: 512 x 512, 100 points (seed 12345): unsigned
     JumpFlood, exact EDT; microseconds per call (latency-bound)."""
@@ -250,7 +288,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only copy 512 MiB device to device 5 times (calibrates HBM byte counters)")
-    ap.add_argument("--only-extra", choices=["jfa3d", "glyphs", "jfa2d", "config1", "seeds"],
+    ap.add_argument("--only-extra", choices=["jfa3d", "glyphs", "jfa2d", "config1", "seeds", "twofield"],
                     help="run only this secondary workload and print its figures (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -282,7 +320,8 @@ def main():
 
     if args.only_extra and args.only_extra != "jfa3d":
         run = {"glyphs": lambda: bench_glyphs(L, torch, dev, args), "jfa2d": lambda: bench_jfa2d(L, torch, dev),
-               "config1": lambda: bench_config1(L, torch, dev), "seeds": lambda: bench_seeds(L, torch, dev)}
+               "config1": lambda: bench_config1(L, torch, dev), "seeds": lambda: bench_seeds(L, torch, dev),
+               "twofield": lambda: bench_two_field(L, torch, dev)}
         print(json.dumps({args.only_extra: run[args.only_extra]()}), flush=True)
         return
 
@@ -391,6 +430,7 @@ def main():
         torch.cuda.empty_cache()
         result["glyphs"] = bench_glyphs(L, torch, dev, args)
         result["jfa2d_signed_8192"] = bench_jfa2d(L, torch, dev)
+        result["two_field_8192"] = bench_two_field(L, torch, dev)
         result["config1_512"] = bench_config1(L, torch, dev)
         result["seeds_2048"] = bench_seeds(L, torch, dev)
 
