@@ -27,6 +27,7 @@ SOURCES = [
     "dol_encode.hip",
     "dol_batch.hip",
     "dol_seeds.hip",
+    "dol_voxel.hip",
     "dol_capi.cpp",
     "dol_scenes.cpp",
     "dol_shim.cpp",

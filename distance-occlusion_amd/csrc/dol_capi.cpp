@@ -51,6 +51,8 @@ void launch_sdf_rg_rgba(int w, int h, const uint32_t* mask, const int16_t* delta
                         uint8_t* out, hipStream_t s);
 void launch_signed_edt(size_t n, const int32_t* dout, const int32_t* din, float* sdf,
                        hipStream_t s);
+void launch_voxelise(int tri_count, const int* indices, const float* vertices, const float* bbox,
+                     int w, int h, int d, uint32_t* mask, hipStream_t s);
 void launch_two_field_2d(int w, int h, const int16_t* d0, const int16_t* d1, float max_len,
                          int type, uint8_t* out, hipStream_t s);
 void launch_two_field_3d(int w, int h, int d, const int16_t* d0, const int16_t* d1, float max_len,
@@ -1078,6 +1080,58 @@ int dolhip_encode_two_field_3d(int w, int h, int d, const int16_t* delta0, const
     launch_two_field_3d(w, h, d, a.as<int16_t>(), b.as<int16_t>(), max_len, type, o.as<uint8_t>(), 0);
     TRY(finish_sync("encode_two_field_3d"));
     return o.download(out, out_bytes);
+}
+
+
+// ---- surface voxeliser (MeshSupport.cpp:369-475)
+
+static int check_voxel_args(int tri_count, const float* vertices, const float* bbox, int w, int h,
+                            int d, const uint32_t* mask)
+{
+    TRY(check_dims(w, h, d));
+    if (tri_count < 0) return fail(DOLHIP_ERR_INVALID_ARG, "tri_count must be >= 0, got %d", tri_count);
+    TRY(check_ptr(bbox, "bbox"));
+    TRY(check_ptr(mask, "mask"));
+    if (tri_count > 0) TRY(check_ptr(vertices, "vertices"));
+    for (int k = 0; k < 3; k++)
+        if (!(bbox[3 + k] > bbox[k]))
+            return fail(DOLHIP_ERR_INVALID_ARG, "bbox max must exceed min on every axis");
+    return DOLHIP_OK;
+}
+
+int dolhip_mask_from_triangles_dev(int tri_count, const int* indices, const float* vertices,
+                                   const float* bbox, int w, int h, int d, uint32_t* mask,
+                                   void* stream)
+{
+    TRY(check_voxel_args(tri_count, vertices, bbox, w, h, d, mask));
+    TRY(require_device());
+    launch_voxelise(tri_count, indices, vertices, bbox, w, h, d, mask, as_stream(stream));
+    return finish(as_stream(stream), "mask_from_triangles");
+}
+
+int dolhip_mask_from_triangles(int tri_count, const int* indices, const float* vertices,
+                               const float* bbox, int w, int h, int d, uint32_t* mask)
+{
+    TRY(check_voxel_args(tri_count, vertices, bbox, w, h, d, mask));
+    TRY(require_device());
+    // vertices referenced: up to the largest index (or 3 per triangle)
+    long long nv = 3LL * tri_count;
+    if (indices) {
+        nv = 0;
+        for (long long i = 0; i < 3LL * tri_count; i++) {
+            if (indices[i] < 0) return fail(DOLHIP_ERR_INVALID_ARG, "negative vertex index %d", indices[i]);
+            nv = indices[i] + 1 > nv ? indices[i] + 1 : nv;
+        }
+    }
+    const size_t mbytes = mask_bytes(w, (long long)h * d);
+    DevBuf m, v, ix;
+    TRY(m.upload(mask, mbytes));
+    if (nv > 0) TRY(v.upload(vertices, (size_t)nv * 12));
+    if (indices && tri_count > 0) TRY(ix.upload(indices, (size_t)tri_count * 12));
+    launch_voxelise(tri_count, indices ? ix.as<int>() : nullptr, v.as<float>(), bbox, w, h, d,
+                    m.as<uint32_t>(), 0);
+    TRY(finish_sync("mask_from_triangles"));
+    return m.download(mask, mbytes);
 }
 
 }  // extern "C"

@@ -93,6 +93,8 @@ _SIGS = {
     "dolhip_encode_sdf_rg_rgba_2d": ([_I, _I, _P, _P, _F, _P], _I),
     "dolhip_signed_border_2d_dev": ([_I, _I, _P, _P, _P, _P], _I),
     "dolhip_signed_border_3d_dev": ([_I, _I, _I, _P, _P, _P, _P], _I),
+    "dolhip_mask_from_triangles": ([_I, _P, _P, _P, _I, _I, _I, _P], _I),
+    "dolhip_mask_from_triangles_dev": ([_I, _P, _P, _P, _I, _I, _I, _P, _P], _I),
     "dolhip_two_field_2d_workspace": ([_I, _I], _SZ),
     "dolhip_two_field_3d_workspace": ([_I, _I, _I], _SZ),
     "dolhip_two_field_deltas_2d": ([_I, _I, _P, _P, _P, _I], _I),
@@ -347,6 +349,17 @@ class DolHip:
         self._call("dolhip_encode_two_field_3d", w, h, d, _addr(np.ascontiguousarray(d0, np.int16)),
                    _addr(np.ascontiguousarray(d1, np.int16)), max_len, image_type, _addr(out))
         return out
+
+    def mask_from_triangles(self, w, h, d, vertices, indices, bbox, mask=None):
+        """ORs the surface voxels of the triangles into mask (a new zero one if None)"""
+        if mask is None:
+            mask = np.zeros(mask_stride(w) * h * d, np.uint32)
+        vertices = np.ascontiguousarray(vertices, np.float32)
+        idx = None if indices is None else np.ascontiguousarray(indices, np.int32)
+        n = (vertices.size // 9) if idx is None else idx.size // 3
+        self._call("dolhip_mask_from_triangles", n, _addr(idx), _addr(vertices),
+                   _addr(np.ascontiguousarray(bbox, np.float32)), w, h, d, _addr(mask))
+        return mask
 
     # ---- host generators
     def mask_add_points_2d(self, w, h, n, seed, mask):

@@ -227,6 +227,22 @@ int dolhip_encode_two_field_3d_dev(int w, int h, int d, const int16_t* delta0,
                                    const int16_t* delta1, float max_len, int type, uint8_t* out,
                                    void* stream);
 
+/* ---- surface voxeliser: CreateBitMaskFromTriangles (MeshSupport.cpp:369-475,
+ * declared MeshSupport.h:44-53) ----------------------------------------------
+ * ORs into mask (w x h x d bitmask layout) every cell the triangles touch:
+ * the reference's triangle/cell test at each cell of a triangle's cell range
+ * (a one-cell range is set without the test).  vertices: float x, y, z
+ * triples; indices: 3 per triangle, or NULL for consecutive vertices (the
+ * reference reads indices even when NULL); bbox: {min x, y, z, max x, y, z},
+ * the volume's extent (the tool passes FindBounds of the mesh,
+ * MeshSupport.cpp:833-860).  The _dev form takes device vertices, indices
+ * and mask (bbox stays a host pointer) and trusts indices to be in range. */
+int dolhip_mask_from_triangles(int tri_count, const int* indices, const float* vertices,
+                               const float* bbox, int w, int h, int d, uint32_t* mask);
+int dolhip_mask_from_triangles_dev(int tri_count, const int* indices, const float* vertices,
+                                   const float* bbox, int w, int h, int d, uint32_t* mask,
+                                   void* stream);
+
 /* ---- batched ragged glyphs (extension) ------------------------------------- */
 /* n masks of uint8 coverage, mask i of size w[i] x h[i] starting at byte
  * offset[i] of `bytes` (and of `out`).  Per mask: threshold (>= threshold XOR

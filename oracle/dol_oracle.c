@@ -1051,6 +1051,171 @@ void oc_two_field_image_3d(int w, int h, int d, const int16_t* d0, const int16_t
         }
 }
 
+/* ------------------------------------------------------------- voxeliser */
+
+/* Surface voxelisation of a triangle list into a bitmask:
+ * CreateBitMaskFromTriangles, MeshSupport.cpp:369-475, with the
+ * triangle/cell test of cTriCellIntersectDelta, MeshSupport.cpp:158-300
+ * (separating axes: the 9 edge x axis products, the triangle's box, its
+ * plane).  Every product and sum is a separate float operation in the order
+ * written there (this file is built without contraction).  indices == NULL
+ * reads consecutive vertices (the reference dereferences indices anyway). */
+typedef struct { float x, y, z; } vox_v3;
+
+static vox_v3 v3_sub(vox_v3 a, vox_v3 b) { vox_v3 r = {a.x - b.x, a.y - b.y, a.z - b.z}; return r; }
+static float v3_dot(vox_v3 a, vox_v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static float vmin_f(float a, float b) { return a < b ? a : b; }
+static float vmax_f(float a, float b) { return b < a ? a : b; }
+static void fswap(float* a, float* b) { float t = *a; *a = *b; *b = t; }
+
+typedef struct {
+    vox_v3 e1, t1, u1, e2, t2, u2, e3, t3, u3, pmin, pmax, hw, n;
+    float ndv_min, ndv_max, nd;
+} vox_tri;
+
+/* one edge's three axis tests: t = lower, u = upper bound of the triangle's
+ * projection less/plus the box's radius */
+static void vox_edge(vox_v3 e, vox_v3 pa, vox_v3 pb, vox_v3 pc, vox_v3 qa, vox_v3 qb, vox_v3 qc,
+                     vox_v3 hw, vox_v3* t, vox_v3* u)
+{
+    /* x from pa, y from pb, z from pc; the other vertex per axis from q* */
+    t->x = e.z * pa.y - e.y * pa.z;
+    t->y = -e.z * pb.x + e.x * pb.z;
+    t->z = e.y * pc.x - e.x * pc.y;
+    u->x = e.z * qa.y - e.y * qa.z;
+    u->y = -e.z * qb.x + e.x * qb.z;
+    u->z = e.y * qc.x - e.x * qc.y;
+    if (t->x >= u->x) fswap(&t->x, &u->x);
+    if (t->y >= u->y) fswap(&t->y, &u->y);
+    if (t->z >= u->z) fswap(&t->z, &u->z);
+    const vox_v3 ae = {fabsf(e.x), fabsf(e.y), fabsf(e.z)};
+    const vox_v3 r = {ae.z * hw.y + ae.y * hw.z, ae.z * hw.x + ae.x * hw.z, ae.y * hw.x + ae.x * hw.y};
+    t->x = t->x - r.x; t->y = t->y - r.y; t->z = t->z - r.z;
+    u->x = u->x + r.x; u->y = u->y + r.y; u->z = u->z + r.z;
+}
+
+static void vox_setup(vox_v3 p1, vox_v3 p2, vox_v3 p3, vox_v3 hw, vox_tri* T)
+{
+    T->hw = hw;
+    T->e1 = v3_sub(p2, p1);
+    T->e2 = v3_sub(p3, p2);
+    T->e3 = v3_sub(p1, p3);
+    /* TriDoubleAreaNormal, MeshSupport.cpp:141-155 */
+    const vox_v3 a = p1, b = p2, c = p3;
+    T->n.x = (a.y - b.y) * (a.z + b.z) + (b.y - c.y) * (b.z + c.z) + (c.y - a.y) * (c.z + a.z);
+    T->n.y = (a.z - b.z) * (a.x + b.x) + (b.z - c.z) * (b.x + c.x) + (c.z - a.z) * (c.x + a.x);
+    T->n.z = (a.x - b.x) * (a.y + b.y) + (b.x - c.x) * (b.y + c.y) + (c.x - a.x) * (c.y + a.y);
+    vox_v3 vmn, vmx;
+    vmn.x = T->n.x > 0.0f ? -hw.x : hw.x;  vmx.x = T->n.x > 0.0f ? hw.x : -hw.x;
+    vmn.y = T->n.y > 0.0f ? -hw.y : hw.y;  vmx.y = T->n.y > 0.0f ? hw.y : -hw.y;
+    vmn.z = T->n.z > 0.0f ? -hw.z : hw.z;  vmx.z = T->n.z > 0.0f ? hw.z : -hw.z;
+    T->ndv_min = v3_dot(T->n, vmn);
+    T->ndv_max = v3_dot(T->n, vmx);
+    T->nd = v3_dot(T->n, p1);
+    /* per edge, the vertex pairs of MeshSupport.cpp:214-251 */
+    vox_edge(T->e1, p1, p1, p2, p3, p3, p3, hw, &T->t1, &T->u1);
+    vox_edge(T->e2, p1, p1, p1, p3, p3, p2, hw, &T->t2, &T->u2);
+    vox_edge(T->e3, p1, p1, p2, p2, p2, p3, hw, &T->t3, &T->u3);
+    T->pmin.x = vmin_f(vmin_f(p1.x, p2.x), p3.x);
+    T->pmin.y = vmin_f(vmin_f(p1.y, p2.y), p3.y);
+    T->pmin.z = vmin_f(vmin_f(p1.z, p2.z), p3.z);
+    T->pmax.x = vmax_f(vmax_f(p1.x, p2.x), p3.x);
+    T->pmax.y = vmax_f(vmax_f(p1.y, p2.y), p3.y);
+    T->pmax.z = vmax_f(vmax_f(p1.z, p2.z), p3.z);
+}
+
+/* IntersectZ / IntersectY / IntersectX, MeshSupport.cpp:279-351 */
+static int vox_hit_z(const vox_tri* T, vox_v3 c)
+{
+    return !(T->pmin.z - c.z > T->hw.z || T->pmax.z - c.z < -T->hw.z);
+}
+
+static int vox_hit_y(const vox_tri* T, vox_v3 c)
+{
+    float dr;
+    dr = T->e1.z * c.y - T->e1.y * c.z;
+    if (T->t1.x > dr || T->u1.x < dr) return 0;
+    dr = T->e2.z * c.y - T->e2.y * c.z;
+    if (T->t2.x > dr || T->u2.x < dr) return 0;
+    dr = T->e3.z * c.y - T->e3.y * c.z;
+    if (T->t3.x > dr || T->u3.x < dr) return 0;
+    return !(T->pmin.y - c.y > T->hw.y || T->pmax.y - c.y < -T->hw.y);
+}
+
+static int vox_hit_x(const vox_tri* T, vox_v3 c)
+{
+    float dr;
+    dr = -T->e1.z * c.x + T->e1.x * c.z;
+    if (T->t1.y > dr || T->u1.y < dr) return 0;
+    dr = T->e1.y * c.x - T->e1.x * c.y;
+    if (T->t1.z > dr || T->u1.z < dr) return 0;
+    dr = -T->e2.z * c.x + T->e2.x * c.z;
+    if (T->t2.y > dr || T->u2.y < dr) return 0;
+    dr = T->e2.y * c.x - T->e2.x * c.y;
+    if (T->t2.z > dr || T->u2.z < dr) return 0;
+    dr = -T->e3.z * c.x + T->e3.x * c.z;
+    if (T->t3.y > dr || T->u3.y < dr) return 0;
+    dr = T->e3.y * c.x - T->e3.x * c.y;
+    if (T->t3.z > dr || T->u3.z < dr) return 0;
+    if (T->pmin.x - c.x > T->hw.x || T->pmax.x - c.x < -T->hw.x) return 0;
+    const float d = T->nd - v3_dot(T->n, c);
+    return T->ndv_min <= d && T->ndv_max >= d;
+}
+
+void oc_mask_from_triangles(int tri_count, const int* indices, const float* vertices,
+                            const float* bbox, int w, int h, int d, uint32_t* mask)
+{
+    const size_t row = (size_t)mask_stride(w), slice = row * h;
+    const vox_v3 whd = {(float)w, (float)h, (float)d};
+    const vox_v3 cmin = {bbox[0], bbox[1], bbox[2]};
+    const vox_v3 ext = {bbox[3] - bbox[0], bbox[4] - bbox[1], bbox[5] - bbox[2]};
+    const vox_v3 cw = {ext.x / whd.x, ext.y / whd.y, ext.z / whd.z};
+    const vox_v3 inv = {whd.x / ext.x, whd.y / ext.y, whd.z / ext.z};
+    const vox_v3 hw = {cw.x * 0.500001f, cw.y * 0.500001f, cw.z * 0.500001f};
+    for (int t = 0; t < tri_count; t++) {
+        const int i1 = indices ? indices[3 * t] : 3 * t, i2 = indices ? indices[3 * t + 1] : 3 * t + 1,
+                  i3 = indices ? indices[3 * t + 2] : 3 * t + 2;
+        const vox_v3 p1 = {vertices[3 * i1], vertices[3 * i1 + 1], vertices[3 * i1 + 2]};
+        const vox_v3 p2 = {vertices[3 * i2], vertices[3 * i2 + 1], vertices[3 * i2 + 2]};
+        const vox_v3 p3 = {vertices[3 * i3], vertices[3 * i3 + 1], vertices[3 * i3 + 2]};
+        /* TriangleBounds, MeshSupport.cpp:353-366 */
+        const vox_v3 bmn = {vmin_f(vmin_f(p1.x, p2.x), p3.x), vmin_f(vmin_f(p1.y, p2.y), p3.y),
+                            vmin_f(vmin_f(p1.z, p2.z), p3.z)};
+        const vox_v3 bmx = {vmax_f(vmax_f(p1.x, p2.x), p3.x), vmax_f(vmax_f(p1.y, p2.y), p3.y),
+                            vmax_f(vmax_f(p1.z, p2.z), p3.z)};
+        int cx0 = (int)floorf((bmn.x - cmin.x) * inv.x), cy0 = (int)floorf((bmn.y - cmin.y) * inv.y),
+            cz0 = (int)floorf((bmn.z - cmin.z) * inv.z);
+        int cx1 = (int)ceilf((bmx.x - cmin.x) * inv.x), cy1 = (int)ceilf((bmx.y - cmin.y) * inv.y),
+            cz1 = (int)ceilf((bmx.z - cmin.z) * inv.z);
+        if (cx0 < 0) cx0 = 0;
+        if (cy0 < 0) cy0 = 0;
+        if (cz0 < 0) cz0 = 0;
+        if (cx1 > w) cx1 = w;
+        if (cy1 > h) cy1 = h;
+        if (cz1 > d) cz1 = d;
+        if (cx0 == cx1) continue;
+        if (cx0 + 1 == cx1 && cy0 + 1 == cy1 && cz0 + 1 == cz1) {   /* single cell: no test */
+            mask[slice * cz0 + row * cy0 + (cx0 >> 5)] |= 1u << (cx0 & 31);
+            continue;
+        }
+        vox_tri T;
+        vox_setup(p1, p2, p3, hw, &T);
+        vox_v3 c = {0.0f, 0.0f, 0.0f};
+        for (int z = cz0; z < cz1; z++) {
+            c.z = cmin.z + cw.z * ((float)z + 0.5f);
+            if (!vox_hit_z(&T, c)) continue;
+            for (int y = cy0; y < cy1; y++) {
+                c.y = cmin.y + cw.y * ((float)y + 0.5f);
+                if (!vox_hit_y(&T, c)) continue;
+                for (int x = cx0; x < cx1; x++) {
+                    c.x = cmin.x + cw.x * ((float)x + 0.5f);
+                    if (vox_hit_x(&T, c)) mask[slice * z + row * y + (x >> 5)] |= 1u << (x & 31);
+                }
+            }
+        }
+    }
+}
+
 /* ------------------------------------------------------------- invariants */
 
 /* CheckDeltas, DistanceOcclusionTool.cpp:371-402: each delta must land in

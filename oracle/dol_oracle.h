@@ -86,6 +86,11 @@ void oc_two_field_image_2d(int w, int h, const int16_t* d0, const int16_t* d1, f
 void oc_two_field_image_3d(int w, int h, int d, const int16_t* d0, const int16_t* d1,
                            float max_len, int type, uint8_t* out);
 
+/* surface voxeliser of MeshSupport.cpp:369-475 (ORs into mask); bbox =
+ * {min xyz, max xyz}; indices NULL reads consecutive vertices */
+void oc_mask_from_triangles(int tri_count, const int* indices, const float* vertices,
+                            const float* bbox, int w, int h, int d, uint32_t* mask);
+
 /* invariants of Tool:371-439 (CheckDeltas); returns number of failing cells */
 long oc_check_deltas_2d(int w, int h, const int16_t* delta, int cw);
 long oc_check_deltas_3d(int w, int h, int d, const int16_t* delta, int cw);

@@ -15,6 +15,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libdolref.so")
+MESH_REF_SO = os.path.join(ROOT, "oracle", "_ref", "libmeshref.so")
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -227,6 +228,17 @@ class Oracle:
                                      _ptr(np.ascontiguousarray(d1, np.int16)), _F(max_len), _I(image_type), _ptr(out))
         return out
 
+    def mask_from_triangles(self, w, h, d, vertices, indices, bbox, mask=None):
+        """ORs the surface voxels of the triangles into mask (a new one if None)"""
+        if mask is None:
+            mask = empty_mask(w, h, d)
+        vertices = np.ascontiguousarray(vertices, np.float32)
+        idx = None if indices is None else np.ascontiguousarray(indices, np.int32)
+        n = (vertices.size // 9) if idx is None else idx.size // 3
+        self.L.oc_mask_from_triangles(_I(n), _ptr(idx), _ptr(vertices),
+                                      _ptr(np.ascontiguousarray(bbox, np.float32)), _I(w), _I(h), _I(d), _ptr(mask))
+        return mask
+
     def dist_2d(self, delta):
         delta = np.ascontiguousarray(delta, dtype=np.int16)
         n = delta.size // 2
@@ -335,5 +347,44 @@ def load_reference():
     """The compiled reference, or None where it was not built."""
     try:
         return Reference()
+    except (FileNotFoundError, OSError):
+        return None
+
+
+class MeshReference:
+    """The reference's MeshSupport.cpp compiled from its sources (oracle/_ref)."""
+
+    def __init__(self, path=MESH_REF_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.L = ctypes.CDLL(path)
+
+    def read_obj(self, path):
+        npos, nidx = ctypes.c_int(), ctypes.c_int()
+        if not self.L.ref_read_obj(path.encode(), ctypes.byref(npos), ctypes.byref(nidx), None, None):
+            raise IOError(path)
+        pos = np.empty((npos.value, 3), np.float32)
+        idx = np.empty(nidx.value, np.int32)
+        self.L.ref_read_obj(path.encode(), ctypes.byref(npos), ctypes.byref(nidx), _ptr(pos), _ptr(idx))
+        return pos, idx
+
+    def find_bounds(self, positions):
+        positions = np.ascontiguousarray(positions, np.float32)
+        out = np.empty(6, np.float32)
+        self.L.ref_find_bounds(_I(positions.size // 3), _ptr(positions), _ptr(out))
+        return out
+
+    def mask_from_triangles(self, w, h, d, vertices, indices, bbox):
+        mask = empty_mask(w, h, d)
+        idx = np.ascontiguousarray(indices, np.int32)
+        self.L.ref_mask_from_triangles(_I(idx.size // 3), _ptr(idx), _ptr(np.ascontiguousarray(vertices, np.float32)),
+                                       _ptr(np.ascontiguousarray(bbox, np.float32)), _I(w), _I(h), _I(d), _ptr(mask))
+        return mask
+
+
+def load_mesh_reference():
+    """The compiled mesh support, or None where it was not built."""
+    try:
+        return MeshReference()
     except (FileNotFoundError, OSError):
         return None

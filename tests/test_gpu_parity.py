@@ -237,6 +237,50 @@ def test_encoders_3d(gpu, oracle, rng, dims):
 # ------------------------------------------------------------- errors
 
 
+# ------------------------------------------------------------- voxeliser
+
+
+def test_mask_from_triangles_golden(gpu, oracle):
+    """The reference's masks of its sample mesh and of a triangle soup."""
+    import os
+
+    from oracle_bind import ROOT
+
+    g = np.load(os.path.join(ROOT, "tests", "golden", "voxel_golden.npz"))
+    for key in sorted(g.files):
+        kind = key.split("_")[0]
+        if "_mask_" not in key:
+            continue
+        w, h, d = (int(v) for v in key.split("_")[2:])
+        got = gpu.mask_from_triangles(w, h, d, g[kind + "_positions"], g[kind + "_indices"], g[kind + "_bbox"])
+        np.testing.assert_array_equal(got, g[key])
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (7, 40, 3), (64, 64, 64), (130, 33, 70)])
+def test_mask_from_triangles(gpu, oracle, rng, dims):
+    w, h, d = dims
+    for n in (1, 50, 2000):
+        pos = rng.uniform(-0.2, 1.2, (3 * n, 3)).astype(np.float32)
+        pos[: 3 * (n // 2)] = (rng.uniform(0, 1, (n // 2, 1, 3)) + rng.normal(0, 0.03, (n // 2, 3, 3))
+                               ).astype(np.float32).reshape(-1, 3)
+        bbox = np.array([0, 0, 0, 1, 1, 1], np.float32)
+        idx = rng.permutation(3 * n).astype(np.int32)
+        np.testing.assert_array_equal(gpu.mask_from_triangles(w, h, d, pos, idx, bbox),
+                                      oracle.mask_from_triangles(w, h, d, pos, idx, bbox))
+        # no index array: consecutive vertices; ORed into an existing mask
+        base = rand_mask(rng, dims, 0.05)
+        np.testing.assert_array_equal(gpu.mask_from_triangles(w, h, d, pos, None, bbox, base.copy()),
+                                      oracle.mask_from_triangles(w, h, d, pos, None, bbox, base.copy()))
+
+
+def test_mask_from_triangles_large_triangle(gpu, oracle):
+    """One triangle across a 256^3 box (about 10^7 cells in its range)."""
+    pos = np.array([[0.01, 0.02, 0.03], [0.97, 0.1, 0.9], [0.3, 0.99, 0.5]], np.float32)
+    bbox = np.array([0, 0, 0, 1, 1, 1], np.float32)
+    np.testing.assert_array_equal(gpu.mask_from_triangles(256, 256, 256, pos, None, bbox),
+                                  oracle.mask_from_triangles(256, 256, 256, pos, None, bbox))
+
+
 # ------------------------------------------------------------- two-field (-S) images
 
 

@@ -263,6 +263,44 @@ def test_two_field_images_known_answers(oracle):
     np.testing.assert_array_equal(oracle.two_field_image_3d(1, 1, 2, e0, e1, 4.0, 0).ravel(), [160, 192])
 
 
+def _voxel_cases():
+    g = np.load(os.path.join(GOLDEN, "voxel_golden.npz"))
+    for key in sorted(g.files):
+        for kind in ("plant", "soup"):
+            if key.startswith(kind + "_mask_"):
+                w, h, d = (int(v) for v in key.split("_")[2:])
+                yield (w, h, d), g[kind + "_positions"], g[kind + "_indices"], g[kind + "_bbox"], g[key]
+
+
+def test_voxeliser_golden(oracle):
+    """The oracle's voxeliser against the reference's masks (the sample mesh
+    data/plant2.obj and a triangle soup; tests/golden/make_voxel_golden.py)."""
+    n = 0
+    for (w, h, d), pos, idx, bbox, exp in _voxel_cases():
+        np.testing.assert_array_equal(oracle.mask_from_triangles(w, h, d, pos, idx, bbox), exp)
+        n += 1
+    assert n == 5
+
+
+def test_voxeliser_live_reference(oracle, rng):
+    """More soups and grids against MeshSupport.cpp compiled from its source."""
+    from oracle_bind import load_mesh_reference
+
+    ref = load_mesh_reference()
+    if ref is None:
+        pytest.skip("oracle/_ref/libmeshref.so not built (reference sources absent)")
+    for trial in range(6):
+        n = int(rng.integers(1, 300))
+        pos = rng.uniform(-0.3, 1.3, (3 * n, 3)).astype(np.float32)
+        if trial % 2:
+            pos[::3] = pos[1::3] + rng.normal(0, 0.02, (n, 3)).astype(np.float32)
+        idx = rng.permutation(3 * n).astype(np.int32)
+        bbox = np.array([0, 0, 0, 1, 1 + trial, 1], np.float32)
+        w, h, d = (int(v) for v in rng.integers(1, 48, 3))
+        np.testing.assert_array_equal(oracle.mask_from_triangles(w, h, d, pos, idx, bbox),
+                                      ref.mask_from_triangles(w, h, d, pos, idx, bbox))
+
+
 def test_mask_bytes_roundtrip(oracle, rng):
     w, h = 77, 13
     data = rng.integers(0, 256, size=w * h, dtype=np.uint8)
