@@ -278,6 +278,67 @@ def bench_seeds(L, torch, dev):
     return {"forward_ms": round(timed(fwd, 10, torch), 4), "backward_ms": round(timed(bwd, 10, torch), 4)}
 
 
+def torus_mesh(nu=1024, nv=512, r0=0.3, r1=0.12):
+    """A torus of 2 * nu * nv triangles centred in the unit box."""
+    u = np.arange(nu) * (2 * np.pi / nu)
+    v = np.arange(nv) * (2 * np.pi / nv)
+    uu, vv = np.meshgrid(u, v, indexing="ij")
+    pos = np.stack([0.5 + (r0 + r1 * np.cos(vv)) * np.cos(uu),
+                    0.5 + (r0 + r1 * np.cos(vv)) * np.sin(uu),
+                    0.5 + r1 * np.sin(vv)], -1).reshape(-1, 3).astype(np.float32)
+    i, j = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    a = i * nv + j
+    b = ((i + 1) % nu) * nv + j
+    c = ((i + 1) % nu) * nv + (j + 1) % nv
+    e = i * nv + (j + 1) % nv
+    idx = np.stack([a, b, c, a, c, e], -1).reshape(-1).astype(np.int32)
+    return pos, idx
+
+
+def bench_voxel(L, torch, dev, cpu=True):
+    """Surface voxelisation into a 512^3 bitmask: the reference's sample mesh
+    (data/plant2.obj, 3496 triangles, kept in tests/golden/voxel_golden.npz)
+    and a 1M-triangle torus; the oracle's voxeliser (one thread) on the
+    sample mesh as the CPU figure."""
+    n = 512
+    g = np.load(os.path.join(ROOT, "tests", "golden", "voxel_golden.npz"))
+    out = {}
+    mask_t = torch.zeros((n // 32) * n * n, dtype=torch.int32, device=dev)
+    meshes = {"plant2_512": (g["plant_positions"], g["plant_indices"], g["plant_bbox"]),
+              "torus1m_512": torus_mesh() + (np.array([0, 0, 0, 1, 1, 1], np.float32),)}
+    for name, (pos, idx, bbox) in meshes.items():
+        p_t = torch.from_numpy(np.ascontiguousarray(pos)).to(dev)
+        i_t = torch.from_numpy(np.ascontiguousarray(idx)).to(dev)
+        tri = idx.size // 3
+        bb = np.ascontiguousarray(bbox, np.float32)
+
+        def step():
+            L.call_dev("mask_from_triangles", tri, i_t, p_t, bb, n, n, n, mask_t)
+
+        mask_t.zero_()
+        step()
+        ms = timed(step, 5, torch)
+        out[name] = {"triangles": tri, "ms": round(ms, 4), "mtriangles_per_s": round(tri / (ms * 1e-3) / 1e6, 2),
+                     "set_voxels": int(torch.count_nonzero(mask_t).item())}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_bind import Oracle
+
+        pos, idx, bbox = meshes["plant2_512"]
+        t0 = time.perf_counter()
+        m = Oracle().mask_from_triangles(n, n, n, pos, idx, bbox)
+        cpu_s = time.perf_counter() - t0
+        same = np.array_equal(m.view(np.int32), mask_cpu_view(L, torch, dev, pos, idx, bbox, n))
+        out["cpu_baseline_plant2_512"] = {"ms": round(cpu_s * 1e3, 1), "cores": 1, "kind": "port",
+                                          "gpu_mask_equal": bool(same)}
+    return out
+
+
+def mask_cpu_view(L, torch, dev, pos, idx, bbox, n):
+    """the GPU's mask of one mesh, on the host"""
+    return L.mask_from_triangles(n, n, n, pos, idx, bbox).view(np.int32)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -288,7 +349,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only copy 512 MiB device to device 5 times (calibrates HBM byte counters)")
-    ap.add_argument("--only-extra", choices=["jfa3d", "glyphs", "jfa2d", "config1", "seeds", "twofield"],
+    ap.add_argument("--only-extra", choices=["jfa3d", "glyphs", "jfa2d", "config1", "seeds", "twofield", "voxel"],
                     help="run only this secondary workload and print its figures (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -321,7 +382,8 @@ def main():
     if args.only_extra and args.only_extra != "jfa3d":
         run = {"glyphs": lambda: bench_glyphs(L, torch, dev, args), "jfa2d": lambda: bench_jfa2d(L, torch, dev),
                "config1": lambda: bench_config1(L, torch, dev), "seeds": lambda: bench_seeds(L, torch, dev),
-               "twofield": lambda: bench_two_field(L, torch, dev)}
+               "twofield": lambda: bench_two_field(L, torch, dev),
+               "voxel": lambda: bench_voxel(L, torch, dev, cpu=not args.no_cpu)}
         print(json.dumps({args.only_extra: run[args.only_extra]()}), flush=True)
         return
 
@@ -431,6 +493,7 @@ def main():
         result["glyphs"] = bench_glyphs(L, torch, dev, args)
         result["jfa2d_signed_8192"] = bench_jfa2d(L, torch, dev)
         result["two_field_8192"] = bench_two_field(L, torch, dev)
+        result["voxel_512"] = bench_voxel(L, torch, dev, cpu=not args.no_cpu)
         result["config1_512"] = bench_config1(L, torch, dev)
         result["seeds_2048"] = bench_seeds(L, torch, dev)
 
