@@ -181,26 +181,67 @@ __global__ void __launch_bounds__(kRowMax) k_jfa3d_rows(int step, int w_arg, int
         src_row = zz * h + yy;
         return zz >= 0 && zz < d && yy >= 0 && yy < h;
     };
-    if (IN_AOS) {
-        const int chunks = (3 * w) / 8;   // 16-byte chunks in a row
-        for (int e = threadIdx.x; e < 9 * chunks; e += blockDim.x) {
-            const int r = e / chunks, k = e - r * chunks;
-            int sr;
-            if (!in_row(r, sr)) continue;
-            ((uint4*)(lds + r * (3 * w / 2)))[k] = ((const uint4*)(in + (size_t)3 * w * sr))[k];
-        }
+    if constexpr (W == 512) {
+        // one thread per cell of the row (blockDim = W): the chunks of the
+        // four iterations are known at compile time up to the thread index,
+        // the staged rows' offsets come from selects rather than multiplies,
+        // and all four loads are issued (rows outside the volume read row 0
+        // instead and are not stored) before the first LDS store
+        constexpr int kCh = IN_AOS ? 3 * W / 8 : W / 4 + W / 8;   // 16-byte chunks a row
+        static_assert((9 * kCh + W - 1) / W == 4, "four chunks a thread");
+        const int zs[3] = {z - step, z, z + step}, ys[3] = {y - step, y, y + step};
+        auto chunk = [&](int m, int& dst) {
+            const int e = threadIdx.x + m * W;
+            const int r = min(e / kCh, 8), k = e - r * kCh;
+            const int zz = r < 3 ? zs[0] : (r < 6 ? zs[1] : zs[2]);
+            const int rr = r - 3 * (r / 3);
+            const int yy = rr == 0 ? ys[0] : (rr == 1 ? ys[1] : ys[2]);
+            const bool ok = e < 9 * kCh && zz >= 0 && zz < d && yy >= 0 && yy < h;
+            const size_t sr = ok ? (size_t)zz * h + yy : 0;
+            size_t off;   // in 16-byte units from `in`
+            if (IN_AOS) {
+                off = sr * (3 * W / 8) + (ok ? k : 0);
+                dst = r * (3 * W / 2) + 4 * k;
+            } else if (k < W / 4) {
+                off = sr * (W / 4) + k;
+                dst = r * W + 4 * k;
+            } else {
+                off = cells / 4 + sr * (W / 8) + (k - W / 4);   // the z plane starts at byte 4 * cells
+                dst = 9 * W + r * (W / 2) + 4 * (k - W / 4);
+            }
+            if (!ok) dst = -1;
+            return off;
+        };
+        const uint4* in4 = (const uint4*)in;
+        int d0, d1, d2, d3;
+        const size_t o0 = chunk(0, d0), o1 = chunk(1, d1), o2 = chunk(2, d2), o3 = chunk(3, d3);
+        const uint4 v0 = in4[o0], v1 = in4[o1], v2 = in4[o2], v3 = in4[o3];
+        if (d0 >= 0) *(uint4*)(lds + d0) = v0;
+        if (d1 >= 0) *(uint4*)(lds + d1) = v1;
+        if (d2 >= 0) *(uint4*)(lds + d2) = v2;
+        if (d3 >= 0) *(uint4*)(lds + d3) = v3;
     } else {
-        const int cxy = w / 4, cz = w / 8;   // 16-byte chunks in an xy row and a z row
-        const uint32_t* pxy = (const uint32_t*)in;
-        const int16_t* pz = in + 2 * cells;
-        for (int e = threadIdx.x; e < 9 * (cxy + cz); e += blockDim.x) {
-            const int r = e / (cxy + cz), k = e - r * (cxy + cz);
-            int sr;
-            if (!in_row(r, sr)) continue;
-            if (k < cxy)
-                ((uint4*)(lds + r * w))[k] = ((const uint4*)(pxy + (size_t)w * sr))[k];
-            else
-                ((uint4*)(lds + 9 * w + r * (w / 2)))[k - cxy] = ((const uint4*)(pz + (size_t)w * sr))[k - cxy];
+        if (IN_AOS) {
+            const int chunks = (3 * w) / 8;   // 16-byte chunks in a row
+            for (int e = threadIdx.x; e < 9 * chunks; e += blockDim.x) {
+                const int r = e / chunks, k = e - r * chunks;
+                int sr;
+                if (!in_row(r, sr)) continue;
+                ((uint4*)(lds + r * (3 * w / 2)))[k] = ((const uint4*)(in + (size_t)3 * w * sr))[k];
+            }
+        } else {
+            const int cxy = w / 4, cz = w / 8;   // 16-byte chunks in an xy row and a z row
+            const uint32_t* pxy = (const uint32_t*)in;
+            const int16_t* pz = in + 2 * cells;
+            for (int e = threadIdx.x; e < 9 * (cxy + cz); e += blockDim.x) {
+                const int r = e / (cxy + cz), k = e - r * (cxy + cz);
+                int sr;
+                if (!in_row(r, sr)) continue;
+                if (k < cxy)
+                    ((uint4*)(lds + r * w))[k] = ((const uint4*)(pxy + (size_t)w * sr))[k];
+                else
+                    ((uint4*)(lds + 9 * w + r * (w / 2)))[k - cxy] = ((const uint4*)(pz + (size_t)w * sr))[k - cxy];
+            }
         }
     }
     __syncthreads();
