@@ -182,40 +182,48 @@ __global__ void __launch_bounds__(kRowMax) k_jfa3d_rows(int step, int w_arg, int
         return zz >= 0 && zz < d && yy >= 0 && yy < h;
     };
     if constexpr (W == 512) {
-        // one thread per cell of the row (blockDim = W): the chunks of the
-        // four iterations are known at compile time up to the thread index,
-        // the staged rows' offsets come from selects rather than multiplies,
-        // and all four loads are issued (rows outside the volume read row 0
-        // instead and are not stored) before the first LDS store
-        constexpr int kCh = IN_AOS ? 3 * W / 8 : W / 4 + W / 8;   // 16-byte chunks a row
-        static_assert((9 * kCh + W - 1) / W == 4, "four chunks a thread");
-        const int zs[3] = {z - step, z, z + step}, ys[3] = {y - step, y, y + step};
-        auto chunk = [&](int m, int& dst) {
-            const int e = threadIdx.x + m * W;
-            const int r = min(e / kCh, 8), k = e - r * kCh;
-            const int zz = r < 3 ? zs[0] : (r < 6 ? zs[1] : zs[2]);
-            const int rr = r - 3 * (r / 3);
-            const int yy = rr == 0 ? ys[0] : (rr == 1 ? ys[1] : ys[2]);
-            const bool ok = e < 9 * kCh && zz >= 0 && zz < d && yy >= 0 && yy < h;
+        // one thread per cell of the row: the 9 x 192 16-byte chunks of the
+        // staged rows are split so that a thread's chunks share their row's
+        // y offset and column: threads t < 384 take column t % 192 of rows g,
+        // g + 3, g + 6 (g = t / 192), threads t >= 384 column t - 384 of rows
+        // 2, 5, 8, and threads t < 192 also column 128 + t % 64 of row
+        // 2 + 3 (t / 64).  Rows outside the volume read row 0 instead and are
+        // not stored; all loads are issued before the first LDS store.
+        constexpr int kCh = 3 * W / 8;   // 192 chunks a row in either layout
+        static_assert(IN_AOS || W / 4 + W / 8 == kCh, "chunks a row");
+        const int t = threadIdx.x;
+        const int g = t < 384 ? (t < 192 ? 0 : 1) : 2;
+        const int k = t < 384 ? t - 192 * g : t - 384;
+        const int ke = 128 + (t & 63), je = t >> 6;   // the extra chunk (t < 192)
+        // source offset (16-byte units) and LDS word of chunk kk of staged
+        // row r = rg + 3 j, whose source row is (z + (j - 1) step, yy)
+        auto chunk = [&](int rg, int j, int kk, int yy, int& dst) -> size_t {
+            const int zz = z + (j - 1) * step;
+            const int r = rg + 3 * j;
+            const bool ok = zz >= 0 && zz < d && yy >= 0 && yy < h;
             const size_t sr = ok ? (size_t)zz * h + yy : 0;
-            size_t off;   // in 16-byte units from `in`
+            size_t off;
             if (IN_AOS) {
-                off = sr * (3 * W / 8) + (ok ? k : 0);
-                dst = r * (3 * W / 2) + 4 * k;
-            } else if (k < W / 4) {
-                off = sr * (W / 4) + k;
-                dst = r * W + 4 * k;
+                off = sr * kCh + kk;
+                dst = r * (3 * W / 2) + 4 * kk;
+            } else if (kk < W / 4) {
+                off = sr * (W / 4) + kk;
+                dst = r * W + 4 * kk;
             } else {
-                off = cells / 4 + sr * (W / 8) + (k - W / 4);   // the z plane starts at byte 4 * cells
-                dst = 9 * W + r * (W / 2) + 4 * (k - W / 4);
+                off = cells / 4 + sr * (W / 8) + (kk - W / 4);   // the z plane starts at byte 4 * cells
+                dst = 9 * W + r * (W / 2) + 4 * (kk - W / 4);
             }
             if (!ok) dst = -1;
             return off;
         };
-        const uint4* in4 = (const uint4*)in;
+        const int yg = y + (g - 1) * step;
         int d0, d1, d2, d3;
-        const size_t o0 = chunk(0, d0), o1 = chunk(1, d1), o2 = chunk(2, d2), o3 = chunk(3, d3);
-        const uint4 v0 = in4[o0], v1 = in4[o1], v2 = in4[o2], v3 = in4[o3];
+        const size_t o0 = chunk(g, 0, k, yg, d0), o1 = chunk(g, 1, k, yg, d1), o2 = chunk(g, 2, k, yg, d2);
+        const size_t o3 = chunk(2, je < 3 ? je : 0, ke, y + step, d3);
+        if (t >= 192) d3 = -1;
+        const uint4* in4 = (const uint4*)in;
+        const uint4 v0 = in4[o0], v1 = in4[o1], v2 = in4[o2];
+        const uint4 v3 = in4[t < 192 ? o3 : 0];
         if (d0 >= 0) *(uint4*)(lds + d0) = v0;
         if (d1 >= 0) *(uint4*)(lds + d1) = v1;
         if (d2 >= 0) *(uint4*)(lds + d2) = v2;
