@@ -294,16 +294,19 @@ __global__ void __launch_bounds__(kRowMax) k_jfa3d_rows(int step, int w_arg, int
     const bool okx[3] = {x - step >= 0, true, x + step < w};
     // byte offsets of the three tap columns, held in registers (the empty
     // asm keeps the compiler from re-forming them in every row block)
-    uint32_t oxy[3], oz[3];
+    typedef __attribute__((address_space(3))) const uint32_t* lds_u32;
+    typedef __attribute__((address_space(3))) const int16_t* lds_i16;
+    lds_u32 pxy[3];
+    lds_i16 pz[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        oxy[k] = 4u * (uint32_t)xs[k];
-        oz[k] = 2u * (uint32_t)xs[k];
-        asm volatile("" : "+v"(oxy[k]), "+v"(oz[k]));
+        pxy[k] = (lds_u32)(lds + xs[k]);
+        pz[k] = (lds_i16)(lds16 + 2 * 9 * w + xs[k]);
+        asm volatile("" : "+v"(pxy[k]), "+v"(pz[k]));
     }
     auto tap = [&](int r, int ic, uint32_t& xy, int& tz) {
-        xy = ((const uint32_t*)((const char*)lds + oxy[ic + 1]))[r * w];
-        tz = ((const int16_t*)((const char*)(lds16 + 2 * 9 * w) + oz[ic + 1]))[r * w];
+        xy = pxy[ic + 1][r * w];
+        tz = pz[ic + 1][r * w];
     };
     const int off = cw * step;
     // candidate of tap t = 9 (kc+1) + 3 (jc+1) + (ic+1): the source offset by
