@@ -35,8 +35,14 @@ __global__ void __launch_bounds__(256) k_jfa2d_x4(int step, int w, int h, int xb
                                                   const uint32_t* __restrict__ in,
                                                   uint32_t* __restrict__ out, int cw)
 {
-    const int y = blockIdx.x / xb;
-    const int x = ((blockIdx.x - y * xb) * 256 + threadIdx.x) * 4;
+    // rows in residue classes mod step (y = r, r + step, r + 2 step, ...)
+    // when step divides h: consecutive rows then share two of their three
+    // source rows, which stay in L2 (the xb blocks of a row go to distinct
+    // XCDs, each XCD keeping its own column range)
+    const int q = blockIdx.x / xb;
+    const int nq = h / step;
+    const int y = nq * step == h ? (q % nq) * step + q / nq : q;
+    const int x = ((blockIdx.x - q * xb) * 256 + threadIdx.x) * 4;
     if (x >= w) return;
     const int off = cw * step;
     const bool up = y - step >= 0, down = y + step < h;
