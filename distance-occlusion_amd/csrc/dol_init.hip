@@ -8,7 +8,8 @@
 // land on its cell.  The windows that can reach cell (x, y[, z]) are those at
 // bit b in {x, x+1}, row pair q in {y-1, y} and slice pair p in {z-1, z}, so a
 // thread needs just the 3x3(x3) pixels around it; cells whose neighbourhood is
-// uniform receive no write and stay at kMaxDelta.
+// uniform receive no write and stay at kMaxDelta.  In 2D the same replay is
+// evaluated for the 32 cells of a mask word at once, as bit masks.
 #include "dol_cells.hpp"
 
 namespace dolhip {
@@ -144,10 +145,75 @@ __device__ __forceinline__ RowBits row_bits(const uint32_t* row, int j, int ws, 
     return b;
 }
 
-// One thread per 32 cells (one mask word of a row).  A cell whose 3x3
-// neighbourhood inside the image is uniform gets no write; such cells are
-// found for the whole word with bit operations, their value is stored in
-// bulk, and only the others run the per-cell replay.
+// Writes of SetBorderCellsCombo (DistanceOcclusionLib.cpp:504-599) to the 32
+// cells of one mask word, for all cells at once: bit i of each word belongs
+// to cell 32j + i.  It is the per-cell replay of border2_cell with every
+// condition evaluated as a bit mask.  Among the writes that land on a cell the
+// last unconditional one wins (order e3, f3, f4, f5; at a word's last bit the
+// reference's loop order makes it e3, f3, f5, f4), and with none of those the
+// first conditional one (e1, e2, f1, f2).
+struct Border2Word {
+    uint32_t any, xn, xp, yn, yp;   // written; x = -1; x = +1; y = -1; y = +1
+};
+
+__device__ __forceinline__ Border2Word border2_word(const RowBits& up, const RowBits& mid,
+                                                    const RowBits& dn, bool has_up, bool has_dn,
+                                                    uint32_t right)
+{
+    uint32_t e1 = 0, e2 = 0, e3 = 0, f1 = 0, f2 = 0, f5 = 0;
+    if (has_up) {   // row pair (y-1, y): this row is the lower one
+        const uint32_t vert0 = up.m ^ mid.m, horz0 = up.m ^ up.l, vprev0 = up.l ^ mid.l;
+        const uint32_t vert1 = up.r ^ mid.r, horz1 = up.r ^ up.m;
+        e1 = (vert0 ^ vprev0) & vprev0 & horz0;
+        e2 = right & (vert1 ^ vert0) & ~vert0 & horz1;
+        e3 = vert0;
+    }
+    const uint32_t horz0 = mid.m ^ mid.l, horz1 = mid.r ^ mid.m;
+    if (has_dn) {   // row pair (y, y+1): this row is the upper one
+        const uint32_t vert0 = mid.m ^ dn.m, vprev0 = mid.l ^ dn.l, vert1 = mid.r ^ dn.r;
+        f1 = (vert0 ^ vprev0) & vprev0 & ~horz0;
+        f2 = right & (vert1 ^ vert0) & ~vert0 & ~horz1;
+        f5 = vert0;
+    }
+    const uint32_t f3 = horz0, f4 = right & horz1;
+    const uint32_t last = 0x80000000u;
+    const uint32_t s5 = f5 & ~(f4 & last), s4 = f4 & ~(f5 & ~last);
+    const uint32_t s3 = f3 & ~f4 & ~f5, se = e3 & ~f3 & ~f4 & ~f5;
+    const uint32_t set = e3 | f3 | f4 | f5;
+    const uint32_t u1 = ~set & e1, u2 = ~set & ~e1 & e2;
+    const uint32_t u3 = ~set & ~e1 & ~e2 & f1, u4 = ~set & ~e1 & ~e2 & ~f1 & f2;
+    Border2Word b;
+    b.any = set | e1 | e2 | f1 | f2;
+    b.xn = s3 | u1 | u3;
+    b.xp = s4 | u2 | u4;
+    b.yn = se | u1 | u2;
+    b.yp = s5 | u3 | u4;
+    return b;
+}
+
+__device__ __forceinline__ uint32_t border2_value(const Border2Word& b, int i)
+{
+    if (!((b.any >> i) & 1u)) return pack2(kMaxDelta2, kMaxDelta2);
+    const int sx = (int)((b.xp >> i) & 1u) - (int)((b.xn >> i) & 1u);
+    const int sy = (int)((b.yp >> i) & 1u) - (int)((b.yn >> i) & 1u);
+    return pack2(sx, sy);
+}
+
+// Border masks of mask word j of row y.
+__device__ __forceinline__ Border2Word border2_masks(const uint32_t* mask, int ws, int w, int h,
+                                                     int j, int y)
+{
+    const RowBits zero{0u, 0u, 0u};
+    const bool has_up = y >= 1, has_dn = y + 1 < h;
+    const RowBits mid = row_bits(mask + (size_t)y * ws, j, ws, w);
+    const RowBits up = has_up ? row_bits(mask + (size_t)(y - 1) * ws, j, ws, w) : zero;
+    const RowBits dn = has_dn ? row_bits(mask + (size_t)(y + 1) * ws, j, ws, w) : zero;
+    const uint32_t right = j == ws - 1 ? ~(1u << ((w - 1) & 31)) : ~0u;
+    return border2_word(up, mid, dn, has_up, has_dn, right);
+}
+
+// One thread per 32 cells (one mask word of a row): the word's fill value is
+// stored in bulk and the written cells are overwritten.
 __global__ void __launch_bounds__(256) k_border_2d_words(int w, int h,
                                                          const uint32_t* __restrict__ mask,
                                                          uint32_t* __restrict__ delta)
@@ -157,16 +223,8 @@ __global__ void __launch_bounds__(256) k_border_2d_words(int w, int h,
     if (gid >= ws * h) return;
     const int y = gid / ws, j = gid - y * ws;
     const int x0 = 32 * j, n = min(32, w - x0);
-    const uint32_t centre = mask[(size_t)y * ws + j];
-    uint32_t diff = 0;
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        const int yy = y - 1 + k;
-        if (yy < 0 || yy >= h) continue;
-        const RowBits b = row_bits(mask + (size_t)yy * ws, j, ws, w);
-        diff |= (centre ^ b.l) | (centre ^ b.m) | (centre ^ b.r);
-    }
-    if (n < 32) diff &= (1u << n) - 1u;
+    const Border2Word b = border2_masks(mask, ws, w, h, j, y);
+    uint32_t any = n < 32 ? b.any & ((1u << n) - 1u) : b.any;
     uint32_t* out = delta + (size_t)y * w + x0;
     const uint32_t fill = pack2(kMaxDelta2, kMaxDelta2);
     if (n == 32 && ((uintptr_t)out & 15) == 0) {
@@ -175,10 +233,52 @@ __global__ void __launch_bounds__(256) k_border_2d_words(int w, int h,
     } else {
         for (int k = 0; k < n; k++) out[k] = fill;
     }
-    while (diff) {
-        const int i = __ffs(diff) - 1;
-        diff &= diff - 1u;
-        out[i] = border2_cell(mask, ws, w, h, x0 + i, y);
+    while (any) {
+        const int i = __ffs(any) - 1;
+        any &= any - 1u;
+        out[i] = border2_value(b, i);
+    }
+}
+
+// Same result for w % 32 == 0, where mask word g covers the flat cells
+// 32g .. 32g+31: a wave's 64 words are 2048 contiguous cells, which it writes
+// as 8 stores of 64 consecutive 16-byte chunks (4 cells a lane).  A chunk
+// whose 8 words have no written cell is the fill; otherwise its lanes take the
+// masks from the word's owning lane by shuffles.
+__global__ void __launch_bounds__(256) k_border_2d_flat(int w, int h,
+                                                        const uint32_t* __restrict__ mask,
+                                                        uint4* __restrict__ delta)
+{
+    const int ws = w >> 5;
+    const int words = ws * h;
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int word0 = gid - lane;
+    Border2Word b{0u, 0u, 0u, 0u, 0u};
+    if (gid < words) {
+        const int y = gid / ws;
+        b = border2_masks(mask, ws, w, h, gid - y * ws, y);
+    }
+    const uint64_t busy = __ballot(b.any != 0u);
+    const uint32_t fill = pack2(kMaxDelta2, kMaxDelta2);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int u = k * 64 + lane;
+        const int g = word0 + (u >> 3);
+        uint4 v = make_uint4(fill, fill, fill, fill);
+        if ((busy >> (8 * k)) & 0xffu) {
+            const int src = u >> 3;
+            Border2Word c;
+            c.any = __shfl(b.any, src);
+            c.xn = __shfl(b.xn, src);
+            c.xp = __shfl(b.xp, src);
+            c.yn = __shfl(b.yn, src);
+            c.yp = __shfl(b.yp, src);
+            const int q = (u & 7) * 4;
+            v = make_uint4(border2_value(c, q), border2_value(c, q + 1), border2_value(c, q + 2),
+                           border2_value(c, q + 3));
+        }
+        if (g < words) delta[(size_t)g * 8 + (u & 7)] = v;
     }
 }
 
@@ -186,8 +286,12 @@ void launch_border_2d(int w, int h, const uint32_t* mask, int16_t* delta, hipStr
 {
     ProfileScope p("border_2d", s);
     const int words = mask_stride(w) * h;
-    hipLaunchKernelGGL(k_border_2d_words, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, w,
-                       h, mask, (uint32_t*)delta);
+    const dim3 grid((unsigned)((words + 255) / 256));
+    if ((w & 31) == 0 && ((uintptr_t)delta & 15) == 0)
+        hipLaunchKernelGGL(k_border_2d_flat, grid, dim3(256), 0, s, w, h, mask, (uint4*)delta);
+    else
+        hipLaunchKernelGGL(k_border_2d_words, grid, dim3(256), 0, s, w, h, mask,
+                           (uint32_t*)delta);
 }
 
 // ------------------------------------------------------------ border init 3D

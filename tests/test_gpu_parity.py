@@ -71,6 +71,15 @@ def test_border_2d(gpu, oracle, reference, rng, dims):
             np.testing.assert_array_equal(got, reference.border_2d(w, h, m))
 
 
+@pytest.mark.parametrize("dims", [(32, 1), (96, 5), (160, 13), (320, 37), (2048, 3), (4096, 2)])
+def test_border_2d_word_rows(gpu, oracle, rng, dims):
+    """Widths that are a multiple of 32 (whole mask words per row): waves
+    spanning several rows, partial last waves."""
+    w, h = dims
+    for m in masks_for(rng, dims):
+        np.testing.assert_array_equal(gpu.init_border_deltas_2d(w, h, m), oracle.border_2d(w, h, m))
+
+
 @pytest.mark.parametrize("dims", SIZES_3D)
 def test_border_3d(gpu, oracle, reference, rng, dims):
     w, h, d = dims
