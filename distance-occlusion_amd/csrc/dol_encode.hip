@@ -281,6 +281,46 @@ __global__ void __launch_bounds__(256) k_two_field_2d(size_t n, const uint32_t* 
     ((uint32_t*)out)[i] = r | (g << 8) | (255u << 24);
 }
 
+// Same, four cells a thread (n % 4 == 0, 16-byte aligned delta images): one
+// 16-byte load from each field and one 4-byte (type 0) or 16-byte store.
+template <int TYPE>
+__device__ __forceinline__ uint32_t two_field_cell(uint32_t a, uint32_t b, float max_len)
+{
+    const float da = dist_of(len2(unpack2x(a), unpack2y(a)));
+    const float db = dist_of(len2(unpack2x(b), unpack2y(b)));
+    if (TYPE == 0) return encode_u8_signed(__fdiv_rn(__fsub_rn(da, db), max_len));
+    const float sx = (float)(unpack2x(a) + unpack2x(b)), sy = (float)(unpack2y(a) + unpack2y(b));
+    uint32_t r, g;
+    if (TYPE == 1) {
+        r = encode_u8_signed(__fdiv_rn(sx, max_len));
+        g = encode_u8_signed(__fdiv_rn(sy, max_len));
+    } else if (TYPE == 2) {
+        r = encode_u8(__fdiv_rn(da, max_len));
+        g = encode_u8(__fdiv_rn(db, max_len));
+    } else {
+        const float dd = __fsub_rn(da, db);
+        r = encode_u8_signed(__fdiv_rn(sx, dd));
+        g = encode_u8_signed(__fdiv_rn(sy, dd));
+    }
+    return r | (g << 8) | (255u << 24);
+}
+
+template <int TYPE>
+__global__ void __launch_bounds__(256) k_two_field_2d_x4(size_t n4, const uint4* __restrict__ d0,
+                                                         const uint4* __restrict__ d1, float max_len,
+                                                         void* __restrict__ out)
+{
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n4) return;
+    const uint4 a = d0[q], b = d1[q];
+    const uint32_t c0 = two_field_cell<TYPE>(a.x, b.x, max_len), c1 = two_field_cell<TYPE>(a.y, b.y, max_len);
+    const uint32_t c2 = two_field_cell<TYPE>(a.z, b.z, max_len), c3 = two_field_cell<TYPE>(a.w, b.w, max_len);
+    if (TYPE == 0)
+        ((uint32_t*)out)[q] = c0 | (c1 << 8) | (c2 << 16) | (c3 << 24);
+    else
+        ((uint4*)out)[q] = make_uint4(c0, c1, c2, c3);
+}
+
 // 3D: the same per slice, blue the z channel of type 1 and 3.  Type 3 reads
 // the first slice's deltas for every slice, as the tool does
 // (DistanceOcclusionTool.cpp:1178-1186 index deltas0/deltas1, not the slice's).
@@ -319,6 +359,19 @@ void launch_two_field_2d(int w, int h, const int16_t* d0, const int16_t* d1, flo
     const dim3 grid(grid1d(n, 256)), block(256);
     const uint32_t *a = (const uint32_t*)d0, *b = (const uint32_t*)d1;
     ProfileScope p("encode_two_field_2d", s);
+    if (n % 4 == 0 && (((uintptr_t)d0 | (uintptr_t)d1) & 15) == 0 &&
+        ((uintptr_t)out & (type == 0 ? 3 : 15)) == 0) {
+        const size_t n4 = n / 4;
+        const dim3 g4(grid1d(n4, 256));
+        const uint4 *a4 = (const uint4*)d0, *b4 = (const uint4*)d1;
+        switch (type) {
+        case 0: hipLaunchKernelGGL(k_two_field_2d_x4<0>, g4, block, 0, s, n4, a4, b4, max_len, out); break;
+        case 1: hipLaunchKernelGGL(k_two_field_2d_x4<1>, g4, block, 0, s, n4, a4, b4, max_len, out); break;
+        case 2: hipLaunchKernelGGL(k_two_field_2d_x4<2>, g4, block, 0, s, n4, a4, b4, max_len, out); break;
+        default: hipLaunchKernelGGL(k_two_field_2d_x4<3>, g4, block, 0, s, n4, a4, b4, max_len, out); break;
+        }
+        return;
+    }
     switch (type) {
     case 0: hipLaunchKernelGGL(k_two_field_2d<0>, grid, block, 0, s, n, a, b, max_len, out); break;
     case 1: hipLaunchKernelGGL(k_two_field_2d<1>, grid, block, 0, s, n, a, b, max_len, out); break;

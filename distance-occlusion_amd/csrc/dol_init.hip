@@ -66,6 +66,20 @@ __global__ void __launch_bounds__(256) k_init_deltas_2d(int w, int xb, const uin
     delta[(size_t)row * w + x] = bit ? 0u : pack2(kMaxDelta2, kMaxDelta2);
 }
 
+// w % 4 == 0: four cells a thread over the flat image, one 16-byte store
+__global__ void __launch_bounds__(256) k_init_deltas_2d_x4(int w, size_t n4, const uint32_t* __restrict__ mask,
+                                                           uint4* __restrict__ delta, uint32_t invert)
+{
+    const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= n4) return;
+    const size_t i = 4 * q, row = i / (size_t)w;
+    const int x = (int)(i - row * (size_t)w);
+    const uint32_t bits = (mask[row * mask_stride(w) + (x >> 5)] >> (x & 31)) ^ (invert ? 15u : 0u);
+    const uint32_t far = pack2(kMaxDelta2, kMaxDelta2);
+    delta[q] = make_uint4(bits & 1u ? 0u : far, bits & 2u ? 0u : far, bits & 4u ? 0u : far,
+                          bits & 8u ? 0u : far);
+}
+
 __global__ void __launch_bounds__(256) k_init_deltas_3d(int w, int xb, const uint32_t* __restrict__ mask,
                                                         int16_t* __restrict__ delta, uint32_t invert)
 {
@@ -84,6 +98,12 @@ void launch_init_deltas_2d(int w, int h, const uint32_t* mask, int16_t* delta, i
 {
     int xb = (w + 255) / 256;
     ProfileScope p("init_deltas_2d", s);
+    if (w % 4 == 0 && ((uintptr_t)delta & 15) == 0) {
+        const size_t n4 = (size_t)w * h / 4;
+        hipLaunchKernelGGL(k_init_deltas_2d_x4, dim3(grid1d(n4, 256)), dim3(256), 0, s, w, n4, mask,
+                           (uint4*)delta, invert ? 1u : 0u);
+        return;
+    }
     hipLaunchKernelGGL(k_init_deltas_2d, dim3((unsigned)(xb * h)), dim3(256), 0, s, w, xb, mask,
                        (uint32_t*)delta, invert ? 1u : 0u);
 }
