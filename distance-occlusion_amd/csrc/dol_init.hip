@@ -3,13 +3,13 @@
 // Border initialisation: the reference builds the signed border seeds by
 // scattering writes from 2x2 (2x2x2) mask windows, row pair after row pair,
 // word after word, and lets later writes win (SetBorderCellsCombo,
-// DistanceOcclusionLib.cpp:504-599 and 1820-2150).  Here every thread owns one
-// output cell and replays, in that same global order, only the writes that
-// land on its cell.  The windows that can reach cell (x, y[, z]) are those at
-// bit b in {x, x+1}, row pair q in {y-1, y} and slice pair p in {z-1, z}, so a
-// thread needs just the 3x3(x3) pixels around it; cells whose neighbourhood is
-// uniform receive no write and stay at kMaxDelta.  In 2D the same replay is
-// evaluated for the 32 cells of a mask word at once, as bit masks.
+// DistanceOcclusionLib.cpp:504-599 and 1820-2150).  Here each output cell's
+// value is found by replaying, in that same global order, only the writes
+// that land on it.  The windows that can reach cell (x, y[, z]) are those at
+// bit b in {x, x+1}, row pair q in {y-1, y} and slice pair p in {z-1, z}, so
+// only the 3x3(x3) pixels around it matter; cells whose neighbourhood is
+// uniform receive no write and stay at kMaxDelta.  The replay is evaluated
+// for the 32 cells of a mask word at once, as bit masks, one thread a word.
 #include "dol_cells.hpp"
 
 namespace dolhip {
@@ -316,159 +316,180 @@ void launch_border_2d(int w, int h, const uint32_t* mask, int16_t* delta, hipStr
 
 // ------------------------------------------------------------ border init 3D
 
-struct Cell3 {
-    int x, y, z;
-    __device__ void set(int vx, int vy, int vz) { x = vx; y = vy; z = vz; }
-};
-
 // Loop kinds of one window, in the reference's order
 // (DistanceOcclusionLib.cpp:1899-1973)
 enum : int { L_BC = 1, L_FXX = 2, L_FY = 4, L_FZ = 8, L_EX = 16, L_EYX = 32, L_EZX = 64 };
 
-// Pixel cube around the cell: bit (dz*9 + dy*3 + dx) is pixel
-// (x-1+dx, y-1+dy, z-1+dz), x = -1 replicating x = 0; outside the volume 0.
-__device__ __forceinline__ int cube_bit(uint32_t cube, int dx, int dy, int dz)
-{
-    return (int)((cube >> (dz * 9 + dy * 3 + dx)) & 1u);
-}
-
-struct Win3 {
-    int ex, ey, ez, ezx, eyx, fx, fy, fz, fxx, bc;
+// The writes that land on cell (x, y, z) come from the 2x2x2 windows at bit
+// b in {x, x+1}, row pair q in {y-1, y} and slice pair p in {z-1, z}
+// (window features: DistanceOcclusionLib.cpp:1862-1894; loops and their
+// order: 1899-1973; the last row's and last slice's extra loops: 1975-2150).
+// Every such write has a value fixed by the loop kind and the cell's place
+// in the window ((cx, cy, cz) below), so the 32 cells of a mask word are
+// replayed together as bit masks: a write's fire mask is its window test
+// over the word, and the cells' state is kept as bit planes (the signs of x,
+// y and z, "written", and "last write was an edge", which is what the face
+// writes' condition abs sum > 1 excludes).  Windows at b = x and b = x + 1
+// alternate loop kind by loop kind; at a word's last bit the reference runs
+// all of the first window's kinds before the second's, so the second
+// window's writes to bit 31 are replayed after the alternating run.  A cell
+// whose neighbourhood inside the volume is uniform gets no write and keeps
+// kMaxDelta3 in all three components.
+struct Planes3 {
+    uint32_t xn, xp, yn, yp, zn, zp, set, edge;
 };
 
-// window whose "x-1" column is cube column kx (0 or 1), rows ky/ky+1,
-// slices kz/kz+1 (DistanceOcclusionLib.cpp:1862-1894)
-__device__ __forceinline__ Win3 window3(uint32_t cube, int kx, int ky, int kz)
+struct Win3b {
+    uint32_t ex, ey, ez, ezx, eyx, fx, fy, fz, fxx, bc;
+};
+
+template <int C>
+__device__ __forceinline__ uint32_t rb_col(const RowBits& b)
 {
-    const int p000 = cube_bit(cube, kx, ky, kz), p001 = cube_bit(cube, kx + 1, ky, kz);
-    const int p010 = cube_bit(cube, kx, ky + 1, kz), p011 = cube_bit(cube, kx + 1, ky + 1, kz);
-    const int p100 = cube_bit(cube, kx, ky, kz + 1), p101 = cube_bit(cube, kx + 1, ky, kz + 1);
-    const int p110 = cube_bit(cube, kx, ky + 1, kz + 1), p111 = cube_bit(cube, kx + 1, ky + 1, kz + 1);
-    Win3 r;
+    return C == 0 ? b.l : (C == 1 ? b.m : b.r);
+}
+
+// rb[dz * 3 + dy]: rows y - 1 + dy of slices z - 1 + dz (zero outside)
+template <int KX, int KY, int KZ>
+__device__ __forceinline__ Win3b window3b(const RowBits* rb)
+{
+    const RowBits &r00 = rb[KZ * 3 + KY], &r01 = rb[KZ * 3 + KY + 1];
+    const RowBits &r10 = rb[(KZ + 1) * 3 + KY], &r11 = rb[(KZ + 1) * 3 + KY + 1];
+    const uint32_t p000 = rb_col<KX>(r00), p001 = rb_col<KX + 1>(r00);
+    const uint32_t p010 = rb_col<KX>(r01), p011 = rb_col<KX + 1>(r01);
+    const uint32_t p100 = rb_col<KX>(r10), p101 = rb_col<KX + 1>(r10);
+    const uint32_t p110 = rb_col<KX>(r11), p111 = rb_col<KX + 1>(r11);
+    Win3b r;
     r.ex = p000 ^ p001;
     r.ey = p000 ^ p010;
     r.ez = p000 ^ p100;
     r.ezx = p001 ^ p101;
-    const int eyz = p100 ^ p110, exy = p010 ^ p011;
+    const uint32_t eyz = p100 ^ p110, exy = p010 ^ p011;
     r.fx = r.ey ^ eyz;
     r.fy = r.ez ^ r.ezx;
     r.fz = r.ex ^ exy;
     r.eyx = p001 ^ p011;
-    const int eyxz = p101 ^ p111;
+    const uint32_t eyxz = p101 ^ p111;
     r.fxx = r.eyx ^ eyxz;
     r.bc = r.fz ^ eyz ^ eyxz;
     return r;
 }
 
-// Apply loop `kind` of window (b, q, p) to the cell (tx, ty, tz relative
-// offsets: the cell sits at (b - bx, q - qy, p - pz) for window origin
-// offsets bx in {0, 1} (b = x or x+1), etc.  Coordinates here are relative to
-// the window: the cell's position inside the 2x2x2 box is (cx, cy, cz) with
-// cx in {0: column b-1, 1: column b}, cy in {0: row q, 1: row q+1},
-// cz in {0: slice p, 1: slice p+1}.
-__device__ __forceinline__ void apply3(Cell3& s, const Win3& v, int kind, int cx, int cy, int cz)
+// bits where the window feature equals the cell's coordinate V (0 or 1)
+template <int V>
+__device__ __forceinline__ uint32_t eqm(uint32_t f)
 {
-    switch (kind) {
-    case L_BC:
-        if (v.bc && cx == v.fx && cy == v.fy && cz == v.fz && s.x == kMaxDelta3)
-            s.set(1 - 2 * v.fx, 1 - 2 * v.fy, 1 - 2 * v.fz);
-        break;
-    case L_FXX:
-        if (v.fxx && cx == 1 && cy == v.ezx && cz == v.eyx &&
-            (s.x == kMaxDelta3 || abs(s.x) + abs(s.y) + abs(s.z) > 1))
-            s.set(0, 1 - 2 * v.ezx, 1 - 2 * v.eyx);
-        break;
-    case L_FY:
-        if (v.fy && cx == v.ez && cy == 0 && cz == v.ex &&
-            (s.x == kMaxDelta3 || abs(s.x) + abs(s.y) + abs(s.z) > 1))
-            s.set(1 - 2 * v.ez, 0, 1 - 2 * v.ex);
-        break;
-    case L_FZ:
-        if (v.fz && cx == v.ey && cy == v.ex && cz == 0 &&
-            (s.x == kMaxDelta3 || abs(s.x) + abs(s.y) + abs(s.z) > 1))
-            s.set(1 - 2 * v.ey, 1 - 2 * v.ex, 0);
-        break;
-    case L_EX:
-        if (v.ex && cy == 0 && cz == 0) {
-            if (cx == 0) s.set(+1, 0, 0);
-            else s.set(-1, 0, 0);
-        }
-        break;
-    case L_EYX:
-        if (v.eyx && cx == 1 && cz == 0) {
-            if (cy == 0) s.set(0, +1, 0);
-            else s.set(0, -1, 0);
-        }
-        break;
-    case L_EZX:
-        if (v.ezx && cx == 1 && cy == 0) {
-            if (cz == 0) s.set(0, 0, +1);
-            else s.set(0, 0, -1);
-        }
-        break;
+    return V ? f : ~f;
+}
+
+// write (VX, VY, VZ) to the cells of m; EDGE: an edge write
+template <int VX, int VY, int VZ, bool EDGE>
+__device__ __forceinline__ void write3(Planes3& s, uint32_t m)
+{
+    s.xn = VX < 0 ? (s.xn | m) : (s.xn & ~m);
+    s.xp = VX > 0 ? (s.xp | m) : (s.xp & ~m);
+    s.yn = VY < 0 ? (s.yn | m) : (s.yn & ~m);
+    s.yp = VY > 0 ? (s.yp | m) : (s.yp & ~m);
+    s.zn = VZ < 0 ? (s.zn | m) : (s.zn & ~m);
+    s.zp = VZ > 0 ? (s.zp | m) : (s.zp & ~m);
+    s.set |= m;
+    s.edge = EDGE ? (s.edge | m) : (s.edge & ~m);
+}
+
+// loop kind KIND of a window to the cells of `bits`, which sit at (CX, CY,
+// CZ) in it: the kind's feature test, the cell's place, and the condition
+// on what the cell holds (corner: unwritten; faces: not an edge)
+template <int KIND, int CX, int CY, int CZ>
+__device__ __forceinline__ void apply3b(Planes3& s, const Win3b& v, uint32_t bits)
+{
+    if (KIND == L_BC) {
+        const uint32_t m = bits & v.bc & eqm<CX>(v.fx) & eqm<CY>(v.fy) & eqm<CZ>(v.fz) & ~s.set;
+        write3<1 - 2 * CX, 1 - 2 * CY, 1 - 2 * CZ, false>(s, m);
+    } else if (KIND == L_FXX) {
+        if (CX != 1) return;
+        const uint32_t m = bits & v.fxx & eqm<CY>(v.ezx) & eqm<CZ>(v.eyx) & ~s.edge;
+        write3<0, 1 - 2 * CY, 1 - 2 * CZ, false>(s, m);
+    } else if (KIND == L_FY) {
+        if (CY != 0) return;
+        const uint32_t m = bits & v.fy & eqm<CX>(v.ez) & eqm<CZ>(v.ex) & ~s.edge;
+        write3<1 - 2 * CX, 0, 1 - 2 * CZ, false>(s, m);
+    } else if (KIND == L_FZ) {
+        if (CZ != 0) return;
+        const uint32_t m = bits & v.fz & eqm<CX>(v.ey) & eqm<CY>(v.ex) & ~s.edge;
+        write3<1 - 2 * CX, 1 - 2 * CY, 0, false>(s, m);
+    } else if (KIND == L_EX) {
+        if (CY != 0 || CZ != 0) return;
+        write3<CX == 0 ? 1 : -1, 0, 0, true>(s, bits & v.ex);
+    } else if (KIND == L_EYX) {
+        if (CX != 1 || CZ != 0) return;
+        write3<0, CY == 0 ? 1 : -1, 0, true>(s, bits & v.eyx);
+    } else if (KIND == L_EZX) {
+        if (CX != 1 || CY != 0) return;
+        write3<0, 0, CZ == 0 ? 1 : -1, true>(s, bits & v.ezx);
     }
 }
 
-// One row-pair step (fixed q, p) for the windows at b = x and b = x + 1 with
-// the loop kinds in `kinds`, ordered as the reference's word loop orders
-// them: loop-major when both bits share a word, bit-major across a word end.
-__device__ __forceinline__ void row_step3(Cell3& s, uint32_t cube, int kinds, bool right,
-                                          bool word_end, int ky, int kz, int cy, int cz)
+template <int KINDS, int KIND, int CX, int CY, int CZ>
+__device__ __forceinline__ void apply3b_if(Planes3& s, const Win3b& v, uint32_t bits)
 {
-    // window at b = x: columns x-1, x -> kx = 0, cell is column b (cx = 1)
-    // window at b = x+1: columns x, x+1 -> kx = 1, cell is column b-1 (cx = 0)
-    const Win3 w0 = window3(cube, 0, ky, kz);
-    const Win3 w1 = window3(cube, 1, ky, kz);
-    if (!word_end) {
-#pragma unroll
-        for (int k = 1; k <= L_EZX; k <<= 1) {
-            if (!(kinds & k)) continue;
-            apply3(s, w0, k, 1, cy, cz);
-            if (right) apply3(s, w1, k, 0, cy, cz);
-        }
-    } else {
-#pragma unroll
-        for (int k = 1; k <= L_EZX; k <<= 1)
-            if (kinds & k) apply3(s, w0, k, 1, cy, cz);
-        if (right) {
-#pragma unroll
-            for (int k = 1; k <= L_EZX; k <<= 1)
-                if (kinds & k) apply3(s, w1, k, 0, cy, cz);
-        }
+    if (KINDS & KIND) apply3b<KIND, CX, CY, CZ>(s, v, bits);
+}
+
+// one row-pair step (fixed q, p) with the loop kinds KINDS: the window at
+// b = x (cell column b, cx = 1) and at b = x + 1 (cell column b - 1, cx = 0;
+// cells with x + 1 < w only); window rows KY, KY + 1 and slices KZ, KZ + 1
+// of the cell's 3x3x3 neighbourhood
+template <int KINDS, int KY, int KZ, int CY, int CZ>
+__device__ __forceinline__ void row_step3b(Planes3& s, const RowBits* rb, uint32_t right)
+{
+    const Win3b w0 = window3b<0, KY, KZ>(rb);
+    const Win3b w1 = window3b<1, KY, KZ>(rb);
+    const uint32_t last = 0x80000000u;
+    const uint32_t inner = right & ~last, end = right & last;
+#define DOL_STEP3B(K)                                   \
+    apply3b_if<KINDS, K, 1, CY, CZ>(s, w0, ~0u);        \
+    apply3b_if<KINDS, K, 0, CY, CZ>(s, w1, inner);
+    DOL_STEP3B(L_BC)
+    DOL_STEP3B(L_FXX)
+    DOL_STEP3B(L_FY)
+    DOL_STEP3B(L_FZ)
+    DOL_STEP3B(L_EX)
+    DOL_STEP3B(L_EYX)
+    DOL_STEP3B(L_EZX)
+#undef DOL_STEP3B
+    if (end) {
+        apply3b_if<KINDS, L_BC, 0, CY, CZ>(s, w1, end);
+        apply3b_if<KINDS, L_FXX, 0, CY, CZ>(s, w1, end);
+        apply3b_if<KINDS, L_FY, 0, CY, CZ>(s, w1, end);
+        apply3b_if<KINDS, L_FZ, 0, CY, CZ>(s, w1, end);
+        apply3b_if<KINDS, L_EX, 0, CY, CZ>(s, w1, end);
+        apply3b_if<KINDS, L_EYX, 0, CY, CZ>(s, w1, end);
+        apply3b_if<KINDS, L_EZX, 0, CY, CZ>(s, w1, end);
     }
 }
 
-// The cell's value given its pixel cube and the cube bits inside the volume:
-// uniform neighbourhoods get no write; otherwise the reference's writes that
-// land on the cell are replayed in order (DistanceOcclusionLib.cpp:1899-1973).
-__device__ __forceinline__ Cell3 border3_from_cube(uint32_t cube, uint32_t inside, bool right,
-                                                   bool word_end, int y, int z, int h, int d)
+// the row steps of slice pair z - 1 + PP: row pairs y - 1 and y, and the
+// last row's extra step
+template <int PP>
+__device__ __forceinline__ void slice_pair3b(Planes3& s, const RowBits* rb, uint32_t right, int y, int h)
 {
-    Cell3 s{kMaxDelta3, kMaxDelta3, kMaxDelta3};
-    const uint32_t in_set = cube & inside;
-    if (in_set != 0 && in_set != inside) {
-        const int all = L_BC | L_FXX | L_FY | L_FZ | L_EX | L_EYX | L_EZX;
-        // slice pairs p = z-1 (cell in slice p+1, cz = 1, kz = 0) then p = z
-        for (int pp = 0; pp < 2; pp++) {
-            const int p = z - 1 + pp;
-            if (p < 0 || p + 1 >= d) continue;
-            const int kz = pp, cz = 1 - pp;
-            for (int qq = 0; qq < 2; qq++) {
-                const int q = y - 1 + qq;
-                if (q < 0 || q + 1 >= h) continue;
-                row_step3(s, cube, all, right, word_end, qq, kz, 1 - qq, cz);
-            }
-            if (y == h - 1)   // last row of the pair: y faces, x and z edges
-                row_step3(s, cube, L_FY | L_EX | L_EZX, right, word_end, 1, kz, 0, cz);
-        }
-        if (z == d - 1) {     // last slice: z faces, x and y edges
-            for (int qq = 0; qq < 2; qq++) {
-                const int q = y - 1 + qq;
-                if (q < 0 || q + 1 >= h) continue;
-                row_step3(s, cube, L_FZ | L_EX | L_EYX, right, word_end, qq, 1, 1 - qq, 0);
-            }
-            if (y == h - 1) row_step3(s, cube, L_EX, right, word_end, 1, 1, 0, 0);
-        }
+    constexpr int kAll = L_BC | L_FXX | L_FY | L_FZ | L_EX | L_EYX | L_EZX;
+    if (y - 1 >= 0) row_step3b<kAll, 0, PP, 1, 1 - PP>(s, rb, right);
+    if (y + 1 < h) row_step3b<kAll, 1, PP, 0, 1 - PP>(s, rb, right);
+    if (y == h - 1) row_step3b<L_FY | L_EX | L_EZX, 1, PP, 0, 1 - PP>(s, rb, right);
+}
+
+// every row step that can reach the word's cells, in the reference's order
+// (slice pairs z - 1 then z, then the last slice's extra steps)
+__device__ __forceinline__ Planes3 border3_word(const RowBits* rb, uint32_t right, int y, int z, int h, int d)
+{
+    Planes3 s{0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+    if (z - 1 >= 0) slice_pair3b<0>(s, rb, right, y, h);
+    if (z + 1 < d) slice_pair3b<1>(s, rb, right, y, h);
+    if (z == d - 1) {
+        if (y - 1 >= 0) row_step3b<L_FZ | L_EX | L_EYX, 0, 1, 1, 0>(s, rb, right);
+        if (y + 1 < h) row_step3b<L_FZ | L_EX | L_EYX, 1, 1, 0, 0>(s, rb, right);
+        if (y == h - 1) row_step3b<L_EX, 1, 1, 0, 0>(s, rb, right);
     }
     return s;
 }
@@ -512,28 +533,85 @@ __global__ void __launch_bounds__(256) k_border_3d_words(int w, int h, int d,
     } else {
         for (int k = 0; k < 3 * n; k++) out[k] = (int16_t)kMaxDelta3;
     }
-    while (diff) {
-        const int i = __ffs(diff) - 1;
-        diff &= diff - 1u;
-        const int x = x0 + i;
-        const bool right = x + 1 < w;
-        uint32_t cube = 0, inside = 0;
+    if (!diff) return;
+    const uint32_t right = j == ws - 1 ? ~(1u << ((w - 1) & 31)) : ~0u;
+    const Planes3 p = border3_word(rb, right, y, z, h, d);
+    uint32_t set = p.set & diff;
+    while (set) {
+        const int i = __ffs(set) - 1;
+        set &= set - 1u;
+        out[3 * i] = (int16_t)((int)((p.xp >> i) & 1u) - (int)((p.xn >> i) & 1u));
+        out[3 * i + 1] = (int16_t)((int)((p.yp >> i) & 1u) - (int)((p.yn >> i) & 1u));
+        out[3 * i + 2] = (int16_t)((int)((p.zp >> i) & 1u) - (int)((p.zn >> i) & 1u));
+    }
+}
+
+// Same result for w % 32 == 0, where mask word g's cells are the flat bytes
+// 192g .. 192g + 191 of the delta image: a workgroup's 256 words are one
+// contiguous 48 KB run, built in LDS (fill, then the written cells) and
+// stored as consecutive 16-byte chunks.
+__global__ void __launch_bounds__(256) k_border_3d_flat(int w, int h, int d,
+                                                        const uint32_t* __restrict__ mask,
+                                                        uint4* __restrict__ delta)
+{
+    constexpr int kChunks = 12;   // 16-byte chunks of a word's 32 cells
+    __shared__ uint4 stage[256 * kChunks];
+    const int ws = w >> 5;
+    const size_t words = (size_t)ws * h * d;
+    const size_t word0 = (size_t)blockIdx.x * 256;
+    const size_t gid = word0 + threadIdx.x;
+    const uint32_t fill = (uint32_t)(uint16_t)kMaxDelta3 * 0x10001u;
+    uint4* mine = stage + threadIdx.x * kChunks;
+#pragma unroll
+    for (int k = 0; k < kChunks; k++) mine[k] = make_uint4(fill, fill, fill, fill);
+    if (gid < words) {
+        const int row = (int)(gid / ws), j = (int)(gid - (size_t)row * ws);
+        const int z = row / h, y = row - z * h;
+        RowBits rb[9];
+        uint32_t diff = 0;
+        const uint32_t centre = mask[gid];
 #pragma unroll
         for (int k = 0; k < 9; k++) {
-            if (!rin[k]) continue;
-            const uint32_t b = ((rb[k].l >> i) & 1u) | (((rb[k].m >> i) & 1u) << 1) |
-                               (right ? ((rb[k].r >> i) & 1u) << 2 : 0u);
-            cube |= b << (3 * k);
-            inside |= (right ? 7u : 3u) << (3 * k);
+            const int zz = z - 1 + k / 3, yy = y - 1 + k % 3;
+            rb[k] = RowBits{0u, 0u, 0u};
+            if (zz >= 0 && zz < d && yy >= 0 && yy < h) {
+                rb[k] = row_bits(mask + ((size_t)zz * h + yy) * ws, j, ws, w);
+                diff |= (centre ^ rb[k].l) | (centre ^ rb[k].m) | (centre ^ rb[k].r);
+            }
         }
-        const Cell3 s = border3_from_cube(cube, inside, right, (x & 31) == 31, y, z, h, d);
-        out[3 * i] = (int16_t)s.x; out[3 * i + 1] = (int16_t)s.y; out[3 * i + 2] = (int16_t)s.z;
+        if (diff) {
+            const uint32_t right = j == ws - 1 ? 0x7fffffffu : ~0u;
+            const Planes3 p = border3_word(rb, right, y, z, h, d);
+            uint32_t set = p.set & diff;
+            int16_t* out = (int16_t*)mine;
+            while (set) {
+                const int i = __ffs(set) - 1;
+                set &= set - 1u;
+                out[3 * i] = (int16_t)((int)((p.xp >> i) & 1u) - (int)((p.xn >> i) & 1u));
+                out[3 * i + 1] = (int16_t)((int)((p.yp >> i) & 1u) - (int)((p.yn >> i) & 1u));
+                out[3 * i + 2] = (int16_t)((int)((p.zp >> i) & 1u) - (int)((p.zn >> i) & 1u));
+            }
+        }
+    }
+    __syncthreads();
+    const size_t n = min((size_t)256, words - word0) * kChunks;
+    uint4* dst = delta + word0 * kChunks;
+#pragma unroll
+    for (int k = 0; k < kChunks; k++) {
+        const size_t c = (size_t)k * 256 + threadIdx.x;
+        if (c < n) dst[c] = stage[c];
     }
 }
 
 void launch_border_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta, hipStream_t s)
 {
     ProfileScope p("border_3d", s);
+    if ((w & 31) == 0 && ((uintptr_t)delta & 15) == 0) {
+        const size_t words = (size_t)(w >> 5) * h * d;
+        hipLaunchKernelGGL(k_border_3d_flat, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s,
+                           w, h, d, mask, (uint4*)delta);
+        return;
+    }
     const size_t words = (size_t)mask_stride(w) * h * d;
     hipLaunchKernelGGL(k_border_3d_words, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, w,
                        h, d, mask, delta);

@@ -90,6 +90,18 @@ def test_border_3d(gpu, oracle, reference, rng, dims):
             np.testing.assert_array_equal(got, reference.border_3d(w, h, d, m))
 
 
+@pytest.mark.parametrize("dims", [(64, 5, 4), (96, 12, 10), (128, 3, 17), (32, 1, 1), (64, 1, 9), (64, 9, 1)])
+def test_border_3d_word_rows(gpu, oracle, reference, rng, dims):
+    """Whole-word rows (word ends inside the row and at its end) and
+    volumes one row or one slice thick."""
+    w, h, d = dims
+    for m in masks_for(rng, dims):
+        got = gpu.init_border_deltas_3d(w, h, d, m)
+        np.testing.assert_array_equal(got, oracle.border_3d(w, h, d, m))
+        if reference is not None:
+            np.testing.assert_array_equal(got, reference.border_3d(w, h, d, m))
+
+
 def test_init_distances(gpu, oracle, rng):
     for w, h in [(1, 1), (33, 17), (100, 64)]:
         m = rand_mask(rng, (w, h), 0.3)
