@@ -727,6 +727,84 @@ void oc_brute_force_3d(int w, int h, int d, int16_t* delta, int cw)
             }
 }
 
+/* ------------------------------------------------------------ fast sweep */
+
+/* Closer(cur, cand) of DistanceOcclusionLib.cpp:624-630 (2177-2183 in 3D):
+ * take the candidate when the current cell is strictly longer. */
+static inline void closer2(int16_t* cur, int x, int y)
+{
+    int16_t c[2] = { wrap16(x), wrap16(y) };
+    if (oc_len2_2(cur) > oc_len2_2(c)) { cur[0] = c[0]; cur[1] = c[1]; }
+}
+
+static inline void closer3(int16_t* cur, int x, int y, int z)
+{
+    int16_t c[3] = { wrap16(x), wrap16(y), wrap16(z) };
+    if (oc_len2_3(cur) > oc_len2_3(c)) { cur[0] = c[0]; cur[1] = c[1]; cur[2] = c[2]; }
+}
+
+/* DistanceOcclusionLib.cpp:633-697.  Four sweeps, one per quadrant
+ * direction (dx, dy) = (1 - 2 (s & 1), 1 - 2 (s >> 1 & 1)); rows in the
+ * sweep's y order from its second row.  Each row first takes the row before
+ * it (offset (0, cy)), then, in the sweep's x order from its second cell, the
+ * cell before it in the row (cx, 0) and the one before that in the previous
+ * row (cx, cy); cx = -cw dx, cy = -cw dy. */
+void oc_fast_sweep_2d(int w, int h, int16_t* delta, int cw)
+{
+    for (int sweep = 0; sweep < 4; sweep++) {
+        const int dx = 1 - 2 * (sweep & 1), dy = 1 - 2 * ((sweep >> 1) & 1);
+        const int cx = -cw * dx, cy = -cw * dy;
+        const int x0 = dx > 0 ? 0 : w - 1, y0 = dy > 0 ? 0 : h - 1;
+        for (int v = 1; v < h; v++) {
+            const int y = y0 + v * dy, yp = y - dy;
+            int16_t* row = delta + 2 * (size_t)y * w;
+            const int16_t* prev = delta + 2 * (size_t)yp * w;
+            for (int u = 0; u < w; u++) {
+                const int x = x0 + u * dx;
+                closer2(row + 2 * x, prev[2 * x], prev[2 * x + 1] + cy);
+            }
+            for (int u = 1; u < w; u++) {
+                const int x = x0 + u * dx, xp = x - dx;
+                closer2(row + 2 * x, row[2 * xp] + cx, row[2 * xp + 1]);
+                closer2(row + 2 * x, prev[2 * xp] + cx, prev[2 * xp + 1] + cy);
+            }
+        }
+    }
+}
+
+/* DistanceOcclusionLib.cpp:2186-2262.  Eight sweeps, one per octant; slices
+ * in the sweep's z order from its second, rows from their second.  A row
+ * first takes, cell by cell, the same row of the previous slice (0, 0, cz)
+ * and then the previous row of its slice (0, cy, 0); then, in x order from
+ * the second cell, the cell before it (cx, 0, 0). */
+void oc_fast_sweep_3d(int w, int h, int d, int16_t* delta, int cw)
+{
+    const size_t slice = (size_t)w * h;
+    for (int sweep = 0; sweep < 8; sweep++) {
+        const int dx = 1 - 2 * (sweep & 1), dy = 1 - 2 * ((sweep >> 1) & 1), dz = 1 - 2 * ((sweep >> 2) & 1);
+        const int cx = -cw * dx, cy = -cw * dy, cz = -cw * dz;
+        const int x0 = dx > 0 ? 0 : w - 1, y0 = dy > 0 ? 0 : h - 1, z0 = dz > 0 ? 0 : d - 1;
+        for (int t = 1; t < d; t++) {
+            const int z = z0 + t * dz, zp = z - dz;
+            for (int v = 1; v < h; v++) {
+                const int y = y0 + v * dy, yp = y - dy;
+                int16_t* row = delta + 3 * (z * slice + (size_t)y * w);
+                const int16_t* back = delta + 3 * (zp * slice + (size_t)y * w);
+                const int16_t* prev = delta + 3 * (z * slice + (size_t)yp * w);
+                for (int u = 0; u < w; u++) {
+                    const int x = x0 + u * dx;
+                    closer3(row + 3 * x, back[3 * x], back[3 * x + 1], back[3 * x + 2] + cz);
+                    closer3(row + 3 * x, prev[3 * x], prev[3 * x + 1] + cy, prev[3 * x + 2]);
+                }
+                for (int u = 1; u < w; u++) {
+                    const int x = x0 + u * dx, xp = x - dx;
+                    closer3(row + 3 * x, row[3 * xp] + cx, row[3 * xp + 1], row[3 * xp + 2]);
+                }
+            }
+        }
+    }
+}
+
 /* ------------------------------------------------- exact squared EDT (ext.) */
 
 /* Not in the reference (which only has the float 2D form above).  The

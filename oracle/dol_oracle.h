@@ -53,6 +53,10 @@ void oc_jfa_3d(int w, int h, int d, int16_t* delta, int cw);
 void oc_jfa_2d_mt(int w, int h, int16_t* delta, int cw, int threads);
 void oc_jfa_3d_mt(int w, int h, int d, int16_t* delta, int cw, int threads);
 
+/* quadrant / octant sweeps, in place (Lib:633-697, 2186-2262) */
+void oc_fast_sweep_2d(int w, int h, int16_t* delta, int cw);
+void oc_fast_sweep_3d(int w, int h, int d, int16_t* delta, int cw);
+
 /* Felzenszwalb float squared distances, in place (Lib:1018-1085) */
 void oc_felzenszwalb_2d(int w, int h, float* dist);
 

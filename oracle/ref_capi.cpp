@@ -43,6 +43,10 @@ void ref_jfa_2d(int w, int h, int16_t* delta, int cw)
 { JumpFlood(w, h, reinterpret_cast<cCellDelta2*>(delta), cw); }
 void ref_jfa_3d(int w, int h, int d, int16_t* delta, int cw)
 { JumpFlood(w, h, d, reinterpret_cast<cCellDelta3*>(delta), cw); }
+void ref_fast_sweep_2d(int w, int h, int16_t* delta, int cw)
+{ FastSweep(w, h, reinterpret_cast<cCellDelta2*>(delta), cw); }
+void ref_fast_sweep_3d(int w, int h, int d, int16_t* delta, int cw)
+{ FastSweep(w, h, d, reinterpret_cast<cCellDelta3*>(delta), cw); }
 void ref_felzenszwalb_2d(int w, int h, float* dist)
 { Felzenszwalb(w, h, dist); }
 void ref_brute_force_2d(int w, int h, int16_t* delta, int cw)

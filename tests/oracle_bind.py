@@ -137,6 +137,16 @@ class Oracle:
         self.L.oc_jfa_3d_mt(_I(w), _I(h), _I(d), _ptr(out), _I(cw), _I(threads))
         return out
 
+    def fast_sweep_2d(self, w, h, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self.L.oc_fast_sweep_2d(_I(w), _I(h), _ptr(out), _I(cw))
+        return out
+
+    def fast_sweep_3d(self, w, h, d, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self.L.oc_fast_sweep_3d(_I(w), _I(h), _I(d), _ptr(out), _I(cw))
+        return out
+
     def jfa_pass_2d(self, step, w, h, delta, cw=1):
         src = np.ascontiguousarray(delta, dtype=np.int16)
         out = np.empty_like(src)
@@ -321,6 +331,16 @@ class Reference:
     def jfa_3d(self, w, h, d, delta, cw=1):
         out = np.ascontiguousarray(delta, dtype=np.int16).copy()
         self.L.ref_jfa_3d(_I(w), _I(h), _I(d), _ptr(out), _I(cw))
+        return out
+
+    def fast_sweep_2d(self, w, h, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self.L.ref_fast_sweep_2d(_I(w), _I(h), _ptr(out), _I(cw))
+        return out
+
+    def fast_sweep_3d(self, w, h, d, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self.L.ref_fast_sweep_3d(_I(w), _I(h), _I(d), _ptr(out), _I(cw))
         return out
 
     def felzenszwalb_2d(self, w, h, dist):

@@ -387,3 +387,77 @@ def test_host_generators_match_oracle(oracle):
     L.mask_add_block_3d(40, 30, 20, 7, a)
     oracle.mask_add_block_3d(40, 30, 20, 7, b)
     np.testing.assert_array_equal(a, b)
+
+
+# ------------------------------------------------------------- FastSweep
+
+
+def _sweep_cases(nd):
+    g = np.load(os.path.join(GOLDEN, "sweep_golden.npz"))
+    k = 0
+    while f"s{nd}_{k}_dims" in g:
+        dims = tuple(int(v) for v in g[f"s{nd}_{k}_dims"])
+        for cw in (1, 2):
+            yield dims, cw, g[f"s{nd}_{k}_cw{cw}_in"], g[f"s{nd}_{k}_cw{cw}_out"]
+        k += 1
+
+
+def test_fast_sweep_golden(oracle):
+    """The oracle's FastSweep against the reference's outputs
+    (tests/golden/make_sweep_golden.py): unsigned starts at cw = 1, border
+    starts at cw = 2."""
+    n = 0
+    for (w, h), cw, start, exp in _sweep_cases(2):
+        np.testing.assert_array_equal(oracle.fast_sweep_2d(w, h, start, cw), exp)
+        n += 1
+    for (w, h, d), cw, start, exp in _sweep_cases(3):
+        np.testing.assert_array_equal(oracle.fast_sweep_3d(w, h, d, start, cw), exp)
+        n += 1
+    assert n == 18
+
+
+def test_fast_sweep_live_reference(oracle, reference, rng):
+    if reference is None:
+        pytest.skip("oracle/_ref/libdolref.so not built (reference sources absent)")
+    for trial in range(8):
+        w, h = (int(v) for v in rng.integers(1, 90, 2))
+        m = pack_mask(rng.random((h, w)) < [0.0, 0.003, 0.05, 0.5][trial % 4])
+        for cw, start in ((1, oracle.init_deltas_2d(w, h, m)), (2, oracle.border_2d(w, h, m))):
+            np.testing.assert_array_equal(oracle.fast_sweep_2d(w, h, start, cw),
+                                          reference.fast_sweep_2d(w, h, start, cw))
+        w, h, d = (int(v) for v in rng.integers(1, 24, 3))
+        m = pack_mask(rng.random((d, h, w)) < [0.0, 0.003, 0.05, 0.5][trial % 4])
+        for cw, start in ((1, oracle.init_deltas_3d(w, h, d, m)), (2, oracle.border_3d(w, h, d, m))):
+            np.testing.assert_array_equal(oracle.fast_sweep_3d(w, h, d, start, cw),
+                                          reference.fast_sweep_3d(w, h, d, start, cw))
+
+
+@pytest.mark.parametrize("dims,seed", [((9, 7), (0, 0)), ((9, 7), (8, 6)), ((31, 12), (17, 5)), ((1, 20), (0, 13))])
+def test_fast_sweep_single_seed_2d(oracle, dims, seed):
+    """One seed: every cell ends holding the exact offset to it, (sx - x, sy - y)
+    (each quadrant sweep carries it along monotone paths)."""
+    w, h = dims
+    bits = np.zeros((h, w), bool)
+    bits[seed[1], seed[0]] = True
+    out = oracle.fast_sweep_2d(w, h, oracle.init_deltas_2d(w, h, pack_mask(bits)), 1)
+    ys, xs = np.mgrid[0:h, 0:w]
+    np.testing.assert_array_equal(out[..., 0], seed[0] - xs)
+    np.testing.assert_array_equal(out[..., 1], seed[1] - ys)
+
+
+@pytest.mark.parametrize("dims,seed", [((6, 5, 4), (0, 4, 3)), ((6, 5, 4), (5, 4, 3)), ((11, 3, 8), (4, 2, 7))])
+def test_fast_sweep_single_seed_3d(oracle, dims, seed):
+    """As in 2D, for a seed on the last row of the last slice (y = h - 1,
+    z = d - 1): the first sweeps carry it along that row, the next ones
+    down every row of that slice, the last four into every slice.  (A sweep
+    never updates its own first row and slice, so a seed elsewhere can leave
+    cells unreached; the reference does the same, which the golden and live
+    comparisons above cover.)"""
+    w, h, d = dims
+    bits = np.zeros((d, h, w), bool)
+    bits[seed[2], seed[1], seed[0]] = True
+    out = oracle.fast_sweep_3d(w, h, d, oracle.init_deltas_3d(w, h, d, pack_mask(bits)), 1)
+    zs, ys, xs = np.mgrid[0:d, 0:h, 0:w]
+    np.testing.assert_array_equal(out[..., 0], seed[0] - xs)
+    np.testing.assert_array_equal(out[..., 1], seed[1] - ys)
+    np.testing.assert_array_equal(out[..., 2], seed[2] - zs)
