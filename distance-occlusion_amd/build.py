@@ -28,6 +28,7 @@ SOURCES = [
     "dol_batch.hip",
     "dol_seeds.hip",
     "dol_voxel.hip",
+    "dol_sweep.hip",
     "dol_capi.cpp",
     "dol_scenes.cpp",
     "dol_shim.cpp",

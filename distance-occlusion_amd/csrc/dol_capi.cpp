@@ -28,6 +28,8 @@ template <class T>
 void launch_init_dist(int w, int h, const uint32_t* mask, T* dist, T max_d, hipStream_t s);
 void launch_border_2d(int w, int h, const uint32_t* mask, int16_t* delta, hipStream_t s);
 void launch_border_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta, hipStream_t s);
+void launch_fast_sweep_2d(int w, int h, int16_t* delta, int cw, hipStream_t s);
+void launch_fast_sweep_3d(int w, int h, int d, int16_t* delta, int cw, hipStream_t s);
 void launch_jfa2d_pass(int step, int w, int h, const int16_t* in, int16_t* out, int cw,
                        hipStream_t s);
 void jfa3d_run(int w, int h, int d, int16_t* delta, int cw, int16_t* tmp, hipStream_t s);
@@ -581,6 +583,55 @@ int dolhip_jump_flood_3d(int w, int h, int d, int16_t* delta, int cw)
     TRY(t.alloc(bytes));
     jfa3d_run(w, h, d, a.as<int16_t>(), cw, t.as<int16_t>(), 0);
     TRY(finish_sync("jump_flood_3d"));
+    return a.download(delta, bytes);
+}
+
+int dolhip_fast_sweep_2d_dev(int w, int h, int16_t* delta, int cw, void* stream)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_cw(cw));
+    if ((uintptr_t)delta & 3) return fail(DOLHIP_ERR_INVALID_ARG, "delta must be 4-byte aligned");
+    TRY(require_device());
+    launch_fast_sweep_2d(w, h, delta, cw, as_stream(stream));
+    return finish(as_stream(stream), "fast_sweep_2d");
+}
+
+int dolhip_fast_sweep_2d(int w, int h, int16_t* delta, int cw)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_cw(cw));
+    TRY(require_device());
+    const size_t bytes = (size_t)w * h * 4;
+    DevBuf a;
+    TRY(a.upload(delta, bytes));
+    launch_fast_sweep_2d(w, h, a.as<int16_t>(), cw, 0);
+    TRY(finish_sync("fast_sweep_2d"));
+    return a.download(delta, bytes);
+}
+
+int dolhip_fast_sweep_3d_dev(int w, int h, int d, int16_t* delta, int cw, void* stream)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_cw(cw));
+    TRY(require_device());
+    launch_fast_sweep_3d(w, h, d, delta, cw, as_stream(stream));
+    return finish(as_stream(stream), "fast_sweep_3d");
+}
+
+int dolhip_fast_sweep_3d(int w, int h, int d, int16_t* delta, int cw)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_cw(cw));
+    TRY(require_device());
+    const size_t bytes = (size_t)w * h * d * 6;
+    DevBuf a;
+    TRY(a.upload(delta, bytes));
+    launch_fast_sweep_3d(w, h, d, a.as<int16_t>(), cw, 0);
+    TRY(finish_sync("fast_sweep_3d"));
     return a.download(delta, bytes);
 }
 

@@ -88,6 +88,10 @@ void DOL::JumpFlood(int w, int h, cCellDelta2 delta[], int cw)
 { require(dolhip_jump_flood_2d(w, h, raw(delta), cw), "JumpFlood"); }
 void DOL::JumpFlood(int w, int h, int d, cCellDelta3 delta[], int cw)
 { require(dolhip_jump_flood_3d(w, h, d, raw(delta), cw), "JumpFlood"); }
+void DOL::FastSweep(int w, int h, cCellDelta2 delta[], int cw)
+{ require(dolhip_fast_sweep_2d(w, h, raw(delta), cw), "FastSweep"); }
+void DOL::FastSweep(int w, int h, int d, cCellDelta3 delta[], int cw)
+{ require(dolhip_fast_sweep_3d(w, h, d, raw(delta), cw), "FastSweep"); }
 
 void DOL::EDT(int w, int h, const uint32_t mask[], int32_t d2[], float dist[], bool invert)
 { require(dolhip_edt_2d(w, h, mask, invert, d2, dist), "EDT"); }

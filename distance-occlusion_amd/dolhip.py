@@ -67,6 +67,10 @@ _SIGS = {
     "dolhip_jump_flood_3d": ([_I, _I, _I, _P, _I], _I),
     "dolhip_jump_flood_3d_workspace": ([_I, _I, _I], _SZ),
     "dolhip_jump_flood_3d_dev": ([_I, _I, _I, _P, _I, _P, _P], _I),
+    "dolhip_fast_sweep_2d": ([_I, _I, _P, _I], _I),
+    "dolhip_fast_sweep_2d_dev": ([_I, _I, _P, _I, _P], _I),
+    "dolhip_fast_sweep_3d": ([_I, _I, _I, _P, _I], _I),
+    "dolhip_fast_sweep_3d_dev": ([_I, _I, _I, _P, _I, _P], _I),
     "dolhip_felzenszwalb_2d": ([_I, _I, _P], _I),
     "dolhip_felzenszwalb_2d_workspace": ([_I, _I], _SZ),
     "dolhip_felzenszwalb_2d_dev": ([_I, _I, _P, _P, _P], _I),
@@ -259,6 +263,16 @@ class DolHip:
     def jump_flood_3d(self, w, h, d, delta, cw=1):
         out = np.ascontiguousarray(delta, dtype=np.int16).copy()
         self._call("dolhip_jump_flood_3d", w, h, d, _addr(out), cw)
+        return out
+
+    def fast_sweep_2d(self, w, h, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self._call("dolhip_fast_sweep_2d", w, h, _addr(out), cw)
+        return out
+
+    def fast_sweep_3d(self, w, h, d, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self._call("dolhip_fast_sweep_3d", w, h, d, _addr(out), cw)
         return out
 
     def felzenszwalb_2d(self, w, h, dist):

@@ -6,9 +6,9 @@
  * functions of DistanceOcclusionLib.h that lie on the data-parallel distance
  * path (mask creation and generators :21-35, cell deltas :41-57, distance
  * initialisation and Felzenszwalb :61-71, delta initialisation and border
- * initialisation :75-82, JumpFlood :92-94, MaskSize/EncodeU8 :124-141).  The
- * sweep methods, Chamfer, BruteForce and the occlusion functions are not part
- * of this path and are not declared here.
+ * initialisation :75-82, FastSweep :84-85, JumpFlood :92-94, MaskSize/EncodeU8
+ * :124-141).  Danielsson, Chamfer, BruteForce and the occlusion functions are
+ * not part of this path and are not declared here.
  *
  * Semantics are the reference's: buffers are host memory owned by the
  * caller and updated in place; every call is synchronous; results are
@@ -74,6 +74,9 @@ namespace DOL
 
     void JumpFlood(int w, int h,        cCellDelta2 delta[], int cw = 1);
     void JumpFlood(int w, int h, int d, cCellDelta3 delta[], int cw = 1);
+
+    void FastSweep(int w, int h,        cCellDelta2 delta[], int cw = 1);
+    void FastSweep(int w, int h, int d, cCellDelta3 delta[], int cw = 1);
 
     // --- Extensions (no counterpart in the reference) ------------------------
     // Exact squared Euclidean distance to the nearest set bit (clear bit if

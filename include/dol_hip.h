@@ -127,6 +127,14 @@ size_t dolhip_jump_flood_3d_workspace(int w, int h, int d);
 int dolhip_jump_flood_3d_dev(int w, int h, int d, int16_t* delta, int cw, void* work,
                              void* stream);
 
+/* FastSweep, in place (DistanceOcclusionLib.h:84-85; .cpp:633-697 2D,
+ * 2186-2262 3D): the four quadrant (eight octant) sweeps, bit-identical.
+ * No workspace; the _dev forms need a 4-byte aligned 2D field. */
+int dolhip_fast_sweep_2d(int w, int h, int16_t* delta, int cw);
+int dolhip_fast_sweep_2d_dev(int w, int h, int16_t* delta, int cw, void* stream);
+int dolhip_fast_sweep_3d(int w, int h, int d, int16_t* delta, int cw);
+int dolhip_fast_sweep_3d_dev(int w, int h, int d, int16_t* delta, int cw, void* stream);
+
 /* Felzenszwalb float squared distances in place, bit-identical to the
  * reference's float arithmetic (DistanceOcclusionLib.h:71, .cpp:1018-1085) */
 int dolhip_felzenszwalb_2d(int w, int h, float* dist);

@@ -4,10 +4,12 @@ compiled reference where it was built), bit-exact, on seeded random masks.
 Sizes straddle the 32-bit word boundaries of the bitmask rows (31, 32, 33,
 63, 64, 65 ...) and include single rows/columns/slices, empty and full masks.
 """
+import os
+
 import numpy as np
 import pytest
 
-from oracle_bind import empty_mask, pack_mask, unpack_mask
+from oracle_bind import ROOT, empty_mask, pack_mask, unpack_mask
 
 pytestmark = pytest.mark.gpu
 
@@ -456,3 +458,64 @@ def test_edt_3d_scene_slab(gpu, oracle):
     m = dolhip.load().scene_shapes_3d(w, h, d, 40, 14, seed=11)
     d2, _ = gpu.edt_3d(w, h, d, m)
     np.testing.assert_array_equal(d2, oracle.edt_3d(w, h, d, m))
+
+
+# ------------------------------------------------------------- FastSweep
+
+
+@pytest.mark.parametrize("dims", [(1, 1), (5, 3), (1, 90), (90, 1), (64, 64), (65, 64), (64, 65),
+                                  (130, 67), (200, 129), (257, 70)])
+def test_fast_sweep_2d(gpu, oracle, rng, dims):
+    w, h = dims
+    for p in (0.0, 0.002, 0.05, 0.5):
+        m = rand_mask(rng, dims, p)
+        for cw, start in ((1, oracle.init_deltas_2d(w, h, m)), (2, oracle.border_2d(w, h, m))):
+            np.testing.assert_array_equal(gpu.fast_sweep_2d(w, h, start, cw), oracle.fast_sweep_2d(w, h, start, cw))
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (5, 4, 3), (17, 9, 6), (64, 64, 3), (70, 65, 5), (33, 130, 4),
+                                  (1, 1, 40), (129, 3, 7)])
+def test_fast_sweep_3d(gpu, oracle, rng, dims):
+    w, h, d = dims
+    for p in (0.0, 0.002, 0.05, 0.5):
+        m = rand_mask(rng, dims, p)
+        for cw, start in ((1, oracle.init_deltas_3d(w, h, d, m)), (2, oracle.border_3d(w, h, d, m))):
+            np.testing.assert_array_equal(gpu.fast_sweep_3d(w, h, d, start, cw),
+                                          oracle.fast_sweep_3d(w, h, d, start, cw))
+
+
+def test_fast_sweep_golden(gpu):
+    """The reference's own outputs (tests/golden/sweep_golden.npz)."""
+    g = np.load(os.path.join(ROOT, "tests", "golden", "sweep_golden.npz"))
+    n = 0
+    for key in g.files:
+        if not key.endswith("_dims"):
+            continue
+        pre = key[: -len("_dims")]
+        dims = tuple(int(v) for v in g[key])
+        for cw in (1, 2):
+            start, exp = g[f"{pre}_cw{cw}_in"], g[f"{pre}_cw{cw}_out"]
+            got = gpu.fast_sweep_2d(*dims, start, cw) if len(dims) == 2 else gpu.fast_sweep_3d(*dims, start, cw)
+            np.testing.assert_array_equal(got, exp)
+            n += 1
+    assert n == 18
+
+
+def test_fast_sweep_large_single_seed(gpu):
+    """Full-size property: one seed, every cell ends with the exact offset to
+    it (2D anywhere; 3D for a seed on the last row of the last slice)."""
+    w, h = 1500, 1100
+    start = np.full((h, w, 2), 32767, np.int16)
+    start[617, 1234] = 0
+    out = gpu.fast_sweep_2d(w, h, start, 1)
+    ys, xs = np.mgrid[0:h, 0:w]
+    np.testing.assert_array_equal(out[..., 0], 1234 - xs)
+    np.testing.assert_array_equal(out[..., 1], 617 - ys)
+    w, h, d = 150, 131, 97
+    start = np.full((d, h, w, 3), 25000, np.int16)
+    start[d - 1, h - 1, 77] = 0
+    out = gpu.fast_sweep_3d(w, h, d, start, 1)
+    zs, ys, xs = np.mgrid[0:d, 0:h, 0:w]
+    np.testing.assert_array_equal(out[..., 0], 77 - xs)
+    np.testing.assert_array_equal(out[..., 1], h - 1 - ys)
+    np.testing.assert_array_equal(out[..., 2], d - 1 - zs)
