@@ -334,6 +334,85 @@ def bench_voxel(L, torch, dev, cpu=True):
     return out
 
 
+def bench_sweep(L, torch, dev, mask3_t=None, cpu=True):
+    """FastSweep (the tool's -f method) after border init, cw = 2: the 8192^2
+    disc/box scene and the 512^3 scene.  CPU figures: the oracle (one thread)
+    on the full 8192^2 field, which also checks the GPU result cell for cell,
+    and on a 128^3 corner of the 3D start field."""
+    out = {}
+    w = h = 8192
+    mask = L.scene_shapes_2d(w, h, 2000, seed=2)
+    m_t = torch.from_numpy(mask.view(np.int32)).to(dev)
+    delta = torch.empty(w * h * 2, dtype=torch.int16, device=dev)
+
+    def step2():
+        L.call_dev("init_border_deltas_2d", w, h, m_t, delta)
+        L.call_dev("fast_sweep_2d", w, h, delta, 2)
+
+    step2()
+    L.profile_enable(True)
+    ms = timed(step2, 3, torch)
+    sweep_ms, _ = L.profile_query("fast_sweep_2d")
+    L.profile_enable(False)
+    launches = 4 * (w // 64 + -(-h // 63) - 1)   # tiles of 64 columns x 63 rows, one launch a diagonal
+    out["fast_sweep_2d_8192"] = {"ms": round(ms, 3), "sweep_ms": round(sweep_ms, 3),
+                                 "gpixels_per_s": round(w * h / (sweep_ms * 1e-3) / 1e9, 3),
+                                 "launches": launches,
+                                 "us_per_launch": round(1e3 * sweep_ms / launches, 2)}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_bind import Oracle
+
+        O = Oracle()
+        L.call_dev("init_border_deltas_2d", w, h, m_t, delta)
+        torch.cuda.synchronize()
+        start = delta.cpu().numpy().reshape(h, w, 2)
+        t0 = time.perf_counter()
+        ref = O.fast_sweep_2d(w, h, start, 2)
+        cpu_s = time.perf_counter() - t0
+        L.call_dev("fast_sweep_2d", w, h, delta, 2)
+        torch.cuda.synchronize()
+        mism = int((delta.cpu().numpy().reshape(h, w, 2) != ref).any(axis=-1).sum())
+        out["fast_sweep_2d_8192"]["cpu_baseline"] = {
+            "ms": round(cpu_s * 1e3, 1), "gpixels_per_s": round(w * h / cpu_s / 1e9, 4), "cores": 1,
+            "kind": "port", "sample": "the full 8192^2 field once", "mismatches_vs_oracle": mism}
+    del delta
+    torch.cuda.empty_cache()
+
+    n = 512
+    if mask3_t is None:
+        mask3_t = torch.from_numpy(L.scene_shapes_3d(n, n, n, 300, 100, seed=3).view(np.int32)).to(dev)
+    d3 = torch.empty(n * n * n * 3, dtype=torch.int16, device=dev)
+
+    def step3():
+        L.call_dev("init_border_deltas_3d", n, n, n, mask3_t, d3)
+        L.call_dev("fast_sweep_3d", n, n, n, d3, 2)
+
+    step3()
+    L.profile_enable(True)
+    ms = timed(step3, 2, torch)
+    sweep_ms, _ = L.profile_query("fast_sweep_3d")
+    L.profile_enable(False)
+    launches = 8 * (n // 64 + -(-n // 63) + n - 2)
+    out["fast_sweep_3d_512"] = {"ms": round(ms, 3), "sweep_ms": round(sweep_ms, 3),
+                                "gvoxels_per_s": round(n ** 3 / (sweep_ms * 1e-3) / 1e9, 3),
+                                "launches": launches, "us_per_launch": round(1e3 * sweep_ms / launches, 2)}
+    if cpu:
+        k = 128
+        L.call_dev("init_border_deltas_3d", n, n, n, mask3_t, d3)
+        torch.cuda.synchronize()
+        corner = np.ascontiguousarray(d3.cpu().numpy().reshape(n, n, n, 3)[:k, :k, :k])
+        t0 = time.perf_counter()
+        O.fast_sweep_3d(k, k, k, corner, 2)
+        cpu_s = time.perf_counter() - t0
+        out["fast_sweep_3d_512"]["cpu_baseline"] = {
+            "ms_128cube": round(cpu_s * 1e3, 1), "gvoxels_per_s": round(k ** 3 / cpu_s / 1e9, 4), "cores": 1,
+            "kind": "port", "sample": "FastSweep of the 128^3 corner of the 512^3 start field"}
+    del d3
+    torch.cuda.empty_cache()
+    return out
+
+
 def mask_cpu_view(L, torch, dev, pos, idx, bbox, n):
     """the GPU's mask of one mesh, on the host"""
     return L.mask_from_triangles(n, n, n, pos, idx, bbox).view(np.int32)
@@ -349,7 +428,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only copy 512 MiB device to device 5 times (calibrates HBM byte counters)")
-    ap.add_argument("--only-extra", choices=["jfa3d", "glyphs", "jfa2d", "config1", "seeds", "twofield", "voxel"],
+    ap.add_argument("--only-extra", choices=["jfa3d", "glyphs", "jfa2d", "config1", "seeds", "twofield", "voxel", "sweep"],
                     help="run only this secondary workload and print its figures (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -383,7 +462,8 @@ def main():
         run = {"glyphs": lambda: bench_glyphs(L, torch, dev, args), "jfa2d": lambda: bench_jfa2d(L, torch, dev),
                "config1": lambda: bench_config1(L, torch, dev), "seeds": lambda: bench_seeds(L, torch, dev),
                "twofield": lambda: bench_two_field(L, torch, dev),
-               "voxel": lambda: bench_voxel(L, torch, dev, cpu=not args.no_cpu)}
+               "voxel": lambda: bench_voxel(L, torch, dev, cpu=not args.no_cpu),
+               "sweep": lambda: bench_sweep(L, torch, dev, cpu=not args.no_cpu)}
         print(json.dumps({args.only_extra: run[args.only_extra]()}), flush=True)
         return
 
@@ -494,6 +574,7 @@ def main():
         result["jfa2d_signed_8192"] = bench_jfa2d(L, torch, dev)
         result["two_field_8192"] = bench_two_field(L, torch, dev)
         result["voxel_512"] = bench_voxel(L, torch, dev, cpu=not args.no_cpu)
+        result["fast_sweep"] = bench_sweep(L, torch, dev, mask_t, cpu=not args.no_cpu)
         result["config1_512"] = bench_config1(L, torch, dev)
         result["seeds_2048"] = bench_seeds(L, torch, dev)
 

@@ -48,12 +48,6 @@ __device__ __forceinline__ uint32_t sw_add(uint32_t xy, uint32_t oxy)
     return __builtin_bit_cast(uint32_t, __builtin_bit_cast(sw_u16x2, xy) + __builtin_bit_cast(sw_u16x2, oxy));
 }
 
-// the value of the lane before (wave_shr:1); lane 0 gets 0
-__device__ __forceinline__ uint32_t from_prev_lane(uint32_t v)
-{
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);
-}
-
 struct Frame {
     int w, x0, dx, y0, dy;
     __device__ size_t row(int v) const { return (size_t)(y0 + v * dy) * w; }
@@ -73,62 +67,95 @@ __device__ __forceinline__ Frame frame(int w, int h, int sweep)
 
 }  // namespace
 
+// lane l's row, rotated: register k holds column (k - l) mod 64, so at step s
+// every lane needs register s mod 64 -- the same for all lanes, which keeps
+// the row in registers through the unrolled steps
+__device__ __forceinline__ int rot(int k, int lane) { return (k - lane) & (kST - 1); }
+
+// the value of the lane before (lane 0 gets 0)
+__device__ __forceinline__ uint32_t from_prev_lane(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);
+}
+
+// No candidate: a length no int16 pair or triple has (2^31 - 1 is neither a
+// sum of two squares, being 3 mod 4, nor of three, being 7 mod 8).
+constexpr int kNoLen = 0x7fffffff;
+
+// Closer() applied to the candidates in order keeps the first of least
+// length: pick that one of (x0, x1, x2, x3) given their lengths
+template <class T>
+__device__ __forceinline__ T first_least(T x0, int l0, T x1, int l1, T x2, int l2, T x3, int l3)
+{
+    const int m = min(min(l0, l1), min(l2, l3));
+    T r = l2 == m ? x2 : x3;
+    r = l1 == m ? x1 : r;
+    return l0 == m ? x0 : r;
+}
+
+// Lane l of a tile holds row v0 - 1 + l: lane 0 the row before the tile,
+// final already (its values are only passed on), lanes 1..63 the tile's 63
+// rows.
+constexpr int kRows = kST - 1;
+
 // 2D: tiles (U, V) with U + V = diag; block b is V = max(0, diag - tu + 1) + b
 __global__ void __launch_bounds__(kST) k_fast_sweep_2d(int w, int h, int sweep, int cw, int diag,
                                                        int tu, uint32_t* __restrict__ delta)
 {
     __shared__ uint32_t tile[kST * kSStride];
-    __shared__ uint32_t top[kST + 1];   // row v0 - 1: [0] column u0 - 1, [1 + a] column u0 + a
     const Frame f = frame(w, h, sweep);
     const uint32_t ox = (uint16_t)(-cw * f.dx), oy = (uint32_t)(uint16_t)(-cw * f.dy) << 16;
     const int V = max(0, diag - (tu - 1)) + (int)blockIdx.x, U = diag - V;
-    const int u0 = U * kST, v0 = V * kST;
-    const int nu = min(kST, w - u0), nv = min(kST, h - v0);
+    const int u0 = U * kST, v0 = V * kRows;
+    const int nu = min(kST, w - u0);
     const int lane = threadIdx.x;
+    const int vr = v0 - 1 + lane;                       // the lane's row
+    const int rlo = max(v0 - 1, 0), rhi = min(v0 + kRows - 1, h - 1);
     const int cu = f.col(u0 + min(lane, nu - 1));
-    for (int r = 0; r < nv; r++)
-        if (lane < nu) tile[r * kSStride + lane] = delta[f.row(v0 + r) + cu];
-    if (v0 > 0) {
-        if (lane < nu) top[1 + lane] = delta[f.row(v0 - 1) + cu];
-        if (lane == 0 && u0 > 0) top[0] = delta[f.row(v0 - 1) + f.col(u0 - 1)];
+    {
+        // all rows in flight at once (rows outside the field repeat the nearest)
+        uint32_t v[kST];
+#pragma unroll
+        for (int r = 0; r < kST; r++) v[r] = delta[f.row(min(max(v0 - 1 + r, rlo), rhi)) + cu];
+#pragma unroll
+        for (int r = 0; r < kST; r++) tile[r * kSStride + lane] = v[r];
     }
+    const bool row_ok = vr >= 0 && vr < h, update = row_ok && lane >= 1 && vr >= 1;
     // the lane's last two results; before its first column, the column
     // before the tile
-    uint32_t mine = (u0 > 0 && lane < nv) ? delta[f.row(v0 + lane) + f.col(u0 - 1)] : 0u;
+    uint32_t mine = (u0 > 0 && row_ok) ? delta[f.row(vr) + f.col(u0 - 1)] : 0u;
     uint32_t mine_prev = 0u;
     __syncthreads();
-    const bool row_ok = lane < nv, update = row_ok && v0 + lane >= 1;
-    uint32_t* trow = tile + lane * kSStride;
-    for (int s = 0; s < kST + kST - 1; s++) {
+    uint32_t R[kST];
+#pragma unroll
+    for (int k = 0; k < kST; k++) R[k] = tile[lane * kSStride + rot(k, lane)];
+#pragma unroll
+    for (int s = 0; s < 2 * kST - 1; s++) {
         const int a = s - lane;
-        uint32_t up = from_prev_lane(mine), dg = from_prev_lane(mine_prev);
-        if (!row_ok || a < 0 || a >= nu) continue;
-        if (lane == 0 && v0 > 0) {
-            up = top[1 + a];
-            dg = top[a];
+        // (a, v - 1) and (a - 1, v - 1), finished by the lane before
+        const uint32_t up = from_prev_lane(mine), dg = from_prev_lane(mine_prev);
+        const uint32_t cur = R[s & (kST - 1)];
+        const uint32_t c0 = sw_add(up, oy), c1 = sw_add(mine, ox), c2 = sw_add(dg, ox | oy);
+        const bool left = u0 + a >= 1;
+        const uint32_t best = first_least(cur, sw_len2(cur, 0), c0, sw_len2(c0, 0), c1,
+                                          left ? sw_len2(c1, 0) : kNoLen, c2, left ? sw_len2(c2, 0) : kNoLen);
+        if (row_ok && a >= 0 && a < nu) {
+            if (update) R[s & (kST - 1)] = best;
+            mine_prev = mine;
+            mine = R[s & (kST - 1)];
         }
-        uint32_t cur = trow[a];
-        if (update) {
-            int lc = sw_len2(cur, 0);
-            const uint32_t c0 = sw_add(up, oy);
-            const int l0 = sw_len2(c0, 0);
-            if (lc > l0) { cur = c0; lc = l0; }
-            if (u0 + a >= 1) {
-                const uint32_t c1 = sw_add(mine, ox);
-                const int l1 = sw_len2(c1, 0);
-                if (lc > l1) { cur = c1; lc = l1; }
-                const uint32_t c2 = sw_add(dg, ox | oy);
-                const int l2 = sw_len2(c2, 0);
-                if (lc > l2) cur = c2;
-            }
-            trow[a] = cur;
-        }
-        mine_prev = mine;
-        mine = cur;
     }
+#pragma unroll
+    for (int k = 0; k < kST; k++) tile[lane * kSStride + rot(k, lane)] = R[k];
     __syncthreads();
-    for (int r = 0; r < nv; r++)
-        if (lane < nu) delta[f.row(v0 + r) + cu] = tile[r * kSStride + lane];
+    {
+        uint32_t v[kST];
+#pragma unroll
+        for (int r = 1; r < kST; r++) v[r] = tile[r * kSStride + lane];
+#pragma unroll
+        for (int r = 1; r < kST; r++)
+            if (v0 - 1 + r <= rhi && lane < nu) delta[f.row(v0 - 1 + r) + cu] = v[r];
+    }
 }
 
 // 3D: tiles (U, V) of slice t = diag - U - V; block b is (U, V) = (b % tu, b / tu).
@@ -140,8 +167,6 @@ __global__ void __launch_bounds__(kST) k_fast_sweep_3d(int w, int h, int d, int 
     __shared__ int16_t tz[kST * kSStride];
     __shared__ uint32_t bxy[kST * kSStride];   // the same cells of the slice before
     __shared__ int16_t bz[kST * kSStride];
-    __shared__ uint32_t top_xy[kST];            // row v0 - 1
-    __shared__ int16_t top_z[kST];
     const int U = (int)blockIdx.x % tu, V = (int)blockIdx.x / tu;
     const int t = diag - U - V;
     if (t < 1 || t >= d) return;   // the sweep's first slice keeps its values
@@ -153,80 +178,103 @@ __global__ void __launch_bounds__(kST) k_fast_sweep_3d(int w, int h, int d, int 
     int16_t* cur_slice = delta + 3 * (slice * (size_t)z);
     const uint32_t ox = (uint16_t)(-cw * f.dx), oy = (uint32_t)(uint16_t)(-cw * f.dy) << 16;
     const int oz = -cw * dz;
-    const int u0 = U * kST, v0 = V * kST;
-    const int nu = min(kST, w - u0), nv = min(kST, h - v0);
+    const int u0 = U * kST, v0 = V * kRows;
+    const int nu = min(kST, w - u0);
     const int lane = threadIdx.x;
+    const int vr = v0 - 1 + lane;
+    const int rlo = max(v0 - 1, 0), rhi = min(v0 + kRows - 1, h - 1);
     const int cu = f.col(u0 + min(lane, nu - 1));
-    auto load = [](const int16_t* c, uint32_t& xy, int16_t& zz) {
-        xy = (uint32_t)(uint16_t)c[0] | ((uint32_t)(uint16_t)c[1] << 16);
-        zz = c[2];
-    };
-    for (int r = 0; r < nv; r++) {
-        if (lane >= nu) continue;
-        const size_t i = 3 * (f.row(v0 + r) + cu);
-        load(cur_slice + i, txy[r * kSStride + lane], tz[r * kSStride + lane]);
-        load(back + i, bxy[r * kSStride + lane], bz[r * kSStride + lane]);
+    auto pack = [](const int16_t* c) { return (uint32_t)(uint16_t)c[0] | ((uint32_t)(uint16_t)c[1] << 16); };
+    {
+        // the rows and the same cells of the slice before, 32 rows at a time
+        // in flight (rows outside the field repeat the nearest)
+#pragma unroll
+        for (int r0 = 0; r0 < kST; r0 += kST / 2) {
+            uint32_t vxy[kST / 2], bv[kST / 2];
+            int16_t vz[kST / 2], bvz[kST / 2];
+#pragma unroll
+            for (int r = 0; r < kST / 2; r++) {
+                const size_t i = 3 * (f.row(min(max(v0 - 1 + r0 + r, rlo), rhi)) + cu);
+                vxy[r] = pack(cur_slice + i);
+                vz[r] = cur_slice[i + 2];
+                bv[r] = pack(back + i);
+                bvz[r] = back[i + 2];
+            }
+#pragma unroll
+            for (int r = 0; r < kST / 2; r++) {
+                txy[(r0 + r) * kSStride + lane] = vxy[r];
+                tz[(r0 + r) * kSStride + lane] = vz[r];
+                bxy[(r0 + r) * kSStride + lane] = bv[r];
+                bz[(r0 + r) * kSStride + lane] = bvz[r];
+            }
+        }
     }
-    if (v0 > 0 && lane < nu) load(cur_slice + 3 * (f.row(v0 - 1) + cu), top_xy[lane], top_z[lane]);
+    const bool row_ok = vr >= 0 && vr < h, update = row_ok && lane >= 1 && vr >= 1;
     uint32_t mine = 0u;
-    int16_t mine_z = 0;
-    if (u0 > 0 && lane < nv) load(cur_slice + 3 * (f.row(v0 + lane) + f.col(u0 - 1)), mine, mine_z);
+    int mine_z = 0;
+    if (u0 > 0 && row_ok) {
+        const int16_t* c = cur_slice + 3 * (f.row(vr) + f.col(u0 - 1));
+        mine = pack(c);
+        mine_z = c[2];
+    }
     __syncthreads();
-    const bool row_ok = lane < nv, update = row_ok && v0 + lane >= 1;
-    uint32_t* trow = txy + lane * kSStride;
-    int16_t* trow_z = tz + lane * kSStride;
+    uint32_t R[kST];
+    int RZ[kST];
+#pragma unroll
+    for (int k = 0; k < kST; k++) {
+        R[k] = txy[lane * kSStride + rot(k, lane)];
+        RZ[k] = tz[lane * kSStride + rot(k, lane)];
+    }
     const uint32_t* brow = bxy + lane * kSStride;
     const int16_t* brow_z = bz + lane * kSStride;
-    for (int s = 0; s < kST + kST - 1; s++) {
-        const int a = s - lane;
-        uint32_t up = from_prev_lane(mine);
-        int up_z = (int)(int16_t)from_prev_lane((uint32_t)(uint16_t)mine_z);
-        if (!row_ok || a < 0 || a >= nu) continue;
-        if (lane == 0 && v0 > 0) {
-            up = top_xy[a];
-            up_z = top_z[a];
+#pragma unroll
+    for (int s = 0; s < 2 * kST - 1; s++) {
+        const int a = s - lane;   // in [-63, 126]: brow + a stays inside bxy
+        // (a, v - 1), finished by the lane before
+        const uint32_t up = from_prev_lane(mine);
+        const int up_z = (int)(int16_t)from_prev_lane((uint32_t)mine_z);
+        const uint32_t cxy = R[s & (kST - 1)];
+        const int cz = RZ[s & (kST - 1)];
+        const uint32_t bxy_ = brow[a];   // the slice before, (0, 0, cz)
+        const int bz_ = wrap16(brow_z[a] + oz);
+        const uint32_t uxy = sw_add(up, oy);   // the row before, (0, cy, 0)
+        const uint32_t lxy = sw_add(mine, ox);   // the cell before, (cx, 0, 0)
+        const int l0 = sw_len2(cxy, cz * cz), l1 = sw_len2(bxy_, bz_ * bz_), l2 = sw_len2(uxy, up_z * up_z);
+        const int l3 = u0 + a >= 1 ? sw_len2(lxy, mine_z * mine_z) : kNoLen;
+        const int m = min(min(l0, l1), min(l2, l3));
+        const int pick = l0 == m ? 0 : (l1 == m ? 1 : (l2 == m ? 2 : 3));
+        const uint32_t nxy = pick == 0 ? cxy : (pick == 1 ? bxy_ : (pick == 2 ? uxy : lxy));
+        const int nz = pick == 0 ? cz : (pick == 1 ? bz_ : (pick == 2 ? up_z : mine_z));
+        if (row_ok && a >= 0 && a < nu) {
+            if (update) {
+                R[s & (kST - 1)] = nxy;
+                RZ[s & (kST - 1)] = nz;
+            }
+            mine = R[s & (kST - 1)];
+            mine_z = RZ[s & (kST - 1)];
         }
-        uint32_t cxy = trow[a];
-        int cz = trow_z[a];
-        if (update) {
-            int lc = sw_len2(cxy, cz * cz);
-            {   // the slice before, (0, 0, cz)
-                const uint32_t c = brow[a];
-                const int c_z = wrap16(brow_z[a] + oz);
-                const int l = sw_len2(c, c_z * c_z);
-                if (lc > l) { cxy = c; cz = c_z; lc = l; }
-            }
-            {   // the row before, (0, cy, 0)
-                const uint32_t c = sw_add(up, oy);
-                const int l = sw_len2(c, up_z * up_z);
-                if (lc > l) { cxy = c; cz = up_z; lc = l; }
-            }
-            if (u0 + a >= 1) {   // the cell before, (cx, 0, 0)
-                const uint32_t c = sw_add(mine, ox);
-                const int l = sw_len2(c, (int)mine_z * mine_z);
-                if (lc > l) { cxy = c; cz = mine_z; }
-            }
-            trow[a] = cxy;
-            trow_z[a] = (int16_t)cz;
-        }
-        mine = cxy;
-        mine_z = (int16_t)cz;
+    }
+#pragma unroll
+    for (int k = 0; k < kST; k++) {
+        txy[lane * kSStride + rot(k, lane)] = R[k];
+        tz[lane * kSStride + rot(k, lane)] = (int16_t)RZ[k];
     }
     __syncthreads();
-    for (int r = 0; r < nv; r++) {
-        if (lane >= nu) continue;
-        int16_t* c = cur_slice + 3 * (f.row(v0 + r) + cu);
+    for (int r = 1; r < kST; r++) {
         const uint32_t xy = txy[r * kSStride + lane];
+        const int16_t zz = tz[r * kSStride + lane];
+        if (v0 - 1 + r > rhi || lane >= nu) continue;
+        int16_t* c = cur_slice + 3 * (f.row(v0 - 1 + r) + cu);
         c[0] = (int16_t)(xy & 0xffffu);
         c[1] = (int16_t)(xy >> 16);
-        c[2] = tz[r * kSStride + lane];
+        c[2] = zz;
     }
 }
 
 void launch_fast_sweep_2d(int w, int h, int16_t* delta, int cw, hipStream_t s)
 {
     ProfileScope p("fast_sweep_2d", s);
-    const int tu = (w + kST - 1) / kST, tv = (h + kST - 1) / kST;
+    const int tu = (w + kST - 1) / kST, tv = (h + kRows - 1) / kRows;
     for (int sweep = 0; sweep < 4; sweep++)
         for (int diag = 0; diag < tu + tv - 1; diag++) {
             const int lo = diag - (tu - 1) > 0 ? diag - (tu - 1) : 0, hi = diag < tv - 1 ? diag : tv - 1;
@@ -238,7 +286,7 @@ void launch_fast_sweep_2d(int w, int h, int16_t* delta, int cw, hipStream_t s)
 void launch_fast_sweep_3d(int w, int h, int d, int16_t* delta, int cw, hipStream_t s)
 {
     ProfileScope p("fast_sweep_3d", s);
-    const int tu = (w + kST - 1) / kST, tv = (h + kST - 1) / kST;
+    const int tu = (w + kST - 1) / kST, tv = (h + kRows - 1) / kRows;
     for (int sweep = 0; sweep < 8; sweep++)
         for (int diag = 0; diag < tu + tv + d - 2; diag++)
             hipLaunchKernelGGL(k_fast_sweep_3d, dim3((unsigned)(tu * tv)), dim3(kST), 0, s, w, h, d,
