@@ -6,8 +6,8 @@
 //
 // Flags as the reference's (DistanceOcclusionTool.cpp:1782-1883): -d distance,
 // -s signed distance with border initialisation, -S signed distance from
-// exterior and interior fields, -m 2 JumpFlood (default here)
-// or 5 Felzenszwalb (-d only), -w image size, -W volume size, -p random points
+// exterior and interior fields, -m 1 FastSweep, 2 JumpFlood (default here)
+// or 5 Felzenszwalb (-d only; FindDeltas, :711-739 and 937-965), -w image size, -W volume size, -p random points
 // (default seed 12345), -b block sides, -l maxLen, -c check.  Flows follow
 // GenerateDistance (:832-883), GenerateDistanceFelzenszwalb (:741-782),
 // GenerateSignedDistance (:1082-1124, 1197-1232) and
@@ -34,6 +34,19 @@ uint8_t EncodeU8Signed(float v)   // DistanceOcclusionTool.cpp:71-75
 }
 
 float Dist(int x, int y, int z = 0) { return sqrtf((float)(x * x + y * y + z * z)); }
+
+// FindDeltas (DistanceOcclusionTool.cpp:711-739, 937-965) for the methods
+// on the GPU path: 1 FastSweep, 2 JumpFlood
+void FindDeltas(int method, int w, int h, cCellDelta2 delta[], int cw = 1)
+{
+    if (method == 1) FastSweep(w, h, delta, cw);
+    else JumpFlood(w, h, delta, cw);
+}
+void FindDeltas(int method, int w, int h, int d, cCellDelta3 delta[], int cw = 1)
+{
+    if (method == 1) FastSweep(w, h, d, delta, cw);
+    else JumpFlood(w, h, d, delta, cw);
+}
 
 bool write_raw(const char* path, const void* p, size_t n)
 {
@@ -98,25 +111,25 @@ int main(int argc, char** argv)
         InitDistancesFromBitMask(w, h, mask, dist.data());
         Felzenszwalb(w, h, dist.data());
         for (float& v : dist) v = sqrtf(v);
-    } else if (mode == 'd' && method == 2) {
+    } else if (mode == 'd' && (method == 1 || method == 2)) {
         if (volume) {
             std::vector<cCellDelta3> delta(n);
             InitDeltasFromBitMask(w, h, d, mask, delta.data());
-            JumpFlood(w, h, d, delta.data());
+            FindDeltas(method, w, h, d, delta.data());
             for (size_t i = 0; i < n; i++) dist[i] = Dist(delta[i].x, delta[i].y, delta[i].z);
         } else {
             std::vector<cCellDelta2> delta(n);
             InitDeltasFromBitMask(w, h, mask, delta.data());
-            JumpFlood(w, h, delta.data());
+            FindDeltas(method, w, h, delta.data());
             for (size_t i = 0; i < n; i++) dist[i] = Dist(delta[i].x, delta[i].y);
         }
-    } else if (mode == 's' && method == 2) {
+    } else if (mode == 's' && (method == 1 || method == 2)) {
         std::vector<uint8_t> sdf(n);
         const int ws = MaskSize(w);
         if (volume) {
             std::vector<cCellDelta3> delta(n);
             InitBorderDeltasFromBitMask(w, h, d, mask, delta.data());
-            JumpFlood(w, h, d, delta.data(), 2);
+            FindDeltas(method, w, h, d, delta.data(), 2);
             for (size_t i = 0; i < n; i++) {
                 const size_t row = i / w;
                 const int x = (int)(i - row * w);
@@ -127,7 +140,7 @@ int main(int argc, char** argv)
         } else {
             std::vector<cCellDelta2> delta(n);
             InitBorderDeltasFromBitMask(w, h, mask, delta.data());
-            JumpFlood(w, h, delta.data(), 2);
+            FindDeltas(method, w, h, delta.data(), 2);
             for (size_t i = 0; i < n; i++) {
                 const size_t row = i / w;
                 const int x = (int)(i - row * w);
@@ -139,7 +152,7 @@ int main(int argc, char** argv)
         if (!write_raw(out, sdf.data(), n)) ok = false;
         delete[] mask;
         return ok ? 0 : -1;
-    } else if (mode == 'S' && method == 2) {
+    } else if (mode == 'S' && (method == 1 || method == 2)) {
         // GenerateSignedDistance (:1082-1124, 1197-1232): exterior and
         // interior fields, then (Dist(d0) - Dist(d1)) / maxLen (:1035-1036)
         std::vector<uint8_t> sdf(n);
@@ -147,16 +160,16 @@ int main(int argc, char** argv)
             std::vector<cCellDelta3> d0(n), d1(n);
             InitDeltasFromBitMask(w, h, d, mask, d0.data(), false);
             InitDeltasFromBitMask(w, h, d, mask, d1.data(), true);
-            JumpFlood(w, h, d, d0.data());
-            JumpFlood(w, h, d, d1.data());
+            FindDeltas(method, w, h, d, d0.data());
+            FindDeltas(method, w, h, d, d1.data());
             for (size_t i = 0; i < n; i++)
                 sdf[i] = EncodeU8Signed((Dist(d0[i].x, d0[i].y, d0[i].z) - Dist(d1[i].x, d1[i].y, d1[i].z)) / maxLen);
         } else {
             std::vector<cCellDelta2> d0(n), d1(n);
             InitDeltasFromBitMask(w, h, mask, d0.data(), false);
             InitDeltasFromBitMask(w, h, mask, d1.data(), true);
-            JumpFlood(w, h, d0.data());
-            JumpFlood(w, h, d1.data());
+            FindDeltas(method, w, h, d0.data());
+            FindDeltas(method, w, h, d1.data());
             for (size_t i = 0; i < n; i++)
                 sdf[i] = EncodeU8Signed((Dist(d0[i].x, d0[i].y) - Dist(d1[i].x, d1[i].y)) / maxLen);
         }

@@ -102,3 +102,26 @@ def test_cli_two_field(gpu, oracle, tmp_path):
 def test_cli_rejects_unknown(gpu, tmp_path):
     r, _ = run(tmp_path, "-s", "-m", "5")
     assert r.returncode != 0
+
+
+def test_cli_fast_sweep(gpu, oracle, tmp_path):
+    """-m 1: the tool's FindDeltas with FastSweep, in -d (checked against
+    the exact EDT, as -c does) and -s flows, 2D and 3D."""
+    r, out = run(tmp_path, "-d", "-m", "1", "-w", "300", "-p", "40", "-c")
+    assert r.returncode == 0, r.stderr
+    m = mask2(oracle, 300, 40)
+    exp = oracle.dist_2d(oracle.fast_sweep_2d(300, 300, oracle.init_deltas_2d(300, 300, m), 1))
+    np.testing.assert_array_equal(np.fromfile(out, np.float32).reshape(300, 300), exp)
+    r, out = run(tmp_path, "-s", "-m", "1", "-w", "256", "-b", "4", "-p", "30")
+    assert r.returncode == 0, r.stderr
+    m = mask2(oracle, 256, 30, 4)
+    d = oracle.fast_sweep_2d(256, 256, oracle.border_2d(256, 256, m), 2)
+    np.testing.assert_array_equal(np.fromfile(out, np.uint8).reshape(256, 256),
+                                  oracle.sdf_u8_2d(256, 256, m, d, 2 * 0.25 * 256))
+    w = 40
+    r, out = run(tmp_path, "-s", "-m", "1", "-W", str(w), "-b", "7", "-p", "10")
+    assert r.returncode == 0, r.stderr
+    m = mask3(oracle, w, 10, 7)
+    d = oracle.fast_sweep_3d(w, w, w, oracle.border_3d(w, w, w, m), 2)
+    np.testing.assert_array_equal(np.fromfile(out, np.uint8).reshape(w, w, w),
+                                  oracle.sdf_u8_3d(w, w, w, m, d, 2 * 0.25 * w))
