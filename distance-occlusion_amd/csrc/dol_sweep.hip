@@ -158,15 +158,24 @@ __global__ void __launch_bounds__(kST) k_fast_sweep_2d(int w, int h, int sweep, 
     }
 }
 
+// 3D tiles are kTW3 columns wide: a tile is then kTW3 + 63 steps, and with
+// the slices adding to the number of diagonals (u + v + t), narrow tiles
+// shorten the whole chain (d + w / kTW3 + h / 63 diagonals of kTW3 + 63
+// steps).  The lane's row is rotated modulo kTW3, as in 2D.
+constexpr int kTW3 = 8;
+constexpr int kS3 = kTW3 + 2;   // LDS row stride: lane l's column s - l sits at
+                                // word (kS3 - 1) l + s, an odd multiple of l
+                                // plus s: a different bank per lane
+
 // 3D: tiles (U, V) of slice t = diag - U - V; block b is (U, V) = (b % tu, b / tu).
 // The sweep's first slice (t = 0) is never updated.
 __global__ void __launch_bounds__(kST) k_fast_sweep_3d(int w, int h, int d, int sweep, int cw, int diag,
                                                        int tu, int16_t* __restrict__ delta)
 {
-    __shared__ uint32_t txy[kST * kSStride];
-    __shared__ int16_t tz[kST * kSStride];
-    __shared__ uint32_t bxy[kST * kSStride];   // the same cells of the slice before
-    __shared__ int16_t bz[kST * kSStride];
+    __shared__ uint32_t txy[kST * kS3];
+    __shared__ int16_t tz[kST * kS3];
+    __shared__ uint32_t bxy[kST * kS3];   // the same cells of the slice before
+    __shared__ int16_t bz[kST * kS3];
     const int U = (int)blockIdx.x % tu, V = (int)blockIdx.x / tu;
     const int t = diag - U - V;
     if (t < 1 || t >= d) return;   // the sweep's first slice keeps its values
@@ -178,35 +187,36 @@ __global__ void __launch_bounds__(kST) k_fast_sweep_3d(int w, int h, int d, int 
     int16_t* cur_slice = delta + 3 * (slice * (size_t)z);
     const uint32_t ox = (uint16_t)(-cw * f.dx), oy = (uint32_t)(uint16_t)(-cw * f.dy) << 16;
     const int oz = -cw * dz;
-    const int u0 = U * kST, v0 = V * kRows;
-    const int nu = min(kST, w - u0);
+    const int u0 = U * kTW3, v0 = V * kRows;
+    const int nu = min(kTW3, w - u0);
     const int lane = threadIdx.x;
     const int vr = v0 - 1 + lane;
     const int rlo = max(v0 - 1, 0), rhi = min(v0 + kRows - 1, h - 1);
-    const int cu = f.col(u0 + min(lane, nu - 1));
     auto pack = [](const int16_t* c) { return (uint32_t)(uint16_t)c[0] | ((uint32_t)(uint16_t)c[1] << 16); };
+    // cell e = lane + 64 i of the staged 64 x kTW3 block: row e / kTW3, column e % kTW3
+    auto cell = [&](int e) {
+        const int r = e / kTW3, c = e % kTW3;
+        return 3 * (f.row(min(max(v0 - 1 + r, rlo), rhi)) + f.col(u0 + min(c, nu - 1)));
+    };
     {
-        // the rows and the same cells of the slice before, 32 rows at a time
-        // in flight (rows outside the field repeat the nearest)
+        // every cell of the block and of the slice before in flight at once
+        uint32_t vxy[kTW3], bv[kTW3];
+        int16_t vz[kTW3], bvz[kTW3];
 #pragma unroll
-        for (int r0 = 0; r0 < kST; r0 += kST / 2) {
-            uint32_t vxy[kST / 2], bv[kST / 2];
-            int16_t vz[kST / 2], bvz[kST / 2];
+        for (int i = 0; i < kTW3; i++) {
+            const size_t o = cell(lane + kST * i);
+            vxy[i] = pack(cur_slice + o);
+            vz[i] = cur_slice[o + 2];
+            bv[i] = pack(back + o);
+            bvz[i] = back[o + 2];
+        }
 #pragma unroll
-            for (int r = 0; r < kST / 2; r++) {
-                const size_t i = 3 * (f.row(min(max(v0 - 1 + r0 + r, rlo), rhi)) + cu);
-                vxy[r] = pack(cur_slice + i);
-                vz[r] = cur_slice[i + 2];
-                bv[r] = pack(back + i);
-                bvz[r] = back[i + 2];
-            }
-#pragma unroll
-            for (int r = 0; r < kST / 2; r++) {
-                txy[(r0 + r) * kSStride + lane] = vxy[r];
-                tz[(r0 + r) * kSStride + lane] = vz[r];
-                bxy[(r0 + r) * kSStride + lane] = bv[r];
-                bz[(r0 + r) * kSStride + lane] = bvz[r];
-            }
+        for (int i = 0; i < kTW3; i++) {
+            const int e = lane + kST * i, k = (e / kTW3) * kS3 + e % kTW3;
+            txy[k] = vxy[i];
+            tz[k] = vz[i];
+            bxy[k] = bv[i];
+            bz[k] = bvz[i];
         }
     }
     const bool row_ok = vr >= 0 && vr < h, update = row_ok && lane >= 1 && vr >= 1;
@@ -218,25 +228,27 @@ __global__ void __launch_bounds__(kST) k_fast_sweep_3d(int w, int h, int d, int 
         mine_z = c[2];
     }
     __syncthreads();
-    uint32_t R[kST];
-    int RZ[kST];
+    uint32_t R[kTW3];
+    int RZ[kTW3];
 #pragma unroll
-    for (int k = 0; k < kST; k++) {
-        R[k] = txy[lane * kSStride + rot(k, lane)];
-        RZ[k] = tz[lane * kSStride + rot(k, lane)];
+    for (int k = 0; k < kTW3; k++) {
+        const int j = lane * kS3 + ((k - lane) & (kTW3 - 1));
+        R[k] = txy[j];
+        RZ[k] = tz[j];
     }
-    const uint32_t* brow = bxy + lane * kSStride;
-    const int16_t* brow_z = bz + lane * kSStride;
+    const uint32_t* brow = bxy + lane * kS3;
+    const int16_t* brow_z = bz + lane * kS3;
 #pragma unroll
-    for (int s = 0; s < 2 * kST - 1; s++) {
-        const int a = s - lane;   // in [-63, 126]: brow + a stays inside bxy
+    for (int s = 0; s < kTW3 + kST - 1; s++) {
+        const int a = s - lane;
+        const int ab = min(max(a, 0), kTW3 - 1);   // the read stays in the lane's row
         // (a, v - 1), finished by the lane before
         const uint32_t up = from_prev_lane(mine);
         const int up_z = (int)(int16_t)from_prev_lane((uint32_t)mine_z);
-        const uint32_t cxy = R[s & (kST - 1)];
-        const int cz = RZ[s & (kST - 1)];
-        const uint32_t bxy_ = brow[a];   // the slice before, (0, 0, cz)
-        const int bz_ = wrap16(brow_z[a] + oz);
+        const uint32_t cxy = R[s & (kTW3 - 1)];
+        const int cz = RZ[s & (kTW3 - 1)];
+        const uint32_t bxy_ = brow[ab];   // the slice before, (0, 0, cz)
+        const int bz_ = wrap16(brow_z[ab] + oz);
         const uint32_t uxy = sw_add(up, oy);   // the row before, (0, cy, 0)
         const uint32_t lxy = sw_add(mine, ox);   // the cell before, (cx, 0, 0)
         const int l0 = sw_len2(cxy, cz * cz), l1 = sw_len2(bxy_, bz_ * bz_), l2 = sw_len2(uxy, up_z * up_z);
@@ -247,27 +259,30 @@ __global__ void __launch_bounds__(kST) k_fast_sweep_3d(int w, int h, int d, int 
         const int nz = pick == 0 ? cz : (pick == 1 ? bz_ : (pick == 2 ? up_z : mine_z));
         if (row_ok && a >= 0 && a < nu) {
             if (update) {
-                R[s & (kST - 1)] = nxy;
-                RZ[s & (kST - 1)] = nz;
+                R[s & (kTW3 - 1)] = nxy;
+                RZ[s & (kTW3 - 1)] = nz;
             }
-            mine = R[s & (kST - 1)];
-            mine_z = RZ[s & (kST - 1)];
+            mine = R[s & (kTW3 - 1)];
+            mine_z = RZ[s & (kTW3 - 1)];
         }
     }
 #pragma unroll
-    for (int k = 0; k < kST; k++) {
-        txy[lane * kSStride + rot(k, lane)] = R[k];
-        tz[lane * kSStride + rot(k, lane)] = (int16_t)RZ[k];
+    for (int k = 0; k < kTW3; k++) {
+        const int j = lane * kS3 + ((k - lane) & (kTW3 - 1));
+        txy[j] = R[k];
+        tz[j] = (int16_t)RZ[k];
     }
     __syncthreads();
-    for (int r = 1; r < kST; r++) {
-        const uint32_t xy = txy[r * kSStride + lane];
-        const int16_t zz = tz[r * kSStride + lane];
-        if (v0 - 1 + r > rhi || lane >= nu) continue;
-        int16_t* c = cur_slice + 3 * (f.row(v0 - 1 + r) + cu);
-        c[0] = (int16_t)(xy & 0xffffu);
-        c[1] = (int16_t)(xy >> 16);
-        c[2] = zz;
+#pragma unroll
+    for (int i = 0; i < kTW3; i++) {
+        const int e = lane + kST * i, r = e / kTW3, c = e % kTW3;
+        const uint32_t xy = txy[r * kS3 + c];
+        const int16_t zz = tz[r * kS3 + c];
+        if (r < 1 || v0 - 1 + r > rhi || c >= nu) continue;
+        int16_t* o = cur_slice + cell(e);
+        o[0] = (int16_t)(xy & 0xffffu);
+        o[1] = (int16_t)(xy >> 16);
+        o[2] = zz;
     }
 }
 
@@ -286,7 +301,7 @@ void launch_fast_sweep_2d(int w, int h, int16_t* delta, int cw, hipStream_t s)
 void launch_fast_sweep_3d(int w, int h, int d, int16_t* delta, int cw, hipStream_t s)
 {
     ProfileScope p("fast_sweep_3d", s);
-    const int tu = (w + kST - 1) / kST, tv = (h + kRows - 1) / kRows;
+    const int tu = (w + kTW3 - 1) / kTW3, tv = (h + kRows - 1) / kRows;
     for (int sweep = 0; sweep < 8; sweep++)
         for (int diag = 0; diag < tu + tv + d - 2; diag++)
             hipLaunchKernelGGL(k_fast_sweep_3d, dim3((unsigned)(tu * tv)), dim3(kST), 0, s, w, h, d,
