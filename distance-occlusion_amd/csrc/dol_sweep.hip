@@ -158,131 +158,107 @@ __global__ void __launch_bounds__(kST) k_fast_sweep_2d(int w, int h, int sweep, 
     }
 }
 
-// 3D tiles are kTW3 columns wide: a tile is then kTW3 + 63 steps, and with
-// the slices adding to the number of diagonals (u + v + t), narrow tiles
-// shorten the whole chain (d + w / kTW3 + h / 63 diagonals of kTW3 + 63
-// steps).  The lane's row is rotated modulo kTW3, as in 2D.
+// 3D: the slices add d to the number of diagonals (u + v + t), so short
+// tiles pay: a wavefront holds four tiles, one per 16-lane DPP row, each
+// kTW3 = 8 columns by kRows3 = 15 rows (lane 0 of the row carrying the row
+// before the tile, as in 2D), which is kTW3 + 15 steps a tile.  A lane keeps
+// its row and the same row of the slice before in registers, rotated modulo
+// kTW3 by its place in the group, so the tile needs no LDS.
 constexpr int kTW3 = 8;
-constexpr int kS3 = kTW3 + 2;   // LDS row stride: lane l's column s - l sits at
-                                // word (kS3 - 1) l + s, an odd multiple of l
-                                // plus s: a different bank per lane
+constexpr int kGroup3 = 16;                 // lanes per tile (one DPP row)
+constexpr int kRows3 = kGroup3 - 1;
+constexpr int kTiles3 = kST / kGroup3;      // tiles per wavefront, stacked in v
 
-// 3D: tiles (U, V) of slice t = diag - U - V; block b is (U, V) = (b % tu, b / tu).
-// The sweep's first slice (t = 0) is never updated.
+// the value of the lane before in the lane's DPP row of 16 (row_shr:1); the
+// row's first lane gets 0
+__device__ __forceinline__ uint32_t from_prev_lane16(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);
+}
+
+// 3D: the tiles (U, V, t) with U + V + t = diag; block b holds U = b % tu and
+// V = kTiles3 * (b / tu) + g in lane group g; the sweep's first slice (t = 0)
+// is never updated.
 __global__ void __launch_bounds__(kST) k_fast_sweep_3d(int w, int h, int d, int sweep, int cw, int diag,
                                                        int tu, int16_t* __restrict__ delta)
 {
-    __shared__ uint32_t txy[kST * kS3];
-    __shared__ int16_t tz[kST * kS3];
-    __shared__ uint32_t bxy[kST * kS3];   // the same cells of the slice before
-    __shared__ int16_t bz[kST * kS3];
-    const int U = (int)blockIdx.x % tu, V = (int)blockIdx.x / tu;
+    const int lane = threadIdx.x, j = lane % kGroup3;
+    const int U = (int)blockIdx.x % tu, V = kTiles3 * ((int)blockIdx.x / tu) + lane / kGroup3;
     const int t = diag - U - V;
-    if (t < 1 || t >= d) return;   // the sweep's first slice keeps its values
     const Frame f = frame(w, h, sweep);
     const int dz = 1 - 2 * ((sweep >> 2) & 1);
-    const int z = (dz > 0 ? 0 : d - 1) + t * dz;
+    const bool tile_ok = t >= 1 && t < d && V * kRows3 < h;
+    // a group without a tile reads slice 0 (and writes nothing)
+    const int z = tile_ok ? (dz > 0 ? 0 : d - 1) + t * dz : 0;
+    const int zb = tile_ok ? z - dz : 0;
     const size_t slice = (size_t)w * h;
-    const int16_t* back = delta + 3 * (slice * (size_t)(z - dz));
+    const int16_t* back = delta + 3 * (slice * (size_t)zb);
     int16_t* cur_slice = delta + 3 * (slice * (size_t)z);
     const uint32_t ox = (uint16_t)(-cw * f.dx), oy = (uint32_t)(uint16_t)(-cw * f.dy) << 16;
     const int oz = -cw * dz;
-    const int u0 = U * kTW3, v0 = V * kRows;
+    const int u0 = U * kTW3, v0 = V * kRows3;
     const int nu = min(kTW3, w - u0);
-    const int lane = threadIdx.x;
-    const int vr = v0 - 1 + lane;
-    const int rlo = max(v0 - 1, 0), rhi = min(v0 + kRows - 1, h - 1);
+    const int vr = v0 - 1 + j;                            // the lane's row
+    const bool row_ok = tile_ok && vr >= 0 && vr < h;
+    const bool update = row_ok && j >= 1 && vr >= 1;
+    const size_t rbase = f.row(min(max(vr, 0), h - 1));
+    auto at = [&](int c) { return 3 * (rbase + f.col(u0 + min(c, nu - 1))); };
     auto pack = [](const int16_t* c) { return (uint32_t)(uint16_t)c[0] | ((uint32_t)(uint16_t)c[1] << 16); };
-    // cell e = lane + 64 i of the staged 64 x kTW3 block: row e / kTW3, column e % kTW3
-    auto cell = [&](int e) {
-        const int r = e / kTW3, c = e % kTW3;
-        return 3 * (f.row(min(max(v0 - 1 + r, rlo), rhi)) + f.col(u0 + min(c, nu - 1)));
-    };
-    {
-        // every cell of the block and of the slice before in flight at once
-        uint32_t vxy[kTW3], bv[kTW3];
-        int16_t vz[kTW3], bvz[kTW3];
+    // register k holds column (k - j) mod kTW3, so step s reads register s mod kTW3
+    uint32_t R[kTW3], B[kTW3];
+    int RZ[kTW3], BZ[kTW3];
 #pragma unroll
-        for (int i = 0; i < kTW3; i++) {
-            const size_t o = cell(lane + kST * i);
-            vxy[i] = pack(cur_slice + o);
-            vz[i] = cur_slice[o + 2];
-            bv[i] = pack(back + o);
-            bvz[i] = back[o + 2];
-        }
-#pragma unroll
-        for (int i = 0; i < kTW3; i++) {
-            const int e = lane + kST * i, k = (e / kTW3) * kS3 + e % kTW3;
-            txy[k] = vxy[i];
-            tz[k] = vz[i];
-            bxy[k] = bv[i];
-            bz[k] = bvz[i];
-        }
+    for (int k = 0; k < kTW3; k++) {
+        const size_t o = at((k - j) & (kTW3 - 1));
+        R[k] = pack(cur_slice + o);
+        RZ[k] = cur_slice[o + 2];
+        B[k] = pack(back + o);
+        BZ[k] = wrap16(back[o + 2] + oz);
     }
-    const bool row_ok = vr >= 0 && vr < h, update = row_ok && lane >= 1 && vr >= 1;
-    uint32_t mine = 0u;
+    uint32_t mine = 0u;   // the lane's last result; first, the column before the tile
     int mine_z = 0;
     if (u0 > 0 && row_ok) {
-        const int16_t* c = cur_slice + 3 * (f.row(vr) + f.col(u0 - 1));
+        const int16_t* c = cur_slice + 3 * (rbase + f.col(u0 - 1));
         mine = pack(c);
         mine_z = c[2];
     }
-    __syncthreads();
-    uint32_t R[kTW3];
-    int RZ[kTW3];
 #pragma unroll
-    for (int k = 0; k < kTW3; k++) {
-        const int j = lane * kS3 + ((k - lane) & (kTW3 - 1));
-        R[k] = txy[j];
-        RZ[k] = tz[j];
-    }
-    const uint32_t* brow = bxy + lane * kS3;
-    const int16_t* brow_z = bz + lane * kS3;
-#pragma unroll
-    for (int s = 0; s < kTW3 + kST - 1; s++) {
-        const int a = s - lane;
-        const int ab = min(max(a, 0), kTW3 - 1);   // the read stays in the lane's row
+    for (int s = 0; s < kTW3 + kGroup3 - 1; s++) {
+        const int a = s - j;
+        const int k = s & (kTW3 - 1);
         // (a, v - 1), finished by the lane before
-        const uint32_t up = from_prev_lane(mine);
-        const int up_z = (int)(int16_t)from_prev_lane((uint32_t)mine_z);
-        const uint32_t cxy = R[s & (kTW3 - 1)];
-        const int cz = RZ[s & (kTW3 - 1)];
-        const uint32_t bxy_ = brow[ab];   // the slice before, (0, 0, cz)
-        const int bz_ = wrap16(brow_z[ab] + oz);
-        const uint32_t uxy = sw_add(up, oy);   // the row before, (0, cy, 0)
+        const uint32_t up = from_prev_lane16(mine);
+        const int up_z = (int)(int16_t)from_prev_lane16((uint32_t)mine_z);
+        const uint32_t cxy = R[k];
+        const int cz = RZ[k];
+        const uint32_t uxy = sw_add(up, oy);     // the row before, (0, cy, 0)
         const uint32_t lxy = sw_add(mine, ox);   // the cell before, (cx, 0, 0)
-        const int l0 = sw_len2(cxy, cz * cz), l1 = sw_len2(bxy_, bz_ * bz_), l2 = sw_len2(uxy, up_z * up_z);
+        // candidates in the reference's order: the slice before (0, 0, cz),
+        // the row before, the cell before
+        const int l0 = sw_len2(cxy, cz * cz), l1 = sw_len2(B[k], BZ[k] * BZ[k]), l2 = sw_len2(uxy, up_z * up_z);
         const int l3 = u0 + a >= 1 ? sw_len2(lxy, mine_z * mine_z) : kNoLen;
         const int m = min(min(l0, l1), min(l2, l3));
         const int pick = l0 == m ? 0 : (l1 == m ? 1 : (l2 == m ? 2 : 3));
-        const uint32_t nxy = pick == 0 ? cxy : (pick == 1 ? bxy_ : (pick == 2 ? uxy : lxy));
-        const int nz = pick == 0 ? cz : (pick == 1 ? bz_ : (pick == 2 ? up_z : mine_z));
+        const uint32_t nxy = pick == 0 ? cxy : (pick == 1 ? B[k] : (pick == 2 ? uxy : lxy));
+        const int nz = pick == 0 ? cz : (pick == 1 ? BZ[k] : (pick == 2 ? up_z : mine_z));
         if (row_ok && a >= 0 && a < nu) {
             if (update) {
-                R[s & (kTW3 - 1)] = nxy;
-                RZ[s & (kTW3 - 1)] = nz;
+                R[k] = nxy;
+                RZ[k] = nz;
             }
-            mine = R[s & (kTW3 - 1)];
-            mine_z = RZ[s & (kTW3 - 1)];
+            mine = R[k];
+            mine_z = RZ[k];
         }
     }
+    if (!update) return;
 #pragma unroll
     for (int k = 0; k < kTW3; k++) {
-        const int j = lane * kS3 + ((k - lane) & (kTW3 - 1));
-        txy[j] = R[k];
-        tz[j] = (int16_t)RZ[k];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < kTW3; i++) {
-        const int e = lane + kST * i, r = e / kTW3, c = e % kTW3;
-        const uint32_t xy = txy[r * kS3 + c];
-        const int16_t zz = tz[r * kS3 + c];
-        if (r < 1 || v0 - 1 + r > rhi || c >= nu) continue;
-        int16_t* o = cur_slice + cell(e);
-        o[0] = (int16_t)(xy & 0xffffu);
-        o[1] = (int16_t)(xy >> 16);
-        o[2] = zz;
+        const int c = (k - j) & (kTW3 - 1);
+        if (c >= nu) continue;
+        int16_t* o = cur_slice + at(c);
+        o[0] = (int16_t)(R[k] & 0xffffu);
+        o[1] = (int16_t)(R[k] >> 16);
+        o[2] = (int16_t)RZ[k];
     }
 }
 
@@ -301,10 +277,11 @@ void launch_fast_sweep_2d(int w, int h, int16_t* delta, int cw, hipStream_t s)
 void launch_fast_sweep_3d(int w, int h, int d, int16_t* delta, int cw, hipStream_t s)
 {
     ProfileScope p("fast_sweep_3d", s);
-    const int tu = (w + kTW3 - 1) / kTW3, tv = (h + kRows - 1) / kRows;
+    const int tu = (w + kTW3 - 1) / kTW3, tv = (h + kRows3 - 1) / kRows3;
+    const int vb = (tv + kTiles3 - 1) / kTiles3;   // wavefronts along v
     for (int sweep = 0; sweep < 8; sweep++)
-        for (int diag = 0; diag < tu + tv + d - 2; diag++)
-            hipLaunchKernelGGL(k_fast_sweep_3d, dim3((unsigned)(tu * tv)), dim3(kST), 0, s, w, h, d,
+        for (int diag = 1; diag < tu + tv + d - 2; diag++)   // diagonal 0 is slice 0 only
+            hipLaunchKernelGGL(k_fast_sweep_3d, dim3((unsigned)(tu * vb)), dim3(kST), 0, s, w, h, d,
                                sweep, cw, diag, tu, delta);
 }
 
