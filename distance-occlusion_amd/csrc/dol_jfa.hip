@@ -37,12 +37,16 @@ __global__ void __launch_bounds__(256) k_jfa2d_x4(int step, int w, int h, int xb
 {
     // rows in residue classes mod step (y = r, r + step, r + 2 step, ...)
     // when step divides h: consecutive rows then share two of their three
-    // source rows, which stay in L2 (the xb blocks of a row go to distinct
-    // XCDs, each XCD keeping its own column range)
-    const int q = blockIdx.x / xb;
+    // source rows.  Workgroups are dealt to the 8 XCDs round-robin; when the
+    // grid splits evenly, XCD c takes the c-th eighth of the rows in that
+    // order, so the rows a row's taps read (and, for steps past a block's
+    // width, the columns its x taps read) are fetched into the same L2.
+    int b = blockIdx.x;
+    if ((gridDim.x & 7) == 0) b = (b & 7) * (gridDim.x >> 3) + (b >> 3);
+    const int q = b / xb;
     const int nq = h / step;
     const int y = nq * step == h ? (q % nq) * step + q / nq : q;
-    const int x = ((blockIdx.x - q * xb) * 256 + threadIdx.x) * 4;
+    const int x = ((b - q * xb) * 256 + threadIdx.x) * 4;
     if (x >= w) return;
     const int off = cw * step;
     const bool up = y - step >= 0, down = y + step < h;
