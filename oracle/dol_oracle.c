@@ -805,6 +805,101 @@ void oc_fast_sweep_3d(int w, int h, int d, int16_t* delta, int cw)
     }
 }
 
+/* ------------------------------------------------------- occlusion sweeps */
+
+/* DistanceOcclusionLib.cpp:1114-1146 (2D, 4 planes) and 2507-2538 (3D, 8
+ * planes): every plane is 1 where the bit is set, else 0. */
+static void init_dirw(int w, size_t rows, int planes, const uint32_t* mask, float* dirw)
+{
+    const size_t cells = (size_t)w * rows;
+    const int ws = mask_stride(w);
+    for (size_t r = 0; r < rows; r++)
+        for (int x = 0; x < w; x++) {
+            const float v = ((mask[r * ws + (x >> 5)] >> (x & 31)) & 1u) ? 1.0f : 0.0f;
+            for (int p = 0; p < planes; p++) dirw[p * cells + r * w + x] = v;
+        }
+}
+void oc_init_dirw_2d(int w, int h, const uint32_t* mask, float* dirw) { init_dirw(w, (size_t)h, 4, mask, dirw); }
+void oc_init_dirw_3d(int w, int h, int d, const uint32_t* mask, float* dirw)
+{
+    init_dirw(w, (size_t)h * d, 8, mask, dirw);
+}
+
+/* QuadrantCombine and the two OctantCombines (Lib:1150-1168, 2543-2567),
+ * the same float terms in the same order. */
+static float quadrant_combine(float a, float b) { return 0.75f * ((a + b) - 1.0f * a * b); }
+static float octant_combine3(float a, float b, float c) { return 0.33333333f * (a + b + c - a * b * c); }
+static float octant_combine6(float oa, float ob, float oc, float ia, float ib, float ic)
+{
+    const float a = ia + (1.0f - ia) * oa;
+    const float b = ib + (1.0f - ib) * ob;
+    const float c = ic + (1.0f - ic) * oc;
+    return 0.3333333f * (a + b + c - a * b * c);
+}
+
+/* Lib:1171-1209.  Plane q is swept from its corner (x direction 1 - 2 (q & 1),
+ * y 1 - 2 (q >> 1 & 1)), rows in order, each cell in x order: the first row
+ * takes the cell before it, the first column the row before it, every other
+ * cell both; v += (1 - v) * QuadrantCombine(before, row before). */
+void oc_occlusion_sweep_2d(int w, int h, float* dirw)
+{
+    const size_t cells = (size_t)w * h;
+    for (int q = 0; q < 4; q++) {
+        float* f = dirw + q * cells;
+        const int dx = 1 - 2 * (q & 1), dy = 1 - 2 * ((q >> 1) & 1);
+        const int x0 = dx > 0 ? 0 : w - 1, y0 = dy > 0 ? 0 : h - 1;
+        for (int v = 0; v < h; v++) {
+            const int y = y0 + v * dy;
+            float* row = f + (size_t)y * w;
+            const float* prev = f + (size_t)(y - dy) * w;
+            for (int u = 0; u < w; u++) {
+                const int x = x0 + u * dx;
+                float c;
+                if (u == 0 && v == 0) continue;
+                if (v == 0) c = quadrant_combine(row[x - dx], 0.0f);
+                else if (u == 0) c = quadrant_combine(prev[x], 0.0f);
+                else c = quadrant_combine(row[x - dx], prev[x]);
+                row[x] += (1.0f - row[x]) * c;
+            }
+        }
+    }
+}
+
+/* Lib:2569-2648 (in place) and 2650-2743 (from `in`), plane q over octant
+ * (1 - 2 (q & 1), 1 - 2 (q >> 1 & 1), 1 - 2 (q >> 2 & 1)); slices, rows and
+ * cells in order; a neighbour before the field's start is 0.  In place:
+ * v += (1 - v) * OctantCombine(cell before, row before, slice before); from
+ * `in`: out = OctantCombine(their out values, their in values). */
+static void occlusion_sweep_3d(int w, int h, int d, const float* in, float* dirw)
+{
+    const size_t cells = (size_t)w * h * d, slice = (size_t)w * h;
+    for (int q = 0; q < 8; q++) {
+        float* f = dirw + q * cells;
+        const float* g = in ? in + q * cells : NULL;
+        const int dx = 1 - 2 * (q & 1), dy = 1 - 2 * ((q >> 1) & 1), dz = 1 - 2 * ((q >> 2) & 1);
+        const int x0 = dx > 0 ? 0 : w - 1, y0 = dy > 0 ? 0 : h - 1, z0 = dz > 0 ? 0 : d - 1;
+        for (int t = 0; t < d; t++)
+            for (int v = 0; v < h; v++)
+                for (int u = 0; u < w; u++) {
+                    if (u == 0 && v == 0 && t == 0) continue;
+                    const int x = x0 + u * dx, y = y0 + v * dy, z = z0 + t * dz;
+                    const size_t i = z * slice + (size_t)y * w + x;
+                    const size_t il = i - dx, iu = i - (ptrdiff_t)dy * w, ib = i - (ptrdiff_t)dz * slice;
+                    const float ol = u ? f[il] : 0.0f, ou = v ? f[iu] : 0.0f, ob = t ? f[ib] : 0.0f;
+                    if (g) {
+                        f[i] = octant_combine6(ol, ou, ob, u ? g[il] : 0.0f, v ? g[iu] : 0.0f, t ? g[ib] : 0.0f);
+                    } else {
+                        f[i] += (1.0f - f[i]) * octant_combine3(ol, ou, ob);
+                    }
+                }
+    }
+}
+void oc_occlusion_sweep_3d(int w, int h, int d, float* dirw) { occlusion_sweep_3d(w, h, d, NULL, dirw); }
+void oc_occlusion_sweep_3d_from(int w, int h, int d, const float* in, float* out)
+{
+    occlusion_sweep_3d(w, h, d, in, out);
+}
+
 /* ------------------------------------------------- exact squared EDT (ext.) */
 
 /* Not in the reference (which only has the float 2D form above).  The

@@ -57,6 +57,13 @@ void oc_jfa_3d_mt(int w, int h, int d, int16_t* delta, int cw, int threads);
 void oc_fast_sweep_2d(int w, int h, int16_t* delta, int cw);
 void oc_fast_sweep_3d(int w, int h, int d, int16_t* delta, int cw);
 
+/* direction fields and occlusion sweeps (Lib:1114-1209, 2507-2769) */
+void oc_init_dirw_2d(int w, int h, const uint32_t* mask, float* dirw4);
+void oc_init_dirw_3d(int w, int h, int d, const uint32_t* mask, float* dirw8);
+void oc_occlusion_sweep_2d(int w, int h, float* dirw4);
+void oc_occlusion_sweep_3d(int w, int h, int d, float* dirw8);
+void oc_occlusion_sweep_3d_from(int w, int h, int d, const float* in8, float* out8);
+
 /* Felzenszwalb float squared distances, in place (Lib:1018-1085) */
 void oc_felzenszwalb_2d(int w, int h, float* dist);
 

@@ -47,6 +47,16 @@ void ref_fast_sweep_2d(int w, int h, int16_t* delta, int cw)
 { FastSweep(w, h, reinterpret_cast<cCellDelta2*>(delta), cw); }
 void ref_fast_sweep_3d(int w, int h, int d, int16_t* delta, int cw)
 { FastSweep(w, h, d, reinterpret_cast<cCellDelta3*>(delta), cw); }
+void ref_init_dirw_2d(int w, int h, const uint32_t* mask, float* dirw)
+{ InitDirWFromBitMask(w, h, mask, dirw); }
+void ref_init_dirw_3d(int w, int h, int d, const uint32_t* mask, float* dirw)
+{ InitDirWFromBitMask(w, h, d, mask, dirw); }
+void ref_occlusion_sweep_2d(int w, int h, float* dirw)
+{ OcclusionSweep(w, h, dirw); }
+void ref_occlusion_sweep_3d(int w, int h, int d, float* dirw)
+{ OcclusionSweep(w, h, d, dirw); }
+void ref_occlusion_sweep_3d_from(int w, int h, int d, const float* in, float* out)
+{ OcclusionSweep(w, h, d, in, out); }
 void ref_felzenszwalb_2d(int w, int h, float* dist)
 { Felzenszwalb(w, h, dist); }
 void ref_brute_force_2d(int w, int h, int16_t* delta, int cw)

@@ -147,6 +147,32 @@ class Oracle:
         self.L.oc_fast_sweep_3d(_I(w), _I(h), _I(d), _ptr(out), _I(cw))
         return out
 
+    def init_dirw_2d(self, w, h, mask):
+        out = np.empty((4, h, w), dtype=np.float32)
+        self.L.oc_init_dirw_2d(_I(w), _I(h), _ptr(mask), _ptr(out))
+        return out
+
+    def init_dirw_3d(self, w, h, d, mask):
+        out = np.empty((8, d, h, w), dtype=np.float32)
+        self.L.oc_init_dirw_3d(_I(w), _I(h), _I(d), _ptr(mask), _ptr(out))
+        return out
+
+    def occlusion_sweep_2d(self, w, h, dirw):
+        out = np.ascontiguousarray(dirw, dtype=np.float32).copy()
+        self.L.oc_occlusion_sweep_2d(_I(w), _I(h), _ptr(out))
+        return out
+
+    def occlusion_sweep_3d(self, w, h, d, dirw):
+        out = np.ascontiguousarray(dirw, dtype=np.float32).copy()
+        self.L.oc_occlusion_sweep_3d(_I(w), _I(h), _I(d), _ptr(out))
+        return out
+
+    def occlusion_sweep_3d_from(self, w, h, d, dirw_in, dirw_out):
+        src = np.ascontiguousarray(dirw_in, dtype=np.float32)
+        out = np.ascontiguousarray(dirw_out, dtype=np.float32).copy()
+        self.L.oc_occlusion_sweep_3d_from(_I(w), _I(h), _I(d), _ptr(src), _ptr(out))
+        return out
+
     def jfa_pass_2d(self, step, w, h, delta, cw=1):
         src = np.ascontiguousarray(delta, dtype=np.int16)
         out = np.empty_like(src)
@@ -341,6 +367,32 @@ class Reference:
     def fast_sweep_3d(self, w, h, d, delta, cw=1):
         out = np.ascontiguousarray(delta, dtype=np.int16).copy()
         self.L.ref_fast_sweep_3d(_I(w), _I(h), _I(d), _ptr(out), _I(cw))
+        return out
+
+    def init_dirw_2d(self, w, h, mask):
+        out = np.empty((4, h, w), dtype=np.float32)
+        self.L.ref_init_dirw_2d(_I(w), _I(h), _ptr(mask), _ptr(out))
+        return out
+
+    def init_dirw_3d(self, w, h, d, mask):
+        out = np.empty((8, d, h, w), dtype=np.float32)
+        self.L.ref_init_dirw_3d(_I(w), _I(h), _I(d), _ptr(mask), _ptr(out))
+        return out
+
+    def occlusion_sweep_2d(self, w, h, dirw):
+        out = np.ascontiguousarray(dirw, dtype=np.float32).copy()
+        self.L.ref_occlusion_sweep_2d(_I(w), _I(h), _ptr(out))
+        return out
+
+    def occlusion_sweep_3d(self, w, h, d, dirw):
+        out = np.ascontiguousarray(dirw, dtype=np.float32).copy()
+        self.L.ref_occlusion_sweep_3d(_I(w), _I(h), _I(d), _ptr(out))
+        return out
+
+    def occlusion_sweep_3d_from(self, w, h, d, dirw_in, dirw_out):
+        src = np.ascontiguousarray(dirw_in, dtype=np.float32)
+        out = np.ascontiguousarray(dirw_out, dtype=np.float32).copy()
+        self.L.ref_occlusion_sweep_3d_from(_I(w), _I(h), _I(d), _ptr(src), _ptr(out))
         return out
 
     def felzenszwalb_2d(self, w, h, dist):

@@ -461,3 +461,68 @@ def test_fast_sweep_single_seed_3d(oracle, dims, seed):
     np.testing.assert_array_equal(out[..., 0], seed[0] - xs)
     np.testing.assert_array_equal(out[..., 1], seed[1] - ys)
     np.testing.assert_array_equal(out[..., 2], seed[2] - zs)
+
+
+# ------------------------------------------------------------- occlusion sweeps
+
+
+def test_occlusion_golden(oracle):
+    """The oracle's direction fields and sweeps against the reference's
+    (tests/golden/make_occlusion_golden.py), bit for bit."""
+    g = np.load(os.path.join(GOLDEN, "occlusion_golden.npz"))
+    n = 0
+    for k in range(4):
+        w, h = (int(v) for v in g[f"o2_{k}_dims"])
+        dw = oracle.init_dirw_2d(w, h, g[f"o2_{k}_mask"])
+        np.testing.assert_array_equal(dw, g[f"o2_{k}_dirw"])
+        np.testing.assert_array_equal(oracle.occlusion_sweep_2d(w, h, dw).view(np.uint32), g[f"o2_{k}_swept"].view(np.uint32))
+        n += 1
+    for k in range(3):
+        w, h, d = (int(v) for v in g[f"o3_{k}_dims"])
+        dw = oracle.init_dirw_3d(w, h, d, g[f"o3_{k}_mask"])
+        np.testing.assert_array_equal(dw, g[f"o3_{k}_dirw"])
+        np.testing.assert_array_equal(oracle.occlusion_sweep_3d(w, h, d, dw).view(np.uint32), g[f"o3_{k}_swept"].view(np.uint32))
+        np.testing.assert_array_equal(oracle.occlusion_sweep_3d_from(w, h, d, dw, g[f"o3_{k}_out0"]).view(np.uint32),
+                                      g[f"o3_{k}_from"].view(np.uint32))
+        n += 1
+    assert n == 7
+
+
+def test_occlusion_live_reference(oracle, reference, rng):
+    if reference is None:
+        pytest.skip("oracle/_ref/libdolref.so not built (reference sources absent)")
+    for trial in range(6):
+        w, h = (int(v) for v in rng.integers(1, 80, 2))
+        m = pack_mask(rng.random((h, w)) < [0.0, 0.01, 0.2][trial % 3])
+        dw = oracle.init_dirw_2d(w, h, m)
+        np.testing.assert_array_equal(dw, reference.init_dirw_2d(w, h, m))
+        np.testing.assert_array_equal(oracle.occlusion_sweep_2d(w, h, dw).view(np.uint32),
+                                      reference.occlusion_sweep_2d(w, h, dw).view(np.uint32))
+        w, h, d = (int(v) for v in rng.integers(1, 20, 3))
+        m = pack_mask(rng.random((d, h, w)) < [0.0, 0.01, 0.2][trial % 3])
+        dw = oracle.init_dirw_3d(w, h, d, m)
+        o0 = rng.random(dw.shape).astype(np.float32)
+        np.testing.assert_array_equal(oracle.occlusion_sweep_3d(w, h, d, dw).view(np.uint32),
+                                      reference.occlusion_sweep_3d(w, h, d, dw).view(np.uint32))
+        np.testing.assert_array_equal(oracle.occlusion_sweep_3d_from(w, h, d, dw, o0).view(np.uint32),
+                                      reference.occlusion_sweep_3d_from(w, h, d, dw, o0).view(np.uint32))
+
+
+def test_occlusion_closed_forms(oracle):
+    """A set pixel at the start of a single row: along the row, each cell of
+    the quadrant swept away from it holds 0.75 times the one before
+    (QuadrantCombine(a, 0) = 0.75 a, and v = 0 + 1 * that); the planes swept
+    towards it stay 0 up to it.  An empty field stays 0, a full one 1."""
+    w = 12
+    bits = np.zeros((1, w), bool)
+    bits[0, 0] = True
+    out = oracle.occlusion_sweep_2d(w, 1, oracle.init_dirw_2d(w, 1, pack_mask(bits)))
+    exp = np.empty(w, np.float32)
+    exp[0] = 1.0
+    for x in range(1, w):
+        exp[x] = np.float32(0.75) * exp[x - 1]
+    np.testing.assert_array_equal(out[0, 0], exp)      # plane 0 sweeps +x
+    np.testing.assert_array_equal(out[1, 0, 1:], 0.0)  # plane 1 sweeps -x: nothing reaches x > 0
+    for fill in (0.0, 1.0):
+        dw = np.full((8, 3, 4, 5), fill, np.float32)
+        np.testing.assert_array_equal(oracle.occlusion_sweep_3d(5, 4, 3, dw), dw)
