@@ -413,6 +413,79 @@ def bench_sweep(L, torch, dev, mask3_t=None, cpu=True):
     return out
 
 
+def bench_occlusion(L, torch, dev, mask3_t=None, cpu=True):
+    """Occlusion direction fields and sweeps (the tool's ambient occlusion
+    input): the 8192^2 disc/box scene, 4 planes; the 512^3 scene, 8 planes in
+    place.  CPU figures: the oracle (one thread) on a 2048^2 corner and a
+    128^3 corner, same work per cell."""
+    out = {}
+    w = h = 8192
+    mask = L.scene_shapes_2d(w, h, 2000, seed=2)
+    m_t = torch.from_numpy(mask.view(np.int32)).to(dev)
+    dw = torch.empty(4 * w * h, dtype=torch.float32, device=dev)
+
+    def step2():
+        L.call_dev("init_dirw_2d", w, h, m_t, dw)
+        L.call_dev("occlusion_sweep_2d", w, h, dw)
+
+    step2()
+    L.profile_enable(True)
+    ms = timed(step2, 3, torch)
+    sweep_ms, _ = L.profile_query("occlusion_sweep")
+    L.profile_enable(False)
+    out["occlusion_2d_8192"] = {"ms": round(ms, 3), "sweep_ms": round(sweep_ms, 3),
+                                "gpixels_per_s": round(w * h / (sweep_ms * 1e-3) / 1e9, 3),
+                                "launches": w // 8 + -(-h // 15) - 1}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_bind import Oracle
+
+        O = Oracle()
+        k = 2048
+        corner = np.ascontiguousarray(mask.reshape(h, w // 32)[:k, : k // 32])
+        start = O.init_dirw_2d(k, k, corner)
+        t0 = time.perf_counter()
+        O.occlusion_sweep_2d(k, k, start)
+        cpu_s = time.perf_counter() - t0
+        out["occlusion_2d_8192"]["cpu_baseline"] = {
+            "ms_2048sq": round(cpu_s * 1e3, 1), "gpixels_per_s": round(k * k / cpu_s / 1e9, 4), "cores": 1,
+            "kind": "port", "sample": "the 2048^2 corner of the scene, 4 planes"}
+    del dw
+    torch.cuda.empty_cache()
+
+    n = 512
+    if mask3_t is None:
+        mask3_t = torch.from_numpy(L.scene_shapes_3d(n, n, n, 300, 100, seed=3).view(np.int32)).to(dev)
+    dw = torch.empty(8 * n ** 3, dtype=torch.float32, device=dev)
+
+    def step3():
+        L.call_dev("init_dirw_3d", n, n, n, mask3_t, dw)
+        L.call_dev("occlusion_sweep_3d", n, n, n, dw)
+
+    step3()
+    L.profile_enable(True)
+    ms = timed(step3, 2, torch)
+    sweep_ms, _ = L.profile_query("occlusion_sweep")
+    L.profile_enable(False)
+    out["occlusion_3d_512"] = {"ms": round(ms, 3), "sweep_ms": round(sweep_ms, 3),
+                               "gvoxels_per_s": round(n ** 3 / (sweep_ms * 1e-3) / 1e9, 3),
+                               "launches": n // 8 + -(-n // 15) + n - 2}
+    if cpu:
+        k = 128
+        m3 = mask3_t.cpu().numpy().view(np.uint32).reshape(n, n, n // 32)
+        corner = np.ascontiguousarray(m3[:k, :k, : k // 32])
+        start = O.init_dirw_3d(k, k, k, corner)
+        t0 = time.perf_counter()
+        O.occlusion_sweep_3d(k, k, k, start)
+        cpu_s = time.perf_counter() - t0
+        out["occlusion_3d_512"]["cpu_baseline"] = {
+            "ms_128cube": round(cpu_s * 1e3, 1), "gvoxels_per_s": round(k ** 3 / cpu_s / 1e9, 4), "cores": 1,
+            "kind": "port", "sample": "the 128^3 corner of the scene, 8 planes"}
+    del dw
+    torch.cuda.empty_cache()
+    return out
+
+
 def mask_cpu_view(L, torch, dev, pos, idx, bbox, n):
     """the GPU's mask of one mesh, on the host"""
     return L.mask_from_triangles(n, n, n, pos, idx, bbox).view(np.int32)
@@ -428,7 +501,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only copy 512 MiB device to device 5 times (calibrates HBM byte counters)")
-    ap.add_argument("--only-extra", choices=["jfa3d", "glyphs", "jfa2d", "config1", "seeds", "twofield", "voxel", "sweep"],
+    ap.add_argument("--only-extra", choices=["jfa3d", "glyphs", "jfa2d", "config1", "seeds", "twofield", "voxel", "sweep", "occlusion"],
                     help="run only this secondary workload and print its figures (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -463,7 +536,8 @@ def main():
                "config1": lambda: bench_config1(L, torch, dev), "seeds": lambda: bench_seeds(L, torch, dev),
                "twofield": lambda: bench_two_field(L, torch, dev),
                "voxel": lambda: bench_voxel(L, torch, dev, cpu=not args.no_cpu),
-               "sweep": lambda: bench_sweep(L, torch, dev, cpu=not args.no_cpu)}
+               "sweep": lambda: bench_sweep(L, torch, dev, cpu=not args.no_cpu),
+               "occlusion": lambda: bench_occlusion(L, torch, dev, cpu=not args.no_cpu)}
         print(json.dumps({args.only_extra: run[args.only_extra]()}), flush=True)
         return
 
@@ -575,6 +649,7 @@ def main():
         result["two_field_8192"] = bench_two_field(L, torch, dev)
         result["voxel_512"] = bench_voxel(L, torch, dev, cpu=not args.no_cpu)
         result["fast_sweep"] = bench_sweep(L, torch, dev, mask_t, cpu=not args.no_cpu)
+        result["occlusion"] = bench_occlusion(L, torch, dev, mask_t, cpu=not args.no_cpu)
         result["config1_512"] = bench_config1(L, torch, dev)
         result["seeds_2048"] = bench_seeds(L, torch, dev)
 

@@ -29,6 +29,7 @@ SOURCES = [
     "dol_seeds.hip",
     "dol_voxel.hip",
     "dol_sweep.hip",
+    "dol_occlusion.hip",
     "dol_capi.cpp",
     "dol_scenes.cpp",
     "dol_shim.cpp",

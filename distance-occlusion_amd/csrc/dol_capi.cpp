@@ -29,6 +29,8 @@ void launch_init_dist(int w, int h, const uint32_t* mask, T* dist, T max_d, hipS
 void launch_border_2d(int w, int h, const uint32_t* mask, int16_t* delta, hipStream_t s);
 void launch_border_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta, hipStream_t s);
 void launch_fast_sweep_2d(int w, int h, int16_t* delta, int cw, hipStream_t s);
+void launch_init_dirw(int w, int h, int d, int planes, const uint32_t* mask, float* dirw, hipStream_t s);
+void launch_occlusion_sweep(int mode, int w, int h, int d, const float* in, float* dirw, hipStream_t s);
 void launch_fast_sweep_3d(int w, int h, int d, int16_t* delta, int cw, hipStream_t s);
 void launch_jfa2d_pass(int step, int w, int h, const int16_t* in, int16_t* out, int cw,
                        hipStream_t s);
@@ -633,6 +635,126 @@ int dolhip_fast_sweep_3d(int w, int h, int d, int16_t* delta, int cw)
     launch_fast_sweep_3d(w, h, d, a.as<int16_t>(), cw, 0);
     TRY(finish_sync("fast_sweep_3d"));
     return a.download(delta, bytes);
+}
+
+int dolhip_init_dirw_2d_dev(int w, int h, const uint32_t* mask, float* dirw4, void* stream)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(dirw4, "dirw4"));
+    TRY(require_device());
+    launch_init_dirw(w, h, 1, 4, mask, dirw4, as_stream(stream));
+    return finish(as_stream(stream), "init_dirw_2d");
+}
+
+int dolhip_init_dirw_2d(int w, int h, const uint32_t* mask, float* dirw4)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(dirw4, "dirw4"));
+    TRY(require_device());
+    const size_t bytes = (size_t)w * h * 4 * sizeof(float);
+    DevBuf m, o;
+    TRY(m.upload(mask, mask_bytes(w, h)));
+    TRY(o.alloc(bytes));
+    launch_init_dirw(w, h, 1, 4, m.as<uint32_t>(), o.as<float>(), 0);
+    TRY(finish_sync("init_dirw_2d"));
+    return o.download(dirw4, bytes);
+}
+
+int dolhip_init_dirw_3d_dev(int w, int h, int d, const uint32_t* mask, float* dirw8, void* stream)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(dirw8, "dirw8"));
+    TRY(require_device());
+    launch_init_dirw(w, h, d, 8, mask, dirw8, as_stream(stream));
+    return finish(as_stream(stream), "init_dirw_3d");
+}
+
+int dolhip_init_dirw_3d(int w, int h, int d, const uint32_t* mask, float* dirw8)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(dirw8, "dirw8"));
+    TRY(require_device());
+    const size_t bytes = (size_t)w * h * d * 8 * sizeof(float);
+    DevBuf m, o;
+    TRY(m.upload(mask, mask_bytes(w, (long long)h * d)));
+    TRY(o.alloc(bytes));
+    launch_init_dirw(w, h, d, 8, m.as<uint32_t>(), o.as<float>(), 0);
+    TRY(finish_sync("init_dirw_3d"));
+    return o.download(dirw8, bytes);
+}
+
+int dolhip_occlusion_sweep_2d_dev(int w, int h, float* dirw4, void* stream)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(dirw4, "dirw4"));
+    TRY(require_device());
+    launch_occlusion_sweep(0, w, h, 1, nullptr, dirw4, as_stream(stream));
+    return finish(as_stream(stream), "occlusion_sweep_2d");
+}
+
+int dolhip_occlusion_sweep_2d(int w, int h, float* dirw4)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(dirw4, "dirw4"));
+    TRY(require_device());
+    const size_t bytes = (size_t)w * h * 4 * sizeof(float);
+    DevBuf a;
+    TRY(a.upload(dirw4, bytes));
+    launch_occlusion_sweep(0, w, h, 1, nullptr, a.as<float>(), 0);
+    TRY(finish_sync("occlusion_sweep_2d"));
+    return a.download(dirw4, bytes);
+}
+
+int dolhip_occlusion_sweep_3d_dev(int w, int h, int d, float* dirw8, void* stream)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(dirw8, "dirw8"));
+    TRY(require_device());
+    launch_occlusion_sweep(1, w, h, d, nullptr, dirw8, as_stream(stream));
+    return finish(as_stream(stream), "occlusion_sweep_3d");
+}
+
+int dolhip_occlusion_sweep_3d(int w, int h, int d, float* dirw8)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(dirw8, "dirw8"));
+    TRY(require_device());
+    const size_t bytes = (size_t)w * h * d * 8 * sizeof(float);
+    DevBuf a;
+    TRY(a.upload(dirw8, bytes));
+    launch_occlusion_sweep(1, w, h, d, nullptr, a.as<float>(), 0);
+    TRY(finish_sync("occlusion_sweep_3d"));
+    return a.download(dirw8, bytes);
+}
+
+int dolhip_occlusion_sweep_3d_from_dev(int w, int h, int d, const float* in8, float* out8, void* stream)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(in8, "in8"));
+    TRY(check_ptr(out8, "out8"));
+    if (in8 == out8) return fail(DOLHIP_ERR_INVALID_ARG, "in8 and out8 must be different fields");
+    TRY(require_device());
+    launch_occlusion_sweep(2, w, h, d, in8, out8, as_stream(stream));
+    return finish(as_stream(stream), "occlusion_sweep_3d_from");
+}
+
+int dolhip_occlusion_sweep_3d_from(int w, int h, int d, const float* in8, float* out8)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(in8, "in8"));
+    TRY(check_ptr(out8, "out8"));
+    TRY(require_device());
+    const size_t bytes = (size_t)w * h * d * 8 * sizeof(float);
+    DevBuf a, b;
+    TRY(a.upload(in8, bytes));
+    TRY(b.upload(out8, bytes));
+    launch_occlusion_sweep(2, w, h, d, a.as<float>(), b.as<float>(), 0);
+    TRY(finish_sync("occlusion_sweep_3d_from"));
+    return b.download(out8, bytes);
 }
 
 size_t dolhip_felzenszwalb_2d_workspace(int w, int h)

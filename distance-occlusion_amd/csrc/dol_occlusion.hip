@@ -1,0 +1,190 @@
+// dol_occlusion.hip -- direction-field initialisation and occlusion sweeps
+// (DistanceOcclusionLib.cpp:1114-1209 2D, 2507-2769 3D) for gfx950.
+//
+// Each of the 4 (2D) or 8 (3D) planes of the direction field is swept from
+// its own corner, and the planes do not interact, so every launch works on
+// all of them.  In a plane's sweep frame (u, v, t from its corner) cell
+// (u, v, t) combines (u - 1, v, t), (u, v - 1, t) and (u, v, t - 1), each
+// already updated, with 0 standing in for a neighbour outside the field;
+// only the corner cell is left as it is.  Cells with equal u + v + t are
+// independent, so the sweep is a wavefront over tiles, as FastSweep's in
+// dol_sweep.hip: one 16-lane DPP row of a wavefront holds an 8-column x
+// 15-row tile of one slice, lane l on row v0 - 1 + l (lane 0 carrying the
+// row before the tile), lanes skewed one column each; a launch is one
+// diagonal U + V + t of tiles.  A lane keeps its row (and the same row of the
+// slice before) in registers, rotated by its place in the group so that
+// every step reads the same register.
+//
+// The float expressions are the reference's, term for term and in its
+// order; the library is built without FMA contraction, so each result is
+// the reference's to the bit.
+#include "dol_common.hpp"
+
+namespace dolhip {
+
+// ------------------------------------------------------------ init
+
+// one thread per cell: every plane gets 1 where the bit is set, else 0
+// (InitDirWFromBitMask, DistanceOcclusionLib.cpp:1114-1146, 2507-2538)
+__global__ void __launch_bounds__(256) k_init_dirw(int w, size_t cells, int planes,
+                                                   const uint32_t* __restrict__ mask,
+                                                   float* __restrict__ dirw)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= cells) return;
+    const size_t row = i / (size_t)w;
+    const int x = (int)(i - row * (size_t)w);
+    const float v = ((mask[row * mask_stride(w) + (x >> 5)] >> (x & 31)) & 1u) ? 1.0f : 0.0f;
+    for (int p = 0; p < planes; p++) dirw[(size_t)p * cells + i] = v;
+}
+
+// planes: 4 (2D) or 8 (3D)
+void launch_init_dirw(int w, int h, int d, int planes, const uint32_t* mask, float* dirw, hipStream_t s)
+{
+    ProfileScope p("init_dirw", s);
+    const size_t cells = (size_t)w * h * d;
+    hipLaunchKernelGGL(k_init_dirw, dim3(grid1d(cells, 256)), dim3(256), 0, s, w, cells, planes, mask,
+                       dirw);
+}
+
+// ------------------------------------------------------------ sweeps
+
+namespace {
+
+constexpr int kOW = 8;                 // tile columns
+constexpr int kOGroup = 16;            // lanes per tile (one DPP row)
+constexpr int kORows = kOGroup - 1;    // tile rows
+constexpr int kOTiles = 64 / kOGroup;  // tiles per wavefront, stacked in v
+
+// QuadrantCombine (DistanceOcclusionLib.cpp:1150-1168)
+__device__ __forceinline__ float quadrant_combine(float a, float b)
+{
+    return 0.75f * ((a + b) - 1.0f * a * b);
+}
+
+// OctantCombine, three and six arguments (DistanceOcclusionLib.cpp:2543-2567)
+__device__ __forceinline__ float octant_combine(float a, float b, float c)
+{
+    return 0.33333333f * (a + b + c - a * b * c);
+}
+
+__device__ __forceinline__ float octant_combine(float oa, float ob, float oc, float ia, float ib,
+                                                float ic)
+{
+    const float a = ia + (1.0f - ia) * oa;
+    const float b = ib + (1.0f - ib) * ob;
+    const float c = ic + (1.0f - ic) * oc;
+    return 0.3333333f * (a + b + c - a * b * c);
+}
+
+// the value of the lane before in the lane's DPP row of 16 (row_shr:1)
+__device__ __forceinline__ float from_prev_lane16(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x111, 0xf,
+                                                                 0xf, false));
+}
+
+}  // namespace
+
+// MODE 0: 2D, in place (OcclusionSweep, :1171-1209); MODE 1: 3D, in place
+// (:2569-2648); MODE 2: 3D, out from in (:2650-2743; `in` read only).
+// Lane group g of block b: 2D, plane b / vb and V = kOTiles * (b % vb) + g,
+// U = diag - V; 3D, plane b / (vb tu), r = b % (vb tu), U = r % tu,
+// V = kOTiles * (r / tu) + g, t = diag - U - V.
+template <int MODE>
+__global__ void __launch_bounds__(64) k_occlusion_sweep(int w, int h, int d, int diag, int tu, int vb,
+                                                        const float* __restrict__ in,
+                                                        float* __restrict__ dirw)
+{
+    const int lane = threadIdx.x, j = lane % kOGroup;
+    const int nb = MODE == 0 ? vb : vb * tu;
+    const int plane = (int)blockIdx.x / nb, r = (int)blockIdx.x % nb;
+    const int V = kOTiles * (MODE == 0 ? r : r / tu) + lane / kOGroup;
+    const int Ud = MODE == 0 ? diag - V : r % tu;
+    const int t = MODE == 0 ? 0 : diag - Ud - V;
+    const size_t cells = (size_t)w * h * d, slice = (size_t)w * h;
+    const int dx = 1 - 2 * (plane & 1), dy = 1 - 2 * ((plane >> 1) & 1), dz = 1 - 2 * ((plane >> 2) & 1);
+    const int x0 = dx > 0 ? 0 : w - 1, y0 = dy > 0 ? 0 : h - 1, z0 = dz > 0 ? 0 : d - 1;
+    const bool tile_ok = Ud >= 0 && Ud < tu && t >= 0 && t < d && V * kORows < h;
+    const int U = tile_ok ? Ud : 0;   // a group without a tile reads tile 0 (and writes nothing)
+    const int z = tile_ok ? z0 + t * dz : 0, zb = tile_ok && t >= 1 ? z - dz : z;
+    float* out = dirw + (size_t)plane * cells;
+    const float* inp = MODE == 2 ? in + (size_t)plane * cells : out;
+    const int u0 = U * kOW, v0 = V * kORows;
+    const int nu = min(kOW, w - u0);
+    const int vr = v0 - 1 + j;                           // the lane's row
+    const bool row_ok = tile_ok && vr >= 0 && vr < h;
+    const bool update = row_ok && j >= 1;
+    const size_t rb = (size_t)(y0 + min(max(vr, 0), h - 1) * dy) * w;
+    auto at = [&](int zz, int c) { return (size_t)zz * slice + rb + (size_t)(x0 + (u0 + min(c, nu - 1)) * dx); };
+    // register k holds column (k - j) mod kOW, so step s reads register s mod kOW
+    float R[kOW], B[kOW], RI[kOW], BI[kOW];
+#pragma unroll
+    for (int k = 0; k < kOW; k++) {
+        const int c = (k - j) & (kOW - 1);
+        R[k] = out[at(z, c)];
+        B[k] = out[at(zb, c)];
+        if (MODE == 2) {
+            RI[k] = inp[at(z, c)];
+            BI[k] = inp[at(zb, c)];
+        }
+    }
+    // the lane's last results (output and, MODE 2, input); first the column
+    // before the tile
+    float mine = 0.0f, mine_in = 0.0f;
+    if (u0 > 0 && row_ok) {
+        const size_t o = (size_t)z * slice + rb + (size_t)(x0 + (u0 - 1) * dx);
+        mine = out[o];
+        if (MODE == 2) mine_in = inp[o];
+    }
+#pragma unroll
+    for (int s = 0; s < kOW + kOGroup - 1; s++) {
+        const int a = s - j;
+        const int k = s & (kOW - 1);
+        const float up = from_prev_lane16(mine);
+        const float up_in = MODE == 2 ? from_prev_lane16(mine_in) : 0.0f;
+        const bool hl = u0 + a >= 1, hu = vr >= 1, hb = t >= 1;   // neighbours inside
+        float v;
+        if (MODE == 0) {
+            // first row: the cell before; first column: the row before;
+            // elsewhere both (:1191-1205)
+            const float q = hl ? quadrant_combine(mine, hu ? up : 0.0f) : quadrant_combine(up, 0.0f);
+            v = R[k] + (1.0f - R[k]) * q;
+        } else if (MODE == 1) {
+            v = R[k] + (1.0f - R[k]) * octant_combine(hl ? mine : 0.0f, hu ? up : 0.0f, hb ? B[k] : 0.0f);
+        } else {
+            v = octant_combine(hl ? mine : 0.0f, hu ? up : 0.0f, hb ? B[k] : 0.0f, hl ? mine_in : 0.0f,
+                               hu ? up_in : 0.0f, hb ? BI[k] : 0.0f);
+        }
+        if (row_ok && a >= 0 && a < nu) {
+            if (update && (hl || hu || hb)) R[k] = v;   // the plane's corner cell stays
+            mine = R[k];
+            if (MODE == 2) mine_in = RI[k];
+        }
+    }
+    if (!update) return;
+#pragma unroll
+    for (int k = 0; k < kOW; k++) {
+        const int c = (k - j) & (kOW - 1);
+        if (c < nu) out[at(z, c)] = R[k];
+    }
+}
+
+// mode 0: 2D (d == 1, 4 planes); 1: 3D in place; 2: 3D from `in` (8 planes)
+void launch_occlusion_sweep(int mode, int w, int h, int d, const float* in, float* dirw, hipStream_t s)
+{
+    ProfileScope p("occlusion_sweep", s);
+    const int tu = (w + kOW - 1) / kOW, tv = (h + kORows - 1) / kORows;
+    const int vb = (tv + kOTiles - 1) / kOTiles;
+    const dim3 grid((unsigned)(mode == 0 ? 4 * vb : 8 * vb * tu));
+    for (int diag = 0; diag < tu + tv + d - 2; diag++) {
+        if (mode == 0)
+            hipLaunchKernelGGL(k_occlusion_sweep<0>, grid, dim3(64), 0, s, w, h, 1, diag, tu, vb, nullptr, dirw);
+        else if (mode == 1)
+            hipLaunchKernelGGL(k_occlusion_sweep<1>, grid, dim3(64), 0, s, w, h, d, diag, tu, vb, nullptr, dirw);
+        else
+            hipLaunchKernelGGL(k_occlusion_sweep<2>, grid, dim3(64), 0, s, w, h, d, diag, tu, vb, in, dirw);
+    }
+}
+
+}  // namespace dolhip

@@ -92,6 +92,16 @@ void DOL::FastSweep(int w, int h, cCellDelta2 delta[], int cw)
 { require(dolhip_fast_sweep_2d(w, h, raw(delta), cw), "FastSweep"); }
 void DOL::FastSweep(int w, int h, int d, cCellDelta3 delta[], int cw)
 { require(dolhip_fast_sweep_3d(w, h, d, raw(delta), cw), "FastSweep"); }
+void DOL::InitDirWFromBitMask(int w, int h, const uint32_t mask[], float dirW4[])
+{ require(dolhip_init_dirw_2d(w, h, mask, dirW4), "InitDirWFromBitMask"); }
+void DOL::InitDirWFromBitMask(int w, int h, int d, const uint32_t mask[], float dirW8[])
+{ require(dolhip_init_dirw_3d(w, h, d, mask, dirW8), "InitDirWFromBitMask"); }
+void DOL::OcclusionSweep(int w, int h, float dirW4[])
+{ require(dolhip_occlusion_sweep_2d(w, h, dirW4), "OcclusionSweep"); }
+void DOL::OcclusionSweep(int w, int h, int d, float dirW8[])
+{ require(dolhip_occlusion_sweep_3d(w, h, d, dirW8), "OcclusionSweep"); }
+void DOL::OcclusionSweep(int w, int h, int d, const float inDirW8[], float outDirW8[])
+{ require(dolhip_occlusion_sweep_3d_from(w, h, d, inDirW8, outDirW8), "OcclusionSweep"); }
 
 void DOL::EDT(int w, int h, const uint32_t mask[], int32_t d2[], float dist[], bool invert)
 { require(dolhip_edt_2d(w, h, mask, invert, d2, dist), "EDT"); }

@@ -71,6 +71,16 @@ _SIGS = {
     "dolhip_fast_sweep_2d_dev": ([_I, _I, _P, _I, _P], _I),
     "dolhip_fast_sweep_3d": ([_I, _I, _I, _P, _I], _I),
     "dolhip_fast_sweep_3d_dev": ([_I, _I, _I, _P, _I, _P], _I),
+    "dolhip_init_dirw_2d": ([_I, _I, _P, _P], _I),
+    "dolhip_init_dirw_2d_dev": ([_I, _I, _P, _P, _P], _I),
+    "dolhip_init_dirw_3d": ([_I, _I, _I, _P, _P], _I),
+    "dolhip_init_dirw_3d_dev": ([_I, _I, _I, _P, _P, _P], _I),
+    "dolhip_occlusion_sweep_2d": ([_I, _I, _P], _I),
+    "dolhip_occlusion_sweep_2d_dev": ([_I, _I, _P, _P], _I),
+    "dolhip_occlusion_sweep_3d": ([_I, _I, _I, _P], _I),
+    "dolhip_occlusion_sweep_3d_dev": ([_I, _I, _I, _P, _P], _I),
+    "dolhip_occlusion_sweep_3d_from": ([_I, _I, _I, _P, _P], _I),
+    "dolhip_occlusion_sweep_3d_from_dev": ([_I, _I, _I, _P, _P, _P], _I),
     "dolhip_felzenszwalb_2d": ([_I, _I, _P], _I),
     "dolhip_felzenszwalb_2d_workspace": ([_I, _I], _SZ),
     "dolhip_felzenszwalb_2d_dev": ([_I, _I, _P, _P, _P], _I),
@@ -273,6 +283,32 @@ class DolHip:
     def fast_sweep_3d(self, w, h, d, delta, cw=1):
         out = np.ascontiguousarray(delta, dtype=np.int16).copy()
         self._call("dolhip_fast_sweep_3d", w, h, d, _addr(out), cw)
+        return out
+
+    def init_dirw_2d(self, w, h, mask):
+        out = np.empty((4, h, w), dtype=np.float32)
+        self._call("dolhip_init_dirw_2d", w, h, _addr(np.ascontiguousarray(mask, np.uint32)), _addr(out))
+        return out
+
+    def init_dirw_3d(self, w, h, d, mask):
+        out = np.empty((8, d, h, w), dtype=np.float32)
+        self._call("dolhip_init_dirw_3d", w, h, d, _addr(np.ascontiguousarray(mask, np.uint32)), _addr(out))
+        return out
+
+    def occlusion_sweep_2d(self, w, h, dirw):
+        out = np.ascontiguousarray(dirw, dtype=np.float32).copy()
+        self._call("dolhip_occlusion_sweep_2d", w, h, _addr(out))
+        return out
+
+    def occlusion_sweep_3d(self, w, h, d, dirw):
+        out = np.ascontiguousarray(dirw, dtype=np.float32).copy()
+        self._call("dolhip_occlusion_sweep_3d", w, h, d, _addr(out))
+        return out
+
+    def occlusion_sweep_3d_from(self, w, h, d, dirw_in, dirw_out):
+        src = np.ascontiguousarray(dirw_in, dtype=np.float32)
+        out = np.ascontiguousarray(dirw_out, dtype=np.float32).copy()
+        self._call("dolhip_occlusion_sweep_3d_from", w, h, d, _addr(src), _addr(out))
         return out
 
     def felzenszwalb_2d(self, w, h, dist):

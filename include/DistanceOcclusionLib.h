@@ -6,9 +6,10 @@
  * functions of DistanceOcclusionLib.h that lie on the data-parallel distance
  * path (mask creation and generators :21-35, cell deltas :41-57, distance
  * initialisation and Felzenszwalb :61-71, delta initialisation and border
- * initialisation :75-82, FastSweep :84-85, JumpFlood :92-94, MaskSize/EncodeU8
- * :124-141).  Danielsson, Chamfer, BruteForce and the occlusion functions are
- * not part of this path and are not declared here.
+ * initialisation :75-82, FastSweep :84-85, JumpFlood :92-94, direction fields
+ * and occlusion sweeps :104-112, MaskSize/EncodeU8 :124-141).  Danielsson,
+ * Chamfer, BruteForce, the shadow sweep and the AO encoders are not part of
+ * this path and are not declared here.
  *
  * Semantics are the reference's: buffers are host memory owned by the
  * caller and updated in place; every call is synchronous; results are
@@ -77,6 +78,13 @@ namespace DOL
 
     void FastSweep(int w, int h,        cCellDelta2 delta[], int cw = 1);
     void FastSweep(int w, int h, int d, cCellDelta3 delta[], int cw = 1);
+
+    // --- Occlusion (DistanceOcclusionLib.h:104-112) ----------------------------
+    void InitDirWFromBitMask(int w, int h,        const uint32_t mask[], float dirW4[]);
+    void InitDirWFromBitMask(int w, int h, int d, const uint32_t mask[], float dirW8[]);
+    void OcclusionSweep(int w, int h,        float dirW4[]);
+    void OcclusionSweep(int w, int h, int d, float dirW8[]);
+    void OcclusionSweep(int w, int h, int d, const float inDirW8[], float outDirW8[]);
 
     // --- Extensions (no counterpart in the reference) ------------------------
     // Exact squared Euclidean distance to the nearest set bit (clear bit if

@@ -135,6 +135,23 @@ int dolhip_fast_sweep_2d_dev(int w, int h, int16_t* delta, int cw, void* stream)
 int dolhip_fast_sweep_3d(int w, int h, int d, int16_t* delta, int cw);
 int dolhip_fast_sweep_3d_dev(int w, int h, int d, int16_t* delta, int cw, void* stream);
 
+/* Direction fields and occlusion sweeps (DistanceOcclusionLib.h:104-112;
+ * .cpp:1114-1209, 2507-2769), float results bit-identical.  dirw4 is 4 planes
+ * of w*h floats, dirw8 8 planes of w*h*d; the sweeps update in place, or
+ * (_from) write out8 from in8, out8's first cell of each plane left as it
+ * was.  No workspace. */
+int dolhip_init_dirw_2d(int w, int h, const uint32_t* mask, float* dirw4);
+int dolhip_init_dirw_2d_dev(int w, int h, const uint32_t* mask, float* dirw4, void* stream);
+int dolhip_init_dirw_3d(int w, int h, int d, const uint32_t* mask, float* dirw8);
+int dolhip_init_dirw_3d_dev(int w, int h, int d, const uint32_t* mask, float* dirw8, void* stream);
+int dolhip_occlusion_sweep_2d(int w, int h, float* dirw4);
+int dolhip_occlusion_sweep_2d_dev(int w, int h, float* dirw4, void* stream);
+int dolhip_occlusion_sweep_3d(int w, int h, int d, float* dirw8);
+int dolhip_occlusion_sweep_3d_dev(int w, int h, int d, float* dirw8, void* stream);
+int dolhip_occlusion_sweep_3d_from(int w, int h, int d, const float* in8, float* out8);
+int dolhip_occlusion_sweep_3d_from_dev(int w, int h, int d, const float* in8, float* out8,
+                                       void* stream);
+
 /* Felzenszwalb float squared distances in place, bit-identical to the
  * reference's float arithmetic (DistanceOcclusionLib.h:71, .cpp:1018-1085) */
 int dolhip_felzenszwalb_2d(int w, int h, float* dist);

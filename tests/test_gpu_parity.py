@@ -519,3 +519,67 @@ def test_fast_sweep_large_single_seed(gpu):
     np.testing.assert_array_equal(out[..., 0], 77 - xs)
     np.testing.assert_array_equal(out[..., 1], h - 1 - ys)
     np.testing.assert_array_equal(out[..., 2], d - 1 - zs)
+
+
+# ------------------------------------------------------------- occlusion sweeps
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+@pytest.mark.parametrize("dims", [(1, 1), (5, 3), (1, 40), (40, 1), (8, 15), (9, 16), (64, 61), (130, 67), (257, 33)])
+def test_occlusion_2d(gpu, oracle, rng, dims):
+    w, h = dims
+    for p in (0.0, 0.01, 0.2, 1.0):
+        m = rand_mask(rng, dims, p)
+        dw = gpu.init_dirw_2d(w, h, m)
+        np.testing.assert_array_equal(dw, oracle.init_dirw_2d(w, h, m))
+        np.testing.assert_array_equal(_bits(gpu.occlusion_sweep_2d(w, h, dw)), _bits(oracle.occlusion_sweep_2d(w, h, dw)))
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (5, 4, 3), (17, 9, 6), (8, 15, 1), (33, 31, 5), (64, 20, 9), (3, 2, 40)])
+def test_occlusion_3d(gpu, oracle, rng, dims):
+    w, h, d = dims
+    for p in (0.0, 0.01, 0.2):
+        m = rand_mask(rng, dims, p)
+        dw = gpu.init_dirw_3d(w, h, d, m)
+        np.testing.assert_array_equal(dw, oracle.init_dirw_3d(w, h, d, m))
+        np.testing.assert_array_equal(_bits(gpu.occlusion_sweep_3d(w, h, d, dw)),
+                                      _bits(oracle.occlusion_sweep_3d(w, h, d, dw)))
+        o0 = rng.random(dw.shape).astype(np.float32)
+        np.testing.assert_array_equal(_bits(gpu.occlusion_sweep_3d_from(w, h, d, dw, o0)),
+                                      _bits(oracle.occlusion_sweep_3d_from(w, h, d, dw, o0)))
+
+
+def test_occlusion_golden(gpu):
+    """The reference's own outputs (tests/golden/occlusion_golden.npz)."""
+    g = np.load(os.path.join(ROOT, "tests", "golden", "occlusion_golden.npz"))
+    for k in range(4):
+        w, h = (int(v) for v in g[f"o2_{k}_dims"])
+        dw = gpu.init_dirw_2d(w, h, g[f"o2_{k}_mask"])
+        np.testing.assert_array_equal(dw, g[f"o2_{k}_dirw"])
+        np.testing.assert_array_equal(_bits(gpu.occlusion_sweep_2d(w, h, dw)), _bits(g[f"o2_{k}_swept"]))
+    for k in range(3):
+        w, h, d = (int(v) for v in g[f"o3_{k}_dims"])
+        dw = gpu.init_dirw_3d(w, h, d, g[f"o3_{k}_mask"])
+        np.testing.assert_array_equal(_bits(gpu.occlusion_sweep_3d(w, h, d, dw)), _bits(g[f"o3_{k}_swept"]))
+        np.testing.assert_array_equal(_bits(gpu.occlusion_sweep_3d_from(w, h, d, dw, g[f"o3_{k}_out0"])),
+                                      _bits(g[f"o3_{k}_from"]))
+
+
+def test_occlusion_large(gpu, oracle, rng):
+    """2048 x 1500 against the oracle, and a full-size 1D closed form: a set
+    pixel at the start of each row decays by 0.75 a cell along plane 0."""
+    w, h = 2048, 1500
+    m = rand_mask(rng, (w, h), 0.002)
+    dw = oracle.init_dirw_2d(w, h, m)
+    np.testing.assert_array_equal(_bits(gpu.occlusion_sweep_2d(w, h, dw)), _bits(oracle.occlusion_sweep_2d(w, h, dw)))
+    w = 3000
+    dw = np.zeros((4, 1, w), np.float32)
+    dw[:, 0, 0] = 1.0
+    exp = np.empty(w, np.float32)
+    exp[0] = 1.0
+    for x in range(1, w):
+        exp[x] = np.float32(0.75) * exp[x - 1]
+    np.testing.assert_array_equal(gpu.occlusion_sweep_2d(w, 1, dw)[0, 0], exp)
