@@ -31,6 +31,7 @@ void launch_border_3d(int w, int h, int d, const uint32_t* mask, int16_t* delta,
 void launch_fast_sweep_2d(int w, int h, int16_t* delta, int cw, hipStream_t s);
 void launch_init_dirw(int w, int h, int d, int planes, const uint32_t* mask, float* dirw, hipStream_t s);
 void launch_occlusion_sweep(int mode, int w, int h, int d, const float* in, float* dirw, hipStream_t s);
+void launch_encode_ao(size_t n, int planes, bool rgba, const float* dirw, uint8_t* out, hipStream_t s);
 void launch_fast_sweep_3d(int w, int h, int d, int16_t* delta, int cw, hipStream_t s);
 void launch_jfa2d_pass(int step, int w, int h, const int16_t* in, int16_t* out, int cw,
                        hipStream_t s);
@@ -755,6 +756,73 @@ int dolhip_occlusion_sweep_3d_from(int w, int h, int d, const float* in8, float*
     launch_occlusion_sweep(2, w, h, d, a.as<float>(), b.as<float>(), 0);
     TRY(finish_sync("occlusion_sweep_3d_from"));
     return b.download(out8, bytes);
+}
+
+// AO images: n cells of `planes` planes into 1 (ao) or 4 (rgba) bytes a cell
+static int encode_ao_dev(size_t n, int planes, bool rgba, const float* dirw, uint8_t* out, void* stream,
+                         const char* what)
+{
+    TRY(check_ptr(dirw, "dirw"));
+    TRY(check_ptr(out, "out"));
+    if (rgba && ((uintptr_t)out & 3)) return fail(DOLHIP_ERR_INVALID_ARG, "rgba must be 4-byte aligned");
+    TRY(require_device());
+    launch_encode_ao(n, planes, rgba, dirw, out, as_stream(stream));
+    return finish(as_stream(stream), what);
+}
+
+static int encode_ao_host(size_t n, int planes, bool rgba, const float* dirw, uint8_t* out, const char* what)
+{
+    TRY(check_ptr(dirw, "dirw"));
+    TRY(check_ptr(out, "out"));
+    TRY(require_device());
+    const size_t ob = n * (rgba ? 4 : 1);
+    DevBuf a, o;
+    TRY(a.upload(dirw, n * planes * sizeof(float)));
+    TRY(o.alloc(ob));
+    launch_encode_ao(n, planes, rgba, a.as<float>(), o.as<uint8_t>(), 0);
+    TRY(finish_sync(what));
+    return o.download(out, ob);
+}
+
+int dolhip_encode_ao_2d(int w, int h, const float* dirw4, uint8_t* ao)
+{
+    TRY(check_dims(w, h));
+    return encode_ao_host((size_t)w * h, 4, false, dirw4, ao, "encode_ao_2d");
+}
+int dolhip_encode_ao_2d_dev(int w, int h, const float* dirw4, uint8_t* ao, void* stream)
+{
+    TRY(check_dims(w, h));
+    return encode_ao_dev((size_t)w * h, 4, false, dirw4, ao, stream, "encode_ao_2d");
+}
+int dolhip_encode_ao_3d(int w, int h, int d, const float* dirw8, uint8_t* ao)
+{
+    TRY(check_dims(w, h, d));
+    return encode_ao_host((size_t)w * h * d, 8, false, dirw8, ao, "encode_ao_3d");
+}
+int dolhip_encode_ao_3d_dev(int w, int h, int d, const float* dirw8, uint8_t* ao, void* stream)
+{
+    TRY(check_dims(w, h, d));
+    return encode_ao_dev((size_t)w * h * d, 8, false, dirw8, ao, stream, "encode_ao_3d");
+}
+int dolhip_encode_dirw_2d(int w, int h, const float* dirw4, uint8_t* rgba)
+{
+    TRY(check_dims(w, h));
+    return encode_ao_host((size_t)w * h, 4, true, dirw4, rgba, "encode_dirw_2d");
+}
+int dolhip_encode_dirw_2d_dev(int w, int h, const float* dirw4, uint8_t* rgba, void* stream)
+{
+    TRY(check_dims(w, h));
+    return encode_ao_dev((size_t)w * h, 4, true, dirw4, rgba, stream, "encode_dirw_2d");
+}
+int dolhip_encode_dirw_3d(int w, int h, int d, const float* dirw8, uint8_t* rgba)
+{
+    TRY(check_dims(w, h, d));
+    return encode_ao_host((size_t)w * h * d, 8, true, dirw8, rgba, "encode_dirw_3d");
+}
+int dolhip_encode_dirw_3d_dev(int w, int h, int d, const float* dirw8, uint8_t* rgba, void* stream)
+{
+    TRY(check_dims(w, h, d));
+    return encode_ao_dev((size_t)w * h * d, 8, true, dirw8, rgba, stream, "encode_dirw_3d");
 }
 
 size_t dolhip_felzenszwalb_2d_workspace(int w, int h)

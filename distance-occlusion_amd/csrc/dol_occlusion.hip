@@ -18,7 +18,7 @@
 // The float expressions are the reference's, term for term and in its
 // order; the library is built without FMA contraction, so each result is
 // the reference's to the bit.
-#include "dol_common.hpp"
+#include "dol_cells.hpp"
 
 namespace dolhip {
 
@@ -185,6 +185,76 @@ void launch_occlusion_sweep(int mode, int w, int h, int d, const float* in, floa
         else
             hipLaunchKernelGGL(k_occlusion_sweep<2>, grid, dim3(64), 0, s, w, h, d, diag, tu, vb, in, dirw);
     }
+}
+
+// ------------------------------------------------------------ AO images
+
+// EncodeAO4 / EncodeAO8 (one byte a cell) and EncodeDirW4 / EncodeDirW8 (RGBA
+// a cell) of DistanceOcclusionLib.h:143-199 over whole fields of n cells,
+// the reference's float terms in its order (the tool's per-cell loops,
+// DistanceOcclusionTool.cpp:1521-1540, 1595-1615)
+// EncodeDirW4 / EncodeDirW8 of one cell as an RGBA word
+template <int PLANES>
+__device__ __forceinline__ uint32_t dirw_rgba(const float* d)
+{
+    uint32_t b0, b1, b2, b3;
+    if constexpr (PLANES == 4) {
+        float fw = d[0] + d[1] + d[2] + d[3];
+        float fx = d[0] - d[1] + d[2] - d[3];
+        float fy = d[0] + d[1] - d[2] - d[3];
+        const float inv = __fdiv_rn(1.0f, sqrt_rn(fx * fx + fy * fy));
+        fx *= inv;
+        fy *= inv;
+        fw *= 0.25f;
+        fx = fw * fx * 0.5f + 0.5f;
+        fy = fw * fy * 0.5f + 0.5f;
+        b0 = encode_u8(fx); b1 = encode_u8(fy); b2 = encode_u8(fw); b3 = 255;
+    } else {
+        float fw = d[0] + d[1] + d[2] + d[3] + d[4] + d[5] + d[6] + d[7];
+        float fx = d[0] - d[1] + d[2] - d[3] + d[4] - d[5] + d[6] - d[7];
+        float fy = d[0] + d[1] - d[2] - d[3] + d[4] + d[5] - d[6] - d[7];
+        float fz = d[0] + d[1] + d[2] + d[3] - d[4] - d[5] - d[6] - d[7];
+        const float inv = __fdiv_rn(1.0f, sqrt_rn(fx * fx + fy * fy + fz * fz));
+        fx *= inv;
+        fy *= inv;
+        fz *= inv;
+        fw *= 0.125f;
+        fx = fw * fx * 0.5f + 0.5f;
+        fy = fw * fy * 0.5f + 0.5f;
+        fz = fw * fz * 0.5f + 0.5f;
+        b0 = encode_u8(fx); b1 = encode_u8(fy); b2 = encode_u8(fz); b3 = encode_u8(fw);
+    }
+    return b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
+}
+
+template <int PLANES, bool DIRW>
+__global__ void __launch_bounds__(256) k_encode_ao(size_t n, const float* __restrict__ dirw,
+                                                   void* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float d[PLANES];
+#pragma unroll
+    for (int p = 0; p < PLANES; p++) d[p] = dirw[(size_t)p * n + i];
+    if (DIRW) {
+        ((uint32_t*)out)[i] = dirw_rgba<PLANES>(d);
+    } else {
+        float s = d[0];
+#pragma unroll
+        for (int p = 1; p < PLANES; p++) s += d[p];
+        ((uint8_t*)out)[i] = encode_u8(1.0f - (PLANES == 4 ? 0.25f : 0.125f) * s);
+    }
+}
+
+// planes 4 or 8; rgba: EncodeDirW images (4 bytes a cell), else EncodeAO (1)
+void launch_encode_ao(size_t n, int planes, bool rgba, const float* dirw, uint8_t* out, hipStream_t s)
+{
+    ProfileScope p("encode_ao", s);
+    const dim3 g(grid1d(n, 256)), b(256);
+    if (planes == 4 && !rgba) hipLaunchKernelGGL((k_encode_ao<4, false>), g, b, 0, s, n, dirw, out);
+    else if (planes == 4) hipLaunchKernelGGL((k_encode_ao<4, true>), g, b, 0, s, n, dirw, out);
+    else if (!rgba) hipLaunchKernelGGL((k_encode_ao<8, false>), g, b, 0, s, n, dirw, out);
+    else hipLaunchKernelGGL((k_encode_ao<8, true>), g, b, 0, s, n, dirw, out);
 }
 
 }  // namespace dolhip

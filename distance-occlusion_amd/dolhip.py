@@ -81,6 +81,14 @@ _SIGS = {
     "dolhip_occlusion_sweep_3d_dev": ([_I, _I, _I, _P, _P], _I),
     "dolhip_occlusion_sweep_3d_from": ([_I, _I, _I, _P, _P], _I),
     "dolhip_occlusion_sweep_3d_from_dev": ([_I, _I, _I, _P, _P, _P], _I),
+    "dolhip_encode_ao_2d": ([_I, _I, _P, _P], _I),
+    "dolhip_encode_ao_2d_dev": ([_I, _I, _P, _P, _P], _I),
+    "dolhip_encode_ao_3d": ([_I, _I, _I, _P, _P], _I),
+    "dolhip_encode_ao_3d_dev": ([_I, _I, _I, _P, _P, _P], _I),
+    "dolhip_encode_dirw_2d": ([_I, _I, _P, _P], _I),
+    "dolhip_encode_dirw_2d_dev": ([_I, _I, _P, _P, _P], _I),
+    "dolhip_encode_dirw_3d": ([_I, _I, _I, _P, _P], _I),
+    "dolhip_encode_dirw_3d_dev": ([_I, _I, _I, _P, _P, _P], _I),
     "dolhip_felzenszwalb_2d": ([_I, _I, _P], _I),
     "dolhip_felzenszwalb_2d_workspace": ([_I, _I], _SZ),
     "dolhip_felzenszwalb_2d_dev": ([_I, _I, _P, _P, _P], _I),
@@ -309,6 +317,15 @@ class DolHip:
         src = np.ascontiguousarray(dirw_in, dtype=np.float32)
         out = np.ascontiguousarray(dirw_out, dtype=np.float32).copy()
         self._call("dolhip_occlusion_sweep_3d_from", w, h, d, _addr(src), _addr(out))
+        return out
+
+    def encode_ao(self, dirw, rgba=False):
+        """AO image of a (4, h, w) or (8, d, h, w) field: bytes, or RGBA (EncodeDirW)."""
+        src = np.ascontiguousarray(dirw, dtype=np.float32)
+        dims = tuple(reversed(src.shape[1:]))
+        out = np.empty(src.shape[1:] + ((4,) if rgba else ()), dtype=np.uint8)
+        kind = "dirw" if rgba else "ao"
+        self._call(f"dolhip_encode_{kind}_{len(dims)}d", *dims, _addr(src), _addr(out))
         return out
 
     def felzenszwalb_2d(self, w, h, dist):

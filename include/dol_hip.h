@@ -152,6 +152,18 @@ int dolhip_occlusion_sweep_3d_from(int w, int h, int d, const float* in8, float*
 int dolhip_occlusion_sweep_3d_from_dev(int w, int h, int d, const float* in8, float* out8,
                                        void* stream);
 
+/* AO images of a swept field (the tool's per-cell EncodeAO4/8 and
+ * EncodeDirW4/8 loops, DistanceOcclusionLib.h:143-199): ao one byte a cell,
+ * rgba four. */
+int dolhip_encode_ao_2d(int w, int h, const float* dirw4, uint8_t* ao);
+int dolhip_encode_ao_2d_dev(int w, int h, const float* dirw4, uint8_t* ao, void* stream);
+int dolhip_encode_ao_3d(int w, int h, int d, const float* dirw8, uint8_t* ao);
+int dolhip_encode_ao_3d_dev(int w, int h, int d, const float* dirw8, uint8_t* ao, void* stream);
+int dolhip_encode_dirw_2d(int w, int h, const float* dirw4, uint8_t* rgba);
+int dolhip_encode_dirw_2d_dev(int w, int h, const float* dirw4, uint8_t* rgba, void* stream);
+int dolhip_encode_dirw_3d(int w, int h, int d, const float* dirw8, uint8_t* rgba);
+int dolhip_encode_dirw_3d_dev(int w, int h, int d, const float* dirw8, uint8_t* rgba, void* stream);
+
 /* Felzenszwalb float squared distances in place, bit-identical to the
  * reference's float arithmetic (DistanceOcclusionLib.h:71, .cpp:1018-1085) */
 int dolhip_felzenszwalb_2d(int w, int h, float* dist);

@@ -895,6 +895,57 @@ static void occlusion_sweep_3d(int w, int h, int d, const float* in, float* dirw
     }
 }
 void oc_occlusion_sweep_3d(int w, int h, int d, float* dirw) { occlusion_sweep_3d(w, h, d, NULL, dirw); }
+
+/* EncodeAO4 / EncodeAO8 (DistanceOcclusionLib.h:143-155): the planes summed
+ * in order, EncodeU8(1 - sum / planes) with the reference's 0.25f / 0.125f;
+ * one byte a cell.  EncodeDirW4 / EncodeDirW8 (:157-199): the signed sums
+ * along x, y (and z) normalised by 1 / sqrtf of their squared length, then
+ * EncodeU8 of w * c * 0.5 + 0.5 for each and of w; RGBA a cell (alpha 255 in
+ * 2D, w in 3D).  The tool writes these images per cell
+ * (DistanceOcclusionTool.cpp:1521-1540, 1595-1615). */
+void oc_encode_ao(size_t n, int planes, const float* dirw, uint8_t* out)
+{
+    for (size_t i = 0; i < n; i++) {
+        float s = dirw[i];
+        for (int p = 1; p < planes; p++) s += dirw[p * n + i];
+        out[i] = oc_encode_u8(1.0f - (planes == 4 ? 0.25f : 0.125f) * s);
+    }
+}
+
+void oc_encode_dirw(size_t n, int planes, const float* dirw, uint8_t* out)
+{
+    for (size_t i = 0; i < n; i++) {
+        float d[8] = {0};
+        for (int p = 0; p < planes; p++) d[p] = dirw[p * n + i];
+        uint8_t* o = out + 4 * i;
+        if (planes == 4) {
+            float fw = d[0] + d[1] + d[2] + d[3];
+            float fx = d[0] - d[1] + d[2] - d[3];
+            float fy = d[0] + d[1] - d[2] - d[3];
+            const float inv = 1.0f / sqrtf(fx * fx + fy * fy);
+            fx *= inv;
+            fy *= inv;
+            fw *= 0.25f;
+            fx = fw * fx * 0.5f + 0.5f;
+            fy = fw * fy * 0.5f + 0.5f;
+            o[0] = oc_encode_u8(fx); o[1] = oc_encode_u8(fy); o[2] = oc_encode_u8(fw); o[3] = 255;
+        } else {
+            float fw = d[0] + d[1] + d[2] + d[3] + d[4] + d[5] + d[6] + d[7];
+            float fx = d[0] - d[1] + d[2] - d[3] + d[4] - d[5] + d[6] - d[7];
+            float fy = d[0] + d[1] - d[2] - d[3] + d[4] + d[5] - d[6] - d[7];
+            float fz = d[0] + d[1] + d[2] + d[3] - d[4] - d[5] - d[6] - d[7];
+            const float inv = 1.0f / sqrtf(fx * fx + fy * fy + fz * fz);
+            fx *= inv;
+            fy *= inv;
+            fz *= inv;
+            fw *= 0.125f;
+            fx = fw * fx * 0.5f + 0.5f;
+            fy = fw * fy * 0.5f + 0.5f;
+            fz = fw * fz * 0.5f + 0.5f;
+            o[0] = oc_encode_u8(fx); o[1] = oc_encode_u8(fy); o[2] = oc_encode_u8(fz); o[3] = oc_encode_u8(fw);
+        }
+    }
+}
 void oc_occlusion_sweep_3d_from(int w, int h, int d, const float* in, float* out)
 {
     occlusion_sweep_3d(w, h, d, in, out);

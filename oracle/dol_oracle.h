@@ -14,6 +14,7 @@
 #ifndef DOL_ORACLE_H
 #define DOL_ORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -63,6 +64,10 @@ void oc_init_dirw_3d(int w, int h, int d, const uint32_t* mask, float* dirw8);
 void oc_occlusion_sweep_2d(int w, int h, float* dirw4);
 void oc_occlusion_sweep_3d(int w, int h, int d, float* dirw8);
 void oc_occlusion_sweep_3d_from(int w, int h, int d, const float* in8, float* out8);
+/* AO images from n cells of 4 or 8 planes: one byte a cell, or RGBA
+ * (Lib.h:143-199) */
+void oc_encode_ao(size_t n, int planes, const float* dirw, uint8_t* out);
+void oc_encode_dirw(size_t n, int planes, const float* dirw, uint8_t* out);
 
 /* Felzenszwalb float squared distances, in place (Lib:1018-1085) */
 void oc_felzenszwalb_2d(int w, int h, float* dist);

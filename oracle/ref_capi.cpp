@@ -66,4 +66,23 @@ void ref_brute_force_3d(int w, int h, int d, int16_t* delta, int cw)
 
 uint8_t ref_encode_u8(float v) { return EncodeU8(v); }
 
+// the header's per-cell AO encoders over n cells of 4 or 8 planes
+void ref_encode_ao(size_t n, int planes, const float* dirw, uint8_t* out)
+{
+    for (size_t i = 0; i < n; i++) {
+        float c[8];
+        for (int p = 0; p < planes; p++) c[p] = dirw[p * n + i];
+        out[i] = planes == 4 ? EncodeAO4(c) : EncodeAO8(c);
+    }
+}
+void ref_encode_dirw(size_t n, int planes, const float* dirw, uint8_t* out)
+{
+    for (size_t i = 0; i < n; i++) {
+        float c[8];
+        for (int p = 0; p < planes; p++) c[p] = dirw[p * n + i];
+        if (planes == 4) EncodeDirW4(c, out + 4 * i);
+        else EncodeDirW8(c, out + 4 * i);
+    }
+}
+
 }

@@ -173,6 +173,15 @@ class Oracle:
         self.L.oc_occlusion_sweep_3d_from(_I(w), _I(h), _I(d), _ptr(src), _ptr(out))
         return out
 
+    def encode_ao(self, dirw, dirw_mode=False):
+        d = np.ascontiguousarray(dirw, dtype=np.float32)
+        planes = d.shape[0]
+        n = d.size // planes
+        out = np.empty((n, 4) if dirw_mode else n, dtype=np.uint8)
+        fn = self.L.oc_encode_dirw if dirw_mode else self.L.oc_encode_ao
+        fn(ctypes.c_size_t(n), _I(planes), _ptr(d), _ptr(out))
+        return out.reshape(d.shape[1:] + ((4,) if dirw_mode else ()))
+
     def jfa_pass_2d(self, step, w, h, delta, cw=1):
         src = np.ascontiguousarray(delta, dtype=np.int16)
         out = np.empty_like(src)
@@ -394,6 +403,15 @@ class Reference:
         out = np.ascontiguousarray(dirw_out, dtype=np.float32).copy()
         self.L.ref_occlusion_sweep_3d_from(_I(w), _I(h), _I(d), _ptr(src), _ptr(out))
         return out
+
+    def encode_ao(self, dirw, dirw_mode=False):
+        d = np.ascontiguousarray(dirw, dtype=np.float32)
+        planes = d.shape[0]
+        n = d.size // planes
+        out = np.empty((n, 4) if dirw_mode else n, dtype=np.uint8)
+        fn = self.L.ref_encode_dirw if dirw_mode else self.L.ref_encode_ao
+        fn(ctypes.c_size_t(n), _I(planes), _ptr(d), _ptr(out))
+        return out.reshape(d.shape[1:] + ((4,) if dirw_mode else ()))
 
     def felzenszwalb_2d(self, w, h, dist):
         out = np.ascontiguousarray(dist, dtype=np.float32).copy()

@@ -583,3 +583,20 @@ def test_occlusion_large(gpu, oracle, rng):
     for x in range(1, w):
         exp[x] = np.float32(0.75) * exp[x - 1]
     np.testing.assert_array_equal(gpu.occlusion_sweep_2d(w, 1, dw)[0, 0], exp)
+
+
+@pytest.mark.parametrize("shape", [(4, 1, 1), (4, 37, 53), (8, 3, 5, 7), (8, 9, 17, 33)])
+def test_ao_encoders(gpu, oracle, rng, shape):
+    """AO images of swept fields (random fields, all-zero and all-one fields)
+    against the oracle, byte for byte."""
+    for fill in (None, 0.0, 1.0):
+        d = rng.random(shape).astype(np.float32) if fill is None else np.full(shape, fill, np.float32)
+        for rgba in (False, True):
+            np.testing.assert_array_equal(gpu.encode_ao(d, rgba), oracle.encode_ao(d, rgba))
+    # a swept field from a mask
+    dims = tuple(reversed(shape[1:]))
+    m = rand_mask(rng, dims, 0.05)
+    dw = oracle.init_dirw_2d(*dims, m) if len(dims) == 2 else oracle.init_dirw_3d(*dims, m)
+    dw = oracle.occlusion_sweep_2d(*dims, dw) if len(dims) == 2 else oracle.occlusion_sweep_3d(*dims, dw)
+    for rgba in (False, True):
+        np.testing.assert_array_equal(gpu.encode_ao(dw, rgba), oracle.encode_ao(dw, rgba))

@@ -526,3 +526,29 @@ def test_occlusion_closed_forms(oracle):
     for fill in (0.0, 1.0):
         dw = np.full((8, 3, 4, 5), fill, np.float32)
         np.testing.assert_array_equal(oracle.occlusion_sweep_3d(5, 4, 3, dw), dw)
+
+
+def test_ao_encoders_live_reference(oracle, reference, rng):
+    """EncodeAO4/8 and EncodeDirW4/8 over whole fields against the reference
+    header's own per-cell functions, including a zero-length direction
+    (1 / sqrtf(0): the bytes the reference's NaN gives)."""
+    if reference is None:
+        pytest.skip("oracle/_ref/libdolref.so not built (reference sources absent)")
+    for planes, shape in ((4, (4, 33, 17)), (8, (8, 5, 9, 7))):
+        for fill in (None, 0.0, 1.0, 0.5):
+            d = rng.random(shape).astype(np.float32) if fill is None else np.full(shape, fill, np.float32)
+            for rgba in (False, True):
+                np.testing.assert_array_equal(oracle.encode_ao(d, rgba), reference.encode_ao(d, rgba))
+
+
+def test_ao_encoder_closed_forms(oracle):
+    """All planes 0: ao 255 (EncodeU8(1)); all 1: ao 0; one plane 1 of 4:
+    EncodeU8(0.75) = 191; direction of plane 0 alone: x = y = 0.25 * 0.5 *
+    (1 / sqrt 2) + 0.5 -> rint(255 * 0.588388) = 150."""
+    z = np.zeros((4, 1, 3), np.float32)
+    np.testing.assert_array_equal(oracle.encode_ao(z), 255)
+    np.testing.assert_array_equal(oracle.encode_ao(z + 1), 0)
+    one = z.copy()
+    one[0] = 1.0
+    np.testing.assert_array_equal(oracle.encode_ao(one), 191)
+    np.testing.assert_array_equal(oracle.encode_ao(one, True)[0, 0], [150, 150, 64, 255])
