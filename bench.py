@@ -423,17 +423,22 @@ def bench_occlusion(L, torch, dev, mask3_t=None, cpu=True):
     mask = L.scene_shapes_2d(w, h, 2000, seed=2)
     m_t = torch.from_numpy(mask.view(np.int32)).to(dev)
     dw = torch.empty(4 * w * h, dtype=torch.float32, device=dev)
+    ao = torch.empty(w * h, dtype=torch.uint8, device=dev)
+    rgba = torch.empty(w * h * 4, dtype=torch.uint8, device=dev)
 
-    def step2():
+    def step2():   # the tool's occlusion output: direction field, sweep, both images
         L.call_dev("init_dirw_2d", w, h, m_t, dw)
         L.call_dev("occlusion_sweep_2d", w, h, dw)
+        L.call_dev("encode_ao_2d", w, h, dw, ao)
+        L.call_dev("encode_dirw_2d", w, h, dw, rgba)
 
     step2()
     L.profile_enable(True)
     ms = timed(step2, 3, torch)
     sweep_ms, _ = L.profile_query("occlusion_sweep")
+    enc_ms, _ = L.profile_query("encode_ao")
     L.profile_enable(False)
-    out["occlusion_2d_8192"] = {"ms": round(ms, 3), "sweep_ms": round(sweep_ms, 3),
+    out["occlusion_2d_8192"] = {"ms": round(ms, 3), "sweep_ms": round(sweep_ms, 3), "encode_ms": round(enc_ms, 4),
                                 "gpixels_per_s": round(w * h / (sweep_ms * 1e-3) / 1e9, 3),
                                 "launches": w // 8 + -(-h // 15) - 1}
     if cpu:
@@ -450,7 +455,7 @@ def bench_occlusion(L, torch, dev, mask3_t=None, cpu=True):
         out["occlusion_2d_8192"]["cpu_baseline"] = {
             "ms_2048sq": round(cpu_s * 1e3, 1), "gpixels_per_s": round(k * k / cpu_s / 1e9, 4), "cores": 1,
             "kind": "port", "sample": "the 2048^2 corner of the scene, 4 planes"}
-    del dw
+    del dw, ao, rgba
     torch.cuda.empty_cache()
 
     n = 512
