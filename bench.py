@@ -615,6 +615,14 @@ def main():
         "hbm_roofline_pct_of_3pass_bytes": round(100 * pct_3pass, 2),
         "kernels": {k: round(v["avg_ms"], 4) for k, v in kernels.items()},
         "roofline": roofline,
+        # both kernels against HBM peak: edt3d_z is HBM-bound, edt3d_xy VALU-bound
+        # (its scan steps), so its HBM fraction is low by construction
+        "roofline_by_kernel": {
+            k: {"bound": "hbm" if k == "edt3d_z" else "valu",
+                "hbm_gbs": round(EDT3D_BYTES_PER_VOXEL[k] * cells / (v["avg_ms"] * 1e-3) / 1e9, 1),
+                "hbm_frac": round(EDT3D_BYTES_PER_VOXEL[k] * cells / (v["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "avg_ms": round(v["avg_ms"], 4)}
+            for k, v in kernels.items()},
     }
 
     # ---- CPU baseline (the oracle, one thread) and full-size parity
