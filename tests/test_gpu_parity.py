@@ -600,3 +600,24 @@ def test_ao_encoders(gpu, oracle, rng, shape):
     dw = oracle.occlusion_sweep_2d(*dims, dw) if len(dims) == 2 else oracle.occlusion_sweep_3d(*dims, dw)
     for rgba in (False, True):
         np.testing.assert_array_equal(gpu.encode_ao(dw, rgba), oracle.encode_ao(dw, rgba))
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_sweeps_random_sizes(gpu, oracle, seed):
+    """FastSweep and the occlusion sweeps on seeded random sizes (ragged tile
+    edges in every direction) and densities, against the oracle."""
+    r = np.random.default_rng(1000 + seed)
+    for _ in range(3):
+        w, h = (int(v) for v in r.integers(1, 300, 2))
+        m = rand_mask(r, (w, h), float(r.choice([0.0, 0.001, 0.02, 0.3])))
+        for cw, start in ((1, oracle.init_deltas_2d(w, h, m)), (2, oracle.border_2d(w, h, m))):
+            np.testing.assert_array_equal(gpu.fast_sweep_2d(w, h, start, cw), oracle.fast_sweep_2d(w, h, start, cw))
+        dw = oracle.init_dirw_2d(w, h, m)
+        np.testing.assert_array_equal(_bits(gpu.occlusion_sweep_2d(w, h, dw)), _bits(oracle.occlusion_sweep_2d(w, h, dw)))
+        w, h, d = (int(v) for v in r.integers(1, 40, 3))
+        m = rand_mask(r, (w, h, d), float(r.choice([0.0, 0.002, 0.05])))
+        start = oracle.border_3d(w, h, d, m)
+        np.testing.assert_array_equal(gpu.fast_sweep_3d(w, h, d, start, 2), oracle.fast_sweep_3d(w, h, d, start, 2))
+        dw = oracle.init_dirw_3d(w, h, d, m)
+        np.testing.assert_array_equal(_bits(gpu.occlusion_sweep_3d(w, h, d, dw)),
+                                      _bits(oracle.occlusion_sweep_3d(w, h, d, dw)))
