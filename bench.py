@@ -511,7 +511,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the path runs on one MI355X; see DESIGN.md for how it would shard)", file=sys.stderr)
+              "(the path runs on one MI355X; volumes would shard by slabs of slices, one per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import torch
