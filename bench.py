@@ -440,7 +440,7 @@ def bench_occlusion(L, torch, dev, mask3_t=None, cpu=True):
     L.profile_enable(False)
     out["occlusion_2d_8192"] = {"ms": round(ms, 3), "sweep_ms": round(sweep_ms, 3), "encode_ms": round(enc_ms, 4),
                                 "gpixels_per_s": round(w * h / (sweep_ms * 1e-3) / 1e9, 3),
-                                "launches": w // 8 + -(-h // 15) - 1}
+                                "launches": w // 64 + -(-h // 63) - 1}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_bind import Oracle

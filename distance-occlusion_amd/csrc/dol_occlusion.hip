@@ -8,12 +8,13 @@
 // already updated, with 0 standing in for a neighbour outside the field;
 // only the corner cell is left as it is.  Cells with equal u + v + t are
 // independent, so the sweep is a wavefront over tiles, as FastSweep's in
-// dol_sweep.hip: one 16-lane DPP row of a wavefront holds an 8-column x
-// 15-row tile of one slice, lane l on row v0 - 1 + l (lane 0 carrying the
-// row before the tile), lanes skewed one column each; a launch is one
-// diagonal U + V + t of tiles.  A lane keeps its row (and the same row of the
-// slice before) in registers, rotated by its place in the group so that
-// every step reads the same register.
+// dol_sweep.hip: in 3D one 16-lane DPP row of a wavefront holds an 8-column
+// x 15-row tile of one slice, in 2D a wavefront holds a 64 x 63 tile; lane l
+// is on row v0 - 1 + l (lane 0 carrying the row before the tile), lanes
+// skewed one column each; a launch is one diagonal U + V (+ t) of tiles.  A
+// lane keeps its row (and in 3D the same row of the slice before) in
+// registers, rotated by its place in the tile so that every step reads the
+// same register.
 //
 // The float expressions are the reference's, term for term and in its
 // order; the library is built without FMA contraction, so each result is
@@ -86,22 +87,19 @@ __device__ __forceinline__ float from_prev_lane16(float v)
 
 }  // namespace
 
-// MODE 0: 2D, in place (OcclusionSweep, :1171-1209); MODE 1: 3D, in place
-// (:2569-2648); MODE 2: 3D, out from in (:2650-2743; `in` read only).
-// Lane group g of block b: 2D, plane b / vb and V = kOTiles * (b % vb) + g,
-// U = diag - V; 3D, plane b / (vb tu), r = b % (vb tu), U = r % tu,
-// V = kOTiles * (r / tu) + g, t = diag - U - V.
+// 3D.  MODE 1: in place (OcclusionSweep, :2569-2648); MODE 2: out from in
+// (:2650-2743; `in` read only).  Lane group g of block b: plane b / (vb tu),
+// r = b % (vb tu), U = r % tu, V = kOTiles * (r / tu) + g, t = diag - U - V.
 template <int MODE>
 __global__ void __launch_bounds__(64) k_occlusion_sweep(int w, int h, int d, int diag, int tu, int vb,
                                                         const float* __restrict__ in,
                                                         float* __restrict__ dirw)
 {
     const int lane = threadIdx.x, j = lane % kOGroup;
-    const int nb = MODE == 0 ? vb : vb * tu;
-    const int plane = (int)blockIdx.x / nb, r = (int)blockIdx.x % nb;
-    const int V = kOTiles * (MODE == 0 ? r : r / tu) + lane / kOGroup;
-    const int Ud = MODE == 0 ? diag - V : r % tu;
-    const int t = MODE == 0 ? 0 : diag - Ud - V;
+    const int plane = (int)blockIdx.x / (vb * tu), r = (int)blockIdx.x % (vb * tu);
+    const int V = kOTiles * (r / tu) + lane / kOGroup;
+    const int Ud = r % tu;
+    const int t = diag - Ud - V;
     const size_t cells = (size_t)w * h * d, slice = (size_t)w * h;
     const int dx = 1 - 2 * (plane & 1), dy = 1 - 2 * ((plane >> 1) & 1), dz = 1 - 2 * ((plane >> 2) & 1);
     const int x0 = dx > 0 ? 0 : w - 1, y0 = dy > 0 ? 0 : h - 1, z0 = dz > 0 ? 0 : d - 1;
@@ -110,6 +108,7 @@ __global__ void __launch_bounds__(64) k_occlusion_sweep(int w, int h, int d, int
     const int z = tile_ok ? z0 + t * dz : 0, zb = tile_ok && t >= 1 ? z - dz : z;
     float* out = dirw + (size_t)plane * cells;
     const float* inp = MODE == 2 ? in + (size_t)plane * cells : out;
+    static_assert(MODE == 1 || MODE == 2, "3D modes");
     const int u0 = U * kOW, v0 = V * kORows;
     const int nu = min(kOW, w - u0);
     const int vr = v0 - 1 + j;                           // the lane's row
@@ -145,12 +144,7 @@ __global__ void __launch_bounds__(64) k_occlusion_sweep(int w, int h, int d, int
         const float up_in = MODE == 2 ? from_prev_lane16(mine_in) : 0.0f;
         const bool hl = u0 + a >= 1, hu = vr >= 1, hb = t >= 1;   // neighbours inside
         float v;
-        if (MODE == 0) {
-            // first row: the cell before; first column: the row before;
-            // elsewhere both (:1191-1205)
-            const float q = hl ? quadrant_combine(mine, hu ? up : 0.0f) : quadrant_combine(up, 0.0f);
-            v = R[k] + (1.0f - R[k]) * q;
-        } else if (MODE == 1) {
+        if (MODE == 1) {
             v = R[k] + (1.0f - R[k]) * octant_combine(hl ? mine : 0.0f, hu ? up : 0.0f, hb ? B[k] : 0.0f);
         } else {
             v = octant_combine(hl ? mine : 0.0f, hu ? up : 0.0f, hb ? B[k] : 0.0f, hl ? mine_in : 0.0f,
@@ -170,17 +164,100 @@ __global__ void __launch_bounds__(64) k_occlusion_sweep(int w, int h, int d, int
     }
 }
 
+// 2D (OcclusionSweep, :1171-1209).  The chain is (tile diagonals) x (steps a
+// tile) for any tile shape, but each launch has a cost of its own, so 2D (no
+// slices to multiply the diagonals) uses 64-column x 63-row tiles, one a
+// wavefront, lane l on row v0 - 1 + l: 258 launches at 8192^2 instead of
+// 1570 with 8 x 15 tiles.  The tile passes through LDS to and from HBM
+// (coalesced), and each lane's row is held in 64 registers rotated by its
+// lane index.  First row: the cell before; first column: the row before;
+// elsewhere both (:1191-1205).
+constexpr int kOB = 64;                // tile columns = lanes
+constexpr int kOBRows = kOB - 1;       // tile rows (lane 0 carries the row before)
+constexpr int kOBStride = kOB + 2;     // LDS row stride: lane l's column s - l at word 65 l + s
+
+__device__ __forceinline__ float from_prev_lane64(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf,
+                                                                 0xf, false));
+}
+
+// block b: plane b / nt, tile V = max(0, diag - tu + 1) + b % nt, U = diag - V
+__global__ void __launch_bounds__(kOB) k_occlusion_sweep_2d(int w, int h, int diag, int tu, int nt,
+                                                            float* __restrict__ dirw)
+{
+    __shared__ float tile[kOB * kOBStride];
+    const int lane = threadIdx.x;
+    const int plane = (int)blockIdx.x / nt;
+    const int V = max(0, diag - (tu - 1)) + (int)blockIdx.x % nt, U = diag - V;
+    const int dx = 1 - 2 * (plane & 1), dy = 1 - 2 * ((plane >> 1) & 1);
+    const int x0 = dx > 0 ? 0 : w - 1, y0 = dy > 0 ? 0 : h - 1;
+    float* f = dirw + (size_t)plane * w * h;
+    const int u0 = U * kOB, v0 = V * kOBRows;
+    const int nu = min(kOB, w - u0);
+    const int rlo = max(v0 - 1, 0), rhi = min(v0 + kOBRows - 1, h - 1);
+    auto row = [&](int v) { return (size_t)(y0 + v * dy) * w; };
+    const int cu = x0 + (u0 + min(lane, nu - 1)) * dx;
+    {
+        // all rows in flight at once (rows outside the field repeat the nearest)
+        float v[kOB];
+#pragma unroll
+        for (int r = 0; r < kOB; r++) v[r] = f[row(min(max(v0 - 1 + r, rlo), rhi)) + cu];
+#pragma unroll
+        for (int r = 0; r < kOB; r++) tile[r * kOBStride + lane] = v[r];
+    }
+    const int vr = v0 - 1 + lane;
+    const bool row_ok = vr >= 0 && vr < h, update = row_ok && lane >= 1;
+    float mine = (u0 > 0 && row_ok) ? f[row(vr) + x0 + (u0 - 1) * dx] : 0.0f;
+    __syncthreads();
+    float R[kOB];
+#pragma unroll
+    for (int k = 0; k < kOB; k++) R[k] = tile[lane * kOBStride + ((k - lane) & (kOB - 1))];
+    const bool hu = vr >= 1;
+#pragma unroll
+    for (int s = 0; s < 2 * kOB - 1; s++) {
+        const int a = s - lane;
+        const float up = from_prev_lane64(mine);
+        const bool hl = u0 + a >= 1;
+        const float q = hl ? quadrant_combine(mine, hu ? up : 0.0f) : quadrant_combine(up, 0.0f);
+        const float v = R[s & (kOB - 1)] + (1.0f - R[s & (kOB - 1)]) * q;
+        if (row_ok && a >= 0 && a < nu) {
+            if (update && (hl || hu)) R[s & (kOB - 1)] = v;   // the plane's corner cell stays
+            mine = R[s & (kOB - 1)];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kOB; k++) tile[lane * kOBStride + ((k - lane) & (kOB - 1))] = R[k];
+    __syncthreads();
+    {
+        float v[kOB];
+#pragma unroll
+        for (int r = 1; r < kOB; r++) v[r] = tile[r * kOBStride + lane];
+#pragma unroll
+        for (int r = 1; r < kOB; r++)
+            if (v0 - 1 + r <= rhi && lane < nu) f[row(v0 - 1 + r) + cu] = v[r];
+    }
+}
+
 // mode 0: 2D (d == 1, 4 planes); 1: 3D in place; 2: 3D from `in` (8 planes)
 void launch_occlusion_sweep(int mode, int w, int h, int d, const float* in, float* dirw, hipStream_t s)
 {
     ProfileScope p("occlusion_sweep", s);
+    if (mode == 0) {
+        const int tu = (w + kOB - 1) / kOB, tv = (h + kOBRows - 1) / kOBRows;
+        for (int diag = 0; diag < tu + tv - 1; diag++) {
+            const int lo = diag - (tu - 1) > 0 ? diag - (tu - 1) : 0, hi = diag < tv - 1 ? diag : tv - 1;
+            const int nt = hi - lo + 1;
+            hipLaunchKernelGGL(k_occlusion_sweep_2d, dim3((unsigned)(4 * nt)), dim3(kOB), 0, s, w, h, diag, tu,
+                               nt, dirw);
+        }
+        return;
+    }
     const int tu = (w + kOW - 1) / kOW, tv = (h + kORows - 1) / kORows;
     const int vb = (tv + kOTiles - 1) / kOTiles;
-    const dim3 grid((unsigned)(mode == 0 ? 4 * vb : 8 * vb * tu));
+    const dim3 grid((unsigned)(8 * vb * tu));
     for (int diag = 0; diag < tu + tv + d - 2; diag++) {
-        if (mode == 0)
-            hipLaunchKernelGGL(k_occlusion_sweep<0>, grid, dim3(64), 0, s, w, h, 1, diag, tu, vb, nullptr, dirw);
-        else if (mode == 1)
+        if (mode == 1)
             hipLaunchKernelGGL(k_occlusion_sweep<1>, grid, dim3(64), 0, s, w, h, d, diag, tu, vb, nullptr, dirw);
         else
             hipLaunchKernelGGL(k_occlusion_sweep<2>, grid, dim3(64), 0, s, w, h, d, diag, tu, vb, in, dirw);
