@@ -268,6 +268,15 @@ static int check_ptr(const void* p, const char* name)
     return DOLHIP_OK;
 }
 
+// a cCellDelta2 field on the device: the kernels access its cells as 32-bit
+// words, so the pointer must be 4-byte aligned (the struct itself only needs 2)
+static int check_cells2(const void* p, const char* name)
+{
+    TRY(check_ptr(p, name));
+    if ((uintptr_t)p & 3) return fail(DOLHIP_ERR_INVALID_ARG, "%s must be 4-byte aligned", name);
+    return DOLHIP_OK;
+}
+
 static int check_cw(int cw)
 {
     if (cw < 1 || cw > 2)
@@ -407,7 +416,7 @@ int dolhip_init_deltas_2d_dev(int w, int h, const uint32_t* mask, int16_t* delta
 {
     TRY(check_dims(w, h));
     TRY(check_ptr(mask, "mask"));
-    TRY(check_ptr(delta, "delta"));
+    TRY(check_cells2(delta, "delta"));
     TRY(require_device());
     launch_init_deltas_2d(w, h, mask, delta, invert, as_stream(stream));
     return finish(as_stream(stream), "init_deltas_2d");
@@ -458,7 +467,7 @@ int dolhip_init_border_deltas_2d_dev(int w, int h, const uint32_t* mask, int16_t
 {
     TRY(check_dims(w, h));
     TRY(check_ptr(mask, "mask"));
-    TRY(check_ptr(delta, "delta"));
+    TRY(check_cells2(delta, "delta"));
     TRY(require_device());
     launch_border_2d(w, h, mask, delta, as_stream(stream));
     return finish(as_stream(stream), "border_2d");
@@ -532,7 +541,7 @@ size_t dolhip_jump_flood_2d_workspace(int w, int h)
 int dolhip_jump_flood_2d_dev(int w, int h, int16_t* delta, int cw, void* work, void* stream)
 {
     TRY(check_dims(w, h));
-    TRY(check_ptr(delta, "delta"));
+    TRY(check_cells2(delta, "delta"));
     TRY(check_cw(cw));
     TRY(require_device());
     void* tmp = nullptr;
@@ -592,9 +601,8 @@ int dolhip_jump_flood_3d(int w, int h, int d, int16_t* delta, int cw)
 int dolhip_fast_sweep_2d_dev(int w, int h, int16_t* delta, int cw, void* stream)
 {
     TRY(check_dims(w, h));
-    TRY(check_ptr(delta, "delta"));
+    TRY(check_cells2(delta, "delta"));
     TRY(check_cw(cw));
-    if ((uintptr_t)delta & 3) return fail(DOLHIP_ERR_INVALID_ARG, "delta must be 4-byte aligned");
     TRY(require_device());
     launch_fast_sweep_2d(w, h, delta, cw, as_stream(stream));
     return finish(as_stream(stream), "fast_sweep_2d");
@@ -999,7 +1007,7 @@ int dolhip_signed_edt_3d(int w, int h, int d, const uint32_t* mask, float* sdf)
 int dolhip_dist_2d_dev(int n, const int16_t* delta, float* out, void* stream)
 {
     if (n < 0) return fail(DOLHIP_ERR_INVALID_ARG, "n must be >= 0");
-    TRY(check_ptr(delta, "delta"));
+    TRY(check_cells2(delta, "delta"));
     TRY(check_ptr(out, "out"));
     TRY(require_device());
     if (n) launch_dist_2d((size_t)n, delta, out, as_stream(stream));
@@ -1011,7 +1019,7 @@ static int signed_dev(int w, int h, int d, const uint32_t* mask, const int16_t* 
 {
     TRY(check_dims(w, h, d));
     TRY(check_ptr(mask, "mask"));
-    TRY(check_ptr(delta, "delta"));
+    TRY(d == 1 ? check_cells2(delta, "delta") : check_ptr(delta, "delta"));
     TRY(check_ptr(out, "out"));
     if (mode == 1 && !(max_len > 0.0f)) return fail(DOLHIP_ERR_INVALID_ARG, "max_len must be > 0");
     TRY(require_device());
@@ -1137,7 +1145,7 @@ int dolhip_signed_border_2d_dev(int w, int h, const uint32_t* mask, int16_t* del
 {
     TRY(check_dims(w, h));
     TRY(check_ptr(mask, "mask"));
-    TRY(check_ptr(delta, "delta"));
+    TRY(check_cells2(delta, "delta"));
     TRY(require_device());
     void* tmp = nullptr;
     TRY(resolve_work(work, dolhip_jump_flood_2d_workspace(w, h), &tmp));
@@ -1181,8 +1189,8 @@ int dolhip_two_field_deltas_2d_dev(int w, int h, const uint32_t* mask, int16_t* 
 {
     TRY(check_dims(w, h));
     TRY(check_ptr(mask, "mask"));
-    TRY(check_ptr(delta0, "delta0"));
-    TRY(check_ptr(delta1, "delta1"));
+    TRY(check_cells2(delta0, "delta0"));
+    TRY(check_cells2(delta1, "delta1"));
     TRY(check_cw(cw));
     TRY(require_device());
     void* tmp = nullptr;
@@ -1262,8 +1270,8 @@ int dolhip_encode_two_field_2d_dev(int w, int h, const int16_t* delta0, const in
                                    float max_len, int type, uint8_t* out, void* stream)
 {
     TRY(check_dims(w, h));
-    TRY(check_ptr(delta0, "delta0"));
-    TRY(check_ptr(delta1, "delta1"));
+    TRY(check_cells2(delta0, "delta0"));
+    TRY(check_cells2(delta1, "delta1"));
     TRY(check_ptr(out, "out"));
     TRY(check_image_type(type, max_len));
     TRY(require_device());

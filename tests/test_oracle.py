@@ -552,3 +552,26 @@ def test_ao_encoder_closed_forms(oracle):
     one[0] = 1.0
     np.testing.assert_array_equal(oracle.encode_ao(one), 191)
     np.testing.assert_array_equal(oracle.encode_ao(one, True)[0, 0], [150, 150, 64, 255])
+
+
+def test_misaligned_2d_fields_rejected():
+    """Device forms that access cCellDelta2 cells as 32-bit words reject a
+    field that is only 2-byte aligned, before touching any device (the
+    addresses below are never dereferenced)."""
+    lib = ctypes.CDLL(os.path.join(ROOT, "distance-occlusion_amd", "libdolhip.so"))
+    lib.dolhip_last_error.restype = ctypes.c_char_p
+    P = ctypes.c_void_p
+    good, odd = P(0x10000), P(0x10002)
+    calls = {
+        "dolhip_jump_flood_2d_dev": lambda p: (8, 8, p, 1, None, None),
+        "dolhip_fast_sweep_2d_dev": lambda p: (8, 8, p, 1, None),
+        "dolhip_init_deltas_2d_dev": lambda p: (8, 8, good, p, 0, None),
+        "dolhip_init_border_deltas_2d_dev": lambda p: (8, 8, good, p, None),
+        "dolhip_dist_2d_dev": lambda p: (64, p, good, None),
+        "dolhip_signed_distance_2d_dev": lambda p: (8, 8, good, p, good, None),
+        "dolhip_encode_two_field_2d_dev": lambda p: (8, 8, p, good, ctypes.c_float(2.0), 0, good, None),
+    }
+    for name, args in calls.items():
+        fn = getattr(lib, name)
+        assert fn(*args(odd)) == 1, name   # DOLHIP_ERR_INVALID_ARG
+        assert b"4-byte aligned" in lib.dolhip_last_error(), name
