@@ -8,8 +8,9 @@
 // already updated, with 0 standing in for a neighbour outside the field;
 // only the corner cell is left as it is.  Cells with equal u + v + t are
 // independent, so the sweep is a wavefront over tiles, as FastSweep's in
-// dol_sweep.hip: in 3D one 16-lane DPP row of a wavefront holds an 8-column
-// x 15-row tile of one slice, in 2D a wavefront holds a 64 x 63 tile; lane l
+// dol_sweep.hip: in 3D one 16-lane DPP row of a wavefront holds a 16-column
+// x 15-row tile of one slice (8 and 32 columns measured slower), in 2D a
+// wavefront holds a 64 x 63 tile; lane l
 // is on row v0 - 1 + l (lane 0 carrying the row before the tile), lanes
 // skewed one column each; a launch is one diagonal U + V (+ t) of tiles.  A
 // lane keeps its row (and in 3D the same row of the slice before) in
@@ -52,7 +53,7 @@ void launch_init_dirw(int w, int h, int d, int planes, const uint32_t* mask, flo
 
 namespace {
 
-constexpr int kOW = 8;                 // tile columns
+constexpr int kOW = 16;                // tile columns
 constexpr int kOGroup = 16;            // lanes per tile (one DPP row)
 constexpr int kORows = kOGroup - 1;    // tile rows
 constexpr int kOTiles = 64 / kOGroup;  // tiles per wavefront, stacked in v
