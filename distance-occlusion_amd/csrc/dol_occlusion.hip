@@ -151,11 +151,11 @@ __global__ void __launch_bounds__(64) k_occlusion_sweep(int w, int h, int d, int
             v = octant_combine(hl ? mine : 0.0f, hu ? up : 0.0f, hb ? B[k] : 0.0f, hl ? mine_in : 0.0f,
                                hu ? up_in : 0.0f, hb ? BI[k] : 0.0f);
         }
-        if (row_ok && a >= 0 && a < nu) {
-            if (update && (hl || hu || hb)) R[k] = v;   // the plane's corner cell stays
-            mine = R[k];
-            if (MODE == 2) mine_in = RI[k];
-        }
+        // selects rather than a branch: every lane runs the same instructions
+        const bool act = row_ok && a >= 0 && a < nu;
+        R[k] = act && update && (hl || hu || hb) ? v : R[k];   // the plane's corner cell stays
+        mine = act ? R[k] : mine;
+        if (MODE == 2) mine_in = act ? RI[k] : mine_in;
     }
     if (!update) return;
 #pragma unroll
@@ -222,10 +222,10 @@ __global__ void __launch_bounds__(kOB) k_occlusion_sweep_2d(int w, int h, int di
         const bool hl = u0 + a >= 1;
         const float q = hl ? quadrant_combine(mine, hu ? up : 0.0f) : quadrant_combine(up, 0.0f);
         const float v = R[s & (kOB - 1)] + (1.0f - R[s & (kOB - 1)]) * q;
-        if (row_ok && a >= 0 && a < nu) {
-            if (update && (hl || hu)) R[s & (kOB - 1)] = v;   // the plane's corner cell stays
-            mine = R[s & (kOB - 1)];
-        }
+        // selects rather than a branch: every lane runs the same instructions
+        const bool act = row_ok && a >= 0 && a < nu;
+        R[s & (kOB - 1)] = act && update && (hl || hu) ? v : R[s & (kOB - 1)];   // the plane's corner cell stays
+        mine = act ? R[s & (kOB - 1)] : mine;
     }
 #pragma unroll
     for (int k = 0; k < kOB; k++) tile[lane * kOBStride + ((k - lane) & (kOB - 1))] = R[k];

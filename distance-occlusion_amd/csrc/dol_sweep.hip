@@ -139,11 +139,12 @@ __global__ void __launch_bounds__(kST) k_fast_sweep_2d(int w, int h, int sweep, 
         const bool left = u0 + a >= 1;
         const uint32_t best = first_least(cur, sw_len2(cur, 0), c0, sw_len2(c0, 0), c1,
                                           left ? sw_len2(c1, 0) : kNoLen, c2, left ? sw_len2(c2, 0) : kNoLen);
-        if (row_ok && a >= 0 && a < nu) {
-            if (update) R[s & (kST - 1)] = best;
-            mine_prev = mine;
-            mine = R[s & (kST - 1)];
-        }
+        // selects rather than a branch: every lane runs the same instructions
+        const bool act = row_ok && a >= 0 && a < nu;
+        const uint32_t nxt = act && update ? best : R[s & (kST - 1)];
+        R[s & (kST - 1)] = nxt;
+        mine_prev = act ? mine : mine_prev;
+        mine = act ? nxt : mine;
     }
 #pragma unroll
     for (int k = 0; k < kST; k++) tile[lane * kSStride + rot(k, lane)] = R[k];
