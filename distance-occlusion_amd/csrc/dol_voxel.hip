@@ -12,9 +12,10 @@
 // without a test, as there.
 //
 // One wavefront takes one triangle at a time (grid-stride): every lane forms
-// the triangle's test data (the same values, so no exchange is needed), then
-// the lanes walk the range's cells 64 at a time and OR the set bits into the
-// mask with atomics (any order gives the same mask).  Every float operation
+// the triangle's test data (the same values, so no exchange is needed), the
+// lanes test the range's (y, z) rows 64 at a time, and for each passing row
+// they test its cells 64 at a time and OR the set bits into the mask with
+// atomics (any order gives the same mask).  Every float operation
 // is a separate rounded operation in the reference's order (no contraction:
 // the library is built with -ffp-contract=off); the grid's cell size, its
 // inverse and half-width come from the host, computed as there.
@@ -158,18 +159,47 @@ __global__ void __launch_bounds__(256) k_voxelise(int tri_count, const int* __re
         if (nx <= 0 || ny <= 0 || nz <= 0) continue;
         TriTest T;
         tri_setup(p1, p2, p3, g.hw, T);
-        const long long cells = (long long)nx * ny * nz;
-        for (long long j = lane; j < cells; j += 64) {
-            const int x = cx0 + (int)(j % nx);
-            const long long r = j / nx;
-            const int y = cy0 + (int)(r % ny), z = cz0 + (int)(r / ny);
-            V3 c;
-            c.z = g.cmin.z + g.cw.z * ((float)z + 0.5f);
-            if (!hit_slice(T, c)) continue;
-            c.y = g.cmin.y + g.cw.y * ((float)y + 0.5f);
-            if (!hit_row(T, c)) continue;
-            c.x = g.cmin.x + g.cw.x * ((float)x + 0.5f);
-            if (hit_cell(T, c)) atomicOr(mask + slice * z + row * y + (x >> 5), 1u << (x & 31));
+        // rows first, as the reference skips a failing slice or row: each lane
+        // tests one (y, z) row, then the wavefront walks the passing rows'
+        // cells 64 at a time
+        const int rows = ny * nz;
+        if ((long long)rows * nx <= 64) {
+            // a small range: one pass over its cells
+            const int j = lane;
+            if (j < rows * nx) {
+                const int x = cx0 + j % nx, r = j / nx;
+                const int y = cy0 + r % ny, z = cz0 + r / ny;
+                V3 c;
+                c.z = g.cmin.z + g.cw.z * ((float)z + 0.5f);
+                c.y = g.cmin.y + g.cw.y * ((float)y + 0.5f);
+                c.x = g.cmin.x + g.cw.x * ((float)x + 0.5f);
+                if (hit_slice(T, c) && hit_row(T, c) && hit_cell(T, c))
+                    atomicOr(mask + slice * z + row * y + (x >> 5), 1u << (x & 31));
+            }
+            continue;
+        }
+        for (int r0 = 0; r0 < rows; r0 += 64) {
+            const int r = r0 + lane;
+            bool pass = false;
+            if (r < rows) {
+                V3 c;
+                c.z = g.cmin.z + g.cw.z * ((float)(cz0 + r / ny) + 0.5f);
+                c.y = g.cmin.y + g.cw.y * ((float)(cy0 + r % ny) + 0.5f);
+                pass = hit_slice(T, c) && hit_row(T, c);
+            }
+            uint64_t live = __ballot(pass);
+            while (live) {
+                const int b = __ffsll((unsigned long long)live) - 1;
+                live &= live - 1;
+                const int rr = r0 + b, y = cy0 + rr % ny, z = cz0 + rr / ny;
+                V3 c;
+                c.z = g.cmin.z + g.cw.z * ((float)z + 0.5f);
+                c.y = g.cmin.y + g.cw.y * ((float)y + 0.5f);
+                for (int x = cx0 + lane; x < cx1; x += 64) {
+                    c.x = g.cmin.x + g.cw.x * ((float)x + 0.5f);
+                    if (hit_cell(T, c)) atomicOr(mask + slice * z + row * y + (x >> 5), 1u << (x & 31));
+                }
+            }
         }
     }
 }
