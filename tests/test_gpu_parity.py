@@ -304,6 +304,32 @@ def test_mask_from_triangles_large_triangle(gpu, oracle):
                                   oracle.mask_from_triangles(256, 256, 256, pos, None, bbox))
 
 
+def test_mask_from_triangles_mid_ranges(gpu, oracle, rng):
+    """Triangles whose cell ranges hold from a few to a few hundred cells (both
+    sides of the one-pass / rows-first switch at 64 cells), some flat in one
+    axis and some on cell-centre planes, on a 64 x 48 x 40 grid."""
+    w, h, d = 64, 48, 40
+    n = 3000
+    centre = rng.uniform(0, 1, (n, 1, 3))
+    size = rng.choice([0.02, 0.05, 0.1, 0.2], (n, 1, 1))
+    pos = centre + size * rng.normal(0, 1, (n, 3, 3))
+    flat = rng.integers(0, 4, n)                       # 3: not flat
+    for a in range(3):
+        sel = flat == a
+        pos[sel, :, a] = pos[sel, :1, a]              # one coordinate shared by the three vertices
+    dims = np.array([w, h, d])
+    on_centre = rng.random(n) < 0.25
+    a = flat[on_centre]
+    keep = a < 3
+    rows = np.nonzero(on_centre)[0][keep]
+    a = a[keep]
+    pos[rows, :, a] = ((np.floor(pos[rows, 0, a] * dims[a]) + 0.5) / dims[a])[:, None]
+    pos = pos.astype(np.float32).reshape(-1, 3)
+    bbox = np.array([0, 0, 0, 1, 1, 1], np.float32)
+    np.testing.assert_array_equal(gpu.mask_from_triangles(w, h, d, pos, None, bbox),
+                                  oracle.mask_from_triangles(w, h, d, pos, None, bbox))
+
+
 # ------------------------------------------------------------- two-field (-S) images
 
 
