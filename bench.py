@@ -522,7 +522,11 @@ def main():
         for _ in range(5):
             b.copy_(a)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_calib_bytes_each_way": 512 << 20, "launches": 5}))
+        # the same copy timed: the rate a plain device copy reaches here
+        # (bytes read + bytes written per second), beside the EDT passes'
+        ms = timed(lambda: b.copy_(a), 20, torch)
+        print(json.dumps({"copy_calib_bytes_each_way": 512 << 20, "launches": 5,
+                          "copy_ms": round(ms, 4), "copy_gb_per_s": round(2 * (512 << 20) / (ms * 1e-3) / 1e9, 1)}))
         return
 
     import dolhip
