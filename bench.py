@@ -393,7 +393,7 @@ def bench_sweep(L, torch, dev, mask3_t=None, cpu=True):
     ms = timed(step3, 2, torch)
     sweep_ms, _ = L.profile_query("fast_sweep_3d")
     L.profile_enable(False)
-    launches = 8 * (n // 8 + -(-n // 15) + n - 3)   # tiles of 8 columns x 15 rows, four a wavefront
+    launches = 8 * (n // 8 + -(-n // 15) + -(-(n - 1) // 2) - 2)   # tiles of 8 x 15 x 2 slices, four a wavefront
     out["fast_sweep_3d_512"] = {"ms": round(ms, 3), "sweep_ms": round(sweep_ms, 3),
                                 "gvoxels_per_s": round(n ** 3 / (sweep_ms * 1e-3) / 1e9, 3),
                                 "launches": launches, "us_per_launch": round(1e3 * sweep_ms / launches, 2)}

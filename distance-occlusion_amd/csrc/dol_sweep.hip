@@ -159,10 +159,10 @@ __global__ void __launch_bounds__(kST) k_fast_sweep_2d(int w, int h, int sweep, 
     }
 }
 
-// 3D: the slices add d to the number of diagonals (u + v + t), so short
-// tiles pay: a wavefront holds four tiles, one per 16-lane DPP row, each
-// kTW3 = 8 columns by kRows3 = 15 rows (lane 0 of the row carrying the row
-// before the tile, as in 2D), which is kTW3 + 15 steps a tile.  A lane keeps
+// 3D: the slices add d / kZC3 to the number of diagonals (u + v + t), so
+// short tiles pay: a wavefront holds four tiles, one per 16-lane DPP row,
+// each kTW3 = 8 columns by kRows3 = 15 rows (lane 0 of the row carrying the
+// row before the tile, as in 2D), which is kTW3 + 15 steps a tile slice.  A lane keeps
 // its row and the same row of the slice before in registers, rotated modulo
 // kTW3 by its place in the group, so the tile needs no LDS.
 constexpr int kTW3 = 8;
@@ -177,24 +177,33 @@ __device__ __forceinline__ uint32_t from_prev_lane16(uint32_t v)
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);
 }
 
-// 3D: the tiles (U, V, t) with U + V + t = diag; block b holds U = b % tu and
-// V = kTiles3 * (b / tu) + g in lane group g; the sweep's first slice (t = 0)
-// is never updated.
+// A tile takes kZC3 consecutive slices in one launch, one after the other:
+// everything it reads (the slice before its first, the row before and the
+// column before in each of its slices) is final when the launch starts, and
+// each slice's result is the next slice's "slice before", kept in registers.
+// That halves the launches, which the slices' chain otherwise dominates.
+constexpr int kZC3 = 2;
+
+// 3D: the tiles (U, V, T) with U + V + T = diag, tile T holding the slices
+// 1 + kZC3 T .. (the sweep's first slice, 0, is never updated); block b holds
+// U = b % tu and V = kTiles3 * (b / tu) + g in lane group g.
 __global__ void __launch_bounds__(kST) k_fast_sweep_3d(int w, int h, int d, int sweep, int cw, int diag,
                                                        int tu, int16_t* __restrict__ delta)
 {
     const int lane = threadIdx.x, j = lane % kGroup3;
     const int U = (int)blockIdx.x % tu, V = kTiles3 * ((int)blockIdx.x / tu) + lane / kGroup3;
-    const int t = diag - U - V;
+    const int T = diag - U - V;
+    const int t0 = 1 + T * kZC3;                          // the tile's first slice
     const Frame f = frame(w, h, sweep);
     const int dz = 1 - 2 * ((sweep >> 2) & 1);
-    const bool tile_ok = t >= 1 && t < d && V * kRows3 < h;
-    // a group without a tile reads slice 0 (and writes nothing)
-    const int z = tile_ok ? (dz > 0 ? 0 : d - 1) + t * dz : 0;
-    const int zb = tile_ok ? z - dz : 0;
+    const bool tile_ok = T >= 0 && t0 < d && V * kRows3 < h;
     const size_t slice = (size_t)w * h;
-    const int16_t* back = delta + 3 * (slice * (size_t)zb);
-    int16_t* cur_slice = delta + 3 * (slice * (size_t)z);
+    // slice t of the sweep; a group without a tile, and a slice past the
+    // volume's last, read slice 0 (and write nothing)
+    auto plane = [&](int t) {
+        const int z = tile_ok && t < d ? (dz > 0 ? 0 : d - 1) + t * dz : 0;
+        return delta + 3 * (slice * (size_t)z);
+    };
     const uint32_t ox = (uint16_t)(-cw * f.dx), oy = (uint32_t)(uint16_t)(-cw * f.dy) << 16;
     const int oz = -cw * dz;
     const int u0 = U * kTW3, v0 = V * kRows3;
@@ -206,60 +215,91 @@ __global__ void __launch_bounds__(kST) k_fast_sweep_3d(int w, int h, int d, int 
     auto at = [&](int c) { return 3 * (rbase + f.col(u0 + min(c, nu - 1))); };
     auto pack = [](const int16_t* c) { return (uint32_t)(uint16_t)c[0] | ((uint32_t)(uint16_t)c[1] << 16); };
     // register k holds column (k - j) mod kTW3, so step s reads register s mod kTW3
-    uint32_t R[kTW3], B[kTW3];
-    int RZ[kTW3], BZ[kTW3];
+    uint32_t R[kZC3][kTW3], B[kTW3];
+    int RZ[kZC3][kTW3], BZ[kTW3];
+    uint32_t first[kZC3];   // per slice, the column before the tile
+    int first_z[kZC3];
+    {
+        const int16_t* back = plane(t0 - 1);
 #pragma unroll
-    for (int k = 0; k < kTW3; k++) {
-        const size_t o = at((k - j) & (kTW3 - 1));
-        R[k] = pack(cur_slice + o);
-        RZ[k] = cur_slice[o + 2];
-        B[k] = pack(back + o);
-        BZ[k] = wrap16(back[o + 2] + oz);
-    }
-    uint32_t mine = 0u;   // the lane's last result; first, the column before the tile
-    int mine_z = 0;
-    if (u0 > 0 && row_ok) {
-        const int16_t* c = cur_slice + 3 * (rbase + f.col(u0 - 1));
-        mine = pack(c);
-        mine_z = c[2];
+        for (int k = 0; k < kTW3; k++) {
+            const size_t o = at((k - j) & (kTW3 - 1));
+            B[k] = pack(back + o);
+            BZ[k] = wrap16(back[o + 2] + oz);
+        }
     }
 #pragma unroll
-    for (int s = 0; s < kTW3 + kGroup3 - 1; s++) {
-        const int a = s - j;
-        const int k = s & (kTW3 - 1);
-        // (a, v - 1), finished by the lane before
-        const uint32_t up = from_prev_lane16(mine);
-        const int up_z = (int)(int16_t)from_prev_lane16((uint32_t)mine_z);
-        const uint32_t cxy = R[k];
-        const int cz = RZ[k];
-        const uint32_t uxy = sw_add(up, oy);     // the row before, (0, cy, 0)
-        const uint32_t lxy = sw_add(mine, ox);   // the cell before, (cx, 0, 0)
-        // candidates in the reference's order: the slice before (0, 0, cz),
-        // the row before, the cell before
-        const int l0 = sw_len2(cxy, cz * cz), l1 = sw_len2(B[k], BZ[k] * BZ[k]), l2 = sw_len2(uxy, up_z * up_z);
-        const int l3 = u0 + a >= 1 ? sw_len2(lxy, mine_z * mine_z) : kNoLen;
-        const int m = min(min(l0, l1), min(l2, l3));
-        const int pick = l0 == m ? 0 : (l1 == m ? 1 : (l2 == m ? 2 : 3));
-        const uint32_t nxy = pick == 0 ? cxy : (pick == 1 ? B[k] : (pick == 2 ? uxy : lxy));
-        const int nz = pick == 0 ? cz : (pick == 1 ? BZ[k] : (pick == 2 ? up_z : mine_z));
-        if (row_ok && a >= 0 && a < nu) {
-            if (update) {
-                R[k] = nxy;
-                RZ[k] = nz;
+    for (int zi = 0; zi < kZC3; zi++) {
+        const int16_t* cur = plane(t0 + zi);
+#pragma unroll
+        for (int k = 0; k < kTW3; k++) {
+            const size_t o = at((k - j) & (kTW3 - 1));
+            R[zi][k] = pack(cur + o);
+            RZ[zi][k] = cur[o + 2];
+        }
+        first[zi] = 0u;
+        first_z[zi] = 0;
+        if (u0 > 0 && row_ok) {
+            const int16_t* c = cur + 3 * (rbase + f.col(u0 - 1));
+            first[zi] = pack(c);
+            first_z[zi] = c[2];
+        }
+    }
+#pragma unroll
+    for (int zi = 0; zi < kZC3; zi++) {
+        const bool live = row_ok && t0 + zi < d;
+        const bool upd = update && t0 + zi < d;
+        uint32_t mine = first[zi];   // the lane's last result; first, the column before the tile
+        int mine_z = first_z[zi];
+#pragma unroll
+        for (int s = 0; s < kTW3 + kGroup3 - 1; s++) {
+            const int a = s - j;
+            const int k = s & (kTW3 - 1);
+            // (a, v - 1), finished by the lane before
+            const uint32_t up = from_prev_lane16(mine);
+            const int up_z = (int)(int16_t)from_prev_lane16((uint32_t)mine_z);
+            const uint32_t cxy = R[zi][k];
+            const int cz = RZ[zi][k];
+            const uint32_t uxy = sw_add(up, oy);     // the row before, (0, cy, 0)
+            const uint32_t lxy = sw_add(mine, ox);   // the cell before, (cx, 0, 0)
+            // candidates in the reference's order: the slice before (0, 0, cz),
+            // the row before, the cell before
+            const int l0 = sw_len2(cxy, cz * cz), l1 = sw_len2(B[k], BZ[k] * BZ[k]), l2 = sw_len2(uxy, up_z * up_z);
+            const int l3 = u0 + a >= 1 ? sw_len2(lxy, mine_z * mine_z) : kNoLen;
+            const int m = min(min(l0, l1), min(l2, l3));
+            const int pick = l0 == m ? 0 : (l1 == m ? 1 : (l2 == m ? 2 : 3));
+            const uint32_t nxy = pick == 0 ? cxy : (pick == 1 ? B[k] : (pick == 2 ? uxy : lxy));
+            const int nz = pick == 0 ? cz : (pick == 1 ? BZ[k] : (pick == 2 ? up_z : mine_z));
+            if (live && a >= 0 && a < nu) {
+                if (upd) {
+                    R[zi][k] = nxy;
+                    RZ[zi][k] = nz;
+                }
+                mine = R[zi][k];
+                mine_z = RZ[zi][k];
             }
-            mine = R[k];
-            mine_z = RZ[k];
+        }
+        // this slice's result, offset by one slice, is the next one's slice before
+#pragma unroll
+        for (int k = 0; k < kTW3; k++) {
+            B[k] = R[zi][k];
+            BZ[k] = wrap16(RZ[zi][k] + oz);
         }
     }
     if (!update) return;
 #pragma unroll
-    for (int k = 0; k < kTW3; k++) {
-        const int c = (k - j) & (kTW3 - 1);
-        if (c >= nu) continue;
-        int16_t* o = cur_slice + at(c);
-        o[0] = (int16_t)(R[k] & 0xffffu);
-        o[1] = (int16_t)(R[k] >> 16);
-        o[2] = (int16_t)RZ[k];
+    for (int zi = 0; zi < kZC3; zi++) {
+        if (t0 + zi >= d) break;
+        int16_t* cur = plane(t0 + zi);
+#pragma unroll
+        for (int k = 0; k < kTW3; k++) {
+            const int c = (k - j) & (kTW3 - 1);
+            if (c >= nu) continue;
+            int16_t* o = cur + at(c);
+            o[0] = (int16_t)(R[zi][k] & 0xffffu);
+            o[1] = (int16_t)(R[zi][k] >> 16);
+            o[2] = (int16_t)RZ[zi][k];
+        }
     }
 }
 
@@ -280,8 +320,10 @@ void launch_fast_sweep_3d(int w, int h, int d, int16_t* delta, int cw, hipStream
     ProfileScope p("fast_sweep_3d", s);
     const int tu = (w + kTW3 - 1) / kTW3, tv = (h + kRows3 - 1) / kRows3;
     const int vb = (tv + kTiles3 - 1) / kTiles3;   // wavefronts along v
+    const int tt = (d - 1 + kZC3 - 1) / kZC3;      // slice groups after slice 0
+    if (tt == 0) return;
     for (int sweep = 0; sweep < 8; sweep++)
-        for (int diag = 1; diag < tu + tv + d - 2; diag++)   // diagonal 0 is slice 0 only
+        for (int diag = 0; diag < tu + tv + tt - 2; diag++)
             hipLaunchKernelGGL(k_fast_sweep_3d, dim3((unsigned)(tu * vb)), dim3(kST), 0, s, w, h, d,
                                sweep, cw, diag, tu, delta);
 }
