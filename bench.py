@@ -474,7 +474,7 @@ def bench_occlusion(L, torch, dev, mask3_t=None, cpu=True):
     L.profile_enable(False)
     out["occlusion_3d_512"] = {"ms": round(ms, 3), "sweep_ms": round(sweep_ms, 3),
                                "gvoxels_per_s": round(n ** 3 / (sweep_ms * 1e-3) / 1e9, 3),
-                               "launches": n // 16 + -(-n // 15) + n - 2}
+                               "launches": n // 16 + -(-n // 15) + -(-n // 2) - 2}
     if cpu:
         k = 128
         m3 = mask3_t.cpu().numpy().view(np.uint32).reshape(n, n, n // 32)

@@ -90,7 +90,15 @@ __device__ __forceinline__ float from_prev_lane16(float v)
 
 // 3D.  MODE 1: in place (OcclusionSweep, :2569-2648); MODE 2: out from in
 // (:2650-2743; `in` read only).  Lane group g of block b: plane b / (vb tu),
-// r = b % (vb tu), U = r % tu, V = kOTiles * (r / tu) + g, t = diag - U - V.
+// r = b % (vb tu), U = r % tu, V = kOTiles * (r / tu) + g, slice group
+// T = diag - U - V (slices kOZ T ..).
+// A tile takes kOZ consecutive slices in one launch, one after the other:
+// what it reads from its neighbours is final when the launch starts, and
+// each slice's result (and, MODE 2, input) is the next slice's "slice
+// before", kept in registers.  That halves the launches, which the slices'
+// chain otherwise dominates.
+constexpr int kOZ = 2;
+
 template <int MODE>
 __global__ void __launch_bounds__(64) k_occlusion_sweep(int w, int h, int d, int diag, int tu, int vb,
                                                         const float* __restrict__ in,
@@ -100,13 +108,16 @@ __global__ void __launch_bounds__(64) k_occlusion_sweep(int w, int h, int d, int
     const int plane = (int)blockIdx.x / (vb * tu), r = (int)blockIdx.x % (vb * tu);
     const int V = kOTiles * (r / tu) + lane / kOGroup;
     const int Ud = r % tu;
-    const int t = diag - Ud - V;
+    const int T = diag - Ud - V;
+    const int t0 = T * kOZ;                               // the tile's first slice
     const size_t cells = (size_t)w * h * d, slice = (size_t)w * h;
     const int dx = 1 - 2 * (plane & 1), dy = 1 - 2 * ((plane >> 1) & 1), dz = 1 - 2 * ((plane >> 2) & 1);
     const int x0 = dx > 0 ? 0 : w - 1, y0 = dy > 0 ? 0 : h - 1, z0 = dz > 0 ? 0 : d - 1;
-    const bool tile_ok = Ud >= 0 && Ud < tu && t >= 0 && t < d && V * kORows < h;
+    const bool tile_ok = Ud >= 0 && Ud < tu && T >= 0 && t0 < d && V * kORows < h;
     const int U = tile_ok ? Ud : 0;   // a group without a tile reads tile 0 (and writes nothing)
-    const int z = tile_ok ? z0 + t * dz : 0, zb = tile_ok && t >= 1 ? z - dz : z;
+    // slice t of the sweep; slices outside the volume (the one before the
+    // first, one past the last) and groups without a tile read slice z0
+    auto zof = [&](int tt) { return tile_ok && tt >= 0 && tt < d ? z0 + tt * dz : z0; };
     float* out = dirw + (size_t)plane * cells;
     const float* inp = MODE == 2 ? in + (size_t)plane * cells : out;
     static_assert(MODE == 1 || MODE == 2, "3D modes");
@@ -118,50 +129,78 @@ __global__ void __launch_bounds__(64) k_occlusion_sweep(int w, int h, int d, int
     const size_t rb = (size_t)(y0 + min(max(vr, 0), h - 1) * dy) * w;
     auto at = [&](int zz, int c) { return (size_t)zz * slice + rb + (size_t)(x0 + (u0 + min(c, nu - 1)) * dx); };
     // register k holds column (k - j) mod kOW, so step s reads register s mod kOW
-    float R[kOW], B[kOW], RI[kOW], BI[kOW];
+    float R[kOZ][kOW], RI[kOZ][kOW], B[kOW], BI[kOW];
+    float first[kOZ], first_in[kOZ];   // per slice, the column before the tile
+    {
+        const int zb = zof(t0 - 1);
 #pragma unroll
-    for (int k = 0; k < kOW; k++) {
-        const int c = (k - j) & (kOW - 1);
-        R[k] = out[at(z, c)];
-        B[k] = out[at(zb, c)];
-        if (MODE == 2) {
-            RI[k] = inp[at(z, c)];
-            BI[k] = inp[at(zb, c)];
+        for (int k = 0; k < kOW; k++) {
+            const int c = (k - j) & (kOW - 1);
+            B[k] = out[at(zb, c)];
+            BI[k] = MODE == 2 ? inp[at(zb, c)] : 0.0f;
         }
     }
-    // the lane's last results (output and, MODE 2, input); first the column
-    // before the tile
-    float mine = 0.0f, mine_in = 0.0f;
-    if (u0 > 0 && row_ok) {
-        const size_t o = (size_t)z * slice + rb + (size_t)(x0 + (u0 - 1) * dx);
-        mine = out[o];
-        if (MODE == 2) mine_in = inp[o];
+#pragma unroll
+    for (int zi = 0; zi < kOZ; zi++) {
+        const int z = zof(t0 + zi);
+#pragma unroll
+        for (int k = 0; k < kOW; k++) {
+            const int c = (k - j) & (kOW - 1);
+            R[zi][k] = out[at(z, c)];
+            RI[zi][k] = MODE == 2 ? inp[at(z, c)] : 0.0f;
+        }
+        first[zi] = 0.0f;
+        first_in[zi] = 0.0f;
+        if (u0 > 0 && row_ok) {
+            const size_t o = (size_t)z * slice + rb + (size_t)(x0 + (u0 - 1) * dx);
+            first[zi] = out[o];
+            if (MODE == 2) first_in[zi] = inp[o];
+        }
     }
 #pragma unroll
-    for (int s = 0; s < kOW + kOGroup - 1; s++) {
-        const int a = s - j;
-        const int k = s & (kOW - 1);
-        const float up = from_prev_lane16(mine);
-        const float up_in = MODE == 2 ? from_prev_lane16(mine_in) : 0.0f;
-        const bool hl = u0 + a >= 1, hu = vr >= 1, hb = t >= 1;   // neighbours inside
-        float v;
-        if (MODE == 1) {
-            v = R[k] + (1.0f - R[k]) * octant_combine(hl ? mine : 0.0f, hu ? up : 0.0f, hb ? B[k] : 0.0f);
-        } else {
-            v = octant_combine(hl ? mine : 0.0f, hu ? up : 0.0f, hb ? B[k] : 0.0f, hl ? mine_in : 0.0f,
-                               hu ? up_in : 0.0f, hb ? BI[k] : 0.0f);
+    for (int zi = 0; zi < kOZ; zi++) {
+        const int t = t0 + zi;
+        const bool live = row_ok && t < d;
+        // the lane's last results (output and, MODE 2, input); first the
+        // column before the tile
+        float mine = first[zi], mine_in = first_in[zi];
+#pragma unroll
+        for (int s = 0; s < kOW + kOGroup - 1; s++) {
+            const int a = s - j;
+            const int k = s & (kOW - 1);
+            const float up = from_prev_lane16(mine);
+            const float up_in = MODE == 2 ? from_prev_lane16(mine_in) : 0.0f;
+            const bool hl = u0 + a >= 1, hu = vr >= 1, hb = t >= 1;   // neighbours inside
+            float v;
+            if (MODE == 1) {
+                v = R[zi][k] + (1.0f - R[zi][k]) * octant_combine(hl ? mine : 0.0f, hu ? up : 0.0f, hb ? B[k] : 0.0f);
+            } else {
+                v = octant_combine(hl ? mine : 0.0f, hu ? up : 0.0f, hb ? B[k] : 0.0f, hl ? mine_in : 0.0f,
+                                   hu ? up_in : 0.0f, hb ? BI[k] : 0.0f);
+            }
+            // selects rather than a branch: every lane runs the same instructions
+            const bool act = live && a >= 0 && a < nu;
+            R[zi][k] = act && update && (hl || hu || hb) ? v : R[zi][k];   // the plane's corner cell stays
+            mine = act ? R[zi][k] : mine;
+            if (MODE == 2) mine_in = act ? RI[zi][k] : mine_in;
         }
-        // selects rather than a branch: every lane runs the same instructions
-        const bool act = row_ok && a >= 0 && a < nu;
-        R[k] = act && update && (hl || hu || hb) ? v : R[k];   // the plane's corner cell stays
-        mine = act ? R[k] : mine;
-        if (MODE == 2) mine_in = act ? RI[k] : mine_in;
+        // this slice's result (and input) is the next one's slice before
+#pragma unroll
+        for (int k = 0; k < kOW; k++) {
+            B[k] = R[zi][k];
+            BI[k] = RI[zi][k];
+        }
     }
     if (!update) return;
 #pragma unroll
-    for (int k = 0; k < kOW; k++) {
-        const int c = (k - j) & (kOW - 1);
-        if (c < nu) out[at(z, c)] = R[k];
+    for (int zi = 0; zi < kOZ; zi++) {
+        if (t0 + zi >= d) break;
+        const int z = zof(t0 + zi);
+#pragma unroll
+        for (int k = 0; k < kOW; k++) {
+            const int c = (k - j) & (kOW - 1);
+            if (c < nu) out[at(z, c)] = R[zi][k];
+        }
     }
 }
 
@@ -257,7 +296,8 @@ void launch_occlusion_sweep(int mode, int w, int h, int d, const float* in, floa
     const int tu = (w + kOW - 1) / kOW, tv = (h + kORows - 1) / kORows;
     const int vb = (tv + kOTiles - 1) / kOTiles;
     const dim3 grid((unsigned)(8 * vb * tu));
-    for (int diag = 0; diag < tu + tv + d - 2; diag++) {
+    const int tt = (d + kOZ - 1) / kOZ;   // slice groups
+    for (int diag = 0; diag < tu + tv + tt - 2; diag++) {
         if (mode == 1)
             hipLaunchKernelGGL(k_occlusion_sweep<1>, grid, dim3(64), 0, s, w, h, d, diag, tu, vb, nullptr, dirw);
         else
