@@ -104,6 +104,24 @@ def bench_jfa3d(L, torch, dev, mask_t, w, h, d, fl):
     }
 
 
+def bench_glyphs_batched(L, torch, dev, args):
+    """The batched glyph launch alone (tuned variant, hipGraph replay): what
+    a kernel trace of the batch should see."""
+    import dolhip
+
+    gw, gh, goff, gdata = L.scene_glyphs(n=20000, min_size=16, max_size=256, seed=4)
+    b = dolhip.Batch(L, gw, gh, goff, threshold=128, less_than=False)
+    d_in = torch.from_numpy(gdata).to(dev)
+    out_b = torch.empty_like(d_in)
+    b.tune(d_in, out_b)
+    cells, waves = b.config()
+    b.run(d_in, out_b, graph=True)
+    torch.cuda.synchronize()
+    bms = timed(lambda: b.run(d_in, out_b, graph=True), 5, torch)
+    b.close()
+    return {"glyphs": len(gw), "batched_ms": round(bms, 3), "tuned": {"lds_cells": cells, "waves_per_group": waves}}
+
+
 def bench_glyphs(L, torch, dev, args):
     """20,000 uint8 masks, w and h uniform in 16..256: signed 8-bit SDF per
     mask, batched (one persistent launch, hipGraph replay, tuned variant)
@@ -252,6 +270,62 @@ def bench_config1(L, torch, dev):
     jfa()
     edt()
     return {"jfa_us": round(timed(jfa, 20, torch) * 1e3, 1), "edt_us": round(timed(edt, 20, torch) * 1e3, 1)}
+
+
+def bench_felzenszwalb(L, torch, dev):
+    """The reference's own Felzenszwalb (float squared distances, in place
+    after InitDistancesFromBitMask) and the exact int32 EDT on the same
+    masks: the 512 x 512 100-point mask and the 8192 x 8192 disc/box scene;
+    milliseconds per call, inputs resident in HBM."""
+    out = {}
+    flt_max = float(np.finfo(np.float32).max)
+    for name, w in (("512", 512), ("8192", 8192)):
+        h = w
+        if w == 512:
+            mask = np.zeros((w // 32) * h, np.uint32)
+            L.mask_add_points_2d(w, h, 100, 12345, mask)
+        else:
+            mask = L.scene_shapes_2d(w, h, 2000, seed=2)
+        m_t = torch.from_numpy(mask.view(np.int32)).to(dev)
+        f = torch.empty(w * h, dtype=torch.float32, device=dev)
+        fw = torch.empty(max(256, L.workspace("felzenszwalb_2d", w, h)), dtype=torch.uint8, device=dev)
+        d2 = torch.empty(w * h, dtype=torch.int32, device=dev)
+        fl = torch.empty(w * h, dtype=torch.float32, device=dev)
+        ew = torch.empty(max(256, L.workspace("edt_2d", w, h)), dtype=torch.uint8, device=dev)
+
+        def felz():
+            L.call_dev("init_distances_f32", w, h, m_t, f, flt_max)
+            L.call_dev("felzenszwalb_2d", w, h, f, fw)
+
+        def edt():
+            L.call_dev("edt_2d", w, h, m_t, 0, d2, fl, ew)
+
+        felz()
+        edt()
+        reps = 20 if w == 512 else 5
+        L.profile_enable(True)
+        fms = timed(felz, reps, torch)
+        ks = {}
+        for k in ("init_distances", "felzenszwalb_cols", "felzenszwalb_rows"):
+            avg, n = L.profile_query(k)
+            ks[k] = {"avg_ms": round(avg, 4), "launches": n}
+        L.profile_enable(False)
+        L.profile_enable(True)
+        ems = timed(edt, reps, torch)
+        ek = {}
+        for k in ("edt2d_xy", "edt_rows"):
+            avg, n = L.profile_query(k)
+            if n:
+                ek[k] = {"avg_ms": round(avg, 4), "launches": n}
+        L.profile_enable(False)
+        # float of the exact result = the reference's float values while dims <= 4096
+        out[f"felzenszwalb_{name}"] = {"ms": round(fms, 4), "gpixels_per_s": round(w * h / (fms * 1e-3) / 1e9, 3),
+                                       "kernels": ks}
+        out[f"edt2d_{name}"] = {"ms": round(ems, 4), "gpixels_per_s": round(w * h / (ems * 1e-3) / 1e9, 3),
+                                "kernels": ek}
+        del f, fw, d2, fl, ew
+        torch.cuda.empty_cache()
+    return out
 
 
 def bench_seeds(L, torch, dev):
@@ -506,7 +580,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only copy 512 MiB device to device 5 times (calibrates HBM byte counters)")
-    ap.add_argument("--only-extra", choices=["jfa3d", "glyphs", "jfa2d", "config1", "seeds", "twofield", "voxel", "sweep", "occlusion"],
+    ap.add_argument("--only-extra", choices=["jfa3d", "glyphs", "glyphs_batched", "jfa2d", "config1", "seeds", "twofield",
+                                             "voxel", "sweep", "occlusion", "felzenszwalb"],
                     help="run only this secondary workload and print its figures (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -542,6 +617,8 @@ def main():
 
     if args.only_extra and args.only_extra != "jfa3d":
         run = {"glyphs": lambda: bench_glyphs(L, torch, dev, args), "jfa2d": lambda: bench_jfa2d(L, torch, dev),
+               "glyphs_batched": lambda: bench_glyphs_batched(L, torch, dev, args),
+               "felzenszwalb": lambda: bench_felzenszwalb(L, torch, dev),
                "config1": lambda: bench_config1(L, torch, dev), "seeds": lambda: bench_seeds(L, torch, dev),
                "twofield": lambda: bench_two_field(L, torch, dev),
                "voxel": lambda: bench_voxel(L, torch, dev, cpu=not args.no_cpu),

@@ -1480,9 +1480,14 @@ long oc_check_deltas_3d(int w, int h, int d, const int16_t* delta, int cw)
 /* --------------------------------------------- continuous seeds (ext.) */
 
 /* Nearest seed to each cell centre (x, y), in float64; ties go to the lowest
- * seed index.  seeds holds n (x, y) float pairs. */
-void oc_seed_nearest_2d(int w, int h, int n, const float* seeds, int32_t* idx, double* dist)
+ * seed index.  seeds holds n (x, y) float pairs.  Rows are split over
+ * `threads` OpenMP threads (each cell's search is independent). */
+void oc_seed_nearest_2d_mt(int w, int h, int n, const float* seeds, int32_t* idx, double* dist,
+                           int threads)
 {
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(threads > 1 ? threads : 1) schedule(dynamic, 4)
+#endif
     for (int y = 0; y < h; y++)
         for (int x = 0; x < w; x++) {
             double best = INFINITY;
@@ -1495,6 +1500,12 @@ void oc_seed_nearest_2d(int w, int h, int n, const float* seeds, int32_t* idx, d
             idx[(size_t)y * w + x] = bi;
             dist[(size_t)y * w + x] = sqrt(best);
         }
+    (void)threads;
+}
+
+void oc_seed_nearest_2d(int w, int h, int n, const float* seeds, int32_t* idx, double* dist)
+{
+    oc_seed_nearest_2d_mt(w, h, n, seeds, idx, dist, 1);
 }
 
 /* d(sum w*d)/dp_k = sum over cells of seed k of -w (c - p_k) / d; zero where d == 0 */
@@ -1513,4 +1524,36 @@ void oc_seed_grad_2d(int w, int h, int n, const float* seeds, const int32_t* idx
             grad[2 * k] -= (double)weight[i] * dx / dd;
             grad[2 * k + 1] -= (double)weight[i] * dy / dd;
         }
+}
+
+/* ------------------------------------------------ batched glyphs (ext.) */
+
+/* Signed 8-bit SDF of each of n byte masks, the tool's border path run mask
+ * by mask: BitMaskFromImageFile's threshold (DistanceOcclusionTool.cpp:646-665),
+ * InitBorderDeltasFromBitMask + JumpFlood(cw = 2)
+ * (DistanceOcclusionTool.cpp:1333-1365) and EncodeU8Signed of
+ * Dist / (2 maxLen) with maxLen = 0.25 max(w, h)
+ * (DistanceOcclusionTool.cpp:1242-1271, 1973-1974).  Mask i (w[i] x h[i]
+ * bytes) starts at offset[i] of bytes and of out.  Masks are split over
+ * `threads` OpenMP threads. */
+void oc_glyph_sdf_batch(int n, const int32_t* w, const int32_t* h, const int64_t* offset,
+                        const uint8_t* bytes, int threshold, int less_than, uint8_t* out,
+                        int threads)
+{
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(threads > 1 ? threads : 1) schedule(dynamic, 1)
+#endif
+    for (int i = 0; i < n; i++) {
+        int gw = w[i], gh = h[i];
+        size_t cells = (size_t)gw * gh;
+        uint32_t* mask = (uint32_t*)calloc((size_t)mask_stride(gw) * gh + 1, sizeof(uint32_t));
+        int16_t* delta = (int16_t*)malloc(cells * 2 * sizeof(int16_t) + 4);
+        oc_mask_from_bytes(gw, gh, bytes + offset[i], threshold, less_than, mask);
+        oc_border_2d(gw, gh, mask, delta);
+        oc_jfa_2d(gw, gh, delta, 2);
+        oc_sdf_u8_2d(gw, gh, mask, delta, 0.5f * (float)(gw > gh ? gw : gh), out + offset[i]);
+        free(delta);
+        free(mask);
+    }
+    (void)threads;
 }

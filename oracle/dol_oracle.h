@@ -113,8 +113,16 @@ long oc_check_deltas_3d(int w, int h, int d, const int16_t* delta, int cw);
 
 /* continuous seeds (extension): nearest seed index + distance, float64 */
 void oc_seed_nearest_2d(int w, int h, int n, const float* seeds, int32_t* idx, double* dist);
+void oc_seed_nearest_2d_mt(int w, int h, int n, const float* seeds, int32_t* idx, double* dist,
+                           int threads);
 void oc_seed_grad_2d(int w, int h, int n, const float* seeds, const int32_t* idx,
                      const float* weight, double* grad);
+
+/* batched glyphs (extension): threshold, border init, JumpFlood(cw = 2) and
+ * the signed byte image of each ragged mask */
+void oc_glyph_sdf_batch(int n, const int32_t* w, const int32_t* h, const int64_t* offset,
+                        const uint8_t* bytes, int threshold, int less_than, uint8_t* out,
+                        int threads);
 
 #ifdef __cplusplus
 }

@@ -297,12 +297,24 @@ class Oracle:
     def check_deltas_3d(self, w, h, d, delta, cw=1):
         return self.L.oc_check_deltas_3d(_I(w), _I(h), _I(d), _ptr(np.ascontiguousarray(delta)), _I(cw))
 
-    def seed_nearest_2d(self, w, h, seeds):
+    def seed_nearest_2d(self, w, h, seeds, threads=1):
         seeds = np.ascontiguousarray(seeds, dtype=np.float32)
         idx = np.empty((h, w), dtype=np.int32)
         dist = np.empty((h, w), dtype=np.float64)
-        self.L.oc_seed_nearest_2d(_I(w), _I(h), _I(seeds.shape[0]), _ptr(seeds), _ptr(idx), _ptr(dist))
+        self.L.oc_seed_nearest_2d_mt(_I(w), _I(h), _I(seeds.shape[0]), _ptr(seeds), _ptr(idx), _ptr(dist),
+                                     _I(threads))
         return idx, dist
+
+    def glyph_sdf_batch(self, widths, heights, offsets, data, threshold=128, less_than=False, threads=1):
+        """signed byte SDF of every ragged glyph mask (threshold, border init, JumpFlood cw = 2, encode)"""
+        w = np.ascontiguousarray(widths, dtype=np.int32)
+        h = np.ascontiguousarray(heights, dtype=np.int32)
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        out = np.empty_like(data)
+        self.L.oc_glyph_sdf_batch(_I(len(w)), _ptr(w), _ptr(h), _ptr(off), _ptr(data), _I(threshold),
+                                  _I(int(less_than)), _ptr(out), _I(threads))
+        return out
 
     def seed_grad_2d(self, w, h, seeds, idx, weight):
         seeds = np.ascontiguousarray(seeds, dtype=np.float32)
@@ -431,6 +443,12 @@ class Reference:
 
 def load_oracle():
     return Oracle()
+
+
+def host_threads():
+    """threads for the oracle's parallel loops: the host's cores, at most 16"""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return max(1, min(16, n))
 
 
 def load_reference():

@@ -125,6 +125,118 @@ def test_published_images(oracle, samples):
         assert np.abs(rg).max() <= 1 and (rg != 0).any(-1).sum() <= limits[name][1]
 
 
+def test_published_images_exact(oracle, samples):
+    """The published images are the sweep method's: border init, FastSweep
+    in half cells (cw = 2) and the tool's encoders reproduce every pixel.
+    The sdf and sdf-rg images use the default maxLen = 0.25 max(w, h)
+    (DistanceOcclusionTool.cpp:1973-1974, doubled at :1364); the sdf-delta
+    images were written with -l 127 (divisor 254), so every channel value is
+    lrintf(128 + x / 2) of an odd half-cell offset x: exact halves rounded
+    to even, which pins the rounding of EncodeU8Signed as well."""
+    for name in ("tiger", "kiwi"):
+        g = samples[f"{name}_grey"]
+        h, w = g.shape
+        m = oracle.mask_from_bytes(w, h, g, 128, True)
+        d = oracle.fast_sweep_2d(w, h, oracle.border_2d(w, h, m), 2)
+        max_len = 2 * 0.25 * max(w, h)
+        np.testing.assert_array_equal(oracle.sdf_u8_2d(w, h, m, d, max_len), samples[f"{name}_sdf"])
+        np.testing.assert_array_equal(oracle.sdf_rg_rgba_2d(w, h, m, d, max_len), samples[f"{name}_sdf_rg"])
+        np.testing.assert_array_equal(oracle.sdf_delta_rgba_2d(w, h, d, 2 * 127.0), samples[f"{name}_sdf_delta"])
+        # and a divisor one step away changes many pixels
+        assert (oracle.sdf_delta_rgba_2d(w, h, d, 254.5) != samples[f"{name}_sdf_delta"]).sum() > 1000
+
+
+def _signed_u8(v):
+    """EncodeU8Signed in float32 (DistanceOcclusionTool.cpp:71-75)"""
+    v = np.clip(np.asarray(v, np.float32), -1, 1).astype(np.float32)
+    return np.rint(np.float32(128) + v * np.float32(127)).astype(np.uint8)
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_signed_dist_2d_half_planes(oracle, axis):
+    """A half plane (x >= c or y >= c set): the border method's deltas are
+    (+-(2k + 1), 0) half cells, so a cell k cells from the edge is at
+    exactly k + 0.5 cells, negative inside; the sdf byte is
+    EncodeU8Signed(+-(2k + 1) / max_len)."""
+    w, h, c = 41, 29, 13
+    yy, xx = np.indices((h, w))
+    coord = xx if axis == 0 else yy
+    inside = coord >= c
+    m = pack_mask(inside)
+    d = oracle.jfa_2d(w, h, oracle.border_2d(w, h, m), 2)
+    k = np.where(inside, coord - c, c - 1 - coord)
+    exp = np.where(inside, -(k + 0.5), k + 0.5).astype(np.float32)
+    np.testing.assert_array_equal(oracle.signed_dist_2d(w, h, m, d), exp)
+    max_len = 20.0
+    np.testing.assert_array_equal(oracle.sdf_u8_2d(w, h, m, d, max_len),
+                                  _signed_u8(np.where(inside, -1, 1) * (2 * k + 1) / np.float32(max_len)))
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_signed_dist_3d_half_spaces(oracle, axis):
+    """As in 2D, for x, y and z half spaces of a volume: signed distances
+    +-(k + 0.5) and their sdf bytes."""
+    w, h, d, c = 19, 13, 11, 5
+    zz, yy, xx = np.indices((d, h, w))
+    coord = (xx, yy, zz)[axis]
+    inside = coord >= c
+    m = pack_mask(inside)
+    delta = oracle.jfa_3d(w, h, d, oracle.border_3d(w, h, d, m), 2)
+    k = np.where(inside, coord - c, c - 1 - coord)
+    exp = np.where(inside, -(k + 0.5), k + 0.5).astype(np.float32)
+    np.testing.assert_array_equal(oracle.signed_dist_3d(w, h, d, m, delta), exp)
+    max_len = 12.0
+    np.testing.assert_array_equal(oracle.sdf_u8_3d(w, h, d, m, delta, max_len),
+                                  _signed_u8(np.where(inside, -1, 1) * (2 * k + 1) / np.float32(max_len)))
+
+
+def test_signed_3d_one_slice_equals_2d(oracle, samples):
+    """A volume one slice deep whose deltas are the 2D deltas with z = 0
+    gives the 2D signed distances and sdf bytes (the tiger sample)."""
+    g = samples["tiger_grey"]
+    h, w = g.shape
+    m = oracle.mask_from_bytes(w, h, g, 128, True)
+    d2 = oracle.jfa_2d(w, h, oracle.border_2d(w, h, m), 2)
+    d3 = np.zeros((1, h, w, 3), np.int16)
+    d3[0, :, :, :2] = d2
+    np.testing.assert_array_equal(oracle.signed_dist_3d(w, h, 1, m, d3)[0], oracle.signed_dist_2d(w, h, m, d2))
+    np.testing.assert_array_equal(oracle.sdf_u8_3d(w, h, 1, m, d3, 155.0)[0], oracle.sdf_u8_2d(w, h, m, d2, 155.0))
+    # the same field spread over z instead of y: a volume 1 wide, (y, x) -> (z, y)
+    d3z = np.zeros((h, w, 1, 3), np.int16)
+    d3z[:, :, 0, 1] = d2[..., 0]
+    d3z[:, :, 0, 2] = d2[..., 1]
+    mz = pack_mask(unpack_mask(m, w, h)[:, :, None])
+    np.testing.assert_array_equal(oracle.signed_dist_3d(1, w, h, mz, d3z)[:, :, 0], oracle.signed_dist_2d(w, h, m, d2))
+
+
+def test_check_deltas_finds_planted_errors(oracle):
+    """CheckDeltas counts exactly the cells whose delta misses a boundary
+    cell: planted errors (pointing at an empty cell, and out of bounds) are
+    found, one count each."""
+    w = h = 64
+    m = empty_mask(w, h)
+    oracle.mask_add_points_2d(w, h, 12, 7, m)
+    good = oracle.jfa_2d(w, h, oracle.init_deltas_2d(w, h, m), 1)
+    assert oracle.check_deltas_2d(w, h, good, 1) == 0
+    bad = good.copy()
+    empty = np.argwhere(~unpack_mask(m, w, h))
+    (y0, x0), (y1, x1) = empty[5], empty[400]
+    bad[y0, x0] = (x1 - x0, y1 - y0)          # lands on a cell that is not a seed
+    bad[3, 3] = (-10, 0)                      # leaves the image
+    assert oracle.check_deltas_2d(w, h, bad, 1) == 2
+    d = 9
+    m3 = empty_mask(w, h, d)
+    oracle.mask_add_points_3d(w, h, d, 20, 3, m3)
+    good3 = oracle.jfa_3d(w, h, d, oracle.init_deltas_3d(w, h, d, m3), 1)
+    assert oracle.check_deltas_3d(w, h, d, good3, 1) == 0
+    bad3 = good3.copy()
+    bad3[2, 2, 2] = (0, 0, 40)                # beyond the last slice
+    empty3 = np.argwhere(~unpack_mask(m3, w, h, d))
+    (z0, y0, x0), (z1, y1, x1) = empty3[1000], empty3[9000]
+    bad3[z0, y0, x0] = (x1 - x0, y1 - y0, z1 - z0)
+    assert oracle.check_deltas_3d(w, h, d, bad3, 1) == 2
+
+
 # ------------------------------------------------ live against oracle/_ref
 
 
