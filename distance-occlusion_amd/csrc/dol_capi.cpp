@@ -35,7 +35,8 @@ void launch_encode_ao(size_t n, int planes, bool rgba, const float* dirw, uint8_
 void launch_fast_sweep_3d(int w, int h, int d, int16_t* delta, int cw, hipStream_t s);
 void launch_jfa2d_pass(int step, int w, int h, const int16_t* in, int16_t* out, int cw,
                        hipStream_t s);
-void jfa3d_run(int w, int h, int d, int16_t* delta, int cw, int16_t* tmp, hipStream_t s);
+size_t jfa3d_workspace(int w, int h, int d);
+void jfa3d_run(int w, int h, int d, int16_t* delta, int cw, void* work, hipStream_t s);
 void launch_jfa3d_pass(int step, int w, int h, int d, const int16_t* in, int16_t* out, int cw,
                        hipStream_t s);
 size_t felz_workspace(int w, int h);
@@ -142,41 +143,55 @@ static int require_device()
     return DOLHIP_OK;
 }
 
-// per-device work buffer that grows as needed
+// Library-owned work buffers for _dev calls made with work == NULL: one per
+// (device, stream), since work on one stream runs in order and can share a
+// buffer while work on two streams may overlap.  A buffer that is too small
+// is replaced by a larger one, and the old one is kept until the process
+// ends rather than freed: work already enqueued with it, or about to be
+// enqueued by a thread that obtained it, may still use it.
 struct WorkBuf {
-    void* ptr = nullptr;
-    size_t bytes = 0;
+    int dev;
+    void* stream;
+    void* ptr;
+    size_t bytes;
 };
 static std::vector<WorkBuf> g_work;
+static std::vector<void*> g_retired_work;
 
-static int library_work(size_t bytes, void** out)
+static int library_work(size_t bytes, void* stream, void** out)
 {
     int dev = 0;
     TRY_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_mutex);
-    if ((int)g_work.size() <= dev) g_work.resize(dev + 1);
-    WorkBuf& wb = g_work[dev];
-    if (wb.bytes < bytes) {
-        if (wb.ptr) {
-            TRY_HIP(hipDeviceSynchronize());
-            TRY_HIP(hipFree(wb.ptr));
-            wb.ptr = nullptr;
-            wb.bytes = 0;
+    WorkBuf* wb = nullptr;
+    for (auto& b : g_work)
+        if (b.dev == dev && b.stream == stream) {
+            wb = &b;
+            break;
         }
-        TRY_HIP(hipMalloc(&wb.ptr, bytes));
-        wb.bytes = bytes;
+    if (!wb) {
+        g_work.push_back({dev, stream, nullptr, 0});
+        wb = &g_work.back();
     }
-    *out = wb.ptr;
+    if (wb->bytes < bytes) {
+        const size_t want = bytes > 2 * wb->bytes ? bytes : 2 * wb->bytes;
+        void* p = nullptr;
+        TRY_HIP(hipMalloc(&p, want));
+        if (wb->ptr) g_retired_work.push_back(wb->ptr);
+        wb->ptr = p;
+        wb->bytes = want;
+    }
+    *out = wb->ptr;
     return DOLHIP_OK;
 }
 
-static int resolve_work(void* given, size_t bytes, void** out)
+static int resolve_work(void* given, size_t bytes, void* stream, void** out)
 {
     if (given) {
         *out = given;
         return DOLHIP_OK;
     }
-    return library_work(bytes, out);
+    return library_work(bytes, stream, out);
 }
 
 // RAII device buffer for the host-pointer entry points
@@ -545,7 +560,7 @@ int dolhip_jump_flood_2d_dev(int w, int h, int16_t* delta, int cw, void* work, v
     TRY(check_cw(cw));
     TRY(require_device());
     void* tmp = nullptr;
-    TRY(resolve_work(work, dolhip_jump_flood_2d_workspace(w, h), &tmp));
+    TRY(resolve_work(work, dolhip_jump_flood_2d_workspace(w, h), stream, &tmp));
     jfa2d_schedule(w, h, delta, cw, (int16_t*)tmp, as_stream(stream));
     return finish(as_stream(stream), "jump_flood_2d");
 }
@@ -567,7 +582,7 @@ int dolhip_jump_flood_2d(int w, int h, int16_t* delta, int cw)
 
 size_t dolhip_jump_flood_3d_workspace(int w, int h, int d)
 {
-    return (w < 1 || h < 1 || d < 1) ? 0 : (size_t)w * h * d * 6;
+    return (w < 1 || h < 1 || d < 1) ? 0 : jfa3d_workspace(w, h, d);
 }
 
 int dolhip_jump_flood_3d_dev(int w, int h, int d, int16_t* delta, int cw, void* work,
@@ -578,8 +593,8 @@ int dolhip_jump_flood_3d_dev(int w, int h, int d, int16_t* delta, int cw, void* 
     TRY(check_cw(cw));
     TRY(require_device());
     void* tmp = nullptr;
-    TRY(resolve_work(work, dolhip_jump_flood_3d_workspace(w, h, d), &tmp));
-    jfa3d_run(w, h, d, delta, cw, (int16_t*)tmp, as_stream(stream));
+    TRY(resolve_work(work, dolhip_jump_flood_3d_workspace(w, h, d), stream, &tmp));
+    jfa3d_run(w, h, d, delta, cw, tmp, as_stream(stream));
     return finish(as_stream(stream), "jump_flood_3d");
 }
 
@@ -592,8 +607,8 @@ int dolhip_jump_flood_3d(int w, int h, int d, int16_t* delta, int cw)
     const size_t bytes = (size_t)w * h * d * 6;
     DevBuf a, t;
     TRY(a.upload(delta, bytes));
-    TRY(t.alloc(bytes));
-    jfa3d_run(w, h, d, a.as<int16_t>(), cw, t.as<int16_t>(), 0);
+    TRY(t.alloc(dolhip_jump_flood_3d_workspace(w, h, d)));
+    jfa3d_run(w, h, d, a.as<int16_t>(), cw, t.p, 0);
     TRY(finish_sync("jump_flood_3d"));
     return a.download(delta, bytes);
 }
@@ -844,7 +859,7 @@ int dolhip_felzenszwalb_2d_dev(int w, int h, float* dist, void* work, void* stre
     TRY(check_ptr(dist, "dist"));
     TRY(require_device());
     void* tmp = nullptr;
-    TRY(resolve_work(work, felz_workspace(w, h), &tmp));
+    TRY(resolve_work(work, felz_workspace(w, h), stream, &tmp));
     launch_felzenszwalb_2d(w, h, dist, tmp, as_stream(stream));
     return finish(as_stream(stream), "felzenszwalb_2d");
 }
@@ -883,7 +898,7 @@ int dolhip_edt_2d_dev(int w, int h, const uint32_t* mask, int invert, int32_t* d
     if (!d2 && !dist) return fail(DOLHIP_ERR_INVALID_ARG, "d2 and dist are both null");
     TRY(require_device());
     void* tmp = nullptr;
-    TRY(resolve_work(work, dolhip_edt_2d_workspace(w, h), &tmp));
+    TRY(resolve_work(work, dolhip_edt_2d_workspace(w, h), stream, &tmp));
     launch_edt_2d(w, h, mask, invert, d2, dist, tmp, as_stream(stream));
     return finish(as_stream(stream), "edt_2d");
 }
@@ -915,7 +930,7 @@ int dolhip_edt_3d_dev(int w, int h, int d, const uint32_t* mask, int invert, int
     if (!d2 && !dist) return fail(DOLHIP_ERR_INVALID_ARG, "d2 and dist are both null");
     TRY(require_device());
     void* tmp = nullptr;
-    TRY(resolve_work(work, dolhip_edt_3d_workspace(w, h, d), &tmp));
+    TRY(resolve_work(work, dolhip_edt_3d_workspace(w, h, d), stream, &tmp));
     launch_edt_3d(w, h, d, mask, invert, d2, dist, tmp, as_stream(stream));
     return finish(as_stream(stream), "edt_3d");
 }
@@ -948,7 +963,7 @@ static int signed_edt_dev(int w, int h, int d, const uint32_t* mask, float* sdf,
     const size_t field = ((size_t)cells * 4 + 255) & ~(size_t)255;
     const size_t inner = d == 1 ? dolhip_edt_2d_workspace(w, h) : dolhip_edt_3d_workspace(w, h, d);
     void* tmp = nullptr;
-    TRY(resolve_work(work, inner + 2 * field, &tmp));
+    TRY(resolve_work(work, inner + 2 * field, (void*)s, &tmp));
     int32_t* fout = (int32_t*)((char*)tmp + inner);
     int32_t* fin = (int32_t*)((char*)tmp + inner + field);
     if (d == 1) {
@@ -1148,7 +1163,7 @@ int dolhip_signed_border_2d_dev(int w, int h, const uint32_t* mask, int16_t* del
     TRY(check_cells2(delta, "delta"));
     TRY(require_device());
     void* tmp = nullptr;
-    TRY(resolve_work(work, dolhip_jump_flood_2d_workspace(w, h), &tmp));
+    TRY(resolve_work(work, dolhip_jump_flood_2d_workspace(w, h), stream, &tmp));
     launch_border_2d(w, h, mask, delta, as_stream(stream));
     jfa2d_schedule(w, h, delta, 2, (int16_t*)tmp, as_stream(stream));
     return finish(as_stream(stream), "signed_border_2d");
@@ -1162,9 +1177,9 @@ int dolhip_signed_border_3d_dev(int w, int h, int d, const uint32_t* mask, int16
     TRY(check_ptr(delta, "delta"));
     TRY(require_device());
     void* tmp = nullptr;
-    TRY(resolve_work(work, dolhip_jump_flood_3d_workspace(w, h, d), &tmp));
+    TRY(resolve_work(work, dolhip_jump_flood_3d_workspace(w, h, d), stream, &tmp));
     launch_border_3d(w, h, d, mask, delta, as_stream(stream));
-    jfa3d_run(w, h, d, delta, 2, (int16_t*)tmp, as_stream(stream));
+    jfa3d_run(w, h, d, delta, 2, tmp, as_stream(stream));
     return finish(as_stream(stream), "signed_border_3d");
 }
 
@@ -1194,7 +1209,7 @@ int dolhip_two_field_deltas_2d_dev(int w, int h, const uint32_t* mask, int16_t* 
     TRY(check_cw(cw));
     TRY(require_device());
     void* tmp = nullptr;
-    TRY(resolve_work(work, dolhip_two_field_2d_workspace(w, h), &tmp));
+    TRY(resolve_work(work, dolhip_two_field_2d_workspace(w, h), stream, &tmp));
     const hipStream_t s = as_stream(stream);
     launch_init_deltas_2d(w, h, mask, delta0, 0, s);
     launch_init_deltas_2d(w, h, mask, delta1, 1, s);
@@ -1235,12 +1250,12 @@ int dolhip_two_field_deltas_3d_dev(int w, int h, int d, const uint32_t* mask, in
     TRY(check_cw(cw));
     TRY(require_device());
     void* tmp = nullptr;
-    TRY(resolve_work(work, dolhip_two_field_3d_workspace(w, h, d), &tmp));
+    TRY(resolve_work(work, dolhip_two_field_3d_workspace(w, h, d), stream, &tmp));
     const hipStream_t s = as_stream(stream);
     launch_init_deltas_3d(w, h, d, mask, delta0, 0, s);
     launch_init_deltas_3d(w, h, d, mask, delta1, 1, s);
-    jfa3d_run(w, h, d, delta0, cw, (int16_t*)tmp, s);
-    jfa3d_run(w, h, d, delta1, cw, (int16_t*)tmp, s);
+    jfa3d_run(w, h, d, delta0, cw, tmp, s);
+    jfa3d_run(w, h, d, delta1, cw, tmp, s);
     return finish(s, "two_field_deltas_3d");
 }
 

@@ -9,6 +9,8 @@
 // odd so the result ends in the caller's buffer) is the host driver's.
 #include "dol_cells.hpp"
 
+#include <type_traits>
+
 namespace dolhip {
 
 // ------------------------------------------------------------------ 2D
@@ -148,21 +150,55 @@ __global__ void __launch_bounds__(256) k_jfa3d(int step, int w, int h, int d, in
     o[0] = (int16_t)bx; o[1] = (int16_t)by; o[2] = (int16_t)bz;
 }
 
-// Row-staged form, for widths w % 8 == 0 up to kRowMax: one workgroup per
-// output row (y, z) copies the (up to) 9 input rows (y + jc*step,
-// z + kc*step) whole into LDS with 16-byte loads, then each thread reads its
-// taps from LDS.  Same taps, order, skips and arithmetic as k_jfa3d.
+// 3D passes for widths w % 8 == 0 up to kRowMax (k_jfa3d_ring below): a
+// workgroup takes a run of output rows, each lane two columns of a row.
+// Same taps, order, skips and results as k_jfa3d.
 //
 // Between passes the field is kept in the same 6 bytes a cell, split into
 // two planes: packed (x, y) int16 pairs for all cells, then z for all cells.
-// A tap then costs one 32-bit and one 16-bit LDS read, x and y are offset
-// with one packed 16-bit add (wrapping as the reference's int16 store does).
-// The first pass reads the caller's interleaved cCellDelta3 layout (and
-// converts the staged rows in LDS) and the last writes it back.
+// The first pass reads the caller's interleaved cCellDelta3 layout and the
+// last writes it back.
+//
+// Each pass also writes one flag byte per output row and 64-cell slot:
+// kRowUnbounded if a cell that is not kMaxDelta3 has a component beyond
+// kFastBound, kRowHasMax if a cell is kMaxDelta3 (k_jfa3d_classify writes
+// them for the caller's field before the first pass).  A run whose source
+// rows' flags show no unbounded cell, with 54 * cw * step within int16,
+// takes the keyed path; otherwise the general path, which follows the
+// reference's arithmetic including its int16 and int32 wraps.
+//
+// Keyed path.  With |components| <= kFastBound and cw * step <= 606 no
+// candidate wraps and every candidate length is below 2^24, so a tap's
+// outcome is the order of its key 27 * len + rank, where rank is the tap's
+// place in the reference's order with the cell itself first (rank 0; the
+// strict ">" keeps the earliest of equal lengths).  The least key wins.  A
+// staged source s on z-row kc holds
+//     Bb = 27 * (sx^2 + sy^2 + (sz + kc*off)^2)
+// and tap (ic, jc) adds K = 27 * ((ic*off)^2 + (jc*off)^2) + rank and the
+// cross term 54 * off * (ic*sx + jc*sy), with one v_dot2_i32_i16 of (sx, sy)
+// and the packed constant (54 ic off, 54 jc off):
+//     key = 27 * ((sx + ic*off)^2 + (sy + jc*off)^2 + (sz + kc*off)^2) + rank.
+// A kMaxDelta3 source (which the reference skips) is staged with Bb =
+// kPoisonB and (sx, sy) = 0, so its key is above every other; so are rows
+// outside the volume and the poison column at index w that taps beyond
+// either end of the row read.  A kMaxDelta3 cell itself has the key
+// kSelfMaxKey (above every real candidate, below every skipped one), or 0
+// where its own 32-bit length wraps negative (then nothing replaces it, as
+// in the reference).  The winner is decoded from the key's rank and rebuilt
+// from its source.
 constexpr int kRowMax = 1024;
+constexpr int kFlagSlots = 16;        // flag bytes a row: one per wave of up to kRowMax cells
+constexpr int kFastBound = 2048;
+constexpr uint32_t kRowUnbounded = 1, kRowHasMax = 2;
+constexpr int kKeyScale = 27;
+constexpr uint32_t kPoisonB = 2700000000u;          // 27 * 1e8
+constexpr uint32_t kSelfMaxKey = 27u * 67108864u;   // 27 * 2^26
+constexpr int kFastMaxOff = 606;                    // 54 * 606 <= 32767
+constexpr int kRsrcWord3 = 0x00020000;              // buffer descriptor word 3 for raw dword access (gfx9)
 
 typedef unsigned short ushort2v __attribute__((ext_vector_type(2)));
 typedef short short2v __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));   // 8-byte aligned: one ds_read_b64
 
 // squared length of (x, y) packed as int16 pairs plus z*z, wrapping in 32
 // bits as len2 does
@@ -172,210 +208,529 @@ __device__ __forceinline__ int len2_packed(uint32_t xy, int z)
     return __builtin_amdgcn_sdot2(v, v, z * z, false);
 }
 
-// W > 0: the row width as a constant (LDS offsets become immediates)
-template <bool IN_AOS, bool OUT_AOS, int W>
-__global__ void __launch_bounds__(kRowMax) k_jfa3d_rows(int step, int w_arg, int h, int d,
-                                                        size_t cells,
-                                                        const int16_t* __restrict__ in,
-                                                        int16_t* __restrict__ out, int cw)
+// rank of tap t = 9 (kc+1) + 3 (jc+1) + (ic+1) in the reference's order with
+// the cell itself (t = 13) first
+__host__ __device__ constexpr int tap_rank(int t) { return t == 13 ? 0 : (t < 13 ? t + 1 : t); }
+// the rank a staged row's B holds: its ic = 0 tap's (on the cell's own row,
+// whose ic = 0 tap is the cell, the ic = -1 tap's)
+__host__ __device__ constexpr int row_rank(int r) { return r == 4 ? tap_rank(12) : tap_rank(3 * r + 1); }
+
+// the flag bytes of source row srow (its first nw slots), folded into one
+// 64-bit word (only bits 0 and 1 of each byte carry anything)
+__device__ __forceinline__ uint64_t row_flags(const uint8_t* __restrict__ flags, int srow, int nw)
 {
+    const uint64_t* fr = (const uint64_t*)(flags + (size_t)srow * kFlagSlots);
+    uint64_t a = fr[0], b = nw > 8 ? fr[1] : 0;
+    if (nw < 8) a &= ~0ull >> (64 - 8 * nw);
+    if (nw > 8 && nw < 16) b &= ~0ull >> (64 - 8 * (nw - 8));
+    return a | b;
+}
+
+__device__ __forceinline__ uint32_t fold_flags(uint64_t f)
+{
+    f |= f >> 32;
+    f |= f >> 16;
+    f |= f >> 8;
+    return (uint32_t)f & 0xffu;
+}
+
+// One output row (y, z) by the general path, which follows the reference's
+// arithmetic including its int16 and int32 wraps: lane t takes columns
+// x = t + c T (c < COLS, T lanes) from its cells cxy[c][r], cz[c][r] of
+// staged row r = 3 (kc+1) + (jc+1), source row (z + kc*step, y + jc*step),
+// present where rv[r] (uniform).  Stores the result (interleaved if OUT_AOS)
+// and the row's flag bytes.  Uses LDS from `lds` on; the caller
+// synchronises before using it again.
+template <bool OUT_AOS, int W, int COLS>
+__device__ __forceinline__ void jfa3d_row_general(uint32_t* lds, int w, int t, int T, size_t row,
+                                                  int step, int cw, const bool (&rv)[9],
+                                                  const uint32_t (&cxy)[COLS][9],
+                                                  const int (&cz)[COLS][9], size_t cells,
+                                                  int16_t* __restrict__ out,
+                                                  uint8_t* __restrict__ flags_out)
+{
+    constexpr bool kAllLive = W > 0 && W % (64 * COLS) == 0;
+    const int off = cw * step;
+    int xc[COLS];
+    bool live[COLS];
+#pragma unroll
+    for (int c = 0; c < COLS; c++) {
+        xc[c] = t + c * T;
+        live[c] = kAllLive || xc[c] < w;
+    }
+    uint32_t bxy[COLS];   // the results
+    int bz[COLS];
+#pragma unroll
+    for (int c = 0; c < COLS; c++) {
+        bxy[c] = 0;
+        bz[c] = 0;
+    }
+    {
+        // [9][w] packed (x, y), then [9][w] int16 z
+        const int16_t* lds16 = (const int16_t*)lds;
+        int16_t* z16 = (int16_t*)lds + 2 * 9 * w;
+#pragma unroll
+        for (int c = 0; c < COLS; c++) {
+            if (!live[c]) continue;
+#pragma unroll
+            for (int r = 0; r < 9; r++) {
+                lds[r * w + xc[c]] = cxy[c][r];
+                z16[r * w + xc[c]] = (int16_t)cz[c][r];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < COLS; c++) {
+            if (!live[c]) continue;
+            const int x = xc[c];
+            // tap ic of staged row r.  The x of a tap is clamped into the row
+            // so every read is in range (an out-of-row tap is masked off
+            // instead of branched around).
+            const int xs[3] = {max(x - step, 0), x, min(x + step, w - 1)};
+            const bool okx[3] = {x - step >= 0, true, x + step < w};
+            // candidate of tap tt = 9 (kc+1) + 3 (jc+1) + (ic+1): the source
+            // offset by (ic, jc, kc) * off, each component wrapped to int16
+            auto candidate = [&](int tt, uint32_t sxy, int sz, uint32_t& oxy, int& oz) {
+                const int ic = tt % 3 - 1, jc = (tt / 3) % 3 - 1, kc = tt / 9 - 1;
+                ushort2v v = __builtin_bit_cast(ushort2v, sxy);
+                v += ushort2v{(unsigned short)(ic * off), (unsigned short)(jc * off)};
+                oxy = __builtin_bit_cast(uint32_t, v);
+                oz = wrap16(sz + kc * off);
+            };
+            // the loop keeps only the best length and its tap; the winner's
+            // value is formed again at the end
+            int bl = len2_packed(cxy[c][4], cz[c][4]);
+            int bt = 13;
+#pragma unroll
+            for (int r = 0; r < 9; r++) {
+                if (!rv[r]) continue;
+#pragma unroll
+                for (int ic = -1; ic <= 1; ic++) {
+                    const int tt = 3 * r + ic + 1;
+                    if (tt == 13) continue;   // the cell itself never beats itself (strict >)
+                    const uint32_t sxy = lds[r * w + xs[ic + 1]];
+                    const int sz = lds16[2 * 9 * w + r * w + xs[ic + 1]];
+                    uint32_t oxy;
+                    int oz;
+                    candidate(tt, sxy, sz, oxy, oz);
+                    const int cl = len2_packed(oxy, oz);
+                    const bool take = okx[ic + 1] && (int16_t)sxy != kMaxDelta3 && bl > cl;
+                    bl = take ? cl : bl;
+                    bt = take ? tt : bt;
+                }
+            }
+            if (bt == 13) {
+                bxy[c] = cxy[c][4];
+                bz[c] = cz[c][4];
+            } else {
+                const int r = bt / 3, ic = bt % 3 - 1;
+                const int xx = xs[ic + 1];
+                candidate(bt, lds[r * w + xx], lds16[2 * 9 * w + r * w + xx], bxy[c], bz[c]);
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < COLS; c++) {
+        if (live[c]) {
+            const size_t i = row * w + xc[c];
+            if (OUT_AOS) {
+                int16_t* o = out + 3 * i;
+                o[0] = (int16_t)bxy[c]; o[1] = (int16_t)(bxy[c] >> 16); o[2] = (int16_t)bz[c];
+            } else {
+                ((uint32_t*)out)[i] = bxy[c];
+                out[2 * cells + i] = (int16_t)bz[c];
+            }
+        }
+        if (flags_out) {
+            const int ox = (int16_t)bxy[c], oy = (int16_t)(bxy[c] >> 16), oz = (int16_t)bz[c];
+            const bool m = ox == kMaxDelta3;
+            const bool unb = !m && (abs(ox) > kFastBound || abs(oy) > kFastBound || abs(oz) > kFastBound);
+            const uint64_t bu = __ballot(live[c] && unb), bm = __ballot(live[c] && m);
+            if ((t & 63) == 0 && xc[c] < w)
+                flags_out[row * kFlagSlots + (xc[c] >> 6)] =
+                    (uint8_t)((bu ? kRowUnbounded : 0u) | (bm ? kRowHasMax : 0u));
+        }
+    }
+}
+
+// columns a lane takes: two (half the lanes, so the scalar work of a row is
+// shared by twice the cells)
+template <int W>
+constexpr int jfa3d_cols() { return 2; }
+// lanes of the row kernels' workgroups: 256 for 512-wide rows, at most 512
+template <int W>
+constexpr int jfa3d_lanes() { return W == 512 ? 256 : kRowMax / 2; }
+
+// The 3D passes on runs of rows with the source rows in an LDS ring.
+//
+// Workgroup b takes a run of up to `run` output rows (y0 + i*step, z),
+// i = 0, 1, ..., of one residue y0 mod step, as k_jfa3d_runs.  Row i reads
+// source y-rows y0 + (i-1) step, y0 + i step and y0 + (i+1) step (each at
+// z - step, z, z + step), so consecutive rows share two of their three
+// y-rows.  The ring holds 3 y-rows x 3 z-rows of staged cells: row i + 1's
+// new y-row replaces row i - 2's in slot (i + 1) mod 3.  A staged cell
+// holds its raw (x, y), its z and
+//     Bb = 27 * (sx^2 + sy^2 + (sz + kc*off)^2)
+// which, unlike the per-row B of jfa3d_row, does not depend on the y offset
+// the cell is read with, so it is formed once per run; a tap adds
+//     K = 27 * ((ic*off)^2 + (jc*off)^2) + rank
+// and the cross term (54 ic off, 54 jc off) . (sx, sy) with one v_dot2.  A
+// kMaxDelta3 cell is staged with (x, y) = 0 and Bb = kPoisonB.  The whole
+// run takes this path when the flags of all its source rows allow it;
+// otherwise the workgroup runs its rows through jfa3d_row's general path.
+// With IN_AOS the cells are read from the caller's interleaved field (the
+// flags then come from k_jfa3d_classify).  The loop is unrolled by the
+// ring's three phases so every LDS offset is an immediate.
+template <bool IN_AOS, bool OUT_AOS, int W>
+__global__ void __launch_bounds__(jfa3d_lanes<W>()) __attribute__((amdgpu_waves_per_eu(W == 512 ? 3 : 2)))
+k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __restrict__ in,
+             int16_t* __restrict__ out, int cw, const uint8_t* __restrict__ flags_in,
+             uint8_t* __restrict__ flags_out, int run, int residues)
+{
+    constexpr int COLS = jfa3d_cols<W>();
+    constexpr bool kAllLive = W > 0 && W % (64 * COLS) == 0;
     const int w = W > 0 ? W : w_arg;
     extern __shared__ __align__(16) uint32_t lds[];
-    // [9][w] packed (x, y), then [9][w] int16 z (IN_AOS: staged first as
-    // [9][3w] int16 and converted in place)
-    const int16_t* lds16 = (const int16_t*)lds;
-    const int row = blockIdx.x;   // z*h + y
-    const int z = row / h, y = row - z * h;
-    auto in_row = [&](int r, int& src_row) {
-        const int zz = z + (r / 3 - 1) * step, yy = y + (r % 3 - 1) * step;
-        src_row = zz * h + yy;
-        return zz >= 0 && zz < d && yy >= 0 && yy < h;
-    };
-    if constexpr (W == 512) {
-        // one thread per cell of the row: the 9 x 192 16-byte chunks of the
-        // staged rows are split so that a thread's chunks share their row's
-        // y offset and column: threads t < 384 take column t % 192 of rows g,
-        // g + 3, g + 6 (g = t / 192), threads t >= 384 column t - 384 of rows
-        // 2, 5, 8, and threads t < 192 also column 128 + t % 64 of row
-        // 2 + 3 (t / 64).  Rows outside the volume read row 0 instead and are
-        // not stored; all loads are issued before the first LDS store.
-        constexpr int kCh = 3 * W / 8;   // 192 chunks a row in either layout
-        static_assert(IN_AOS || W / 4 + W / 8 == kCh, "chunks a row");
-        const int t = threadIdx.x;
-        const int g = t < 384 ? (t < 192 ? 0 : 1) : 2;
-        const int k = t < 384 ? t - 192 * g : t - 384;
-        const int ke = 128 + (t & 63), je = t >> 6;   // the extra chunk (t < 192)
-        // source offset (16-byte units) and LDS word of chunk kk of staged
-        // row r = rg + 3 j, whose source row is (z + (j - 1) step, yy)
-        auto chunk = [&](int rg, int j, int kk, int yy, int& dst) -> size_t {
-            const int zz = z + (j - 1) * step;
-            const int r = rg + 3 * j;
-            const bool ok = zz >= 0 && zz < d && yy >= 0 && yy < h;
-            const size_t sr = ok ? (size_t)zz * h + yy : 0;
-            size_t off;
-            if (IN_AOS) {
-                off = sr * kCh + kk;
-                dst = r * (3 * W / 2) + 4 * kk;
-            } else if (kk < W / 4) {
-                off = sr * (W / 4) + kk;
-                dst = r * W + 4 * kk;
-            } else {
-                off = cells / 4 + sr * (W / 8) + (kk - W / 4);   // the z plane starts at byte 4 * cells
-                dst = 9 * W + r * (W / 2) + 4 * (kk - W / 4);
-            }
-            if (!ok) dst = -1;
-            return off;
-        };
-        const int yg = y + (g - 1) * step;
-        int d0, d1, d2, d3;
-        const size_t o0 = chunk(g, 0, k, yg, d0), o1 = chunk(g, 1, k, yg, d1), o2 = chunk(g, 2, k, yg, d2);
-        const size_t o3 = chunk(2, je < 3 ? je : 0, ke, y + step, d3);
-        if (t >= 192) d3 = -1;
-        const uint4* in4 = (const uint4*)in;
-        const uint4 v0 = in4[o0], v1 = in4[o1], v2 = in4[o2];
-        const uint4 v3 = in4[t < 192 ? o3 : 0];
-        if (d0 >= 0) *(uint4*)(lds + d0) = v0;
-        if (d1 >= 0) *(uint4*)(lds + d1) = v1;
-        if (d2 >= 0) *(uint4*)(lds + d2) = v2;
-        if (d3 >= 0) *(uint4*)(lds + d3) = v3;
-    } else {
+    int q = blockIdx.x;
+    if ((gridDim.x & 7) == 0) q = (q & 7) * (gridDim.x >> 3) + (q >> 3);
+    const int z = q % d;
+    const int qq = q / d;
+    const int r0 = qq % residues, ch = qq / residues;
+    const int y0 = r0 + ch * run * step;
+    if (y0 >= h) return;
+    const int n = min(run, (h - 1 - y0) / step + 1);
+    const int t = threadIdx.x, T = blockDim.x;
+    const int off = cw * step;
+    int xc[COLS];
+    bool live[COLS];
+    unsigned xl[COLS];
+#pragma unroll
+    for (int c = 0; c < COLS; c++) {
+        xc[c] = t + c * T;
+        live[c] = kAllLive || xc[c] < w;
+        xl[c] = live[c] ? xc[c] : w - 1;
+    }
+    const bool zv[3] = {z - step >= 0, true, z + step < d};
+    const int nw = (w + 63) / 64;
+    // this lane's cells of source row sr (rows outside the volume are never
+    // read): the row's address in a buffer descriptor, the lane's offset in
+    // one register
+    auto load = [&](int sr, uint32_t (&vxy)[COLS], int (&vz)[COLS]) {
+        sr = __builtin_amdgcn_readfirstlane(sr);
         if (IN_AOS) {
-            const int chunks = (3 * w) / 8;   // 16-byte chunks in a row
-            for (int e = threadIdx.x; e < 9 * chunks; e += blockDim.x) {
-                const int r = e / chunks, k = e - r * chunks;
-                int sr;
-                if (!in_row(r, sr)) continue;
-                ((uint4*)(lds + r * (3 * w / 2)))[k] = ((const uint4*)(in + (size_t)3 * w * sr))[k];
+            const auto rr = __builtin_amdgcn_make_buffer_rsrc((void*)(in + (size_t)sr * w * 3), 0, w * 6, kRsrcWord3);
+#pragma unroll
+            for (int c = 0; c < COLS; c++) {
+                const uint32_t lo = __builtin_amdgcn_raw_buffer_load_b16(rr, xl[c] * 6, 0, 0);
+                const uint32_t hi = __builtin_amdgcn_raw_buffer_load_b16(rr, xl[c] * 6 + 2, 0, 0);
+                vxy[c] = lo | hi << 16;
+                vz[c] = (int16_t)__builtin_amdgcn_raw_buffer_load_b16(rr, xl[c] * 6 + 4, 0, 0);
             }
         } else {
-            const int cxy = w / 4, cz = w / 8;   // 16-byte chunks in an xy row and a z row
-            const uint32_t* pxy = (const uint32_t*)in;
-            const int16_t* pz = in + 2 * cells;
-            for (int e = threadIdx.x; e < 9 * (cxy + cz); e += blockDim.x) {
-                const int r = e / (cxy + cz), k = e - r * (cxy + cz);
-                int sr;
-                if (!in_row(r, sr)) continue;
-                if (k < cxy)
-                    ((uint4*)(lds + r * w))[k] = ((const uint4*)(pxy + (size_t)w * sr))[k];
-                else
-                    ((uint4*)(lds + 9 * w + r * (w / 2)))[k - cxy] = ((const uint4*)(pz + (size_t)w * sr))[k - cxy];
+            const size_t base = (size_t)sr * w;
+            const auto rxy = __builtin_amdgcn_make_buffer_rsrc((void*)((const uint32_t*)in + base), 0, w * 4, kRsrcWord3);
+            const auto rz = __builtin_amdgcn_make_buffer_rsrc((void*)(in + 2 * cells + base), 0, w * 2, kRsrcWord3);
+#pragma unroll
+            for (int c = 0; c < COLS; c++) {
+                vxy[c] = __builtin_amdgcn_raw_buffer_load_b32(rxy, xl[c] * 4, 0, 0);
+                vz[c] = (int16_t)__builtin_amdgcn_raw_buffer_load_b16(rz, xl[c] * 2, 0, 0);
             }
         }
+    };
+    auto y_ok = [&](int g) { const int yy = y0 + g * step; return yy >= 0 && yy < h; };
+
+    // the path of the whole run, from the flags of every source row
+    uint64_t f = 0;
+    for (int g = -1; g <= n; g++) {
+        if (!y_ok(g)) continue;
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            if (zv[k]) f |= row_flags(flags_in, (z + (k - 1) * step) * h + y0 + g * step, nw);
     }
-    __syncthreads();
-    const int x = threadIdx.x;
-    if (IN_AOS) {
-        // turn the staged rows into the planar form in place: each thread
-        // takes its own column of the 9 rows into registers, then writes it
-        // back (rows that were not staged carry garbage and are never read)
-        // cell x's 6 bytes start at byte 6x, which is 4-aligned only for
-        // even x: read the two aligned words that hold them and take the
-        // halves (an unaligned LDS access would be split and serialised)
-        uint32_t cxy[9];
-        int16_t cz[9];
-        if (x < w) {
-            const int k0 = (3 * x) >> 1;   // word holding the cell's first byte
-            const bool odd = x & 1;
+    const uint32_t fl = fold_flags(f);
+    if ((fl & kRowUnbounded) || off > kFastMaxOff) {
+        // general path, one row at a time
+        for (int i = 0; i < n; i++) {
+            const int y = y0 + i * step;
+            bool rv[9];
+            uint32_t cxy[COLS][9];
+            int cz[COLS][9];
 #pragma unroll
             for (int r = 0; r < 9; r++) {
-                const uint32_t* wrow = lds + r * (3 * w / 2);
-                const uint32_t a = wrow[k0], b = wrow[k0 + 1];
-                cxy[r] = odd ? __builtin_amdgcn_alignbit(b, a, 16) : a;
-                cz[r] = (int16_t)(odd ? (b >> 16) : (b & 0xffffu));
+                const int k = r / 3, jc = r % 3 - 1;
+                rv[r] = zv[k] && y_ok(i + jc);
+                uint32_t vxy[COLS] = {};
+                int vz[COLS] = {};
+                if (rv[r]) load((z + (k - 1) * step) * h + y + jc * step, vxy, vz);
+#pragma unroll
+                for (int c = 0; c < COLS; c++) {
+                    cxy[c][r] = vxy[c];
+                    cz[c][r] = vz[c];
+                }
+            }
+            jfa3d_row_general<OUT_AOS, W, COLS>(lds, w, t, T, (size_t)z * h + y, step, cw, rv, cxy, cz,
+                                                cells, out, flags_out);
+            __syncthreads();
+        }
+        return;
+    }
+    const bool any_max = (fl & kRowHasMax) != 0;
+
+    // LDS: ring [3 z][3 slots][w + 1] (xy, Bb) pairs, the same of int16 z,
+    // then the decoding tables of the three phases
+    const int stride = w + 1;
+    u32x2* pb = (u32x2*)lds;
+    int16_t* zs = (int16_t*)(pb + 9 * stride);
+    uint4* tbl = (uint4*)(lds + ((9 * stride * 10 + 15) / 16) * 4);
+    for (int e = t; e < 81; e += T) {
+        // phase p = i mod 3, rank: the winner's cell index relative to
+        // (slot 0, column x) and the offsets added to its (x, y) and z
+        const int p = e / 27, rk = e % 27;
+        const int tt = rk == 0 ? 13 : (rk <= 13 ? rk - 1 : rk);
+        const int r = tt / 3, ic = tt % 3 - 1, jc = r % 3 - 1, kc = r / 3 - 1;
+        const int slot = (kc + 1) * 3 + (p + jc + 3) % 3;
+        const uint32_t dxy = (uint32_t)(uint16_t)(ic * off) | (uint32_t)(uint16_t)(jc * off) << 16;
+        tbl[e] = make_uint4((uint32_t)(slot * stride + ic * step), dxy, (uint32_t)(kc * off), 0u);
+    }
+    if (t < 9) pb[t * stride + w] = u32x2{0u, kPoisonB};   // the poison column
+    // stage this lane's cells of y-row g of every z-row into ring slot g mod 3;
+    // a row outside the volume is staged as poison, so the taps need no test
+    auto stage = [&](int g, const uint32_t (&vxy)[3][COLS], const int (&vz)[3][COLS]) {
+        const int slot = (g + 3) % 3;
+        const bool gy = y_ok(g);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            if (!zv[k] || !gy) {
+#pragma unroll
+                for (int c = 0; c < COLS; c++)
+                    if (live[c]) pb[(k * 3 + slot) * stride + xc[c]] = u32x2{0u, kPoisonB};
+                continue;
+            }
+            const int kc = k - 1;
+#pragma unroll
+            for (int c = 0; c < COLS; c++) {
+                const int czr = vz[k][c] + kc * off;
+                const int len = __builtin_amdgcn_sdot2(__builtin_bit_cast(short2v, vxy[k][c]),
+                                                       __builtin_bit_cast(short2v, vxy[k][c]), __mul24(czr, czr), false);
+                uint32_t B = __umul24((uint32_t)len, kKeyScale);
+                uint32_t X = vxy[k][c];
+                if (any_max) {
+                    const bool m = (int16_t)X == kMaxDelta3;
+                    X = m ? 0u : X;
+                    B = m ? kPoisonB : B;
+                }
+                if (live[c]) {
+                    pb[(k * 3 + slot) * stride + xc[c]] = u32x2{X, B};
+                    zs[(k * 3 + slot) * stride + xc[c]] = (int16_t)vz[k][c];
+                }
             }
         }
-        __syncthreads();
-        if (x < w) {
-            int16_t* z16 = (int16_t*)lds + 2 * 9 * w;
-#pragma unroll
-            for (int r = 0; r < 9; r++) {
-                lds[r * w + x] = cxy[r];
-                z16[r * w + x] = cz[r];
-            }
-        }
-        __syncthreads();
-    }
-    if (x >= w) return;
-    // tap ic of staged row r.  The x of a tap is clamped into the row so
-    // every read is in range (an out-of-row tap is masked off instead of
-    // branched around), and the per-ic bases are formed once so that row
-    // offsets fold into the LDS instructions' immediates when W is known.
-    const int xs[3] = {max(x - step, 0), x, min(x + step, w - 1)};
-    const bool okx[3] = {x - step >= 0, true, x + step < w};
-    // byte offsets of the three tap columns, held in registers (the empty
-    // asm keeps the compiler from re-forming them in every row block)
-    typedef __attribute__((address_space(3))) const uint32_t* lds_u32;
-    typedef __attribute__((address_space(3))) const int16_t* lds_i16;
-    lds_u32 pxy[3];
-    lds_i16 pz[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        pxy[k] = (lds_u32)(lds + xs[k]);
-        pz[k] = (lds_i16)(lds16 + 2 * 9 * w + xs[k]);
-        asm volatile("" : "+v"(pxy[k]), "+v"(pz[k]));
-    }
-    auto tap = [&](int r, int ic, uint32_t& xy, int& tz) {
-        xy = pxy[ic + 1][r * w];
-        tz = pz[ic + 1][r * w];
     };
-    const int off = cw * step;
-    // candidate of tap t = 9 (kc+1) + 3 (jc+1) + (ic+1): the source offset by
-    // (ic, jc, kc) * off, each component wrapped to int16
-    auto candidate = [&](int t, uint32_t sxy, int sz, uint32_t& cxy, int& cz) {
-        const int ic = t % 3 - 1, jc = (t / 3) % 3 - 1, kc = t / 9 - 1;
-        ushort2v v = __builtin_bit_cast(ushort2v, sxy);
-        v += ushort2v{(unsigned short)(ic * off), (unsigned short)(jc * off)};
-        cxy = __builtin_bit_cast(uint32_t, v);
-        cz = wrap16(sz + kc * off);
-    };
-    // the loop keeps only the best length and its tap; the winner's value is
-    // formed again at the end
-    uint32_t sxy;
-    int sz;
-    tap(4, 0, sxy, sz);
-    int bl = len2_packed(sxy, sz);
-    int bt = 13;
-#pragma unroll
-    for (int kc = -1; kc <= 1; kc++) {
-        const int zz = z + kc * step;
-        if (zz < 0 || zz >= d) continue;
-#pragma unroll
-        for (int jc = -1; jc <= 1; jc++) {
-            const int yy = y + jc * step;
-            if (yy < 0 || yy >= h) continue;
-            const int r = (kc + 1) * 3 + (jc + 1);
-#pragma unroll
-            for (int ic = -1; ic <= 1; ic++) {
-                const int t = 3 * r + ic + 1;
-                if (t == 13) continue;   // the cell itself never beats itself (strict >)
-                tap(r, ic, sxy, sz);
-                uint32_t cxy;
-                int cz;
-                candidate(t, sxy, sz, cxy, cz);
-                const int cl = len2_packed(cxy, cz);
-                const bool take = okx[ic + 1] && (int16_t)sxy != kMaxDelta3 && bl > cl;
-                bl = take ? cl : bl;
-                bt = take ? t : bt;
-            }
-        }
-    }
-    uint32_t bxy;
-    int bz;
+    // raw cells of the middle z-row of the current and the next y-row (the
+    // output cell itself and its successor)
+    uint32_t self_xy[2][COLS];
+    int self_z[2][COLS];
     {
-        const int r = bt / 3, ic = bt % 3 - 1;
-        const int xx = xs[ic + 1];
-        sxy = lds[r * w + xx];
-        sz = lds16[2 * 9 * w + r * w + xx];
-        candidate(bt, sxy, sz, bxy, bz);
+        uint32_t vxy[3][COLS] = {};
+        int vz[3][COLS] = {};
+#pragma unroll
+        for (int g = -1; g <= 1; g++) {
+            if (y_ok(g)) {
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+                    if (zv[k]) load((z + (k - 1) * step) * h + y0 + g * step, vxy[k], vz[k]);
+            }
+            stage(g, vxy, vz);
+            if (g >= 0) {
+#pragma unroll
+                for (int c = 0; c < COLS; c++) {
+                    self_xy[g][c] = vxy[1][c];
+                    self_z[g][c] = vz[1][c];
+                }
+            }
+        }
     }
-    const size_t i = (size_t)row * w + x;
-    if (OUT_AOS) {
-        int16_t* o = out + 3 * i;
-        o[0] = (int16_t)bxy; o[1] = (int16_t)(bxy >> 16); o[2] = (int16_t)bz;
-    } else {
-        ((uint32_t*)out)[i] = bxy;
-        out[2 * cells + i] = (int16_t)bz;
+    // the uniform per-tap constants
+    const uint32_t c54 = (uint32_t)__builtin_amdgcn_readfirstlane(2 * kKeyScale * off) & 0xffffu;
+    const uint32_t c54n = (0x10000u - c54) & 0xffffu;
+    const uint32_t k1 = (uint32_t)__builtin_amdgcn_readfirstlane(kKeyScale * off * off);
+    // the next y-row's cells, in flight during a row
+    uint32_t nxy[3][COLS] = {};
+    int nz[3][COLS] = {};
+    auto prefetch = [&](int g) {
+        if (g >= n + 1 || !y_ok(g)) return;
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            if (zv[k]) load((z + (k - 1) * step) * h + y0 + g * step, nxy[k], nz[k]);
+    };
+    prefetch(2);
+    __syncthreads();
+    auto body = [&](int i, auto P) {
+        constexpr int p = decltype(P)::value;   // i mod 3
+        if (i > 0) {
+            // y-row i + 1 into the slot of y-row i - 2, and the row after
+            // it into flight
+            if (i + 1 <= n) stage(i + 1, nxy, nz);
+#pragma unroll
+            for (int c = 0; c < COLS; c++) {
+                self_xy[0][c] = self_xy[1][c];
+                self_z[0][c] = self_z[1][c];
+                self_xy[1][c] = nxy[1][c];
+                self_z[1][c] = nz[1][c];
+            }
+            prefetch(i + 2);
+            __syncthreads();
+        }
+        const int y = y0 + i * step;
+        uint32_t bxy[COLS];
+        int bz[COLS];
+#pragma unroll
+        for (int c = 0; c < COLS; c++) {
+            bxy[c] = 0;
+            bz[c] = 0;
+            if (!live[c]) continue;
+            const int x = xc[c];
+            uint32_t best;
+            {
+                const int sx = (int16_t)self_xy[0][c];
+                const int l = len2_packed(self_xy[0][c], self_z[0][c]);
+                best = sx == kMaxDelta3 ? (l < 0 ? 0u : kSelfMaxKey) : __umul24((uint32_t)l, kKeyScale);
+            }
+            const int colm = x - step >= 0 ? x - step : w;
+            const int colp = x + step < w ? x + step : w;
+            typedef __attribute__((address_space(3))) const u32x2* lds_u2;
+            lds_u2 bm = (lds_u2)(pb + colm), b0 = (lds_u2)(pb + x), bp = (lds_u2)(pb + colp);
+            asm volatile("" : "+v"(bm), "+v"(b0), "+v"(bp));
+            auto key = [&](int r, int ic, const u32x2& v) {
+                const int tt = 3 * r + ic + 1, jc = r % 3 - 1;
+                const uint32_t K = (uint32_t)((ic * ic + jc * jc)) * k1 + (uint32_t)tap_rank(tt);
+                const uint32_t cx = ic > 0 ? c54 : ic < 0 ? c54n : 0u;
+                const uint32_t cy = jc > 0 ? c54 : jc < 0 ? c54n : 0u;
+                return (uint32_t)__builtin_amdgcn_sdot2(__builtin_bit_cast(short2v, v.x),
+                                                        __builtin_bit_cast(short2v, cx | cy << 16),
+                                                        (int)(v.y + K), false);
+            };
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                u32x2 nb[3][3];
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const int r = 3 * k + j;
+                    const int slot = k * 3 + (p + j - 1 + 3) % 3;
+                    nb[j][0] = bm[slot * stride];
+                    nb[j][1] = r != 4 ? b0[slot * stride] : u32x2{0u, kPoisonB};
+                    nb[j][2] = bp[slot * stride];
+                }
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const int r = 3 * k + j;
+                    if (r == 4) {
+                        best = min(best, min(key(r, -1, nb[j][0]), key(r, 1, nb[j][2])));
+                    } else {
+                        best = min(best, min(key(r, -1, nb[j][0]), key(r, 0, nb[j][1])));
+                        best = min(best, key(r, 1, nb[j][2]));
+                    }
+                }
+            }
+            // the winner, rebuilt from its source (rank 0: the cell itself,
+            // whose staged copy may have been cleared as a skipped source)
+            const uint32_t rank = best % (uint32_t)kKeyScale;
+            const uint4 e = tbl[p * 27 + rank];
+            const int idx = x + (int)e.x;
+            ushort2v v = __builtin_bit_cast(ushort2v, pb[idx].x);
+            v += __builtin_bit_cast(ushort2v, e.y);
+            bxy[c] = rank ? __builtin_bit_cast(uint32_t, v) : self_xy[0][c];
+            bz[c] = rank ? zs[idx] + (int)e.z : self_z[0][c];
+        }
+        const size_t row = (size_t)z * h + y;
+#pragma unroll
+        for (int c = 0; c < COLS; c++) {
+            if (live[c]) {
+                const size_t o = row * w + xc[c];
+                if (OUT_AOS) {
+                    int16_t* op = out + 3 * o;
+                    op[0] = (int16_t)bxy[c]; op[1] = (int16_t)(bxy[c] >> 16); op[2] = (int16_t)bz[c];
+                } else {
+                    ((uint32_t*)out)[o] = bxy[c];
+                    out[2 * cells + o] = (int16_t)bz[c];
+                }
+            }
+            if (flags_out) {
+                const int ox = (int16_t)bxy[c], oy = (int16_t)(bxy[c] >> 16), oz = (int16_t)bz[c];
+                const bool m = ox == kMaxDelta3;
+                const bool unb = !m && (abs(ox) > kFastBound || abs(oy) > kFastBound || abs(oz) > kFastBound);
+                const uint64_t bu = __ballot(live[c] && unb), bmx = __ballot(live[c] && m);
+                if ((t & 63) == 0 && xc[c] < w)
+                    flags_out[row * kFlagSlots + (xc[c] >> 6)] =
+                        (uint8_t)((bu ? kRowUnbounded : 0u) | (bmx ? kRowHasMax : 0u));
+            }
+        }
+        __syncthreads();
+    };
+    for (int i = 0; i < n; i += 3) {
+        body(i, std::integral_constant<int, 0>{});
+        if (i + 1 < n) body(i + 1, std::integral_constant<int, 1>{});
+        if (i + 2 < n) body(i + 2, std::integral_constant<int, 2>{});
     }
+}
+
+// Flag bytes of the caller's interleaved field, in the passes' format, for
+// the first pass.  Widths that are a multiple of 64: a lane reads 8 cells
+// (48 bytes) with three 16-byte loads and 8 lanes make one 64-cell slot.
+__global__ void __launch_bounds__(256) k_jfa3d_classify64(size_t cells, int w,
+                                                         const uint4* __restrict__ in,
+                                                         uint8_t* __restrict__ flags)
+{
+    const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;   // cells 8g .. 8g + 7
+    bool unb = false, m = false;
+    if (8 * g < cells) {
+        const uint4 a = in[3 * g], b = in[3 * g + 1], c = in[3 * g + 2];
+        const uint32_t v[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            // cell k's components are the 16-bit halves 3k, 3k + 1, 3k + 2
+            int comp[3];
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int hw = 3 * k + j;
+                comp[j] = (int16_t)(v[hw >> 1] >> (16 * (hw & 1)));
+            }
+            const bool mk = comp[0] == kMaxDelta3;
+            m |= mk;
+            unb |= !mk && (abs(comp[0]) > kFastBound || abs(comp[1]) > kFastBound || abs(comp[2]) > kFastBound);
+        }
+    }
+    const uint64_t bu = __ballot(unb), bm = __ballot(m);
+    const int lane = threadIdx.x & 63;
+    if ((lane & 7) == 0 && 8 * g < cells) {
+        const int sh = lane & ~7;
+        const size_t cell = 8 * g, row = cell / (size_t)w;
+        const int slot = (int)(cell - row * w) >> 6;
+        flags[row * kFlagSlots + slot] = (uint8_t)((((bu >> sh) & 0xff) ? kRowUnbounded : 0u) |
+                                                   (((bm >> sh) & 0xff) ? kRowHasMax : 0u));
+    }
+}
+
+// Flag bytes of the caller's interleaved field, in the passes' format, for
+// the first pass: one 64-lane wave per 64-cell slot (any width).
+__global__ void __launch_bounds__(256) k_jfa3d_classify(int w, int rows, const int16_t* __restrict__ in,
+                                                       uint8_t* __restrict__ flags)
+{
+    const int nw = (w + 63) / 64;
+    const long long slot = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);   // (row, wave of row)
+    if (slot >= (long long)rows * nw) return;
+    const int row = (int)(slot / nw), wv = (int)(slot - (long long)row * nw);
+    const int x = wv * 64 + (threadIdx.x & 63);
+    bool unb = false, m = false;
+    if (x < w) {
+        const int16_t* c = in + 3 * ((size_t)row * w + x);
+        const int ox = c[0], oy = c[1], oz = c[2];
+        m = ox == kMaxDelta3;
+        unb = !m && (abs(ox) > kFastBound || abs(oy) > kFastBound || abs(oz) > kFastBound);
+    }
+    const uint64_t bu = __ballot(unb), bm = __ballot(m);
+    if ((threadIdx.x & 63) == 0)
+        flags[(size_t)row * kFlagSlots + wv] = (uint8_t)((bu ? kRowUnbounded : 0u) | (bm ? kRowHasMax : 0u));
 }
 
 void launch_jfa3d_pass(int step, int w, int h, int d, const int16_t* in, int16_t* out, int cw,
@@ -387,27 +742,55 @@ void launch_jfa3d_pass(int step, int w, int h, int d, const int16_t* in, int16_t
                        d, xb, in, out, cw);
 }
 
-static void launch_jfa3d_rows(bool in_aos, bool out_aos, int step, int w, int h, int d,
-                              const int16_t* in, int16_t* out, int cw, hipStream_t s)
+// dynamic LDS of the run kernel: the larger of the keyed path's ring of
+// (xy, Bb) pairs and z with its decoding tables, and the general path's
+// planes
+static size_t jfa3d_rows_lds(int w)
+{
+    const size_t general = (size_t)9 * w * 6;
+    const size_t keyed = ((size_t)9 * (w + 1) * 10 + 15) / 16 * 16 + 81 * 16;
+    return general > keyed ? general : keyed;
+}
+
+constexpr int kRun = 8;   // output rows a workgroup of k_jfa3d_runs takes
+
+static void launch_jfa3d_rows(bool first, bool out_aos, int step, int w, int h, int d,
+                              const int16_t* in, int16_t* out, int cw, const uint8_t* flags_in,
+                              uint8_t* flags_out, hipStream_t s)
 {
     ProfileScope p("jfa3d_pass", s);
-    const int threads = (w + 63) / 64 * 64;
-    const size_t lds = (size_t)9 * w * 6;
+    const int cols = w == 512 ? jfa3d_cols<512>() : jfa3d_cols<0>();
+    const int threads = (w + 64 * cols - 1) / (64 * cols) * 64;
+    const size_t lds = jfa3d_rows_lds(w);
     const size_t cells = (size_t)w * h * d;
-    const dim3 grid((unsigned)((size_t)h * d));
-#define DOL_JFA3D_ROWS(W)                                                                          \
-    if (in_aos && out_aos)                                                                         \
-        hipLaunchKernelGGL((k_jfa3d_rows<true, true, W>), grid, dim3(threads), lds, s, step, w, h, \
-                           d, cells, in, out, cw);                                                 \
-    else if (in_aos)                                                                               \
-        hipLaunchKernelGGL((k_jfa3d_rows<true, false, W>), grid, dim3(threads), lds, s, step, w,   \
-                           h, d, cells, in, out, cw);                                              \
-    else if (out_aos)                                                                              \
-        hipLaunchKernelGGL((k_jfa3d_rows<false, true, W>), grid, dim3(threads), lds, s, step, w,   \
-                           h, d, cells, in, out, cw);                                              \
-    else                                                                                           \
-        hipLaunchKernelGGL((k_jfa3d_rows<false, false, W>), grid, dim3(threads), lds, s, step, w,  \
-                           h, d, cells, in, out, cw);
+    // rows of more than 712 cells need over 64 KiB of LDS, which a kernel
+    // gets only after asking for it
+    auto want_lds = [&](const void* fn) {
+        if (lds > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    };
+    const int residues = step < h ? step : h;
+    const int runs = ((h + step - 1) / step + kRun - 1) / kRun;
+    const dim3 grid_runs((unsigned)((size_t)d * residues * runs));
+    if (first) {
+        // the caller's field has no flags: classify it first
+        if (w % 64 == 0 && ((uintptr_t)in & 15) == 0) {
+            hipLaunchKernelGGL(k_jfa3d_classify64, dim3((unsigned)((cells / 8 + 255) / 256)), dim3(256), 0, s,
+                               cells, w, (const uint4*)in, (uint8_t*)flags_in);
+        } else {
+            const int nw = (w + 63) / 64;
+            const long long slots = (long long)h * d * nw;
+            hipLaunchKernelGGL(k_jfa3d_classify, dim3((unsigned)((slots + 3) / 4)), dim3(256), 0, s, w, h * d,
+                               in, (uint8_t*)flags_in);
+        }
+    }
+#define DOL_JFA3D_ROWS(W)                                                                            \
+    {                                                                                                \
+        auto fn = first ? (out_aos ? k_jfa3d_ring<true, true, W> : k_jfa3d_ring<true, false, W>)     \
+                        : (out_aos ? k_jfa3d_ring<false, true, W> : k_jfa3d_ring<false, false, W>);  \
+        want_lds((const void*)fn);                                                                   \
+        hipLaunchKernelGGL(fn, grid_runs, dim3(threads), lds, s, step, w, h, d, cells, in, out, cw,  \
+                           flags_in, flags_out, kRun, residues);                                     \
+    }
     if (w == 512) {
         DOL_JFA3D_ROWS(512)
     } else {
@@ -424,23 +807,37 @@ static int floor_pow2_host(int v)
     return (int)((int32_t)(u + 1u) >> 1);
 }
 
+// bytes of the JumpFlood 3D workspace: the second field, then the two flag
+// arrays of the row kernels (kFlagSlots bytes per row each)
+size_t jfa3d_workspace(int w, int h, int d)
+{
+    const size_t field = ((size_t)w * h * d * 6 + 255) / 256 * 256;
+    return field + 2 * (size_t)kFlagSlots * h * d;
+}
+
 // The pass schedule of DistanceOcclusionLib.cpp:2450-2472: steps
-// FloorPow2(max dim - 1) down to 1, ping-ponging between delta and tmp, plus
-// one step-1 pass when the count is odd so the result ends in delta.
-void jfa3d_run(int w, int h, int d, int16_t* delta, int cw, int16_t* tmp, hipStream_t s)
+// FloorPow2(max dim - 1) down to 1, ping-ponging between delta and the
+// workspace's field, plus one step-1 pass when the count is odd so the
+// result ends in delta.
+void jfa3d_run(int w, int h, int d, int16_t* delta, int cw, void* work, hipStream_t s)
 {
     int m = w >= h ? w : h;
     m = m >= d ? m : d;
     int steps[40], n = 0;
     for (int step = floor_pow2_host(m - 1); step > 0; step /= 2) steps[n++] = step;
     if (n & 1) steps[n++] = 1;
+    int16_t* tmp = (int16_t*)work;
+    uint8_t* flags[2];
+    flags[0] = (uint8_t*)work + ((size_t)w * h * d * 6 + 255) / 256 * 256;
+    flags[1] = flags[0] + (size_t)kFlagSlots * h * d;
     const bool rows = w % 8 == 0 && w <= kRowMax && ((uintptr_t)delta & 15) == 0 &&
                       ((uintptr_t)tmp & 15) == 0;
     int16_t* a = delta;
     int16_t* b = tmp;
     for (int k = 0; k < n; k++) {
         if (rows)
-            launch_jfa3d_rows(k == 0, k == n - 1, steps[k], w, h, d, a, b, cw, s);
+            launch_jfa3d_rows(k == 0, k == n - 1, steps[k], w, h, d, a, b, cw, flags[(k + 1) & 1],
+                              k == n - 1 ? nullptr : flags[k & 1], s);
         else
             launch_jfa3d_pass(steps[k], w, h, d, a, b, cw, s);
         int16_t* t = a; a = b; b = t;

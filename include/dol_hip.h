@@ -23,8 +23,15 @@
  *                           Where a pass needs a temporary, `work` may point
  *                           to device memory of at least the size the matching
  *                           *_workspace() call returns, or be NULL, in which
- *                           case the library uses a per-device buffer of its
- *                           own that grows as needed (allocated synchronously).
+ *                           case the library uses a buffer of its own, one per
+ *                           (device, stream): calls on one stream share it in
+ *                           stream order, calls on different streams use
+ *                           different ones.  It grows as needed (hipMalloc,
+ *                           which may synchronise); a grown-out buffer is kept
+ *                           until the process ends, since work enqueued with it
+ *                           may still run.  Pass your own `work` under graph
+ *                           capture, or when a stream handle may be reused for
+ *                           a new stream while the old one still has work.
  *
  * Every call returns a dolhip_status; dolhip_last_error() gives the message
  * of the last failure on the calling thread.  The reference's functions

@@ -154,6 +154,23 @@ def test_jfa_3d_staged_rows(gpu, oracle, rng, dims, cw):
         np.testing.assert_array_equal(gpu.jump_flood_3d(w, h, d, start, cw), oracle.jfa_3d(w, h, d, start, cw))
 
 
+@pytest.mark.parametrize("bound", [3, 2047, 2049, 4000, 32767])
+@pytest.mark.parametrize("cw", [1, 2])
+def test_jfa_3d_rows_any_field(gpu, oracle, rng, bound, cw):
+    """Caller fields that no initialiser makes: components uniform in
+    [-bound, bound] with a fifth of the cells kMaxDelta3 (their y and z
+    random).  Fields within 2048 take the keyed path of the row kernel, the
+    others the general one, pass by pass; both must give the reference's
+    wrapped int16 results."""
+    for dims in [(64, 9, 7), (96, 33, 5), (512, 3, 4)]:
+        w, h, d = dims
+        f = rng.integers(-bound, bound + 1, size=(d, h, w, 3)).astype(np.int16)
+        mx = rng.random((d, h, w)) < 0.2
+        f[mx, 0] = 25000
+        np.testing.assert_array_equal(gpu.jump_flood_3d(w, h, d, f, cw), oracle.jfa_3d(w, h, d, f, cw),
+                                      err_msg=f"{dims}")
+
+
 def test_jfa_2d_config1(gpu, oracle, reference):
     """The tool's -p 100 case: 512x512, 100 points from seed 12345."""
     w = h = 512
