@@ -580,6 +580,52 @@ int dolhip_jump_flood_2d(int w, int h, int16_t* delta, int cw)
     return a.download(delta, bytes);
 }
 
+// ---- JumpFlood 2D without allocation
+//
+// The second field of the passes lives in `scratch`, a static device array
+// of DOLHIP_NOALLOC_SCRATCH_BYTES that every device holds from module load
+// until the process ends.  Host code reaches each device's copy through the
+// address hipGetSymbolAddress gives with that device current (the array's
+// name is only a host-side stand-in), looked up on the first call on a
+// device and kept.
+
+static __device__ __align__(256) uint8_t scratch[DOLHIP_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices];
+
+static int scratch_address(void** out)
+{
+    int dev = 0;
+    TRY_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(DOLHIP_ERR_DEVICE, "device %d beyond the %d the library tracks", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        TRY_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return DOLHIP_OK;
+}
+
+int dolhip_jump_flood_2d_noalloc(int w, int h, int16_t* delta, int cw, void* stream)
+{
+    TRY(check_dims(w, h));
+    TRY(check_cells2(delta, "delta"));
+    TRY(check_cw(cw));
+    if ((size_t)w * h * 4 > (size_t)DOLHIP_NOALLOC_SCRATCH_BYTES)
+        return fail(DOLHIP_ERR_INVALID_ARG,
+                    "dolhip_jump_flood_2d_noalloc: %d x %d cells need %zu bytes of scratch, more than its %d",
+                    w, h, (size_t)w * h * 4, DOLHIP_NOALLOC_SCRATCH_BYTES);
+    TRY(require_device());
+    void* tmp = nullptr;
+    TRY(scratch_address(&tmp));
+    jfa2d_schedule(w, h, delta, cw, (int16_t*)tmp, as_stream(stream));
+    return finish(as_stream(stream), "jump_flood_2d_noalloc");
+}
+
 size_t dolhip_jump_flood_3d_workspace(int w, int h, int d)
 {
     return (w < 1 || h < 1 || d < 1) ? 0 : jfa3d_workspace(w, h, d);

@@ -64,6 +64,7 @@ _SIGS = {
     "dolhip_jump_flood_2d": ([_I, _I, _P, _I], _I),
     "dolhip_jump_flood_2d_workspace": ([_I, _I], _SZ),
     "dolhip_jump_flood_2d_dev": ([_I, _I, _P, _I, _P, _P], _I),
+    "dolhip_jump_flood_2d_noalloc": ([_I, _I, _P, _I, _P], _I),
     "dolhip_jump_flood_3d": ([_I, _I, _I, _P, _I], _I),
     "dolhip_jump_flood_3d_workspace": ([_I, _I, _I], _SZ),
     "dolhip_jump_flood_3d_dev": ([_I, _I, _I, _P, _I, _P, _P], _I),

@@ -128,6 +128,17 @@ int dolhip_init_distances_i16(int w, int h, const uint32_t* mask, int16_t* dist,
 int dolhip_jump_flood_2d(int w, int h, int16_t* delta, int cw);
 size_t dolhip_jump_flood_2d_workspace(int w, int h);
 int dolhip_jump_flood_2d_dev(int w, int h, int16_t* delta, int cw, void* work, void* stream);
+/* JumpFlood 2D on a device field, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The second field of the passes is a static device array of
+ * DOLHIP_NOALLOC_SCRATCH_BYTES (16 MiB) that each device holds until the
+ * process ends, so w*h may be at most 4,194,304 cells (2048 x 2048); larger
+ * fields are rejected with DOLHIP_ERR_INVALID_ARG.  Every call on a device
+ * uses that device's one array: calls that overlap on one device (several
+ * streams or threads) race. */
+#define DOLHIP_NOALLOC_SCRATCH_BYTES (16 << 20)
+int dolhip_jump_flood_2d_noalloc(int w, int h, int16_t* delta, int cw, void* stream);
 /* JumpFlood 3D, in place (DistanceOcclusionLib.h:93, .cpp:2396-2472) */
 int dolhip_jump_flood_3d(int w, int h, int d, int16_t* delta, int cw);
 size_t dolhip_jump_flood_3d_workspace(int w, int h, int d);
