@@ -30,6 +30,8 @@ SOURCES = [
     "dol_voxel.hip",
     "dol_sweep.hip",
     "dol_occlusion.hip",
+    "dol_brute.hip",
+    "dol_raster.hip",
     "dol_capi.cpp",
     "dol_scenes.cpp",
     "dol_shim.cpp",

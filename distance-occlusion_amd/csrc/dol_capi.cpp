@@ -33,6 +33,14 @@ void launch_init_dirw(int w, int h, int d, int planes, const uint32_t* mask, flo
 void launch_occlusion_sweep(int mode, int w, int h, int d, const float* in, float* dirw, hipStream_t s);
 void launch_encode_ao(size_t n, int planes, bool rgba, const float* dirw, uint8_t* out, hipStream_t s);
 void launch_fast_sweep_3d(int w, int h, int d, int16_t* delta, int cw, hipStream_t s);
+void launch_brute_force_2d(int w, int h, int16_t* delta, int cw, hipStream_t s);
+void launch_brute_force_3d(int w, int h, int d, int16_t* delta, int cw, hipStream_t s);
+void launch_init_occ(int w, int h, const uint32_t* mask, float* occ, float scale, hipStream_t s);
+void launch_shadow_sweep(float gx, float gy, int w, int h, const float* occ, float* shadow, hipStream_t s);
+template <class T>
+void launch_chamfer(int w, int h, T* dist, hipStream_t s);
+void launch_danielsson_2d(int w, int h, int16_t* delta, int cw, hipStream_t s);
+void launch_danielsson_3d(int w, int h, int d, int16_t* delta, int cw, hipStream_t s);
 void launch_jfa2d_pass(int step, int w, int h, const int16_t* in, int16_t* out, int cw,
                        hipStream_t s);
 size_t jfa3d_workspace(int w, int h, int d);
@@ -292,10 +300,12 @@ static int check_cells2(const void* p, const char* name)
     return DOLHIP_OK;
 }
 
+// cell width of the delta units: 1 for whole cells, 2 for the half cells of
+// the border methods; any positive value the int16 deltas can hold
 static int check_cw(int cw)
 {
-    if (cw < 1 || cw > 2)
-        return fail(DOLHIP_ERR_INVALID_ARG, "cw must be 1 (cell units) or 2 (half-cell units), got %d", cw);
+    if (cw < 1 || cw > 16384)
+        return fail(DOLHIP_ERR_INVALID_ARG, "cw must be in 1 .. 16384, got %d", cw);
     return DOLHIP_OK;
 }
 
@@ -705,6 +715,231 @@ int dolhip_fast_sweep_3d(int w, int h, int d, int16_t* delta, int cw)
     launch_fast_sweep_3d(w, h, d, a.as<int16_t>(), cw, 0);
     TRY(finish_sync("fast_sweep_3d"));
     return a.download(delta, bytes);
+}
+
+// ---- BruteForce
+
+static int check_brute(long long cells)
+{
+    if (cells >= (1LL << 31))
+        return fail(DOLHIP_ERR_INVALID_ARG, "BruteForce takes at most 2^31 - 1 cells");
+    return DOLHIP_OK;
+}
+
+int dolhip_brute_force_2d_dev(int w, int h, int16_t* delta, int cw, void* stream)
+{
+    TRY(check_dims(w, h));
+    TRY(check_cells2(delta, "delta"));
+    TRY(check_cw(cw));
+    TRY(check_brute((long long)w * h));
+    TRY(require_device());
+    launch_brute_force_2d(w, h, delta, cw, as_stream(stream));
+    return finish(as_stream(stream), "brute_force_2d");
+}
+
+int dolhip_brute_force_2d(int w, int h, int16_t* delta, int cw)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_cw(cw));
+    TRY(check_brute((long long)w * h));
+    TRY(require_device());
+    const size_t bytes = (size_t)w * h * 4;
+    DevBuf a;
+    TRY(a.upload(delta, bytes));
+    launch_brute_force_2d(w, h, a.as<int16_t>(), cw, 0);
+    TRY(finish_sync("brute_force_2d"));
+    return a.download(delta, bytes);
+}
+
+int dolhip_brute_force_3d_dev(int w, int h, int d, int16_t* delta, int cw, void* stream)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_cw(cw));
+    TRY(check_brute((long long)w * h * d));
+    TRY(require_device());
+    launch_brute_force_3d(w, h, d, delta, cw, as_stream(stream));
+    return finish(as_stream(stream), "brute_force_3d");
+}
+
+int dolhip_brute_force_3d(int w, int h, int d, int16_t* delta, int cw)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_cw(cw));
+    TRY(check_brute((long long)w * h * d));
+    TRY(require_device());
+    const size_t bytes = (size_t)w * h * d * 6;
+    DevBuf a;
+    TRY(a.upload(delta, bytes));
+    launch_brute_force_3d(w, h, d, a.as<int16_t>(), cw, 0);
+    TRY(finish_sync("brute_force_3d"));
+    return a.download(delta, bytes);
+}
+
+// ---- raster-order methods: one workgroup walks the lines in order
+
+// the LDS row copy of Danielsson holds a row of cells (4 B in 2D, 6 B in 3D)
+static int check_row_lds(int w, int bytes_per_cell, const char* what)
+{
+    if ((long long)w * bytes_per_cell + 16 > 160 * 1024)
+        return fail(DOLHIP_ERR_INVALID_ARG, "%s takes rows of at most %d cells", what,
+                    (160 * 1024 - 16) / bytes_per_cell);
+    return DOLHIP_OK;
+}
+
+int dolhip_danielsson_2d_dev(int w, int h, int16_t* delta, int cw, void* stream)
+{
+    TRY(check_dims(w, h));
+    TRY(check_cells2(delta, "delta"));
+    TRY(check_cw(cw));
+    TRY(check_row_lds(w, 4, "Danielsson 2D"));
+    TRY(require_device());
+    launch_danielsson_2d(w, h, delta, cw, as_stream(stream));
+    return finish(as_stream(stream), "danielsson_2d");
+}
+
+int dolhip_danielsson_2d(int w, int h, int16_t* delta, int cw)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_cw(cw));
+    TRY(check_row_lds(w, 4, "Danielsson 2D"));
+    TRY(require_device());
+    const size_t bytes = (size_t)w * h * 4;
+    DevBuf a;
+    TRY(a.upload(delta, bytes));
+    launch_danielsson_2d(w, h, a.as<int16_t>(), cw, 0);
+    TRY(finish_sync("danielsson_2d"));
+    return a.download(delta, bytes);
+}
+
+int dolhip_danielsson_3d_dev(int w, int h, int d, int16_t* delta, int cw, void* stream)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_cw(cw));
+    TRY(check_row_lds(w, 6, "Danielsson 3D"));
+    TRY(require_device());
+    launch_danielsson_3d(w, h, d, delta, cw, as_stream(stream));
+    return finish(as_stream(stream), "danielsson_3d");
+}
+
+int dolhip_danielsson_3d(int w, int h, int d, int16_t* delta, int cw)
+{
+    TRY(check_dims(w, h, d));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_cw(cw));
+    TRY(check_row_lds(w, 6, "Danielsson 3D"));
+    TRY(require_device());
+    const size_t bytes = (size_t)w * h * d * 6;
+    DevBuf a;
+    TRY(a.upload(delta, bytes));
+    launch_danielsson_3d(w, h, d, a.as<int16_t>(), cw, 0);
+    TRY(finish_sync("danielsson_3d"));
+    return a.download(delta, bytes);
+}
+
+extern "C++" {
+// Chamfer's multiplier (DistanceOcclusionLib.cpp:912-1006 return d0): distances
+// come out in units of 1/3 cell
+static const int kChamferMultiplier = 3;
+
+template <class T>
+static int chamfer_dev(int w, int h, T* dist, int* multiplier, void* stream)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(dist, "distances"));
+    TRY(require_device());
+    launch_chamfer<T>(w, h, dist, as_stream(stream));
+    TRY(finish(as_stream(stream), "chamfer"));
+    if (multiplier) *multiplier = kChamferMultiplier;
+    return DOLHIP_OK;
+}
+
+template <class T>
+static int chamfer_host(int w, int h, T* dist, int* multiplier)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(dist, "distances"));
+    TRY(require_device());
+    const size_t bytes = (size_t)w * h * sizeof(T);
+    DevBuf a;
+    TRY(a.upload(dist, bytes));
+    launch_chamfer<T>(w, h, a.as<T>(), 0);
+    TRY(finish_sync("chamfer"));
+    TRY(a.download(dist, bytes));
+    if (multiplier) *multiplier = kChamferMultiplier;
+    return DOLHIP_OK;
+}
+}  // extern "C++"
+
+int dolhip_chamfer_i16(int w, int h, int16_t* dist, int* multiplier)
+{
+    return chamfer_host(w, h, dist, multiplier);
+}
+int dolhip_chamfer_i32(int w, int h, int32_t* dist, int* multiplier)
+{
+    return chamfer_host(w, h, dist, multiplier);
+}
+int dolhip_chamfer_i16_dev(int w, int h, int16_t* dist, int* multiplier, void* stream)
+{
+    return chamfer_dev(w, h, dist, multiplier, stream);
+}
+int dolhip_chamfer_i32_dev(int w, int h, int32_t* dist, int* multiplier, void* stream)
+{
+    return chamfer_dev(w, h, dist, multiplier, stream);
+}
+
+int dolhip_init_occ_dev(int w, int h, const uint32_t* mask, float* occ, float scale, void* stream)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(occ, "occlusion"));
+    TRY(require_device());
+    launch_init_occ(w, h, mask, occ, scale, as_stream(stream));
+    return finish(as_stream(stream), "init_occ");
+}
+
+int dolhip_init_occ(int w, int h, const uint32_t* mask, float* occ, float scale)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(occ, "occlusion"));
+    TRY(require_device());
+    const size_t bytes = (size_t)w * h * sizeof(float);
+    DevBuf m, o;
+    TRY(m.upload(mask, mask_bytes(w, h)));
+    TRY(o.alloc(bytes));
+    launch_init_occ(w, h, m.as<uint32_t>(), o.as<float>(), scale, 0);
+    TRY(finish_sync("init_occ"));
+    return o.download(occ, bytes);
+}
+
+int dolhip_shadow_sweep_dev(float gx, float gy, int w, int h, const float* occ, float* shadow, void* stream)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(occ, "occlusion"));
+    TRY(check_ptr(shadow, "shadow"));
+    TRY(require_device());
+    launch_shadow_sweep(gx, gy, w, h, occ, shadow, as_stream(stream));
+    return finish(as_stream(stream), "shadow_sweep");
+}
+
+int dolhip_shadow_sweep(float gx, float gy, int w, int h, const float* occ, float* shadow)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(occ, "occlusion"));
+    TRY(check_ptr(shadow, "shadow"));
+    TRY(require_device());
+    const size_t bytes = (size_t)w * h * sizeof(float);
+    DevBuf i, o;
+    TRY(i.upload(occ, bytes));
+    TRY(o.alloc(bytes));
+    launch_shadow_sweep(gx, gy, w, h, i.as<float>(), o.as<float>(), 0);
+    TRY(finish_sync("shadow_sweep"));
+    return o.download(shadow, bytes);
 }
 
 int dolhip_init_dirw_2d_dev(int w, int h, const uint32_t* mask, float* dirw4, void* stream)

@@ -2,7 +2,7 @@
 // over the C ABI (include/dol_hip.h).
 //
 // Each function keeps the reference's signature, its in-place host-memory
-// contract and its synchronous behaviour (DistanceOcclusionLib.h:21-98).  The
+// contract and its synchronous behaviour (DistanceOcclusionLib.h:21-117).  The
 // reference reports nothing, so a failure here -- a bad argument the reference
 // would have run into undefined behaviour with, or no usable device -- prints
 // the library's message and aborts.
@@ -92,6 +92,33 @@ void DOL::FastSweep(int w, int h, cCellDelta2 delta[], int cw)
 { require(dolhip_fast_sweep_2d(w, h, raw(delta), cw), "FastSweep"); }
 void DOL::FastSweep(int w, int h, int d, cCellDelta3 delta[], int cw)
 { require(dolhip_fast_sweep_3d(w, h, d, raw(delta), cw), "FastSweep"); }
+void DOL::Danielsson(int w, int h, cCellDelta2 delta[], int cw)
+{ require(dolhip_danielsson_2d(w, h, raw(delta), cw), "Danielsson"); }
+void DOL::Danielsson(int w, int h, int d, cCellDelta3 delta[], int cw)
+{ require(dolhip_danielsson_3d(w, h, d, raw(delta), cw), "Danielsson"); }
+void DOL::BruteForce(int w, int h, cCellDelta2 delta[], int cw)
+{ require(dolhip_brute_force_2d(w, h, raw(delta), cw), "BruteForce"); }
+void DOL::BruteForce(int w, int h, int d, cCellDelta3 delta[], int cw)
+{ require(dolhip_brute_force_3d(w, h, d, raw(delta), cw), "BruteForce"); }
+
+int DOL::Chamfer(int w, int h, int16_t distances[])
+{
+    int m = 0;
+    require(dolhip_chamfer_i16(w, h, distances, &m), "Chamfer");
+    return m;
+}
+int DOL::Chamfer(int w, int h, int32_t distances[])
+{
+    int m = 0;
+    require(dolhip_chamfer_i32(w, h, distances, &m), "Chamfer");
+    return m;
+}
+
+void DOL::InitOccFromBitMask(int w, int h, const uint32_t mask[], float occlusion[], float scale)
+{ require(dolhip_init_occ(w, h, mask, occlusion, scale), "InitOccFromBitMask"); }
+void DOL::ShadowSweep(float gx, float gy, int w, int h, const float* occlusion, float* shadow)
+{ require(dolhip_shadow_sweep(gx, gy, w, h, occlusion, shadow), "ShadowSweep"); }
+
 void DOL::InitDirWFromBitMask(int w, int h, const uint32_t mask[], float dirW4[])
 { require(dolhip_init_dirw_2d(w, h, mask, dirW4), "InitDirWFromBitMask"); }
 void DOL::InitDirWFromBitMask(int w, int h, int d, const uint32_t mask[], float dirW8[])

@@ -72,6 +72,22 @@ _SIGS = {
     "dolhip_fast_sweep_2d_dev": ([_I, _I, _P, _I, _P], _I),
     "dolhip_fast_sweep_3d": ([_I, _I, _I, _P, _I], _I),
     "dolhip_fast_sweep_3d_dev": ([_I, _I, _I, _P, _I, _P], _I),
+    "dolhip_brute_force_2d": ([_I, _I, _P, _I], _I),
+    "dolhip_brute_force_2d_dev": ([_I, _I, _P, _I, _P], _I),
+    "dolhip_brute_force_3d": ([_I, _I, _I, _P, _I], _I),
+    "dolhip_brute_force_3d_dev": ([_I, _I, _I, _P, _I, _P], _I),
+    "dolhip_danielsson_2d": ([_I, _I, _P, _I], _I),
+    "dolhip_danielsson_2d_dev": ([_I, _I, _P, _I, _P], _I),
+    "dolhip_danielsson_3d": ([_I, _I, _I, _P, _I], _I),
+    "dolhip_danielsson_3d_dev": ([_I, _I, _I, _P, _I, _P], _I),
+    "dolhip_chamfer_i16": ([_I, _I, _P, _P], _I),
+    "dolhip_chamfer_i16_dev": ([_I, _I, _P, _P, _P], _I),
+    "dolhip_chamfer_i32": ([_I, _I, _P, _P], _I),
+    "dolhip_chamfer_i32_dev": ([_I, _I, _P, _P, _P], _I),
+    "dolhip_init_occ": ([_I, _I, _P, _P, _F], _I),
+    "dolhip_init_occ_dev": ([_I, _I, _P, _P, _F, _P], _I),
+    "dolhip_shadow_sweep": ([_F, _F, _I, _I, _P, _P], _I),
+    "dolhip_shadow_sweep_dev": ([_F, _F, _I, _I, _P, _P, _P], _I),
     "dolhip_init_dirw_2d": ([_I, _I, _P, _P], _I),
     "dolhip_init_dirw_2d_dev": ([_I, _I, _P, _P, _P], _I),
     "dolhip_init_dirw_3d": ([_I, _I, _I, _P, _P], _I),
@@ -292,6 +308,48 @@ class DolHip:
     def fast_sweep_3d(self, w, h, d, delta, cw=1):
         out = np.ascontiguousarray(delta, dtype=np.int16).copy()
         self._call("dolhip_fast_sweep_3d", w, h, d, _addr(out), cw)
+        return out
+
+    def brute_force_2d(self, w, h, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self._call("dolhip_brute_force_2d", w, h, _addr(out), cw)
+        return out
+
+    def brute_force_3d(self, w, h, d, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self._call("dolhip_brute_force_3d", w, h, d, _addr(out), cw)
+        return out
+
+    def danielsson_2d(self, w, h, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self._call("dolhip_danielsson_2d", w, h, _addr(out), cw)
+        return out
+
+    def danielsson_3d(self, w, h, d, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self._call("dolhip_danielsson_3d", w, h, d, _addr(out), cw)
+        return out
+
+    def chamfer(self, w, h, dist):
+        """Chamfer in place on int16 or int32 distances; returns (out, multiplier)."""
+        dist = np.asarray(dist)
+        if dist.dtype not in (np.int16, np.int32):
+            raise TypeError("Chamfer takes int16 or int32 distances")
+        out = np.ascontiguousarray(dist).copy()
+        mult = ctypes.c_int(0)
+        name = "dolhip_chamfer_i16" if out.dtype == np.int16 else "dolhip_chamfer_i32"
+        self._call(name, w, h, _addr(out), ctypes.addressof(mult))
+        return out, mult.value
+
+    def init_occ(self, w, h, mask, scale=1.0):
+        out = np.empty((h, w), dtype=np.float32)
+        self._call("dolhip_init_occ", w, h, _addr(np.ascontiguousarray(mask, np.uint32)), _addr(out), scale)
+        return out
+
+    def shadow_sweep(self, gx, gy, w, h, occ):
+        occ = np.ascontiguousarray(occ, dtype=np.float32)
+        out = np.empty((h, w), dtype=np.float32)
+        self._call("dolhip_shadow_sweep", gx, gy, w, h, _addr(occ), _addr(out))
         return out
 
     def init_dirw_2d(self, w, h, mask):

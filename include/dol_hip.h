@@ -153,6 +153,42 @@ int dolhip_fast_sweep_2d_dev(int w, int h, int16_t* delta, int cw, void* stream)
 int dolhip_fast_sweep_3d(int w, int h, int d, int16_t* delta, int cw);
 int dolhip_fast_sweep_3d_dev(int w, int h, int d, int16_t* delta, int cw, void* stream);
 
+/* BruteForce, in place (DistanceOcclusionLib.h:96-97; .cpp:1087-1110 2D,
+ * 2475-2505 3D): the reference's raster order of sources kept exactly (a
+ * cell that is a boundary cell at its turn relaxes the whole field), run by
+ * one workgroup; (sources x cells) work, for checking.  At most 2^31 - 1
+ * cells. */
+int dolhip_brute_force_2d(int w, int h, int16_t* delta, int cw);
+int dolhip_brute_force_2d_dev(int w, int h, int16_t* delta, int cw, void* stream);
+int dolhip_brute_force_3d(int w, int h, int d, int16_t* delta, int cw);
+int dolhip_brute_force_3d_dev(int w, int h, int d, int16_t* delta, int cw, void* stream);
+
+/* Danielsson, in place (DistanceOcclusionLib.h:88-89; .cpp:702-778 2D,
+ * 2264-2394 3D), bit-identical.  Its scans along a row keep a running choice
+ * that is not associative, so one workgroup walks the rows (slices) in
+ * order; rows of at most 40,956 cells (2D) or 27,304 cells (3D). */
+int dolhip_danielsson_2d(int w, int h, int16_t* delta, int cw);
+int dolhip_danielsson_2d_dev(int w, int h, int16_t* delta, int cw, void* stream);
+int dolhip_danielsson_3d(int w, int h, int d, int16_t* delta, int cw);
+int dolhip_danielsson_3d_dev(int w, int h, int d, int16_t* delta, int cw, void* stream);
+
+/* Chamfer 3-4, in place (DistanceOcclusionLib.h:65-66; .cpp:912-1006):
+ * *multiplier (if not null) gets 3, the divisor to cells.  Exact (each row a
+ * min-scan); sums taken in 64 bits. */
+int dolhip_chamfer_i16(int w, int h, int16_t* dist, int* multiplier);
+int dolhip_chamfer_i16_dev(int w, int h, int16_t* dist, int* multiplier, void* stream);
+int dolhip_chamfer_i32(int w, int h, int32_t* dist, int* multiplier);
+int dolhip_chamfer_i32_dev(int w, int h, int32_t* dist, int* multiplier, void* stream);
+
+/* Occlusion image and shadow sweep (DistanceOcclusionLib.h:101-103;
+ * .cpp:2773-2876): occ = scale on set bits, else 0; shadow from occ for a
+ * light along (gx, gy), float results bit-identical. */
+int dolhip_init_occ(int w, int h, const uint32_t* mask, float* occ, float scale);
+int dolhip_init_occ_dev(int w, int h, const uint32_t* mask, float* occ, float scale, void* stream);
+int dolhip_shadow_sweep(float gx, float gy, int w, int h, const float* occ, float* shadow);
+int dolhip_shadow_sweep_dev(float gx, float gy, int w, int h, const float* occ, float* shadow,
+                            void* stream);
+
 /* Direction fields and occlusion sweeps (DistanceOcclusionLib.h:104-112;
  * .cpp:1114-1209, 2507-2769), float results bit-identical.  dirw4 is 4 planes
  * of w*h floats, dirw8 8 planes of w*h*d; the sweeps update in place, or

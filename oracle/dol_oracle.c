@@ -1557,3 +1557,211 @@ void oc_glyph_sdf_batch(int n, const int32_t* w, const int32_t* h, const int64_t
     }
     (void)threads;
 }
+
+/* ------------------------------------------- raster-order methods (off the
+ * data-parallel path; the tool's -m 2 and -m 4 methods and its shadows) */
+
+/* Danielsson 2D, DistanceOcclusionLib.cpp:702-778: a down sweep (each row from
+ * its second takes the row above with (0, -cw), then a left-to-right scan
+ * with (-cw, 0) and a right-to-left scan with (+cw, 0)), then an up sweep
+ * (from the row below with (0, +cw), then the same two scans). */
+static void danielsson_rows(int w, int h, int16_t* delta, int cw)
+{
+    for (int j = 1; j < h; j++) {
+        const int16_t* row0 = delta + 2 * (size_t)(j - 1) * w;
+        int16_t* row1 = delta + 2 * (size_t)j * w;
+        for (int i = 0; i < w; i++) closer2(row1 + 2 * i, row0[2 * i], row0[2 * i + 1] - cw);
+        for (int i = 1; i < w; i++) closer2(row1 + 2 * i, row1[2 * (i - 1)] - cw, row1[2 * (i - 1) + 1]);
+        for (int i = w - 2; i >= 0; i--) closer2(row1 + 2 * i, row1[2 * (i + 1)] + cw, row1[2 * (i + 1) + 1]);
+    }
+    for (int j = h - 1; j > 0; j--) {
+        const int16_t* row0 = delta + 2 * (size_t)j * w;
+        int16_t* row1 = delta + 2 * (size_t)(j - 1) * w;
+        for (int i = 0; i < w; i++) closer2(row1 + 2 * i, row0[2 * i], row0[2 * i + 1] + cw);
+        for (int i = 1; i < w; i++) closer2(row1 + 2 * i, row1[2 * (i - 1)] - cw, row1[2 * (i - 1) + 1]);
+        for (int i = w - 2; i >= 0; i--) closer2(row1 + 2 * i, row1[2 * (i + 1)] + cw, row1[2 * (i + 1) + 1]);
+    }
+}
+
+void oc_danielsson_2d(int w, int h, int16_t* delta, int cw) { danielsson_rows(w, h, delta, cw); }
+
+/* DanielssonSlice, DistanceOcclusionLib.cpp:2264-2346: the 2D sweeps of one
+ * slice, z carried along */
+static void danielsson_slice(int w, int h, int16_t* s, int cw)
+{
+    for (int j = 1; j < h; j++) {
+        const int16_t* row0 = s + 3 * (size_t)(j - 1) * w;
+        int16_t* row1 = s + 3 * (size_t)j * w;
+        for (int i = 0; i < w; i++) closer3(row1 + 3 * i, row0[3 * i], row0[3 * i + 1] - cw, row0[3 * i + 2]);
+        for (int i = 1; i < w; i++)
+            closer3(row1 + 3 * i, row1[3 * (i - 1)] - cw, row1[3 * (i - 1) + 1], row1[3 * (i - 1) + 2]);
+        for (int i = w - 2; i >= 0; i--)
+            closer3(row1 + 3 * i, row1[3 * (i + 1)] + cw, row1[3 * (i + 1) + 1], row1[3 * (i + 1) + 2]);
+    }
+    for (int j = h - 1; j > 0; j--) {
+        const int16_t* row0 = s + 3 * (size_t)j * w;
+        int16_t* row1 = s + 3 * (size_t)(j - 1) * w;
+        for (int i = 0; i < w; i++) closer3(row1 + 3 * i, row0[3 * i], row0[3 * i + 1] + cw, row0[3 * i + 2]);
+        for (int i = 1; i < w; i++)
+            closer3(row1 + 3 * i, row1[3 * (i - 1)] - cw, row1[3 * (i - 1) + 1], row1[3 * (i - 1) + 2]);
+        for (int i = w - 2; i >= 0; i--)
+            closer3(row1 + 3 * i, row1[3 * (i + 1)] + cw, row1[3 * (i + 1) + 1], row1[3 * (i + 1) + 2]);
+    }
+}
+
+/* Danielsson 3D, DistanceOcclusionLib.cpp:2348-2394: each slice from the
+ * second takes the slice before with (0, 0, -cw) and is swept as a 2D slice;
+ * then the same from the last slice back with (0, 0, +cw).  The first slice
+ * of each direction is not swept on its own. */
+void oc_danielsson_3d(int w, int h, int d, int16_t* delta, int cw)
+{
+    const size_t n = (size_t)w * h;
+    for (int k = 1; k < d; k++) {
+        const int16_t* s0 = delta + 3 * (k - 1) * n;
+        int16_t* s1 = delta + 3 * k * n;
+        for (size_t i = 0; i < n; i++) closer3(s1 + 3 * i, s0[3 * i], s0[3 * i + 1], s0[3 * i + 2] - cw);
+        danielsson_slice(w, h, s1, cw);
+    }
+    for (int k = d - 2; k >= 0; k--) {
+        const int16_t* s0 = delta + 3 * (k + 1) * n;
+        int16_t* s1 = delta + 3 * k * n;
+        for (size_t i = 0; i < n; i++) closer3(s1 + 3 * i, s0[3 * i], s0[3 * i + 1], s0[3 * i + 2] + cw);
+        danielsson_slice(w, h, s1, cw);
+    }
+}
+
+/* Chamfer (Borgefors 3-4), DistanceOcclusionLib.cpp:912-1006, on int64
+ * copies of the values: d(x) = min(d(x), neighbour + 3 or + 4), a forward
+ * raster pass (first row: left neighbour; other rows: the three above, then
+ * the left one) and a reverse pass (last row: right neighbour; other rows:
+ * the three below, then the right one).  Rows of one cell have no diagonal
+ * neighbours.  Returns 3, the divisor of the result.  The sums are taken
+ * without overflow (the reference's int32 form overflows on values within
+ * 4 of INT32_MAX). */
+static int chamfer64(int w, int h, int64_t* v)
+{
+    const int64_t d0 = 3, d1 = 4;
+#define CHM(A, B) do { if ((A) > (B)) (A) = (B); } while (0)
+    for (int x = 1; x < w; x++) CHM(v[x], v[x - 1] + d0);
+    for (int y = 1; y < h; y++) {
+        int64_t* r0 = v + (size_t)(y - 1) * w;
+        int64_t* r1 = v + (size_t)y * w;
+        CHM(r1[0], r0[0] + d0);
+        if (w > 1) CHM(r1[0], r0[1] + d1);
+        for (int x = 1; x < w - 1; x++) {
+            CHM(r1[x], r0[x - 1] + d1);
+            CHM(r1[x], r0[x] + d0);
+            CHM(r1[x], r0[x + 1] + d1);
+            CHM(r1[x], r1[x - 1] + d0);
+        }
+        if (w > 1) {
+            CHM(r1[w - 1], r0[w - 2] + d1);
+            CHM(r1[w - 1], r0[w - 1] + d0);
+            CHM(r1[w - 1], r1[w - 2] + d0);
+        }
+    }
+    {
+        int64_t* r0 = v + (size_t)(h - 1) * w;
+        for (int x = w - 2; x >= 0; x--) CHM(r0[x], r0[x + 1] + d0);
+    }
+    for (int y = h - 2; y >= 0; y--) {
+        int64_t* r0 = v + (size_t)(y + 1) * w;
+        int64_t* r1 = v + (size_t)y * w;
+        CHM(r1[w - 1], r0[w - 1] + d0);
+        if (w > 1) CHM(r1[w - 1], r0[w - 2] + d1);
+        for (int x = w - 2; x > 0; x--) {
+            CHM(r1[x], r0[x + 1] + d1);
+            CHM(r1[x], r0[x] + d0);
+            CHM(r1[x], r0[x - 1] + d1);
+            CHM(r1[x], r1[x + 1] + d0);
+        }
+        if (w > 1) {
+            CHM(r1[0], r0[1] + d1);
+            CHM(r1[0], r0[0] + d0);
+            CHM(r1[0], r1[1] + d0);
+        }
+    }
+#undef CHM
+    return (int)d0;
+}
+
+int oc_chamfer_i16(int w, int h, int16_t* dist)
+{
+    size_t n = (size_t)w * h;
+    int64_t* v = (int64_t*)malloc(n * sizeof(int64_t) + 8);
+    for (size_t i = 0; i < n; i++) v[i] = dist[i];
+    int r = chamfer64(w, h, v);
+    for (size_t i = 0; i < n; i++) dist[i] = (int16_t)v[i];
+    free(v);
+    return r;
+}
+
+int oc_chamfer_i32(int w, int h, int32_t* dist)
+{
+    size_t n = (size_t)w * h;
+    int64_t* v = (int64_t*)malloc(n * sizeof(int64_t) + 8);
+    for (size_t i = 0; i < n; i++) v[i] = dist[i];
+    int r = chamfer64(w, h, v);
+    for (size_t i = 0; i < n; i++) dist[i] = (int32_t)v[i];
+    free(v);
+    return r;
+}
+
+/* InitOccFromBitMask, DistanceOcclusionLib.cpp:2864-2867: scale where a bit is
+ * set, 0 elsewhere */
+void oc_init_occ(int w, int h, const uint32_t* mask, float* occ, float scale)
+{
+    int ws = mask_stride(w);
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++)
+            occ[(size_t)y * w + x] = ((mask[(size_t)y * ws + (x >> 5)] >> (x & 31)) & 1) ? scale : 0.0f;
+}
+
+/* ShadowSweep, DistanceOcclusionLib.cpp:2773-2876.  Light along (gx, gy): the
+ * sweep runs along y (rows) when |gx| <= |gy|, else along x (columns), from
+ * the side the light comes from (s = 0 when the component is positive).  The
+ * first line copies the occlusion; every later cell takes
+ * m = prev[i - di] g1 + prev[i] g0 (the first cell of a line m = prev[first] g0)
+ * and stores m + (1 - m) occ, g1 = |minor / major| and g0 = 1 - g1, in the
+ * reference's float operation order. */
+static void shadow_lines(float g, int sx, int sy, int w, int h, int rows, const float* occ, float* sh)
+{
+    const float g1 = g, g0 = 1.0f - g1;
+    const int dx = 1 - 2 * sx, dy = 1 - 2 * sy;
+    /* a line is a row (rows) or a column; ib, ie, di walk along it; jb, dj
+     * step from line to line */
+    int ib, ie, di, jb, dj, lines;
+    if (rows) {
+        di = dx; ib = sx * (w - 1); ie = ib + di * w;
+        dj = dy * w; jb = sy * (h - 1) * w; lines = h;
+    } else {
+        const int wh = w * h;
+        ib = (wh - w) * sy; ie = ib + wh * dy; di = dy * w;
+        jb = (w - 1) * sx; dj = dx; lines = w;
+    }
+    const float* i0 = occ + jb;
+    float* o0 = sh + jb;
+    for (int i = ib; i != ie; i += di) o0[i] = i0[i];
+    for (int j = 1; j < lines; j++) {
+        const float* i1 = i0 + dj;
+        float* o1 = o0 + dj;
+        float m = o0[ib] * g0;
+        o1[ib] = m + (1.0f - m) * i1[ib];
+        for (int i = ib + di; i != ie; i += di) {
+            float mm = o0[i - di] * g1 + o0[i] * g0;
+            o1[i] = mm + (1.0f - mm) * i1[i];
+        }
+        i0 = i1;
+        o0 = o1;
+    }
+}
+
+void oc_shadow_sweep(float gx, float gy, int w, int h, const float* occ, float* shadow)
+{
+    const int sx = gx > 0.0f ? 0 : 1;
+    const int sy = gy > 0.0f ? 0 : 1;
+    if (fabsf(gx) <= fabsf(gy))
+        shadow_lines(fabsf(gx / gy), sx, sy, w, h, 1, occ, shadow);
+    else
+        shadow_lines(fabsf(gy / gx), sx, sy, w, h, 0, occ, shadow);
+}

@@ -118,6 +118,16 @@ void oc_seed_nearest_2d_mt(int w, int h, int n, const float* seeds, int32_t* idx
 void oc_seed_grad_2d(int w, int h, int n, const float* seeds, const int32_t* idx,
                      const float* weight, double* grad);
 
+/* raster-order methods off the data-parallel path (Lib:702-778, 912-1013,
+ * 2264-2394, 2773-2876): Danielsson, Chamfer (returns its divisor 3), the
+ * occlusion init and the shadow sweep */
+void oc_danielsson_2d(int w, int h, int16_t* delta, int cw);
+void oc_danielsson_3d(int w, int h, int d, int16_t* delta, int cw);
+int  oc_chamfer_i16(int w, int h, int16_t* dist);
+int  oc_chamfer_i32(int w, int h, int32_t* dist);
+void oc_init_occ(int w, int h, const uint32_t* mask, float* occ, float scale);
+void oc_shadow_sweep(float gx, float gy, int w, int h, const float* occ, float* shadow);
+
 /* batched glyphs (extension): threshold, border init, JumpFlood(cw = 2) and
  * the signed byte image of each ragged mask */
 void oc_glyph_sdf_batch(int n, const int32_t* w, const int32_t* h, const int64_t* offset,

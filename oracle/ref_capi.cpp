@@ -57,6 +57,16 @@ void ref_occlusion_sweep_3d(int w, int h, int d, float* dirw)
 { OcclusionSweep(w, h, d, dirw); }
 void ref_occlusion_sweep_3d_from(int w, int h, int d, const float* in, float* out)
 { OcclusionSweep(w, h, d, in, out); }
+void ref_danielsson_2d(int w, int h, int16_t* delta, int cw)
+{ Danielsson(w, h, reinterpret_cast<cCellDelta2*>(delta), cw); }
+void ref_danielsson_3d(int w, int h, int d, int16_t* delta, int cw)
+{ Danielsson(w, h, d, reinterpret_cast<cCellDelta3*>(delta), cw); }
+int ref_chamfer_i16(int w, int h, int16_t* dist) { return Chamfer(w, h, dist); }
+int ref_chamfer_i32(int w, int h, int32_t* dist) { return Chamfer(w, h, dist); }
+void ref_init_occ(int w, int h, const uint32_t* mask, float* occ, float scale)
+{ InitOccFromBitMask(w, h, mask, occ, scale); }
+void ref_shadow_sweep(float gx, float gy, int w, int h, const float* occ, float* shadow)
+{ ShadowSweep(gx, gy, w, h, occ, shadow); }
 void ref_felzenszwalb_2d(int w, int h, float* dist)
 { Felzenszwalb(w, h, dist); }
 void ref_brute_force_2d(int w, int h, int16_t* delta, int cw)

@@ -284,6 +284,34 @@ class Oracle:
                                       _ptr(np.ascontiguousarray(bbox, np.float32)), _I(w), _I(h), _I(d), _ptr(mask))
         return mask
 
+    def danielsson_2d(self, w, h, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self.L.oc_danielsson_2d(_I(w), _I(h), _ptr(out), _I(cw))
+        return out
+
+    def danielsson_3d(self, w, h, d, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self.L.oc_danielsson_3d(_I(w), _I(h), _I(d), _ptr(out), _I(cw))
+        return out
+
+    def chamfer(self, w, h, dist):
+        """(result, divisor) of Chamfer on an int16 or int32 field"""
+        out = np.ascontiguousarray(dist).copy()
+        fn = self.L.oc_chamfer_i16 if out.dtype == np.int16 else self.L.oc_chamfer_i32
+        r = fn(_I(w), _I(h), _ptr(out))
+        return out, r
+
+    def init_occ(self, w, h, mask, scale=1.0):
+        out = np.empty((h, w), dtype=np.float32)
+        self.L.oc_init_occ(_I(w), _I(h), _ptr(mask), _ptr(out), _F(scale))
+        return out
+
+    def shadow_sweep(self, gx, gy, w, h, occ):
+        occ = np.ascontiguousarray(occ, dtype=np.float32)
+        out = np.empty_like(occ)
+        self.L.oc_shadow_sweep(_F(gx), _F(gy), _I(w), _I(h), _ptr(occ), _ptr(out))
+        return out
+
     def dist_2d(self, delta):
         delta = np.ascontiguousarray(delta, dtype=np.int16)
         n = delta.size // 2
@@ -428,6 +456,33 @@ class Reference:
     def felzenszwalb_2d(self, w, h, dist):
         out = np.ascontiguousarray(dist, dtype=np.float32).copy()
         self.L.ref_felzenszwalb_2d(_I(w), _I(h), _ptr(out))
+        return out
+
+    def danielsson_2d(self, w, h, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self.L.ref_danielsson_2d(_I(w), _I(h), _ptr(out), _I(cw))
+        return out
+
+    def danielsson_3d(self, w, h, d, delta, cw=1):
+        out = np.ascontiguousarray(delta, dtype=np.int16).copy()
+        self.L.ref_danielsson_3d(_I(w), _I(h), _I(d), _ptr(out), _I(cw))
+        return out
+
+    def chamfer(self, w, h, dist):
+        out = np.ascontiguousarray(dist).copy()
+        fn = self.L.ref_chamfer_i16 if out.dtype == np.int16 else self.L.ref_chamfer_i32
+        r = fn(_I(w), _I(h), _ptr(out))
+        return out, r
+
+    def init_occ(self, w, h, mask, scale=1.0):
+        out = np.empty((h, w), dtype=np.float32)
+        self.L.ref_init_occ(_I(w), _I(h), _ptr(mask), _ptr(out), _F(scale))
+        return out
+
+    def shadow_sweep(self, gx, gy, w, h, occ):
+        occ = np.ascontiguousarray(occ, dtype=np.float32)
+        out = np.empty_like(occ)
+        self.L.ref_shadow_sweep(_F(gx), _F(gy), _I(w), _I(h), _ptr(occ), _ptr(out))
         return out
 
     def brute_force_2d(self, w, h, delta, cw=1):

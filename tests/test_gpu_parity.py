@@ -392,8 +392,11 @@ def test_invalid_arguments(gpu):
         gpu.jump_flood_2d(0, 5, np.zeros((5, 0, 2), np.int16))
     assert e.value.code == dolhip.ERR_INVALID_ARG
     with pytest.raises(dolhip.DolHipError) as e:
-        gpu.jump_flood_2d(4, 4, np.zeros((4, 4, 2), np.int16), cw=3)
+        gpu.jump_flood_2d(4, 4, np.zeros((4, 4, 2), np.int16), cw=16385)
     assert e.value.code == dolhip.ERR_INVALID_ARG
+    for cw in (0, -2):
+        with pytest.raises(dolhip.DolHipError):
+            gpu.jump_flood_3d(2, 2, 2, np.zeros((2, 2, 2, 3), np.int16), cw=cw)
     with pytest.raises(dolhip.DolHipError):
         gpu._call("dolhip_init_deltas_2d", 4, 4, None, None, 0)
 

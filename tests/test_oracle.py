@@ -687,3 +687,132 @@ def test_misaligned_2d_fields_rejected():
         fn = getattr(lib, name)
         assert fn(*args(odd)) == 1, name   # DOLHIP_ERR_INVALID_ARG
         assert b"4-byte aligned" in lib.dolhip_last_error(), name
+
+
+# ------------------------------------------------ raster-order methods off the path
+
+
+def test_offpath_golden(oracle):
+    """Danielsson 2D/3D, Chamfer int16/int32, InitOccFromBitMask and
+    ShadowSweep against the reference's outputs
+    (tests/golden/make_offpath_golden.py), bit for bit."""
+    g = np.load(os.path.join(GOLDEN, "offpath_golden.npz"))
+    lights = [tuple(float(v) for v in l) for l in g["lights"]]
+    for k in range(5):
+        w, h = (int(v) for v in g[f"d2_{k}_dims"])
+        for cw in (1, 2):
+            np.testing.assert_array_equal(oracle.danielsson_2d(w, h, g[f"d2_{k}_cw{cw}_in"], cw),
+                                          g[f"d2_{k}_cw{cw}_out"])
+        for name in ("i16", "i32"):
+            res, div = oracle.chamfer(w, h, g[f"c_{k}_{name}_in"])
+            np.testing.assert_array_equal(res, g[f"c_{k}_{name}_out"])
+            assert div == int(g[f"c_{k}_div"])
+        occ = oracle.init_occ(w, h, g[f"d2_{k}_mask"], float(g[f"o_{k}_scale"]))
+        np.testing.assert_array_equal(occ, g[f"o_{k}_occ"])
+        for j, (gx, gy) in enumerate(lights):
+            np.testing.assert_array_equal(oracle.shadow_sweep(gx, gy, w, h, occ).view(np.uint32),
+                                          g[f"o_{k}_shadow{j}"].view(np.uint32))
+    for k in range(4):
+        w, h, d = (int(v) for v in g[f"d3_{k}_dims"])
+        for cw in (1, 2):
+            np.testing.assert_array_equal(oracle.danielsson_3d(w, h, d, g[f"d3_{k}_cw{cw}_in"], cw),
+                                          g[f"d3_{k}_cw{cw}_out"])
+
+
+def test_offpath_live_reference(oracle, reference, rng):
+    if reference is None:
+        pytest.skip("oracle/_ref/libdolref.so not built (reference sources absent)")
+    for trial in range(10):
+        w, h = (int(v) for v in rng.integers(2, 70, 2))
+        m = pack_mask(rng.random((h, w)) < [0.0, 0.01, 0.2, 0.6][trial % 4])
+        for cw, start in ((1, oracle.init_deltas_2d(w, h, m)), (2, oracle.border_2d(w, h, m))):
+            np.testing.assert_array_equal(oracle.danielsson_2d(w, h, start, cw), reference.danielsson_2d(w, h, start, cw))
+        bits = unpack_mask(m, w, h)
+        for dt, mx in ((np.int16, 32767 - 16), (np.int32, 2**31 - 1 - 16)):
+            start = np.where(bits, 0, mx).astype(dt)
+            a, ra = oracle.chamfer(w, h, start)
+            b, rb = reference.chamfer(w, h, start)
+            np.testing.assert_array_equal(a, b)
+            assert ra == rb == 3
+        occ = oracle.init_occ(w, h, m, 0.37)
+        np.testing.assert_array_equal(occ, reference.init_occ(w, h, m, 0.37))
+        for gx, gy in ((0.2, 0.9), (-1.0, 0.3), (0.5, -0.5), (-0.1, -2.0)):
+            np.testing.assert_array_equal(oracle.shadow_sweep(gx, gy, w, h, occ).view(np.uint32),
+                                          reference.shadow_sweep(gx, gy, w, h, occ).view(np.uint32))
+        w, h, d = (int(v) for v in rng.integers(1, 16, 3))
+        m3 = pack_mask(rng.random((d, h, w)) < [0.0, 0.02, 0.3][trial % 3])
+        for cw, start in ((1, oracle.init_deltas_3d(w, h, d, m3)), (2, oracle.border_3d(w, h, d, m3))):
+            np.testing.assert_array_equal(oracle.danielsson_3d(w, h, d, start, cw),
+                                          reference.danielsson_3d(w, h, d, start, cw))
+
+
+def test_offpath_closed_forms(oracle):
+    """Chamfer on one seed gives the 3-4 weighted distance 3 max + (4 - 3) min
+    of |dx|, |dy| (Borgefors); Danielsson on one seed gives its exact offset;
+    a shadow of an all-zero image is zero and of a full one is one."""
+    w, h = 21, 13
+    bits = np.zeros((h, w), bool)
+    bits[6, 9] = True
+    m = pack_mask(bits)
+    start = np.where(bits, 0, 32767 - 16).astype(np.int16)
+    res, div = oracle.chamfer(w, h, start)
+    yy, xx = np.indices((h, w))
+    ax, ay = np.abs(xx - 9), np.abs(yy - 6)
+    np.testing.assert_array_equal(res, 3 * np.maximum(ax, ay) + np.minimum(ax, ay))
+    assert div == 3
+    dl = oracle.danielsson_2d(w, h, oracle.init_deltas_2d(w, h, m), 1)
+    np.testing.assert_array_equal(dl[..., 0], 9 - xx)
+    np.testing.assert_array_equal(dl[..., 1], 6 - yy)
+    for fill in (0.0, 1.0):
+        occ = np.full((h, w), fill, np.float32)
+        np.testing.assert_array_equal(oracle.shadow_sweep(0.3, 1.0, w, h, occ), occ)
+
+
+# ------------------------------------------------ compositions of pinned functions
+
+
+def test_jfa_pass_composes_to_pinned_jfa(oracle, rng):
+    """The single-pass checkers, run on the reference's schedule (FloorPow2
+    halving, one extra step-1 pass when the count is odd, Lib:887-908,
+    2450-2472), give the JumpFlood the golden vectors pin."""
+    def schedule(n):
+        steps = []
+        s = 1 << (n - 1).bit_length() - 1 if n > 1 else 0
+        while s > 0:
+            steps.append(s)
+            s //= 2
+        return steps + ([1] if len(steps) % 2 else [])
+
+    for dims in ((37, 20), (64, 5), (1, 9)):
+        w, h = dims
+        a = oracle.init_deltas_2d(w, h, pack_mask(rng.random((h, w)) < 0.05))
+        for cw in (1, 2):
+            b = a.copy()
+            for s in schedule(max(w, h)):
+                b = oracle.jfa_pass_2d(s, w, h, b, cw)
+            np.testing.assert_array_equal(b, oracle.jfa_2d(w, h, a, cw))
+    for dims in ((9, 7, 5), (16, 3, 12)):
+        w, h, d = dims
+        a = oracle.init_deltas_3d(w, h, d, pack_mask(rng.random((d, h, w)) < 0.05))
+        for cw in (1, 2):
+            b = a.copy()
+            for s in schedule(max(w, h, d)):
+                b = oracle.jfa_pass_3d(s, w, h, d, b, cw)
+            np.testing.assert_array_equal(b, oracle.jfa_3d(w, h, d, a, cw))
+
+
+def test_glyph_batch_composes(oracle, rng):
+    """The batched glyph checker is the tool's border path run mask by mask
+    (threshold, border init, JumpFlood cw = 2, EncodeU8Signed with
+    maxLen = 0.25 max(w, h)), each step pinned above."""
+    ws = rng.integers(1, 70, 12)
+    hs = rng.integers(1, 70, 12)
+    offs = np.concatenate([[0], np.cumsum(ws * hs)[:-1]]).astype(np.int64)
+    data = rng.integers(0, 256, int((ws * hs).sum())).astype(np.uint8)
+    got = oracle.glyph_sdf_batch(ws, hs, offs, data, 128, False, threads=2)
+    for w, h, o in zip(ws, hs, offs):
+        w, h, o = int(w), int(h), int(o)
+        m = oracle.mask_from_bytes(w, h, data[o:o + w * h], 128, False)
+        dl = oracle.jfa_2d(w, h, oracle.border_2d(w, h, m), 2)
+        exp = oracle.sdf_u8_2d(w, h, m, dl, 0.5 * max(w, h))
+        np.testing.assert_array_equal(got[o:o + w * h], exp.ravel())
