@@ -17,22 +17,27 @@ namespace dolhip {
 // Within a 32-bit word the reference runs the corner loop, then the
 // horizontal loop, then the vertical loop over ascending bits, so when bits
 // x and x+1 share a word their writes interleave loop by loop; across a word
-// end all of bit x's writes come first.  `mask` may be global or LDS.
-__device__ __forceinline__ uint32_t border2_cell(const uint32_t* mask, int ws, int w, int h, int x,
-                                                 int y)
+// end all of bit x's writes come first.  bit(xx, yy) gives the occupancy of
+// an in-range pixel (from a bitmask in global memory or LDS, or thresholded
+// bytes); the word boundaries are the reference's mask words either way.
+template <class Bit>
+__device__ __forceinline__ uint32_t border2_cell_by(Bit bit, int w, int h, int x, int y)
 {
     const bool right = x + 1 < w;
     const bool word_end = (x & 31) == 31;
     int c[3][3];
+    // every read at an in-range pixel (rows and the right column clamped),
+    // then masked: no branches around the reads
 #pragma unroll
     for (int r = 0; r < 3; r++) {
         const int yy = y - 1 + r;
         const bool ok = yy >= 0 && yy < h;
-        const uint32_t* row = mask + (size_t)(ok ? yy : 0) * ws;
-        const int xl = x > 0 ? x - 1 : 0;
-        c[r][0] = ok ? (int)((row[xl >> 5] >> (xl & 31)) & 1u) : 0;
-        c[r][1] = ok ? (int)((row[x >> 5] >> (x & 31)) & 1u) : 0;
-        c[r][2] = (ok && right) ? (int)((row[(x + 1) >> 5] >> ((x + 1) & 31)) & 1u) : 0;
+        const int ys = min(max(yy, 0), h - 1);
+        const int xl = x > 0 ? x - 1 : 0, xr = right ? x + 1 : x;
+        const int b0 = bit(xl, ys), b1 = bit(x, ys), b2 = bit(xr, ys);
+        c[r][0] = ok ? b0 : 0;
+        c[r][1] = ok ? b1 : 0;
+        c[r][2] = (ok && right) ? b2 : 0;
     }
     // rows outside the image never take part in a window; a uniform
     // neighbourhood over the rows inside gets no write
@@ -98,6 +103,124 @@ __device__ __forceinline__ uint32_t border2_cell(const uint32_t* mask, int ws, i
     return pack2(sx, sy);
 }
 
+// the same from a bitmask of row stride ws words (global or LDS)
+__device__ __forceinline__ uint32_t border2_cell(const uint32_t* mask, int ws, int w, int h, int x,
+                                                 int y)
+{
+    return border2_cell_by(
+        [&](int xx, int yy) { return (int)((mask[(size_t)yy * ws + (xx >> 5)] >> (xx & 31)) & 1u); }, w, h,
+        x, y);
+}
+
+// Pixel x-1, x and x+1 of the 32 cells of mask word j of a row, as three
+// words (bit i = cell 32j + i): x = -1 replicates x = 0, and the right
+// neighbour of the row's last cell is made equal to the cell so that it
+// never counts.
+struct RowBits {
+    uint32_t l, m, r;
+};
+
+__device__ __forceinline__ RowBits row_bits(const uint32_t* row, int j, int ws, int w)
+{
+    RowBits b;
+    b.m = row[j];
+    const uint32_t prev = j > 0 ? row[j - 1] : 0u, next = j + 1 < ws ? row[j + 1] : 0u;
+    b.l = (b.m << 1) | (j > 0 ? prev >> 31 : (b.m & 1u));
+    b.r = (b.m >> 1) | (next << 31);
+    if (j == ws - 1) {
+        const uint32_t last = 1u << ((w - 1) & 31);
+        b.r = (b.r & ~last) | (b.m & last);
+    }
+    return b;
+}
+
+// Writes of SetBorderCellsCombo (DistanceOcclusionLib.cpp:504-599) to the 32
+// cells of one mask word, for all cells at once: bit i of each word belongs
+// to cell 32j + i.  It is the per-cell replay of border2_cell with every
+// condition evaluated as a bit mask.  Among the writes that land on a cell the
+// last unconditional one wins (order e3, f3, f4, f5; at a word's last bit the
+// reference's loop order makes it e3, f3, f5, f4), and with none of those the
+// first conditional one (e1, e2, f1, f2).
+struct Border2Word {
+    uint32_t any, xn, xp, yn, yp;   // written; x = -1; x = +1; y = -1; y = +1
+};
+
+__device__ __forceinline__ Border2Word border2_word(const RowBits& up, const RowBits& mid,
+                                                    const RowBits& dn, bool has_up, bool has_dn,
+                                                    uint32_t right)
+{
+    uint32_t e1 = 0, e2 = 0, e3 = 0, f1 = 0, f2 = 0, f5 = 0;
+    if (has_up) {   // row pair (y-1, y): this row is the lower one
+        const uint32_t vert0 = up.m ^ mid.m, horz0 = up.m ^ up.l, vprev0 = up.l ^ mid.l;
+        const uint32_t vert1 = up.r ^ mid.r, horz1 = up.r ^ up.m;
+        e1 = (vert0 ^ vprev0) & vprev0 & horz0;
+        e2 = right & (vert1 ^ vert0) & ~vert0 & horz1;
+        e3 = vert0;
+    }
+    const uint32_t horz0 = mid.m ^ mid.l, horz1 = mid.r ^ mid.m;
+    if (has_dn) {   // row pair (y, y+1): this row is the upper one
+        const uint32_t vert0 = mid.m ^ dn.m, vprev0 = mid.l ^ dn.l, vert1 = mid.r ^ dn.r;
+        f1 = (vert0 ^ vprev0) & vprev0 & ~horz0;
+        f2 = right & (vert1 ^ vert0) & ~vert0 & ~horz1;
+        f5 = vert0;
+    }
+    const uint32_t f3 = horz0, f4 = right & horz1;
+    const uint32_t last = 0x80000000u;
+    const uint32_t s5 = f5 & ~(f4 & last), s4 = f4 & ~(f5 & ~last);
+    const uint32_t s3 = f3 & ~f4 & ~f5, se = e3 & ~f3 & ~f4 & ~f5;
+    const uint32_t set = e3 | f3 | f4 | f5;
+    const uint32_t u1 = ~set & e1, u2 = ~set & ~e1 & e2;
+    const uint32_t u3 = ~set & ~e1 & ~e2 & f1, u4 = ~set & ~e1 & ~e2 & ~f1 & f2;
+    Border2Word b;
+    b.any = set | e1 | e2 | f1 | f2;
+    b.xn = s3 | u1 | u3;
+    b.xp = s4 | u2 | u4;
+    b.yn = se | u1 | u2;
+    b.yp = s5 | u3 | u4;
+    return b;
+}
+
+__device__ __forceinline__ uint32_t border2_value(const Border2Word& b, int i)
+{
+    if (!((b.any >> i) & 1u)) return pack2(kMaxDelta2, kMaxDelta2);
+    const int sx = (int)((b.xp >> i) & 1u) - (int)((b.xn >> i) & 1u);
+    const int sy = (int)((b.yp >> i) & 1u) - (int)((b.yn >> i) & 1u);
+    return pack2(sx, sy);
+}
+
+// The same three words for mask word j of a row stored in a bitmask with no
+// row padding: the row's pixel x is bit start + x of `flat` (bit b in word
+// b >> 5).  Word start + 32 j - 1 >> 5 may be index -1 (the row's first word
+// at the bitmask's first bit), so flat[-1] must be readable; its value is
+// never used.  Bits past the row's end are left as they are: each bit of
+// border2_word depends only on the same bit of its inputs, so they reach only
+// cells past the row.
+__device__ __forceinline__ RowBits row_bits_flat(const uint32_t* flat, int start, int j, int ws, int w)
+{
+    const int p = start + 32 * j - 1;     // the bit left of the word's first cell
+    const int q = p >> 5, sh = p & 31;
+    const uint32_t A = flat[q], B = flat[q + 1], C = flat[q + 2];
+    auto funnel = [&](int o) {            // bits p - sh + o .. + 31, o in sh .. sh + 2
+        const uint32_t lo = o < 32 ? A : B, hi = o < 32 ? B : C;
+        return __builtin_amdgcn_alignbit(hi, lo, (uint32_t)(o & 31));
+    };
+    RowBits b;
+    b.m = funnel(sh + 1);
+    b.l = j > 0 ? funnel(sh) : (b.m << 1) | (b.m & 1u);
+    b.r = funnel(sh + 2);
+    if (j == ws - 1) {
+        const uint32_t last = 1u << ((w - 1) & 31);
+        b.r = (b.r & ~last) | (b.m & last);
+    }
+    return b;
+}
+
+// occupancy of a byte under the tool's threshold (DistanceOcclusionTool.cpp:646-665)
+__device__ __forceinline__ int byte_bit(uint8_t v, int threshold, int less_than)
+{
+    return ((int)v >= threshold) != (less_than != 0);
+}
+
 // ------------------------------------------------------------ JumpFlood 2D
 
 // The cell is kept packed: x and y as the int16 halves of one 32-bit word.
@@ -108,10 +231,14 @@ __device__ __forceinline__ uint32_t border2_cell(const uint32_t* mask, int ws, i
 typedef unsigned short jfa_u16x2 __attribute__((ext_vector_type(2)));
 typedef short jfa_i16x2 __attribute__((ext_vector_type(2)));
 
+// (the three-operand v_dot2_i32_i16 with an inline 0: the builtin selects
+// the two-operand v_dot2c form, which needs a register zeroed by an extra
+// move before every use)
 __device__ __forceinline__ int len2_packed(uint32_t xy)
 {
-    const jfa_i16x2 v = __builtin_bit_cast(jfa_i16x2, xy);
-    return __builtin_amdgcn_sdot2(v, v, 0, false);
+    int r;
+    asm("v_dot2_i32_i16 %0, %1, %1, 0" : "=v"(r) : "v"(xy));
+    return r;
 }
 
 __device__ __forceinline__ void jfa2_tap(uint32_t& best, int& bl, uint32_t v, int ox, int oy)

@@ -143,82 +143,6 @@ template void launch_init_dist<int16_t>(int, int, const uint32_t*, int16_t*, int
 
 // ------------------------------------------------------------ border init 2D
 
-// Pixel x-1, x and x+1 of the 32 cells of mask word j of a row, as three
-// words (bit i = cell 32j + i): x = -1 replicates x = 0, and the right
-// neighbour of the row's last cell is made equal to the cell so that it
-// never counts.
-struct RowBits {
-    uint32_t l, m, r;
-};
-
-__device__ __forceinline__ RowBits row_bits(const uint32_t* row, int j, int ws, int w)
-{
-    RowBits b;
-    b.m = row[j];
-    const uint32_t prev = j > 0 ? row[j - 1] : 0u, next = j + 1 < ws ? row[j + 1] : 0u;
-    b.l = (b.m << 1) | (j > 0 ? prev >> 31 : (b.m & 1u));
-    b.r = (b.m >> 1) | (next << 31);
-    if (j == ws - 1) {
-        const uint32_t last = 1u << ((w - 1) & 31);
-        b.r = (b.r & ~last) | (b.m & last);
-    }
-    return b;
-}
-
-// Writes of SetBorderCellsCombo (DistanceOcclusionLib.cpp:504-599) to the 32
-// cells of one mask word, for all cells at once: bit i of each word belongs
-// to cell 32j + i.  It is the per-cell replay of border2_cell with every
-// condition evaluated as a bit mask.  Among the writes that land on a cell the
-// last unconditional one wins (order e3, f3, f4, f5; at a word's last bit the
-// reference's loop order makes it e3, f3, f5, f4), and with none of those the
-// first conditional one (e1, e2, f1, f2).
-struct Border2Word {
-    uint32_t any, xn, xp, yn, yp;   // written; x = -1; x = +1; y = -1; y = +1
-};
-
-__device__ __forceinline__ Border2Word border2_word(const RowBits& up, const RowBits& mid,
-                                                    const RowBits& dn, bool has_up, bool has_dn,
-                                                    uint32_t right)
-{
-    uint32_t e1 = 0, e2 = 0, e3 = 0, f1 = 0, f2 = 0, f5 = 0;
-    if (has_up) {   // row pair (y-1, y): this row is the lower one
-        const uint32_t vert0 = up.m ^ mid.m, horz0 = up.m ^ up.l, vprev0 = up.l ^ mid.l;
-        const uint32_t vert1 = up.r ^ mid.r, horz1 = up.r ^ up.m;
-        e1 = (vert0 ^ vprev0) & vprev0 & horz0;
-        e2 = right & (vert1 ^ vert0) & ~vert0 & horz1;
-        e3 = vert0;
-    }
-    const uint32_t horz0 = mid.m ^ mid.l, horz1 = mid.r ^ mid.m;
-    if (has_dn) {   // row pair (y, y+1): this row is the upper one
-        const uint32_t vert0 = mid.m ^ dn.m, vprev0 = mid.l ^ dn.l, vert1 = mid.r ^ dn.r;
-        f1 = (vert0 ^ vprev0) & vprev0 & ~horz0;
-        f2 = right & (vert1 ^ vert0) & ~vert0 & ~horz1;
-        f5 = vert0;
-    }
-    const uint32_t f3 = horz0, f4 = right & horz1;
-    const uint32_t last = 0x80000000u;
-    const uint32_t s5 = f5 & ~(f4 & last), s4 = f4 & ~(f5 & ~last);
-    const uint32_t s3 = f3 & ~f4 & ~f5, se = e3 & ~f3 & ~f4 & ~f5;
-    const uint32_t set = e3 | f3 | f4 | f5;
-    const uint32_t u1 = ~set & e1, u2 = ~set & ~e1 & e2;
-    const uint32_t u3 = ~set & ~e1 & ~e2 & f1, u4 = ~set & ~e1 & ~e2 & ~f1 & f2;
-    Border2Word b;
-    b.any = set | e1 | e2 | f1 | f2;
-    b.xn = s3 | u1 | u3;
-    b.xp = s4 | u2 | u4;
-    b.yn = se | u1 | u2;
-    b.yp = s5 | u3 | u4;
-    return b;
-}
-
-__device__ __forceinline__ uint32_t border2_value(const Border2Word& b, int i)
-{
-    if (!((b.any >> i) & 1u)) return pack2(kMaxDelta2, kMaxDelta2);
-    const int sx = (int)((b.xp >> i) & 1u) - (int)((b.xn >> i) & 1u);
-    const int sy = (int)((b.yp >> i) & 1u) - (int)((b.yn >> i) & 1u);
-    return pack2(sx, sy);
-}
-
 // Border masks of mask word j of row y.
 __device__ __forceinline__ Border2Word border2_masks(const uint32_t* mask, int ws, int w, int h,
                                                      int j, int y)

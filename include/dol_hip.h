@@ -6,7 +6,10 @@
  * border initialisation, JumpFlood in 2D and 3D, Felzenszwalb, plus an exact
  * integer squared EDT in 2D and 3D, the output encoders of the reference
  * tool, batched ragged glyph SDFs and a continuous-seed distance with its
- * gradient.
+ * gradient; and, so that the whole reference header can be served, its
+ * sweep and raster-order methods (FastSweep, Danielsson, Chamfer,
+ * BruteForce, the direction and occlusion sweeps, the occlusion image and
+ * shadow sweep, DistanceOcclusionLib.h:65-117).
  *
  * Layouts are the reference's (DistanceOcclusionLib.h:16-57):
  *   mask    uint32 words, rows padded to ceil(w/32) words, LSB first,
@@ -356,13 +359,15 @@ int dolhip_glyph_sdf_u8_dev(int w, int h, const uint8_t* bytes, int threshold, i
 /* every mask of the batch through dolhip_glyph_sdf_u8_dev, one after another
  * on `stream` (the per-image form of the same work) */
 int dolhip_batch_per_image_dev(dolhip_batch* b, const uint8_t* bytes, uint8_t* out, void* stream);
-/* tile configuration chosen by the tuner for this batch (for reports) */
+/* the chosen variant's LDS capacity in cells (masks above it run as per-step
+ * launches) and wavefronts a workgroup (for reports) */
 int dolhip_batch_config(const dolhip_batch* b, int* tile_cells, int* waves_per_group);
 /* select kernel variant `variant` (0 <= variant < count) for later calls;
  * with variant < 0 returns the number of variants instead */
 int dolhip_batch_set_variant(dolhip_batch* b, int variant);
-/* times every (threads, cells per thread) variant on this batch and keeps the
- * fastest for later dolhip_batch_sdf_u8_dev calls; overwrites `out` */
+/* times every (threads, groups of 64 cells a wavefront) variant on this batch
+ * and keeps the fastest for later dolhip_batch_sdf_u8_dev calls; overwrites
+ * `out` */
 int dolhip_batch_tune(dolhip_batch* b, const uint8_t* bytes, uint8_t* out, void* stream);
 
 /* ---- continuous seeds (extension) ---------------------------------------- */

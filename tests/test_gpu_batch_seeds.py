@@ -108,6 +108,40 @@ def test_batch_edge_sizes(gpu, torch, oracle):
     b.close()
 
 
+def test_batch_many_small_and_uniform(gpu, torch, oracle):
+    """Hundreds of small masks share each bin (several to a wavefront's
+    groups), with empty and full masks among them (no border cell: every cell
+    keeps the reference's wrapped MAX values and encodes to 1 or 255), and
+    the dark-is-inside threshold."""
+    import dolhip
+
+    rng = np.random.default_rng(11)
+    n = 700
+    w = rng.integers(1, 48, n).astype(np.int32)
+    h = rng.integers(1, 48, n).astype(np.int32)
+    off = np.concatenate([[0], np.cumsum(w.astype(np.int64) * h)[:-1]])
+    data = rng.integers(0, 256, int((w.astype(np.int64) * h).sum())).astype(np.uint8)
+    for k in range(0, n, 7):   # every 7th mask uniform
+        data[off[k]:off[k] + w[k] * h[k]] = 255 if k % 2 else 0
+    for less_than in (False, True):
+        exp = np.empty(data.size, np.uint8)
+        for i in range(n):
+            ww, hh, o = int(w[i]), int(h[i]), int(off[i])
+            m = oracle.mask_from_bytes(ww, hh, data[o:o + ww * hh], 128, less_than)
+            d = oracle.jfa_2d(ww, hh, oracle.border_2d(ww, hh, m), 2)
+            exp[o:o + ww * hh] = oracle.sdf_u8_2d(ww, hh, m, d, 0.5 * max(ww, hh)).reshape(-1)
+        b = dolhip.Batch(gpu, w, h, off, threshold=128, less_than=less_than)
+        d_in = torch.from_numpy(data).cuda()
+        d_out = torch.empty_like(d_in)
+        for v in range(b.variants()):
+            b.set_variant(v)
+            d_out.fill_(7)
+            b.run(d_in, d_out)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(d_out.cpu().numpy(), exp, err_msg=f"variant {v} less_than {less_than}")
+        b.close()
+
+
 def test_batch_rejects_bad_sizes(gpu):
     import dolhip
 
