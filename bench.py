@@ -12,11 +12,17 @@ Also reported on the same line:
                 its mean duration (HIP events on the launch stream, recorded
                 during the timed steps) against 8 TB/s
   cpu_baseline  the oracle's exact separable EDT (oracle/dol_oracle.c) on
-                the same volume, one thread, on this host; its output is
-                also compared with the GPU's, voxel for voxel
-  extras        the signed 3D JumpFlood pipeline (border init + JumpFlood in
-                half-cell units + float distances) on the same volume, and
-                batched glyph SDFs against per-image launches
+                the same volume, one thread, on this host (median of 3, CPU
+                model and cores stated); its output is also compared with
+                the GPU's, voxel for voxel
+  extras        the other workloads: signed 3D JumpFlood on the same volume,
+                batched glyph SDFs against per-image launches, 8192^2 signed
+                JumpFlood and two-field images, Felzenszwalb, the 512^2
+                100-point case, seeds, voxeliser, sweeps; each carries a
+                digest of its output and (unless --no-cpu) a parity field
+                against the oracle
+  cpu_reference the reference's JumpFlood 2D/3D (one thread and OpenMP) and
+                Felzenszwalb timed on bounded samples on this host
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extras]
 """
@@ -73,7 +79,47 @@ def load_traffic(kernel):
     return t.get(kernel)
 
 
-def bench_jfa3d(L, torch, dev, mask_t, w, h, d, fl):
+def host_info():
+    """the host's CPU model and the cores this process may use"""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return model, cores
+
+
+def median_time(fn, runs):
+    """one untimed warm-up run, then the median of `runs` timed runs (s)"""
+    fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def digest(t):
+    """sha256 (first 16 hex digits) of a tensor's bytes, for comparing runs"""
+    import hashlib
+
+    return hashlib.sha256(t.cpu().contiguous().numpy().tobytes()).hexdigest()[:16]
+
+
+def oracle_mod():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind
+
+    return oracle_bind
+
+
+def bench_jfa3d(L, torch, dev, mask_t, w, h, d, fl, cpu=True):
     """Signed 3D JumpFlood on the headline volume: border init + JumpFlood
     in half cells (cw = 2) + signed float distances."""
     cells = w * h * d
@@ -94,14 +140,25 @@ def bench_jfa3d(L, torch, dev, mask_t, w, h, d, fl):
     L.profile_enable(False)
     passes = jk["jfa3d_pass"]["launches"] // 2
     bpv = 0.125 + 6 + passes * 12 + 4
-    return {
+    out = {
         "ms": round(jms, 3),
         "gvoxels_per_s": round(cells / (jms * 1e-3) / 1e9, 3),
         "kernels": jk,
         "passes": passes,
         "algorithmic_bytes_per_voxel": bpv,
         "hbm_roofline_pct": round(100 * bpv * cells / (jms * 1e-3) / 1e9 / HBM_PEAK_GBS, 2),
+        "digest_deltas": digest(delta),
     }
+    if cpu:
+        # the whole volume against the oracle: border init, then JumpFlood on
+        # the host's threads
+        ob = oracle_mod()
+        O = ob.load_oracle()
+        m = mask_t.cpu().numpy().view(np.uint32)
+        ref = O.jfa_3d(w, h, d, O.border_3d(w, h, d, m), 2, threads=ob.host_threads())
+        got = delta.cpu().numpy().reshape(d, h, w, 3)
+        out["parity"] = {"delta_mismatches_vs_oracle": int((got != ref).any(axis=-1).sum())}
+    return out
 
 
 def bench_glyphs_batched(L, torch, dev, args):
@@ -147,19 +204,11 @@ def bench_glyphs(L, torch, dev, args):
     torch.cuda.synchronize()
     pms = (time.perf_counter() - t0) * 1e3
     same = bool(torch.equal(out_b, out_p))
-    # oracle on a sample of masks
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_bind import load_oracle
-
-    O = load_oracle()
-    host = out_b.cpu().numpy()
-    bad = 0
-    for i in range(0, n, 200):
-        ww, hh, o = int(gw[i]), int(gh[i]), int(goff[i])
-        m = O.mask_from_bytes(ww, hh, gdata[o:o + ww * hh], 128, False)
-        dd = O.jfa_2d(ww, hh, O.border_2d(ww, hh, m), 2)
-        exp = O.sdf_u8_2d(ww, hh, m, dd, 0.5 * max(ww, hh)).reshape(-1)
-        bad += int((host[o:o + ww * hh] != exp).sum())
+    # every mask against the oracle, on the host's threads
+    ob = oracle_mod()
+    O = ob.load_oracle()
+    exp = O.glyph_sdf_batch(gw, gh, goff, gdata, 128, False, threads=ob.host_threads())
+    bad = int((out_b.cpu().numpy() != exp).sum())
     b.close()
     return {
         "glyphs": n,
@@ -170,12 +219,13 @@ def bench_glyphs(L, torch, dev, args):
         "per_image_glyphs_per_s": round(n / (pms * 1e-3), 1),
         "speedup": round(pms / bms, 2),
         "batched_equals_per_image": same,
-        "oracle_sample_mismatched_bytes": bad,
+        "parity": {"mismatched_bytes_vs_oracle": bad},
+        "digest": digest(out_b),
         "tuned": {"lds_cells": cells, "waves_per_group": waves},
     }
 
 
-def bench_jfa2d(L, torch, dev):
+def bench_jfa2d(L, torch, dev, cpu=True):
     """8192 x 8192 bitmask (2000 discs and boxes): signed border init +
     JumpFlood (cw = 2) + signed float distances."""
     w = h = 8192
@@ -199,17 +249,25 @@ def bench_jfa2d(L, torch, dev):
     L.profile_enable(False)
     passes = ks["jfa2d_pass"]["launches"] // 3
     bpp = 0.125 + 4 + passes * 8 + 4
-    return {
+    res = {
         "ms": round(ms, 3),
         "gpixels_per_s": round(w * h / (ms * 1e-3) / 1e9, 3),
         "kernels": ks,
         "passes": passes,
         "algorithmic_bytes_per_pixel": bpp,
         "hbm_roofline_pct": round(100 * bpp * w * h / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 2),
+        "digest_deltas": digest(delta),
     }
+    if cpu:
+        ob = oracle_mod()
+        O = ob.load_oracle()
+        ref = O.jfa_2d(w, h, O.border_2d(w, h, mask), 2, threads=ob.host_threads())
+        got = delta.cpu().numpy().reshape(h, w, 2)
+        res["parity"] = {"delta_mismatches_vs_oracle": int((got != ref).any(axis=-1).sum())}
+    return res
 
 
-def bench_two_field(L, torch, dev):
+def bench_two_field(L, torch, dev, cpu=True):
     """The tool's -S mode on the 8192 x 8192 disc/box scene: exterior and
     interior fields (unsigned init + JumpFlood cw = 1 each) and the sdf byte
     image."""
@@ -237,17 +295,27 @@ def bench_two_field(L, torch, dev):
     # per field: mask in, init out, the passes' reads and writes; then two
     # fields in and one byte out
     bpp = 2 * (0.125 + 4 + passes * 8) + 8 + 1
-    return {
+    res = {
         "ms": round(ms, 3),
         "gpixels_per_s": round(w * h / (ms * 1e-3) / 1e9, 3),
         "kernels": ks,
         "passes_per_field": passes,
         "algorithmic_bytes_per_pixel": bpp,
         "hbm_roofline_pct": round(100 * bpp * w * h / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 2),
+        "digest_image": digest(out),
     }
+    if cpu:
+        ob = oracle_mod()
+        O = ob.load_oracle()
+        t = ob.host_threads()
+        e0 = O.jfa_2d(w, h, O.init_deltas_2d(w, h, mask, False), 1, threads=t)
+        e1 = O.jfa_2d(w, h, O.init_deltas_2d(w, h, mask, True), 1, threads=t)
+        ref = O.two_field_image_2d(w, h, e0, e1, 0.25 * w, 0)
+        res["parity"] = {"mismatched_bytes_vs_oracle": int((out.cpu().numpy().reshape(-1) != ref.reshape(-1)).sum())}
+    return res
 
 
-def bench_config1(L, torch, dev):
+def bench_config1(L, torch, dev, cpu=True):
     """The tool's -p 100 case: 512 x 512, 100 points (seed 12345): unsigned
     JumpFlood, exact EDT; microseconds per call (latency-bound)."""
     w = h = 512
@@ -269,10 +337,23 @@ def bench_config1(L, torch, dev):
 
     jfa()
     edt()
-    return {"jfa_us": round(timed(jfa, 20, torch) * 1e3, 1), "edt_us": round(timed(edt, 20, torch) * 1e3, 1)}
+    res = {"jfa_us": round(timed(jfa, 20, torch) * 1e3, 1), "edt_us": round(timed(edt, 20, torch) * 1e3, 1),
+           "digest_deltas": digest(delta), "digest_d2": digest(d2)}
+    if cpu:
+        # JumpFlood against the oracle; the exact EDT against BruteForce (the
+        # reference's checker, here the oracle's restatement)
+        O = oracle_mod().load_oracle()
+        i0 = O.init_deltas_2d(w, h, mask)
+        jd = O.jfa_2d(w, h, i0, 1)
+        bf = O.brute_force_2d(w, h, i0, 1).astype(np.int64)
+        res["parity"] = {
+            "jfa_delta_mismatches_vs_oracle": int((delta.cpu().numpy().reshape(h, w, 2) != jd).any(axis=-1).sum()),
+            "edt_mismatches_vs_brute_force": int((d2.cpu().numpy().reshape(h, w) != (bf ** 2).sum(-1)).sum()),
+        }
+    return res
 
 
-def bench_felzenszwalb(L, torch, dev):
+def bench_felzenszwalb(L, torch, dev, cpu=True):
     """The reference's own Felzenszwalb (float squared distances, in place
     after InitDistancesFromBitMask) and the exact int32 EDT on the same
     masks: the 512 x 512 100-point mask and the 8192 x 8192 disc/box scene;
@@ -318,17 +399,23 @@ def bench_felzenszwalb(L, torch, dev):
             if n:
                 ek[k] = {"avg_ms": round(avg, 4), "launches": n}
         L.profile_enable(False)
-        # float of the exact result = the reference's float values while dims <= 4096
         out[f"felzenszwalb_{name}"] = {"ms": round(fms, 4), "gpixels_per_s": round(w * h / (fms * 1e-3) / 1e9, 3),
-                                       "kernels": ks}
+                                       "kernels": ks, "digest": digest(f)}
         out[f"edt2d_{name}"] = {"ms": round(ems, 4), "gpixels_per_s": round(w * h / (ems * 1e-3) / 1e9, 3),
-                                "kernels": ek}
+                                "kernels": ek, "digest": digest(d2)}
+        if cpu:
+            O = oracle_mod().load_oracle()
+            fref = O.felzenszwalb_2d(w, h, O.init_distances_f32(w, h, mask))
+            out[f"felzenszwalb_{name}"]["parity"] = {
+                "float_bit_mismatches_vs_oracle": int((f.cpu().numpy().view(np.uint32) != fref.reshape(-1).view(np.uint32)).sum())}
+            out[f"edt2d_{name}"]["parity"] = {
+                "mismatches_vs_oracle": int((d2.cpu().numpy() != O.edt_2d(w, h, mask).reshape(-1)).sum())}
         del f, fw, d2, fl, ew
         torch.cuda.empty_cache()
     return out
 
 
-def bench_seeds(L, torch, dev):
+def bench_seeds(L, torch, dev, cpu=True):
     """2048 x 2048 grid, 4096 continuous seeds: exact nearest seed + float
     distances (forward) and the seed gradient of sum(w * d) (backward)."""
     w = h = 2048
@@ -349,7 +436,77 @@ def bench_seeds(L, torch, dev):
 
     fwd()
     bwd()
-    return {"forward_ms": round(timed(fwd, 10, torch), 4), "backward_ms": round(timed(bwd, 10, torch), 4)}
+    res = {"forward_ms": round(timed(fwd, 10, torch), 4), "backward_ms": round(timed(bwd, 10, torch), 4),
+           "digest_index": digest(idx)}
+    if cpu:
+        # the index field against the oracle's float64 nearest seed; the
+        # gradient against the oracle's float64 gradient
+        ob = oracle_mod()
+        O = ob.load_oracle()
+        ridx, _ = O.seed_nearest_2d(w, h, seeds, threads=ob.host_threads())
+        rgrad = O.seed_grad_2d(w, h, seeds, ridx, wt)
+        g = grad.cpu().numpy()
+        res["parity"] = {"index_mismatches_vs_oracle": int((idx.cpu().numpy() != ridx).sum()),
+                         "grad_max_rel_err_vs_fp64": float(np.abs(g - rgrad).max() / np.abs(rgrad).max())}
+    return res
+
+
+def cpu_reference(L, mask3, jfa2d=None, jfa3d=None, felz=None):
+    """The reference's own on-path functions timed on this host: JumpFlood 2D
+    and 3D (signed border start, cw = 2) single-threaded and with its OpenMP
+    loops on the host's cores, Felzenszwalb single-threaded (its OpenMP form
+    races).  The reference is oracle/_ref (compiled from its sources, with and
+    without -fopenmp) where present, else the oracle's restatement (kind
+    "port", single-threaded).  Bounded samples: corners of the 8192^2 and
+    512^3 scenes; warm-up 1, median of 5.  The GPU rates of the same work on
+    the full configs are put beside."""
+    ob = oracle_mod()
+    model, cores = host_info()
+    ref, omp = ob.load_reference(), ob.load_reference(omp=True)
+    kind = "reference" if ref is not None else "port"
+    one = ref if ref is not None else ob.load_oracle()
+    threads = ob.host_threads()
+    if omp is not None:
+        threads = omp.set_threads(threads)
+    O = ob.load_oracle()
+    out = {"cpu_model": model, "host_cores": cores, "kind": kind, "runs": "warm-up 1, median of 5"}
+
+    k = 4096
+    m2 = L.scene_shapes_2d(8192, 8192, 2000, seed=2).reshape(8192, 256)
+    c2 = np.ascontiguousarray(m2[:k, : k // 32])
+    b2 = O.border_2d(k, k, c2)
+    t1 = median_time(lambda: one.jfa_2d(k, k, b2, 2), 5)
+    e = {"sample": f"{k}^2 corner of the 8192^2 disc/box scene, signed border start, cw = 2",
+         "single_thread": {"s": round(t1, 3), "gpixels_per_s": round(k * k / t1 / 1e9, 5), "cores": 1}}
+    if omp is not None:
+        tm = median_time(lambda: omp.jfa_2d(k, k, b2, 2), 5)
+        e["openmp"] = {"s": round(tm, 3), "gpixels_per_s": round(k * k / tm / 1e9, 5), "cores": threads}
+    if jfa2d:
+        e["gpu_gpixels_per_s_8192"] = jfa2d.get("gpixels_per_s")
+    out["jump_flood_2d"] = e
+
+    k3 = 128
+    m3 = mask3.reshape(512, 512, 16)
+    c3 = np.ascontiguousarray(m3[:k3, :k3, : k3 // 32])
+    b3 = O.border_3d(k3, k3, k3, c3)
+    t1 = median_time(lambda: one.jfa_3d(k3, k3, k3, b3, 2), 5)
+    e = {"sample": f"{k3}^3 corner of the 512^3 scene, signed border start, cw = 2",
+         "single_thread": {"s": round(t1, 3), "gvoxels_per_s": round(k3 ** 3 / t1 / 1e9, 5), "cores": 1}}
+    if omp is not None:
+        tm = median_time(lambda: omp.jfa_3d(k3, k3, k3, b3, 2), 5)
+        e["openmp"] = {"s": round(tm, 3), "gvoxels_per_s": round(k3 ** 3 / tm / 1e9, 5), "cores": threads}
+    if jfa3d:
+        e["gpu_gvoxels_per_s_512"] = jfa3d.get("gvoxels_per_s")
+    out["jump_flood_3d"] = e
+
+    f0 = O.init_distances_f32(k, k, c2)
+    t1 = median_time(lambda: one.felzenszwalb_2d(k, k, f0), 5)
+    e = {"sample": f"{k}^2 corner of the 8192^2 disc/box scene, after InitDistancesFromBitMask",
+         "single_thread": {"s": round(t1, 3), "gpixels_per_s": round(k * k / t1 / 1e9, 5), "cores": 1}}
+    if felz:
+        e["gpu_gpixels_per_s_8192"] = felz.get("felzenszwalb_8192", {}).get("gpixels_per_s")
+    out["felzenszwalb_2d"] = e
+    return out
 
 
 def torus_mesh(nu=1024, nv=512, r0=0.3, r1=0.12):
@@ -616,11 +773,14 @@ def main():
     cells = w * h * d
 
     if args.only_extra and args.only_extra != "jfa3d":
-        run = {"glyphs": lambda: bench_glyphs(L, torch, dev, args), "jfa2d": lambda: bench_jfa2d(L, torch, dev),
+        cpu = not args.no_cpu
+        run = {"glyphs": lambda: bench_glyphs(L, torch, dev, args),
+               "jfa2d": lambda: bench_jfa2d(L, torch, dev, cpu=cpu),
                "glyphs_batched": lambda: bench_glyphs_batched(L, torch, dev, args),
-               "felzenszwalb": lambda: bench_felzenszwalb(L, torch, dev),
-               "config1": lambda: bench_config1(L, torch, dev), "seeds": lambda: bench_seeds(L, torch, dev),
-               "twofield": lambda: bench_two_field(L, torch, dev),
+               "felzenszwalb": lambda: bench_felzenszwalb(L, torch, dev, cpu=cpu),
+               "config1": lambda: bench_config1(L, torch, dev, cpu=cpu),
+               "seeds": lambda: bench_seeds(L, torch, dev, cpu=cpu),
+               "twofield": lambda: bench_two_field(L, torch, dev, cpu=cpu),
                "voxel": lambda: bench_voxel(L, torch, dev, cpu=not args.no_cpu),
                "sweep": lambda: bench_sweep(L, torch, dev, cpu=not args.no_cpu),
                "occlusion": lambda: bench_occlusion(L, torch, dev, cpu=not args.no_cpu)}
@@ -632,7 +792,8 @@ def main():
     mask_t = torch.from_numpy(mask.view(np.int32)).to(dev)
     if args.only_extra == "jfa3d":
         fl3 = torch.empty(cells, dtype=torch.float32, device=dev)
-        print(json.dumps({"jfa3d_signed": bench_jfa3d(L, torch, dev, mask_t, w, h, d, fl3)}), flush=True)
+        print(json.dumps({"jfa3d_signed": bench_jfa3d(L, torch, dev, mask_t, w, h, d, fl3, cpu=not args.no_cpu)}),
+              flush=True)
         return
     d2 = torch.empty(cells, dtype=torch.int32, device=dev)
     fl = torch.empty(cells, dtype=torch.float32, device=dev)
@@ -713,18 +874,25 @@ def main():
 
         O = load_oracle()
         print("# cpu baseline", file=sys.stderr, flush=True)
-        t0 = time.perf_counter()
         ref = O.edt_3d(w, h, d, mask)
-        cpu_s = time.perf_counter() - t0
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            O.edt_3d(w, h, d, mask)
+            ts.append(time.perf_counter() - t0)
+        cpu_s = float(np.median(ts))
         got = d2.cpu().numpy().reshape(d, h, w)
         mism = int((got != ref).sum())
+        model, ncores = host_info()
         result["cpu_baseline"] = {
             "value": round(cells / cpu_s / 1e9, 5),
             "unit": "Gvoxels/s",
             "cores": 1,
             "kind": "port",
-            "sample": f"the full {N}^3 volume once (exact separable EDT of oracle/dol_oracle.c, "
-                      f"{cpu_s:.1f} s on {os.cpu_count()} host cores, one used)",
+            "sample": f"the full {N}^3 volume (exact separable EDT of oracle/dol_oracle.c; the reference has "
+                      f"no 3D EDT), warm-up 1, median of 3: {cpu_s:.2f} s",
+            "cpu_model": model,
+            "host_cores": ncores,
         }
         result["parity"] = {"edt3d_int32_mismatches_vs_oracle": mism}
         if mism:
@@ -735,17 +903,23 @@ def main():
     # ---- extras (configs of the other workloads; not the headline value)
     if not args.no_extras:
         del work
-        result["jfa3d_signed"] = bench_jfa3d(L, torch, dev, mask_t, w, h, d, fl)
+        cpu = not args.no_cpu
+        result["jfa3d_signed"] = bench_jfa3d(L, torch, dev, mask_t, w, h, d, fl, cpu=cpu)
         del d2, fl
         torch.cuda.empty_cache()
         result["glyphs"] = bench_glyphs(L, torch, dev, args)
-        result["jfa2d_signed_8192"] = bench_jfa2d(L, torch, dev)
-        result["two_field_8192"] = bench_two_field(L, torch, dev)
+        result["jfa2d_signed_8192"] = bench_jfa2d(L, torch, dev, cpu=cpu)
+        result["two_field_8192"] = bench_two_field(L, torch, dev, cpu=cpu)
+        result["felzenszwalb"] = bench_felzenszwalb(L, torch, dev, cpu=cpu)
         result["voxel_512"] = bench_voxel(L, torch, dev, cpu=not args.no_cpu)
         result["fast_sweep"] = bench_sweep(L, torch, dev, mask_t, cpu=not args.no_cpu)
         result["occlusion"] = bench_occlusion(L, torch, dev, mask_t, cpu=not args.no_cpu)
-        result["config1_512"] = bench_config1(L, torch, dev)
-        result["seeds_2048"] = bench_seeds(L, torch, dev)
+        result["config1_512"] = bench_config1(L, torch, dev, cpu=cpu)
+        result["seeds_2048"] = bench_seeds(L, torch, dev, cpu=cpu)
+        if cpu:
+            print("# cpu reference timings", file=sys.stderr, flush=True)
+            result["cpu_reference"] = cpu_reference(L, mask, result["jfa2d_signed_8192"], result["jfa3d_signed"],
+                                                    result["felzenszwalb"])
 
     print(json.dumps(result), flush=True)
 

@@ -11,9 +11,27 @@
 // pairs, which is the cCellDelta2/cCellDelta3 layout (DistanceOcclusionLib.h:41-53).
 #include "DistanceOcclusionLib.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 using namespace DOL;
 
 extern "C" {
+
+// threads of the reference's OpenMP loops in the -fopenmp build
+// (libdolref_omp.so: JumpFlood's row loops, DistanceOcclusionLib.cpp:784, 2402);
+// returns the count in effect, 1 in the plain build
+int ref_set_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+    return n > 0 ? n : 1;
+#else
+    (void)n;
+    return 1;
+#endif
+}
 
 void ref_mask_add_points_2d(int w, int h, int n, uint32_t seed, uint32_t* mask)
 { BitMaskAddPoints(w, h, n, seed, mask); }

@@ -15,6 +15,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libdolref.so")
+REF_OMP_SO = os.path.join(ROOT, "oracle", "_ref", "libdolref_omp.so")
 MESH_REF_SO = os.path.join(ROOT, "oracle", "_ref", "libmeshref.so")
 
 _P = ctypes.c_void_p
@@ -361,6 +362,10 @@ class Reference:
             raise FileNotFoundError(path)
         self.L = ctypes.CDLL(path)
 
+    def set_threads(self, n):
+        """threads of the OpenMP loops (the -fopenmp build); returns the count in effect"""
+        return int(self.L.ref_set_threads(_I(n)))
+
     def mask_add_points_2d(self, w, h, n, seed, mask):
         self.L.ref_mask_add_points_2d(_I(w), _I(h), _I(n), _U(seed), _ptr(mask))
 
@@ -506,10 +511,11 @@ def host_threads():
     return max(1, min(16, n))
 
 
-def load_reference():
-    """The compiled reference, or None where it was not built."""
+def load_reference(omp=False):
+    """The compiled reference (its -fopenmp build with omp=True), or None where
+    it was not built."""
     try:
-        return Reference()
+        return Reference(REF_OMP_SO if omp else REF_SO)
     except (FileNotFoundError, OSError):
         return None
 
