@@ -95,10 +95,10 @@ CLI_BIN = os.path.join(HERE, "cli", "distocc_hip")
 
 
 def build_cli(verbose=False):
-    """The distocc_hip command (DOL:: interface over libdolhip.so)."""
+    """The distocc_hip command (DOL:: interface and C ABI over libdolhip.so)."""
     cxx = shutil.which("g++") or "c++"
     cmd = [cxx, "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), CLI_SRC,
-           "-L" + HERE, "-ldolhip", "-Wl,-rpath," + HERE, "-o", CLI_BIN]
+           "-L" + HERE, "-ldolhip", "-Wl,-rpath,$ORIGIN/..", "-o", CLI_BIN]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

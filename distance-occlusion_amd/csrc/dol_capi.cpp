@@ -41,6 +41,12 @@ template <class T>
 void launch_chamfer(int w, int h, T* dist, hipStream_t s);
 void launch_danielsson_2d(int w, int h, int16_t* delta, int cw, hipStream_t s);
 void launch_danielsson_3d(int w, int h, int d, int16_t* delta, int cw, hipStream_t s);
+void launch_border_image(int w, int h, int d, int dims, const uint32_t* mask, const int16_t* delta,
+                         float max_len, int type, uint8_t* out, hipStream_t s);
+void launch_distance_u8(size_t n, int dims, const int16_t* delta, float max_len, uint8_t* out, hipStream_t s);
+template <class T>
+void launch_scalar_u8(size_t n, const T* v, float max_len, uint8_t* out, hipStream_t s);
+void launch_sqrt_f32(size_t n, const float* v, float* out, hipStream_t s);
 void launch_jfa2d_pass(int step, int w, int h, const int16_t* in, int16_t* out, int cw,
                        hipStream_t s);
 size_t jfa3d_workspace(int w, int h, int d);
@@ -940,6 +946,184 @@ int dolhip_shadow_sweep(float gx, float gy, int w, int h, const float* occ, floa
     launch_shadow_sweep(gx, gy, w, h, i.as<float>(), o.as<float>(), 0);
     TRY(finish_sync("shadow_sweep"));
     return o.download(shadow, bytes);
+}
+
+// ---- the tool's image writers
+
+static int check_image_type(int type)
+{
+    if (type < 0 || type > 3)
+        return fail(DOLHIP_ERR_INVALID_ARG, "image type must be 0 (sdf), 1 (sdf-delta), 2 (sdf-rg) or 3 (sdf-angles)");
+    return DOLHIP_OK;
+}
+
+static int check_rgba(const void* p, int type)
+{
+    if (type != 0 && ((uintptr_t)p & 3)) return fail(DOLHIP_ERR_INVALID_ARG, "an RGBA image must be 4-byte aligned");
+    return DOLHIP_OK;
+}
+
+// d = 0: a 2D field
+static int border_image(int w, int h, int d, const uint32_t* mask, const int16_t* delta, float max_len, int type,
+                        uint8_t* out, void* stream, bool dev)
+{
+    const int dd = d > 0 ? d : 1, dims = d > 0 ? 3 : 2;
+    TRY(check_dims(w, h, dd));
+    TRY(check_ptr(mask, "mask"));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_ptr(out, "out"));
+    TRY(check_image_type(type));
+    TRY(require_device());
+    const size_t cells = (size_t)w * h * dd, obytes = cells * (type == 0 ? 1 : 4);
+    if (dev) {
+        TRY(check_rgba(out, type));
+        launch_border_image(w, h, dd, dims, mask, delta, max_len, type, out, as_stream(stream));
+        return finish(as_stream(stream), "encode_border_image");
+    }
+    DevBuf m, a, o;
+    TRY(m.upload(mask, mask_bytes(w, (long long)h * dd)));
+    TRY(a.upload(delta, cells * dims * 2));
+    TRY(o.alloc(obytes));
+    launch_border_image(w, h, dd, dims, m.as<uint32_t>(), a.as<int16_t>(), max_len, type, o.as<uint8_t>(), 0);
+    TRY(finish_sync("encode_border_image"));
+    return o.download(out, obytes);
+}
+
+int dolhip_encode_border_image_2d(int w, int h, const uint32_t* mask, const int16_t* delta, float max_len,
+                                  int type, uint8_t* out)
+{
+    return border_image(w, h, 0, mask, delta, max_len, type, out, nullptr, false);
+}
+int dolhip_encode_border_image_2d_dev(int w, int h, const uint32_t* mask, const int16_t* delta, float max_len,
+                                      int type, uint8_t* out, void* stream)
+{
+    return border_image(w, h, 0, mask, delta, max_len, type, out, stream, true);
+}
+int dolhip_encode_border_image_3d(int w, int h, int d, const uint32_t* mask, const int16_t* delta,
+                                  float max_len, int type, uint8_t* out)
+{
+    if (d < 1) return fail(DOLHIP_ERR_INVALID_ARG, "d must be positive");
+    return border_image(w, h, d, mask, delta, max_len, type, out, nullptr, false);
+}
+int dolhip_encode_border_image_3d_dev(int w, int h, int d, const uint32_t* mask, const int16_t* delta,
+                                      float max_len, int type, uint8_t* out, void* stream)
+{
+    if (d < 1) return fail(DOLHIP_ERR_INVALID_ARG, "d must be positive");
+    return border_image(w, h, d, mask, delta, max_len, type, out, stream, true);
+}
+
+static int distance_u8(int w, int h, int d, const int16_t* delta, float max_len, uint8_t* out, void* stream,
+                       bool dev)
+{
+    const int dd = d > 0 ? d : 1, dims = d > 0 ? 3 : 2;
+    TRY(check_dims(w, h, dd));
+    TRY(check_ptr(delta, "delta"));
+    TRY(check_ptr(out, "out"));
+    TRY(require_device());
+    const size_t cells = (size_t)w * h * dd;
+    if (dev) {
+        launch_distance_u8(cells, dims, delta, max_len, out, as_stream(stream));
+        return finish(as_stream(stream), "encode_distance_u8");
+    }
+    DevBuf a, o;
+    TRY(a.upload(delta, cells * dims * 2));
+    TRY(o.alloc(cells));
+    launch_distance_u8(cells, dims, a.as<int16_t>(), max_len, o.as<uint8_t>(), 0);
+    TRY(finish_sync("encode_distance_u8"));
+    return o.download(out, cells);
+}
+
+int dolhip_encode_distance_u8_2d(int w, int h, const int16_t* delta, float max_len, uint8_t* out)
+{
+    return distance_u8(w, h, 0, delta, max_len, out, nullptr, false);
+}
+int dolhip_encode_distance_u8_2d_dev(int w, int h, const int16_t* delta, float max_len, uint8_t* out,
+                                     void* stream)
+{
+    return distance_u8(w, h, 0, delta, max_len, out, stream, true);
+}
+int dolhip_encode_distance_u8_3d(int w, int h, int d, const int16_t* delta, float max_len, uint8_t* out)
+{
+    if (d < 1) return fail(DOLHIP_ERR_INVALID_ARG, "d must be positive");
+    return distance_u8(w, h, d, delta, max_len, out, nullptr, false);
+}
+int dolhip_encode_distance_u8_3d_dev(int w, int h, int d, const int16_t* delta, float max_len, uint8_t* out,
+                                     void* stream)
+{
+    if (d < 1) return fail(DOLHIP_ERR_INVALID_ARG, "d must be positive");
+    return distance_u8(w, h, d, delta, max_len, out, stream, true);
+}
+
+extern "C++" {
+template <class T>
+static int scalar_u8(int w, int h, const T* v, float max_len, uint8_t* out, void* stream, bool dev)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(v, "distances"));
+    TRY(check_ptr(out, "out"));
+    TRY(require_device());
+    const size_t n = (size_t)w * h;
+    if (dev) {
+        launch_scalar_u8<T>(n, v, max_len, out, as_stream(stream));
+        return finish(as_stream(stream), "encode_scalar_u8");
+    }
+    DevBuf a, o;
+    TRY(a.upload(v, n * sizeof(T)));
+    TRY(o.alloc(n));
+    launch_scalar_u8<T>(n, a.as<T>(), max_len, o.as<uint8_t>(), 0);
+    TRY(finish_sync("encode_scalar_u8"));
+    return o.download(out, n);
+}
+}  // extern "C++"
+
+int dolhip_encode_scalar_u8_f32(int w, int h, const float* v, float max_len, uint8_t* out)
+{
+    return scalar_u8(w, h, v, max_len, out, nullptr, false);
+}
+int dolhip_encode_scalar_u8_f32_dev(int w, int h, const float* v, float max_len, uint8_t* out, void* stream)
+{
+    return scalar_u8(w, h, v, max_len, out, stream, true);
+}
+int dolhip_encode_scalar_u8_i32(int w, int h, const int32_t* v, float max_len, uint8_t* out)
+{
+    return scalar_u8(w, h, v, max_len, out, nullptr, false);
+}
+int dolhip_encode_scalar_u8_i32_dev(int w, int h, const int32_t* v, float max_len, uint8_t* out, void* stream)
+{
+    return scalar_u8(w, h, v, max_len, out, stream, true);
+}
+int dolhip_encode_scalar_u8_i16(int w, int h, const int16_t* v, float max_len, uint8_t* out)
+{
+    return scalar_u8(w, h, v, max_len, out, nullptr, false);
+}
+int dolhip_encode_scalar_u8_i16_dev(int w, int h, const int16_t* v, float max_len, uint8_t* out, void* stream)
+{
+    return scalar_u8(w, h, v, max_len, out, stream, true);
+}
+
+int dolhip_sqrt_f32_dev(int w, int h, const float* v, float* out, void* stream)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(v, "in"));
+    TRY(check_ptr(out, "out"));
+    TRY(require_device());
+    launch_sqrt_f32((size_t)w * h, v, out, as_stream(stream));
+    return finish(as_stream(stream), "sqrt_f32");
+}
+
+int dolhip_sqrt_f32(int w, int h, const float* v, float* out)
+{
+    TRY(check_dims(w, h));
+    TRY(check_ptr(v, "in"));
+    TRY(check_ptr(out, "out"));
+    TRY(require_device());
+    const size_t n = (size_t)w * h;
+    DevBuf a, o;
+    TRY(a.upload(v, n * 4));
+    TRY(o.alloc(n * 4));
+    launch_sqrt_f32(n, a.as<float>(), o.as<float>(), 0);
+    TRY(finish_sync("sqrt_f32"));
+    return o.download(out, n * 4);
 }
 
 int dolhip_init_dirw_2d_dev(int w, int h, const uint32_t* mask, float* dirw4, void* stream)

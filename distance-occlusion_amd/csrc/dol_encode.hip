@@ -414,4 +414,126 @@ void launch_signed_edt(size_t n, const int32_t* dout, const int32_t* din, float*
     hipLaunchKernelGGL(k_signed_edt, dim3(grid1d(n, 256)), dim3(256), 0, s, n, dout, din, sdf);
 }
 
+// ---- every image of the border method (DistanceOcclusionTool.cpp:1240-1331
+// 2D, 1367-1469 3D; DIMS = 2 or 3 int16 components a cell): TYPE 0 sdf, one
+// byte a cell, EncodeU8Signed(-/+ Dist / maxLen) negative on set bits; 1
+// sdf-delta, the components over maxLen, signed; 2 sdf-rg, EncodeU8(Dist /
+// maxLen) in red outside and green inside; 3 sdf-angles, the components over
+// Dist, signed.  RGBA alpha 255, blue 0 in 2D.  One row of a slice a
+// blockIdx.y, so the mask bit needs no division.
+template <int TYPE, int DIMS>
+__global__ void __launch_bounds__(256) k_border_image(int w, const uint32_t* __restrict__ mask,
+                                                      const int16_t* __restrict__ delta, float max_len,
+                                                      uint8_t* __restrict__ out)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= w) return;
+    const size_t row = blockIdx.y, i = row * w + x;
+    const int16_t* c = delta + (size_t)DIMS * i;
+    const int c0 = c[0], c1 = c[1], c2 = DIMS == 3 ? c[2] : 0;
+    const float dist = dist_of(DIMS == 3 ? len2(c0, c1, c2) : len2(c0, c1));
+    const int b = mask_bit(mask, mask_stride(w), row, x);
+    if (TYPE == 0) {
+        const float v = __fdiv_rn(dist, max_len);
+        out[i] = encode_u8_signed(b ? -v : v);
+        return;
+    }
+    uint32_t r, g, bl;
+    if (TYPE == 1) {
+        r = encode_u8_signed(__fdiv_rn((float)c0, max_len));
+        g = encode_u8_signed(__fdiv_rn((float)c1, max_len));
+        bl = DIMS == 3 ? encode_u8_signed(__fdiv_rn((float)c2, max_len)) : 0u;
+    } else if (TYPE == 2) {
+        const uint32_t e = encode_u8(__fdiv_rn(dist, max_len));
+        r = b ? 0u : e;
+        g = b ? e : 0u;
+        bl = 0u;
+    } else {
+        r = encode_u8_signed(__fdiv_rn((float)c0, dist));
+        g = encode_u8_signed(__fdiv_rn((float)c1, dist));
+        bl = DIMS == 3 ? encode_u8_signed(__fdiv_rn((float)c2, dist)) : 0u;
+    }
+    ((uint32_t*)out)[i] = r | (g << 8) | (bl << 16) | (255u << 24);
+}
+
+void launch_border_image(int w, int h, int d, int dims, const uint32_t* mask, const int16_t* delta,
+                         float max_len, int type, uint8_t* out, hipStream_t s)
+{
+    ProfileScope p("encode_border_image", s);
+    const dim3 grid((unsigned)((w + 255) / 256), (unsigned)((size_t)h * d)), block(256);
+#define DOLHIP_BI(T, D) hipLaunchKernelGGL((k_border_image<T, D>), grid, block, 0, s, w, mask, delta, max_len, out)
+    if (dims == 2) {
+        switch (type) {
+        case 0: DOLHIP_BI(0, 2); break;
+        case 1: DOLHIP_BI(1, 2); break;
+        case 2: DOLHIP_BI(2, 2); break;
+        default: DOLHIP_BI(3, 2); break;
+        }
+    } else {
+        switch (type) {
+        case 0: DOLHIP_BI(0, 3); break;
+        case 1: DOLHIP_BI(1, 3); break;
+        case 2: DOLHIP_BI(2, 3); break;
+        default: DOLHIP_BI(3, 3); break;
+        }
+    }
+#undef DOLHIP_BI
+}
+
+// ---- unsigned distance images (DistanceOcclusionTool.cpp:678-698, 886-899):
+// EncodeU8(Dist / maxLen) of a delta field, EncodeU8(v / maxLen) of scalar
+// distances (float, int32, int16; integers converted to float first, as the
+// division there does), and sqrtf of float squared distances (:753-754)
+template <int DIMS>
+__global__ void __launch_bounds__(256) k_distance_u8(size_t n, const int16_t* __restrict__ delta, float max_len,
+                                                     uint8_t* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int16_t* c = delta + (size_t)DIMS * i;
+    const int l2 = DIMS == 3 ? len2(c[0], c[1], c[2]) : len2(c[0], c[1]);
+    out[i] = encode_u8(__fdiv_rn(dist_of(l2), max_len));
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_scalar_u8(size_t n, const T* __restrict__ v, float max_len,
+                                                   uint8_t* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    out[i] = encode_u8(__fdiv_rn((float)v[i], max_len));
+}
+
+__global__ void __launch_bounds__(256) k_sqrt_f32(size_t n, const float* __restrict__ v, float* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    out[i] = sqrt_rn(v[i]);
+}
+
+void launch_distance_u8(size_t n, int dims, const int16_t* delta, float max_len, uint8_t* out, hipStream_t s)
+{
+    ProfileScope p("encode_distance_u8", s);
+    if (dims == 2)
+        hipLaunchKernelGGL(k_distance_u8<2>, dim3(grid1d(n, 256)), dim3(256), 0, s, n, delta, max_len, out);
+    else
+        hipLaunchKernelGGL(k_distance_u8<3>, dim3(grid1d(n, 256)), dim3(256), 0, s, n, delta, max_len, out);
+}
+
+template <class T>
+void launch_scalar_u8(size_t n, const T* v, float max_len, uint8_t* out, hipStream_t s)
+{
+    ProfileScope p("encode_scalar_u8", s);
+    hipLaunchKernelGGL(k_scalar_u8<T>, dim3(grid1d(n, 256)), dim3(256), 0, s, n, v, max_len, out);
+}
+template void launch_scalar_u8<float>(size_t, const float*, float, uint8_t*, hipStream_t);
+template void launch_scalar_u8<int32_t>(size_t, const int32_t*, float, uint8_t*, hipStream_t);
+template void launch_scalar_u8<int16_t>(size_t, const int16_t*, float, uint8_t*, hipStream_t);
+
+void launch_sqrt_f32(size_t n, const float* v, float* out, hipStream_t s)
+{
+    ProfileScope p("sqrt_f32", s);
+    hipLaunchKernelGGL(k_sqrt_f32, dim3(grid1d(n, 256)), dim3(256), 0, s, n, v, out);
+}
+
 }  // namespace dolhip

@@ -76,6 +76,22 @@ _SIGS = {
     "dolhip_brute_force_2d_dev": ([_I, _I, _P, _I, _P], _I),
     "dolhip_brute_force_3d": ([_I, _I, _I, _P, _I], _I),
     "dolhip_brute_force_3d_dev": ([_I, _I, _I, _P, _I, _P], _I),
+    "dolhip_encode_border_image_2d": ([_I, _I, _P, _P, _F, _I, _P], _I),
+    "dolhip_encode_border_image_2d_dev": ([_I, _I, _P, _P, _F, _I, _P, _P], _I),
+    "dolhip_encode_border_image_3d": ([_I, _I, _I, _P, _P, _F, _I, _P], _I),
+    "dolhip_encode_border_image_3d_dev": ([_I, _I, _I, _P, _P, _F, _I, _P, _P], _I),
+    "dolhip_encode_distance_u8_2d": ([_I, _I, _P, _F, _P], _I),
+    "dolhip_encode_distance_u8_2d_dev": ([_I, _I, _P, _F, _P, _P], _I),
+    "dolhip_encode_distance_u8_3d": ([_I, _I, _I, _P, _F, _P], _I),
+    "dolhip_encode_distance_u8_3d_dev": ([_I, _I, _I, _P, _F, _P, _P], _I),
+    "dolhip_encode_scalar_u8_f32": ([_I, _I, _P, _F, _P], _I),
+    "dolhip_encode_scalar_u8_f32_dev": ([_I, _I, _P, _F, _P, _P], _I),
+    "dolhip_encode_scalar_u8_i32": ([_I, _I, _P, _F, _P], _I),
+    "dolhip_encode_scalar_u8_i32_dev": ([_I, _I, _P, _F, _P, _P], _I),
+    "dolhip_encode_scalar_u8_i16": ([_I, _I, _P, _F, _P], _I),
+    "dolhip_encode_scalar_u8_i16_dev": ([_I, _I, _P, _F, _P, _P], _I),
+    "dolhip_sqrt_f32": ([_I, _I, _P, _P], _I),
+    "dolhip_sqrt_f32_dev": ([_I, _I, _P, _P, _P], _I),
     "dolhip_danielsson_2d": ([_I, _I, _P, _I], _I),
     "dolhip_danielsson_2d_dev": ([_I, _I, _P, _I, _P], _I),
     "dolhip_danielsson_3d": ([_I, _I, _I, _P, _I], _I),
@@ -318,6 +334,41 @@ class DolHip:
     def brute_force_3d(self, w, h, d, delta, cw=1):
         out = np.ascontiguousarray(delta, dtype=np.int16).copy()
         self._call("dolhip_brute_force_3d", w, h, d, _addr(out), cw)
+        return out
+
+    def border_image(self, w, h, d, mask, delta, max_len, image_type):
+        """the border method's image: type 0 sdf (bytes), 1 sdf-delta, 2 sdf-rg,
+        3 sdf-angles (RGBA); d = 0 for a 2D field"""
+        mask = np.ascontiguousarray(mask, np.uint32)
+        delta = np.ascontiguousarray(delta, np.int16)
+        dd = max(d, 1)
+        out = np.empty((dd, h, w) if image_type == 0 else (dd, h, w, 4), dtype=np.uint8)
+        if d == 0:
+            self._call("dolhip_encode_border_image_2d", w, h, _addr(mask), _addr(delta), max_len, image_type, _addr(out))
+            return out[0]
+        self._call("dolhip_encode_border_image_3d", w, h, d, _addr(mask), _addr(delta), max_len, image_type, _addr(out))
+        return out
+
+    def distance_u8(self, w, h, d, delta, max_len):
+        delta = np.ascontiguousarray(delta, np.int16)
+        out = np.empty((max(d, 1), h, w), dtype=np.uint8)
+        if d == 0:
+            self._call("dolhip_encode_distance_u8_2d", w, h, _addr(delta), max_len, _addr(out))
+            return out[0]
+        self._call("dolhip_encode_distance_u8_3d", w, h, d, _addr(delta), max_len, _addr(out))
+        return out
+
+    def scalar_u8(self, w, h, values, max_len):
+        v = np.ascontiguousarray(values)
+        name = {np.dtype(np.float32): "f32", np.dtype(np.int32): "i32", np.dtype(np.int16): "i16"}[v.dtype]
+        out = np.empty((h, w), dtype=np.uint8)
+        self._call(f"dolhip_encode_scalar_u8_{name}", w, h, _addr(v), max_len, _addr(out))
+        return out
+
+    def sqrt_f32(self, w, h, values):
+        v = np.ascontiguousarray(values, np.float32)
+        out = np.empty((h, w), dtype=np.float32)
+        self._call("dolhip_sqrt_f32", w, h, _addr(v), _addr(out))
         return out
 
     def danielsson_2d(self, w, h, delta, cw=1):

@@ -281,6 +281,38 @@ int dolhip_encode_sdf_delta_rgba_2d(int w, int h, const int16_t* delta, float ma
 int dolhip_encode_sdf_rg_rgba_2d(int w, int h, const uint32_t* mask, const int16_t* delta,
                                  float max_len, uint8_t* out);
 
+/* Every image of the border method: WriteDistances, DistanceOcclusionTool.cpp:1240-1331
+ * (2D) and 1367-1469 (3D, the slices one after another).  type 0 sdf (one byte a
+ * cell), 1 sdf-delta, 2 sdf-rg, 3 sdf-angles (RGBA8, 4-byte aligned); max_len is
+ * the divisor passed there (twice the tool's maxLen).  Bit-identical. */
+int dolhip_encode_border_image_2d(int w, int h, const uint32_t* mask, const int16_t* delta, float max_len,
+                                  int type, uint8_t* out);
+int dolhip_encode_border_image_2d_dev(int w, int h, const uint32_t* mask, const int16_t* delta, float max_len,
+                                      int type, uint8_t* out, void* stream);
+int dolhip_encode_border_image_3d(int w, int h, int d, const uint32_t* mask, const int16_t* delta,
+                                  float max_len, int type, uint8_t* out);
+int dolhip_encode_border_image_3d_dev(int w, int h, int d, const uint32_t* mask, const int16_t* delta,
+                                      float max_len, int type, uint8_t* out, void* stream);
+/* The unsigned distance images (WriteDistances, DistanceOcclusionTool.cpp:678-698
+ * 2D, 886-899 3D): EncodeU8(Dist(delta) / max_len) of a delta field, and
+ * EncodeU8(v / max_len) of scalar distances (Felzenszwalb's float roots; Chamfer's
+ * int16 values with max_len times its multiplier); sqrtf of float squared
+ * distances (:753-754). */
+int dolhip_encode_distance_u8_2d(int w, int h, const int16_t* delta, float max_len, uint8_t* out);
+int dolhip_encode_distance_u8_2d_dev(int w, int h, const int16_t* delta, float max_len, uint8_t* out,
+                                     void* stream);
+int dolhip_encode_distance_u8_3d(int w, int h, int d, const int16_t* delta, float max_len, uint8_t* out);
+int dolhip_encode_distance_u8_3d_dev(int w, int h, int d, const int16_t* delta, float max_len, uint8_t* out,
+                                     void* stream);
+int dolhip_encode_scalar_u8_f32(int w, int h, const float* v, float max_len, uint8_t* out);
+int dolhip_encode_scalar_u8_f32_dev(int w, int h, const float* v, float max_len, uint8_t* out, void* stream);
+int dolhip_encode_scalar_u8_i32(int w, int h, const int32_t* v, float max_len, uint8_t* out);
+int dolhip_encode_scalar_u8_i32_dev(int w, int h, const int32_t* v, float max_len, uint8_t* out, void* stream);
+int dolhip_encode_scalar_u8_i16(int w, int h, const int16_t* v, float max_len, uint8_t* out);
+int dolhip_encode_scalar_u8_i16_dev(int w, int h, const int16_t* v, float max_len, uint8_t* out, void* stream);
+int dolhip_sqrt_f32(int w, int h, const float* v, float* out);
+int dolhip_sqrt_f32_dev(int w, int h, const float* v, float* out, void* stream);
+
 /* ---- signed SDF pipelines (DistanceOcclusionTool.cpp:1333-1365, 1471-1498) --- */
 /* border init + JumpFlood(cw = 2), delta left in `delta` */
 int dolhip_signed_border_2d_dev(int w, int h, const uint32_t* mask, int16_t* delta, void* work,

@@ -1208,6 +1208,84 @@ void oc_sdf_rg_rgba_2d(int w, int h, const uint32_t* mask, const int16_t* delta,
         }
 }
 
+/* Every image of the border method: WriteDistances 2D,
+ * DistanceOcclusionTool.cpp:1240-1331, and 3D, 1367-1469 (d = 1 is the 2D
+ * writer: the same per-cell formulas, z absent).  type 0 sdf (one byte a
+ * cell: EncodeU8Signed of -/+ Dist / maxLen, negative on set bits), 1
+ * sdf-delta (the delta's components over maxLen, signed; 2D: blue 0), 2
+ * sdf-rg (EncodeU8(Dist / maxLen) in red outside, green inside), 3 sdf-angles
+ * (the delta's components over Dist, signed; 2D: blue 0); RGBA alpha 255.
+ * max_len is the value passed there (twice the tool's maxLen). */
+void oc_border_image(int w, int h, int d, int dims, const uint32_t* mask, const int16_t* delta,
+                     float max_len, int type, uint8_t* out)
+{
+    int ws = mask_stride(w);
+    const int nc = dims;   /* int16 components a cell */
+    for (int z = 0; z < d; z++)
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                size_t r = (size_t)z * h + y, i = r * w + x;
+                const int16_t* c = delta + (size_t)nc * i;
+                int l2 = dims == 2 ? oc_len2_2(c) : oc_len2_3(c);
+                float dist = sqrtf((float)l2);
+                int b = mask_bit(mask, ws, r, x);
+                if (type == 0) {
+                    float v = dist / max_len;
+                    out[i] = oc_encode_u8_signed(b ? -v : v);
+                    continue;
+                }
+                uint8_t* o = out + 4 * i;
+                if (type == 1) {
+                    o[0] = oc_encode_u8_signed(c[0] / max_len);
+                    o[1] = oc_encode_u8_signed(c[1] / max_len);
+                    o[2] = dims == 3 ? oc_encode_u8_signed(c[2] / max_len) : 0;
+                } else if (type == 2) {
+                    o[b] = oc_encode_u8(dist / max_len);
+                    o[b ^ 1] = 0;
+                    o[2] = 0;
+                } else {
+                    o[0] = oc_encode_u8_signed(c[0] / dist);
+                    o[1] = oc_encode_u8_signed(c[1] / dist);
+                    o[2] = dims == 3 ? oc_encode_u8_signed(c[2] / dist) : 0;
+                }
+                o[3] = 255;
+            }
+}
+
+/* Unsigned distance image of the delta methods, WriteDistances
+ * DistanceOcclusionTool.cpp:678-686 (2D) and 886-899 (3D, slice by slice):
+ * EncodeU8(Dist / maxLen). */
+void oc_distance_u8(size_t n, int dims, const int16_t* delta, float max_len, uint8_t* out)
+{
+    for (size_t i = 0; i < n; i++) {
+        int l2 = dims == 2 ? oc_len2_2(delta + 2 * i) : oc_len2_3(delta + 3 * i);
+        out[i] = oc_encode_u8(sqrtf((float)l2) / max_len);
+    }
+}
+
+/* ... and of the scalar methods, DistanceOcclusionTool.cpp:688-697:
+ * EncodeU8(distances[i] / maxLen) for float, int32 and int16 distances
+ * (Felzenszwalb's float roots, Chamfer's int16 values with maxLen times its
+ * multiplier, :753-754, :827). */
+void oc_scalar_u8_f32(size_t n, const float* v, float max_len, uint8_t* out)
+{
+    for (size_t i = 0; i < n; i++) out[i] = oc_encode_u8(v[i] / max_len);
+}
+void oc_scalar_u8_i32(size_t n, const int32_t* v, float max_len, uint8_t* out)
+{
+    for (size_t i = 0; i < n; i++) out[i] = oc_encode_u8(v[i] / max_len);
+}
+void oc_scalar_u8_i16(size_t n, const int16_t* v, float max_len, uint8_t* out)
+{
+    for (size_t i = 0; i < n; i++) out[i] = oc_encode_u8(v[i] / max_len);
+}
+
+/* sqrtf of each value, DistanceOcclusionTool.cpp:753-754 */
+void oc_sqrt_f32(size_t n, const float* v, float* out)
+{
+    for (size_t i = 0; i < n; i++) out[i] = sqrtf(v[i]);
+}
+
 /* Two-field signed images of the tool's -S mode: WriteDistances,
  * DistanceOcclusionTool.cpp:1029-1080 (2D).  d0 is the exterior field, d1 the
  * interior one; type 0 sdf (1 byte a cell), 1 sdf-delta, 2 sdf-rg,
@@ -1473,6 +1551,93 @@ long oc_check_deltas_3d(int w, int h, int d, const int16_t* delta, int cw)
                 }
                 const int16_t* r = delta + 3 * (((size_t)(cz / cw) * h + cy / cw) * w + cx / cw);
                 if (!(abs(r[0]) < cw && abs(r[1]) < cw && abs(r[2]) < cw)) bad++;
+            }
+    return bad;
+}
+
+/* CheckBorderDeltas 2D, DistanceOcclusionTool.cpp:441-500, with the loops
+ * over x < w and y < h (the reference runs x to h and y to w, which reads
+ * outside the mask and field unless w == h).  Occupancy reads clamp to the
+ * image (Occupied, :442-456).  A boundary cell (|x|, |y| < 2) must see the
+ * other occupancy at its delta, and when the delta is diagonal the same
+ * occupancy at its four axis neighbours (no closer edge); any other cell the
+ * same occupancy at all nine.  Returns the number of failed expectations. */
+static int occ2_clamped(const uint32_t* mask, int w, int h, int x, int y)
+{
+    x = x < 0 ? 0 : (x >= w ? w - 1 : x);
+    y = y < 0 ? 0 : (y >= h ? h - 1 : y);
+    return mask_bit(mask, mask_stride(w), (size_t)y, x);
+}
+
+long oc_check_border_deltas_2d(int w, int h, const uint32_t* mask, const int16_t* delta)
+{
+    long bad = 0;
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            const int16_t* c = delta + 2 * ((size_t)y * w + x);
+            const int occ = occ2_clamped(mask, w, h, x, y);
+            if (abs(c[0]) < 2 && abs(c[1]) < 2) {
+                bad += occ2_clamped(mask, w, h, x + c[0], y + c[1]) != !occ;
+                if (c[0] != 0 && c[1] != 0) {
+                    bad += occ2_clamped(mask, w, h, x - 1, y) != occ;
+                    bad += occ2_clamped(mask, w, h, x + 1, y) != occ;
+                    bad += occ2_clamped(mask, w, h, x, y - 1) != occ;
+                    bad += occ2_clamped(mask, w, h, x, y + 1) != occ;
+                }
+            } else {
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dx = -1; dx <= 1; dx++) bad += occ2_clamped(mask, w, h, x + dx, y + dy) != occ;
+            }
+        }
+    return bad;
+}
+
+/* CheckBorderDeltas 3D, DistanceOcclusionTool.cpp:502-599, loops over x < w and
+ * y < h (the reference swaps them as in 2D).  Neighbours outside the volume
+ * pass (Expect, :508-521).  A boundary cell (every |component| < 2) must see
+ * the other occupancy at its delta; a box diagonal the same occupancy at the
+ * twelve face diagonals; each nonzero pair of components (xy, yz, zx) the
+ * same occupancy at that plane's four axis neighbours; any other cell the
+ * same occupancy at all 27. */
+static int expect3(const uint32_t* mask, int w, int h, int d, int x, int y, int z, int value)
+{
+    if (x < 0 || x >= w || y < 0 || y >= h || z < 0 || z >= d) return 1;
+    return mask_bit(mask, mask_stride(w), (size_t)z * h + y, x) == value;
+}
+
+long oc_check_border_deltas_3d(int w, int h, int d, const uint32_t* mask, const int16_t* delta)
+{
+    static const int face[12][3] = { { -1, -1, 0 }, { -1, 1, 0 }, { 1, -1, 0 }, { 1, 1, 0 },
+                                      { -1, 0, -1 }, { -1, 0, 1 }, { 1, 0, -1 }, { 1, 0, 1 },
+                                      { 0, -1, -1 }, { 0, -1, 1 }, { 0, 1, -1 }, { 0, 1, 1 } };
+    long bad = 0;
+    for (int z = 0; z < d; z++)
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                const int16_t* c = delta + 3 * (((size_t)z * h + y) * w + x);
+                const int occ = mask_bit(mask, mask_stride(w), (size_t)z * h + y, x);
+                if (abs(c[0]) < 2 && abs(c[1]) < 2 && abs(c[2]) < 2) {
+                    bad += !expect3(mask, w, h, d, x + c[0], y + c[1], z + c[2], !occ);
+                    if (c[0] && c[1] && c[2])
+                        for (int k = 0; k < 12; k++)
+                            bad += !expect3(mask, w, h, d, x + face[k][0], y + face[k][1], z + face[k][2], occ);
+                    if (c[0] && c[1]) {
+                        bad += !expect3(mask, w, h, d, x - 1, y, z, occ) + !expect3(mask, w, h, d, x + 1, y, z, occ);
+                        bad += !expect3(mask, w, h, d, x, y - 1, z, occ) + !expect3(mask, w, h, d, x, y + 1, z, occ);
+                    }
+                    if (c[1] && c[2]) {
+                        bad += !expect3(mask, w, h, d, x, y - 1, z, occ) + !expect3(mask, w, h, d, x, y + 1, z, occ);
+                        bad += !expect3(mask, w, h, d, x, y, z - 1, occ) + !expect3(mask, w, h, d, x, y, z + 1, occ);
+                    }
+                    if (c[2] && c[0]) {
+                        bad += !expect3(mask, w, h, d, x - 1, y, z, occ) + !expect3(mask, w, h, d, x + 1, y, z, occ);
+                        bad += !expect3(mask, w, h, d, x, y, z - 1, occ) + !expect3(mask, w, h, d, x, y, z + 1, occ);
+                    }
+                } else {
+                    for (int dz = -1; dz <= 1; dz++)
+                        for (int dy = -1; dy <= 1; dy++)
+                            for (int dx = -1; dx <= 1; dx++) bad += !expect3(mask, w, h, d, x + dx, y + dy, z + dz, occ);
+                }
             }
     return bad;
 }

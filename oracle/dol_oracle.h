@@ -134,7 +134,22 @@ void oc_glyph_sdf_batch(int n, const int32_t* w, const int32_t* h, const int64_t
                         const uint8_t* bytes, int threshold, int less_than, uint8_t* out,
                         int threads);
 
+/* image writers of the tool (DistanceOcclusionTool.cpp:678-698, 1240-1331, 1367-1469) */
+void oc_border_image(int w, int h, int d, int dims, const uint32_t* mask, const int16_t* delta,
+                     float max_len, int type, uint8_t* out);
+void oc_distance_u8(size_t n, int dims, const int16_t* delta, float max_len, uint8_t* out);
+void oc_scalar_u8_f32(size_t n, const float* v, float max_len, uint8_t* out);
+void oc_scalar_u8_i32(size_t n, const int32_t* v, float max_len, uint8_t* out);
+void oc_scalar_u8_i16(size_t n, const int16_t* v, float max_len, uint8_t* out);
+void oc_sqrt_f32(size_t n, const float* v, float* out);
+
+/* CheckBorderDeltas with the x / y loop swap fixed (DistanceOcclusionTool.cpp:441-599):
+ * number of failed expectations */
+long oc_check_border_deltas_2d(int w, int h, const uint32_t* mask, const int16_t* delta);
+long oc_check_border_deltas_3d(int w, int h, int d, const uint32_t* mask, const int16_t* delta);
+
 #ifdef __cplusplus
 }
 #endif
+
 #endif

@@ -262,6 +262,50 @@ class Oracle:
         self.L.oc_sdf_rg_rgba_2d(_I(w), _I(h), _ptr(mask), _ptr(np.ascontiguousarray(delta)), _F(max_len), _ptr(out))
         return out
 
+    def check_border_deltas_2d(self, w, h, mask, delta):
+        f = self.L.oc_check_border_deltas_2d
+        f.restype = ctypes.c_long
+        return int(f(_I(w), _I(h), _ptr(np.ascontiguousarray(mask, np.uint32)),
+                     _ptr(np.ascontiguousarray(delta, np.int16))))
+
+    def check_border_deltas_3d(self, w, h, d, mask, delta):
+        f = self.L.oc_check_border_deltas_3d
+        f.restype = ctypes.c_long
+        return int(f(_I(w), _I(h), _I(d), _ptr(np.ascontiguousarray(mask, np.uint32)),
+                     _ptr(np.ascontiguousarray(delta, np.int16))))
+
+    def border_image(self, w, h, d, mask, delta, max_len, image_type):
+        """the border method's image (type 0 sdf, 1 sdf-delta, 2 sdf-rg, 3
+        sdf-angles); d = 0 for a 2D field"""
+        dims = 2 if d == 0 else 3
+        dd = max(d, 1)
+        shape = (dd, h, w) if image_type == 0 else (dd, h, w, 4)
+        out = np.empty(shape, dtype=np.uint8)
+        self.L.oc_border_image(_I(w), _I(h), _I(dd), _I(dims), _ptr(np.ascontiguousarray(mask, np.uint32)),
+                               _ptr(np.ascontiguousarray(delta, np.int16)), _F(max_len), _I(image_type), _ptr(out))
+        return out[0] if d == 0 else out
+
+    def distance_u8(self, dims, delta, max_len):
+        delta = np.ascontiguousarray(delta, np.int16)
+        n = delta.size // dims
+        out = np.empty(n, dtype=np.uint8)
+        self.L.oc_distance_u8(ctypes.c_size_t(n), _I(dims), _ptr(delta), _F(max_len), _ptr(out))
+        return out
+
+    def scalar_u8(self, values, max_len):
+        v = np.ascontiguousarray(values)
+        out = np.empty(v.size, dtype=np.uint8)
+        fn = {np.dtype(np.float32): self.L.oc_scalar_u8_f32, np.dtype(np.int32): self.L.oc_scalar_u8_i32,
+              np.dtype(np.int16): self.L.oc_scalar_u8_i16}[v.dtype]
+        fn(ctypes.c_size_t(v.size), _ptr(v), _F(max_len), _ptr(out))
+        return out
+
+    def sqrt_f32(self, values):
+        v = np.ascontiguousarray(values, np.float32)
+        out = np.empty_like(v)
+        self.L.oc_sqrt_f32(ctypes.c_size_t(v.size), _ptr(v), _ptr(out))
+        return out
+
     def two_field_image_2d(self, w, h, d0, d1, max_len, image_type):
         out = np.empty((h, w) if image_type == 0 else (h, w, 4), dtype=np.uint8)
         self.L.oc_two_field_image_2d(_I(w), _I(h), _ptr(np.ascontiguousarray(d0, np.int16)),

@@ -816,3 +816,115 @@ def test_glyph_batch_composes(oracle, rng):
         dl = oracle.jfa_2d(w, h, oracle.border_2d(w, h, m), 2)
         exp = oracle.sdf_u8_2d(w, h, m, dl, 0.5 * max(w, h))
         np.testing.assert_array_equal(got[o:o + w * h], exp.ravel())
+
+
+# ------------------------------------------------ the tool's image writers
+
+
+def _tool_mask(oracle, dims, sides, points, seed):
+    """the tool's mask: CreateBitMask, BitMaskAddBlock(sides), then
+    BitMaskAddPoints(points, seed) (DistanceOcclusionTool.cpp:1921-1924)"""
+    m = empty_mask(*dims)
+    if len(dims) == 2:
+        if sides:
+            oracle.mask_add_block_2d(*dims, sides, m)
+        if points:
+            oracle.mask_add_points_2d(*dims, points, seed, m)
+    else:
+        if sides:
+            oracle.mask_add_block_3d(*dims, sides, m)
+        if points:
+            oracle.mask_add_points_3d(*dims, points, seed, m)
+    return m
+
+
+def test_image_writers_golden(oracle):
+    """The oracle's image writers against the images the reference's own tool
+    wrote for the same masks (tests/golden/make_image_golden.py): the border
+    method's four image types in 2D and 3D, the unsigned JumpFlood image in 2D
+    and 3D, and the Chamfer and Felzenszwalb images, pixel for pixel."""
+    g = np.load(os.path.join(GOLDEN, "image_golden.npz"))
+    keys = sorted(k[:-5] for k in g.files if k.endswith("_dims"))
+    assert len(keys) == 12
+    for key in keys:
+        dims = tuple(int(v) for v in g[f"{key}_dims"])
+        sides, points, seed, method, itype, ml = (int(v) for v in g[f"{key}_gen"])
+        img = g[f"{key}_img"]
+        w, h = dims[0], dims[1]
+        d = dims[2] if len(dims) == 3 else 0
+        m = _tool_mask(oracle, dims, sides, points, seed)
+        max_len = float(ml) if ml >= 0 else 0.25 * max(w, h)
+        if key.startswith("s"):
+            if d:
+                delta = oracle.jfa_3d(w, h, d, oracle.border_3d(w, h, d, m), 2)
+            else:
+                delta = oracle.jfa_2d(w, h, oracle.border_2d(w, h, m), 2)
+            got = oracle.border_image(w, h, d, m, delta, 2 * max_len, itype)
+            if not d:
+                got = got[None]
+        elif method == 2:
+            if d:
+                delta = oracle.jfa_3d(w, h, d, oracle.init_deltas_3d(w, h, d, m), 1)
+                got = oracle.distance_u8(3, delta, max_len).reshape(d, h, w)
+            else:
+                delta = oracle.jfa_2d(w, h, oracle.init_deltas_2d(w, h, m), 1)
+                got = oracle.distance_u8(2, delta, max_len).reshape(1, h, w)
+        elif method == 4:
+            bits = unpack_mask(m, w, h)
+            start = np.where(bits, 0, 32767 - 16).astype(np.int16)   # InitDistancesFromBitMask(.., INT16_MAX - 16)
+            dist, cm = oracle.chamfer(w, h, start)
+            got = oracle.scalar_u8(dist, max_len * cm).reshape(1, h, w)
+        else:
+            f = oracle.felzenszwalb_2d(w, h, oracle.init_distances_f32(w, h, m))
+            got = oracle.scalar_u8(oracle.sqrt_f32(f), max_len).reshape(1, h, w)
+        np.testing.assert_array_equal(got, img, err_msg=key)
+
+
+def test_check_border_deltas(oracle, reference, rng):
+    """CheckBorderDeltas (x / y loops fixed) passes on the reference's border
+    init, 2D and 3D, square and not, and catches planted errors: a boundary
+    delta pointing at a cell of the same occupancy, a non-boundary value next
+    to an edge, a diagonal where an axis neighbour has the other occupancy."""
+    for dims in ((40, 40), (37, 23), (5, 70), (1, 9)):
+        w, h = dims
+        for p in (0.0, 0.05, 0.4, 1.0):
+            m = pack_mask(rng.random((h, w)) < p)
+            b = oracle.border_2d(w, h, m)
+            if reference is not None:
+                np.testing.assert_array_equal(b, reference.border_2d(w, h, m))
+            assert oracle.check_border_deltas_2d(w, h, m, b) == 0
+    for dims in ((12, 12, 12), (17, 9, 6), (33, 5, 3)):
+        w, h, d = dims
+        for p in (0.0, 0.1, 0.5):
+            m = pack_mask(rng.random((d, h, w)) < p)
+            b = oracle.border_3d(w, h, d, m)
+            assert oracle.check_border_deltas_3d(w, h, d, m, b) == 0
+    # planted errors (2D): a vertical edge between x = 9 and x = 10
+    w, h = 20, 8
+    bits = np.zeros((h, w), bool)
+    bits[:, 10:] = True
+    m = pack_mask(bits)
+    b = oracle.border_2d(w, h, m)
+    assert oracle.check_border_deltas_2d(w, h, m, b) == 0
+    bad = b.copy()
+    bad[3, 9] = (-1, 0)          # points at x = 8, also clear
+    assert oracle.check_border_deltas_2d(w, h, m, bad) > 0
+    bad = b.copy()
+    bad[3, 9] = (32767, 32767)   # non-boundary value beside the edge
+    assert oracle.check_border_deltas_2d(w, h, m, bad) > 0
+    bad = b.copy()
+    bad[3, 9] = (1, 1)           # diagonal although (x + 1, y) is the other side
+    assert oracle.check_border_deltas_2d(w, h, m, bad) > 0
+    # and in 3D: a plane z = 4 | 5
+    w, h, d = 6, 5, 10
+    bits = np.zeros((d, h, w), bool)
+    bits[5:] = True
+    m = pack_mask(bits)
+    b = oracle.border_3d(w, h, d, m)
+    assert oracle.check_border_deltas_3d(w, h, d, m, b) == 0
+    bad = b.copy()
+    bad[4, 2, 3] = (0, 0, -1)
+    assert oracle.check_border_deltas_3d(w, h, d, m, bad) > 0
+    bad = b.copy()
+    bad[4, 2, 3] = (25000, 25000, 25000)
+    assert oracle.check_border_deltas_3d(w, h, d, m, bad) > 0
