@@ -394,7 +394,7 @@ def bench_felzenszwalb(L, torch, dev, cpu=True):
         L.profile_enable(True)
         ems = timed(edt, reps, torch)
         ek = {}
-        for k in ("edt2d_xy", "edt_rows"):
+        for k in ("edt2d_xy", "edt_rows", "edt_cols"):
             avg, n = L.profile_query(k)
             if n:
                 ek[k] = {"avg_ms": round(avg, 4), "launches": n}

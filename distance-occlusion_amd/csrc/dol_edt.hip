@@ -61,8 +61,12 @@ __device__ void felz_line(int n, const float* __restrict__ f, size_t fs, float* 
     }
 }
 
+// one wavefront a workgroup: the lines (one a thread) spread over as many CUs
+// as there are wavefronts, each with an L1 of its own for its dependent reads
+constexpr int kFelzThreads = 64;
+
 // column pass: line x reads dist[q*w + x], writes td[x*h + q]
-__global__ void __launch_bounds__(256) k_felz_cols(int w, int h, const float* __restrict__ dist,
+__global__ void __launch_bounds__(kFelzThreads) k_felz_cols(int w, int h, const float* __restrict__ dist,
                                                    float* __restrict__ td, int* __restrict__ v,
                                                    float* __restrict__ z)
 {
@@ -72,7 +76,7 @@ __global__ void __launch_bounds__(256) k_felz_cols(int w, int h, const float* __
 }
 
 // row pass: line y reads td[q*h + y], writes dist[y*w + q]
-__global__ void __launch_bounds__(256) k_felz_rows(int w, int h, const float* __restrict__ td,
+__global__ void __launch_bounds__(kFelzThreads) k_felz_rows(int w, int h, const float* __restrict__ td,
                                                    float* __restrict__ dist, int* __restrict__ v,
                                                    float* __restrict__ z)
 {
@@ -101,13 +105,13 @@ void launch_felzenszwalb_2d(int w, int h, float* dist, void* work, hipStream_t s
     float* z = (float*)(p + 2 * felz_part(cells * 4));
     {
         ProfileScope ps("felzenszwalb_cols", s);
-        hipLaunchKernelGGL(k_felz_cols, dim3(grid1d((size_t)w, 256)), dim3(256), 0, s, w, h, dist, td,
-                           v, z);
+        hipLaunchKernelGGL(k_felz_cols, dim3(grid1d((size_t)w, kFelzThreads)), dim3(kFelzThreads), 0, s, w, h,
+                           dist, td, v, z);
     }
     {
         ProfileScope ps("felzenszwalb_rows", s);
-        hipLaunchKernelGGL(k_felz_rows, dim3(grid1d((size_t)h, 256)), dim3(256), 0, s, w, h, td, dist,
-                           v, z);
+        hipLaunchKernelGGL(k_felz_rows, dim3(grid1d((size_t)h, kFelzThreads)), dim3(kFelzThreads), 0, s, w, h,
+                           td, dist, v, z);
     }
 }
 
@@ -115,49 +119,77 @@ void launch_felzenszwalb_2d(int w, int h, float* dist, void* work, hipStream_t s
 
 // Per-row distance to the nearest set bit (clear bit if invert), squared;
 // kEdtInf where the row has none.  Bits past w in the last word are ignored.
-__global__ void __launch_bounds__(256) k_edt_rows(int w, int xb, const uint32_t* __restrict__ mask,
-                                                  uint32_t invert, int32_t* __restrict__ g)
+// One workgroup a row (rows of up to 1024 words): the row's words in LDS, a
+// bitmap of its non-empty words (one ballot per 64 words), then for each word
+// the distances from its bit 0 to the nearest set bit in the words before and
+// after it (a search of the bitmap, 32 words a step), and for each cell the
+// least of those and of the nearest set bits in its own word (leading and
+// trailing zero counts).
+constexpr int kRowWords = 1024;
+constexpr int kFarBit = 1 << 30;   // no set bit on that side
+
+__global__ void __launch_bounds__(256) k_edt_rows(int w, const uint32_t* __restrict__ mask, uint32_t invert,
+                                                  int32_t* __restrict__ g)
 {
-    const int row = blockIdx.x / xb;
-    const int x = (blockIdx.x - row * xb) * 256 + threadIdx.x;
-    if (x >= w) return;
-    const int ws = mask_stride(w);
-    const uint32_t* r = mask + (size_t)row * ws;
-    const int j = x >> 5, i = x & 31;
+    __shared__ uint32_t words[kRowWords];
+    __shared__ uint32_t nzb[kRowWords / 32];
+    __shared__ int dl[kRowWords], dr[kRowWords];
+    const size_t row = blockIdx.x;
+    const int ws = mask_stride(w), nb = (ws + 31) >> 5;
     const int tail = w - 32 * (ws - 1);
     const uint32_t last_valid = tail >= 32 ? 0xffffffffu : ((1u << tail) - 1u);
-    auto word = [&](int jj) -> uint32_t {
-        uint32_t m = r[jj] ^ invert;
-        return jj == ws - 1 ? (m & last_valid) : m;
-    };
-    const uint32_t m = word(j);
-    int best = 0x7fffffff;
-    // nearest at or left of x
-    uint32_t lo = m & (i == 31 ? 0xffffffffu : ((2u << i) - 1u));
-    if (lo) {
-        best = i - (31 - __clz(lo));
-    } else {
-        for (int jj = j - 1; jj >= 0; jj--) {
-            uint32_t t = word(jj);
-            if (t) { best = x - (32 * jj + 31 - __clz(t)); break; }
+    const uint32_t* r = mask + row * ws;
+    const int tid = threadIdx.x, lane = tid & 63;
+    for (int base = 0; base < ws; base += 256) {
+        const int j = base + tid;
+        uint32_t m = 0;
+        if (j < ws) {
+            m = r[j] ^ invert;
+            if (j == ws - 1) m &= last_valid;
+            words[j] = m;
+        }
+        const uint64_t bal = __ballot(m != 0u);
+        if (j < ws + 63 && (lane == 0 || lane == 32)) {
+            const int bw = j >> 5;
+            if (bw < nb) nzb[bw] = lane == 0 ? (uint32_t)bal : (uint32_t)(bal >> 32);
         }
     }
-    // nearest at or right of x
-    uint32_t hi = m & ~((1u << i) - 1u);
-    if (hi) {
-        int dx = __ffs(hi) - 1 - i;
-        if (dx < best) best = dx;
-    } else {
-        for (int jj = j + 1; jj < ws; jj++) {
-            uint32_t t = word(jj);
-            if (t) {
-                int dx = 32 * jj + __ffs(t) - 1 - x;
-                if (dx < best) best = dx;
+    __syncthreads();
+    for (int j = tid; j < ws; j += 256) {
+        int left = kFarBit, right = kFarBit;
+        // the nearest non-empty word below j
+        for (int k = (j - 1) >> 5, top = (j - 1) & 31; j > 0 && k >= 0; k--, top = 31) {
+            // bits of nzb[k] at positions <= top
+            const uint32_t bits = nzb[k] & (top == 31 ? 0xffffffffu : ((2u << top) - 1u));
+            if (bits) {
+                const int kl = 32 * k + (31 - __clz(bits));
+                left = 32 * (j - kl) - (31 - __clz(words[kl]));
                 break;
             }
         }
+        // and above
+        for (int k = (j + 1) >> 5, from = (j + 1) & 31; j + 1 < ws && k < nb; k++, from = 0) {
+            const uint32_t bits = nzb[k] & (0xffffffffu << from);
+            if (bits) {
+                const int kr = 32 * k + __ffs(bits) - 1;
+                right = 32 * (kr - j) + __ffs(words[kr]) - 1;
+                break;
+            }
+        }
+        dl[j] = left;
+        dr[j] = right;
     }
-    g[(size_t)row * w + x] = best == 0x7fffffff ? kEdtInf : best * best;
+    __syncthreads();
+    for (int x = tid; x < w; x += 256) {
+        const int j = x >> 5, i = x & 31;
+        const uint32_t m = words[j];
+        int best = min(dl[j] + i, dr[j] - i);
+        const uint32_t lo = m << (31 - i);   // bits at or left of x, x at the top
+        if (lo) best = min(best, (int)__clz(lo));
+        const uint32_t hi = m >> i;          // bits at or right of x, x at the bottom
+        if (hi) best = min(best, __ffs(hi) - 1);
+        g[row * w + x] = best >= kFarBit / 2 ? kEdtInf : best * best;
+    }
 }
 
 // Lower envelope along one axis of length n.  Line c starts at
@@ -170,10 +202,11 @@ __global__ void __launch_bounds__(256) k_edt_lines(int n, long long lines, long 
                                                    long long outer, long long stride,
                                                    const int32_t* __restrict__ f,
                                                    int32_t* __restrict__ d2, float* __restrict__ fl,
-                                                   int* __restrict__ stk)
+                                                   int* __restrict__ stk, const uint8_t* __restrict__ only)
 {
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= lines) return;
+    if (only && !only[c]) return;   // only the lines the scan left
     const long long base = (c / inner) * outer + (c % inner);
     const int32_t* fp = f + base;
     int* st = stk + c;
@@ -237,6 +270,93 @@ __global__ void __launch_bounds__(256) k_edt_lines(int n, long long lines, long 
     }
 }
 
+// Outward scan along lines of any length: d(p) = min_q f(q) + (p - q)^2 by
+// reading f at p -/+ k for k = 1, 2, ... until k^2 reaches every output's best
+// so far (no sample further out can improve it) or the line is exhausted.
+// Lane = x (64 consecutive lines of one plane: coalesced reads of every row);
+// a wavefront computes kScanRows consecutive outputs of its lines, kScanGroup
+// steps at a time from a sliding window of rows.  Values are exact int32
+// (kEdtInf = no site; the additions saturate, so a missing site never wins).
+// A wavefront that would need more than kScanCap steps (long empty stretches)
+// marks its lines in `flags`, and those lines are redone by k_edt_lines.
+constexpr int kScanRows = 8;
+constexpr int kScanGroup = 8;
+constexpr int kScanCap = 1024;
+
+__device__ __forceinline__ uint32_t add_sat_u32(uint32_t a, uint32_t b)
+{
+    return __builtin_elementwise_add_sat(a, b);
+}
+
+__global__ void __launch_bounds__(256) k_edt_scan(int n, int w, long long plane_stride, long long stride,
+                                                  const int32_t* __restrict__ f, int32_t* __restrict__ d2,
+                                                  float* __restrict__ fl, uint8_t* __restrict__ flags)
+{
+    constexpr int R = kScanRows, G = kScanGroup, M = G + R - 1;
+    const int lane = threadIdx.x & 63;
+    const int x = blockIdx.x * 64 + lane;
+    const long long plane = blockIdx.y;
+    const int p0 = (blockIdx.z * 4 + (threadIdx.x >> 6)) * R;   // the wavefront's first output row
+    if (p0 >= n) return;
+    const bool active = x < w;
+    const int32_t* fp = f + plane * plane_stride + (active ? x : 0);
+    auto at = [&](int q) -> uint32_t { return (q >= 0 && q < n) ? (uint32_t)fp[stride * q] : 0xffffffffu; };
+    uint32_t best[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) best[r] = active ? at(p0 + r) : 0u;
+    const int last = min(p0 + R, n) - 1;   // the wavefront's last output row
+    int k = 1;
+    bool capped = false;
+    for (;;) {
+        uint32_t top = 0;
+#pragma unroll
+        for (int r = 0; r < R; r++) top = max(top, best[r]);
+        // samples left at distance k from some output row: below its last row
+        // or above its first
+        const bool left = last - k >= 0 || p0 + k < n;
+        const bool more = active && left && (uint32_t)k * (uint32_t)k < top;
+        if (__builtin_amdgcn_ballot_w64(more) == 0) break;
+        if (k > kScanCap) {
+            capped = true;
+            break;
+        }
+        // rows p0 + r - k - t (below) are a[G - 1 - t + r], rows p0 + r + k + t (above) b[t + r]
+        uint32_t a[M], b[M];
+#pragma unroll
+        for (int u = 0; u < M; u++) {
+            a[u] = at(p0 - k - (G - 1) + u);
+            b[u] = at(p0 + k + u);
+        }
+#pragma unroll
+        for (int t = 0; t < G; t++) {
+            const uint32_t sq = (uint32_t)((k + t) * (k + t));
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                best[r] = min(best[r], min(add_sat_u32(a[G - 1 - t + r], sq), add_sat_u32(b[t + r], sq)));
+        }
+        k += G;
+    }
+    if (!active) return;
+    if (capped) {   // the wavefront's lines are redone whole (every writer stores 1)
+        flags[plane * w + x] = 1;
+        return;
+    }
+    const long long base = plane * plane_stride + x;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        if (p0 + r >= n) break;
+        const int32_t v = best[r] >= (uint32_t)kEdtInf ? kEdtInf : (int32_t)best[r];
+        if (d2) d2[base + stride * (p0 + r)] = v;
+        if (fl) fl[base + stride * (p0 + r)] = sqrt_d2(v);
+    }
+}
+
+__global__ void k_clear_flags(size_t n, uint8_t* __restrict__ flags)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) flags[i] = 0;
+}
+
 static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 bool edt_tile_fits(int w, int h, int d);
@@ -249,29 +369,38 @@ size_t edt_tile_workspace_3d(int w, int h, int d);
 
 // Tiled path (every dimension <= 512): 3D needs the y-pass fields only.
 // Line path: row field g and the line stack; 3D also the y-pass field.
+// Line path: the row field g, the line stack, for 3D the y-pass field, and a
+// flag byte a line for the lines the scan leaves to the stack kernel.
 size_t edt_workspace(int w, int h, int d)
 {
     const size_t cells = (size_t)w * h * d;
     if (edt_tile_fits(w, h, d)) return d == 1 ? 256 : edt_tile_workspace_3d(w, h, d);
-    return align256(cells * 4) * (d == 1 ? 2 : 3);
+    const size_t lines = (size_t)w * (d > h ? d : h);
+    return align256(cells * 4) * (d == 1 ? 2 : 3) + align256(lines);
 }
 
-static void edt_lines(int n, long long lines, long long inner, long long outer, long long stride,
-                      const int32_t* f, int32_t* d2, float* fl, int* stk, hipStream_t s,
+// The transform along `planes` x w lines of length n: line (plane, x) starts
+// at plane * plane_stride + x and steps by `stride`.  The scan first; the
+// lines it flags again with the stack kernel (whose line c = plane * w + x
+// has the same start with inner = w, outer = plane_stride).
+static void edt_lines(int n, int w, long long planes, long long plane_stride, long long stride,
+                      const int32_t* f, int32_t* d2, float* fl, int* stk, uint8_t* flags, hipStream_t s,
                       const char* name)
 {
     ProfileScope ps(name, s);
-    hipLaunchKernelGGL(k_edt_lines, dim3(grid1d((size_t)lines, 256)), dim3(256), 0, s, n, lines,
-                       inner, outer, stride, f, d2, fl, stk);
+    const size_t lines = (size_t)planes * w;
+    hipLaunchKernelGGL(k_clear_flags, dim3(grid1d(lines, 256)), dim3(256), 0, s, lines, flags);
+    const dim3 grid((unsigned)((w + 63) / 64), (unsigned)planes, (unsigned)((n + 4 * kScanRows - 1) / (4 * kScanRows)));
+    hipLaunchKernelGGL(k_edt_scan, grid, dim3(256), 0, s, n, w, plane_stride, stride, f, d2, fl, flags);
+    hipLaunchKernelGGL(k_edt_lines, dim3(grid1d(lines, 256)), dim3(256), 0, s, n, (long long)lines,
+                       (long long)w, plane_stride, stride, f, d2, fl, stk, (const uint8_t*)flags);
 }
 
 static void edt_rows(int w, long long rows, const uint32_t* mask, int invert, int32_t* g,
                      hipStream_t s)
 {
-    const int xb = (w + 255) / 256;
     ProfileScope ps("edt_rows", s);
-    hipLaunchKernelGGL(k_edt_rows, dim3((unsigned)(xb * rows)), dim3(256), 0, s, w, xb, mask,
-                       invert ? 0xffffffffu : 0u, g);
+    hipLaunchKernelGGL(k_edt_rows, dim3((unsigned)rows), dim3(256), 0, s, w, mask, invert ? 0xffffffffu : 0u, g);
 }
 
 void launch_edt_2d(int w, int h, const uint32_t* mask, int invert, int32_t* d2, float* fl,
@@ -284,8 +413,10 @@ void launch_edt_2d(int w, int h, const uint32_t* mask, int invert, int32_t* d2, 
     const size_t part = align256((size_t)w * h * 4);
     int32_t* g = (int32_t*)work;
     int* stk = (int*)((char*)work + part);
+    uint8_t* flags = (uint8_t*)work + 2 * part;
     edt_rows(w, h, mask, invert, g, s);
-    edt_lines(h, w, w, 0, w, g, d2, fl, stk, s, "edt_cols");
+    // columns: one plane of w lines, stride w
+    edt_lines(h, w, 1, 0, w, g, d2, fl, stk, flags, s, "edt_cols");
 }
 
 void launch_edt_3d(int w, int h, int d, const uint32_t* mask, int invert, int32_t* d2, float* fl,
@@ -300,11 +431,12 @@ void launch_edt_3d(int w, int h, int d, const uint32_t* mask, int invert, int32_
     int32_t* g = (int32_t*)work;
     int32_t* mid = (int32_t*)((char*)work + part);
     int* stk = (int*)((char*)work + 2 * part);
+    uint8_t* flags = (uint8_t*)work + 3 * part;
     edt_rows(w, (long long)h * d, mask, invert, g, s);
-    // y lines: (x, z) starts at z*w*h + x, stride w
-    edt_lines(h, (long long)w * d, w, sl, w, g, mid, nullptr, stk, s, "edt_cols");
-    // z lines: (x, y) starts at y*w + x, stride w*h
-    edt_lines(d, sl, sl, 0, sl, mid, d2, fl, stk, s, "edt_slabs");
+    // y lines: plane z, (x, z) starts at z*w*h + x, stride w
+    edt_lines(h, w, d, sl, w, g, mid, nullptr, stk, flags, s, "edt_cols");
+    // z lines: plane y, (x, y) starts at y*w + x, stride w*h
+    edt_lines(d, w, h, w, sl, mid, d2, fl, stk, flags, s, "edt_slabs");
 }
 
 }  // namespace dolhip

@@ -667,3 +667,36 @@ def test_sweeps_random_sizes(gpu, oracle, seed):
         dw = oracle.init_dirw_3d(w, h, d, m)
         np.testing.assert_array_equal(_bits(gpu.occlusion_sweep_3d(w, h, d, dw)),
                                       _bits(oracle.occlusion_sweep_3d(w, h, d, dw)))
+
+
+# ------------------------------------------------------------- long lines (scan + stack fallback)
+
+
+@pytest.mark.parametrize("dims", [(1100, 700), (3000, 40), (40, 3000), (600, 3), (2049, 1)])
+def test_edt_2d_long_lines(gpu, oracle, rng, dims):
+    """Lines above 512 samples: the outward scan, and the stack kernel for the
+    lines whose scan reaches its cap (long empty stretches: one site, empty
+    and full masks), both polarities."""
+    w, h = dims
+    masks = [rand_mask(rng, dims, p) for p in (0.0, 0.001, 0.02, 0.3, 1.0)]
+    one = np.zeros((h, w), bool)
+    one[h // 3, (2 * w) // 3] = True
+    masks.append(pack_mask(one))
+    for m in masks:
+        for inv in (False, True):
+            d2, fl = gpu.edt_2d(w, h, m, inv)
+            exp = oracle.edt_2d(w, h, m, inv)
+            np.testing.assert_array_equal(d2, exp)
+            expf = np.where(exp == 0x7FFFFFFF, np.sqrt(np.float32(np.finfo(np.float32).max)),
+                            np.sqrt(exp.astype(np.float32)))
+            np.testing.assert_array_equal(fl, expf.astype(np.float32))
+
+
+@pytest.mark.parametrize("dims", [(600, 5, 7), (5, 600, 4), (4, 5, 1030), (530, 20, 9)])
+def test_edt_3d_long_lines(gpu, oracle, rng, dims):
+    w, h, d = dims
+    for p in (0.0, 0.002, 0.05, 1.0):
+        m = rand_mask(rng, dims, p)
+        for inv in (False, True):
+            d2, _ = gpu.edt_3d(w, h, d, m, inv)
+            np.testing.assert_array_equal(d2, oracle.edt_3d(w, h, d, m, inv))
