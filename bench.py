@@ -814,6 +814,7 @@ def main():
         avg, n = L.profile_query(k)
         if n:
             kernels[k] = {"avg_ms": avg, "launches": n}
+    redo_ms, redo_n = L.profile_query("edt3d_redo")   # exact redo of failed wide tiles (none here: flags only)
     L.profile_enable(False)
     print(f"# edt3d {ms:.3f} ms/step, kernels {kernels}", file=sys.stderr, flush=True)
 
@@ -856,6 +857,7 @@ def main():
         "pipeline_bytes_per_voxel": bpv,
         "hbm_roofline_pct_of_3pass_bytes": round(100 * pct_3pass, 2),
         "kernels": {k: round(v["avg_ms"], 4) for k, v in kernels.items()},
+        "redo_ms": round(redo_ms, 4) if redo_n else None,
         "roofline": roofline,
         # both kernels against HBM peak: edt3d_z is HBM-bound, edt3d_xy VALU-bound
         # (its scan steps), so its HBM fraction is low by construction

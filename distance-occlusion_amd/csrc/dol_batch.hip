@@ -83,7 +83,6 @@ struct BigTile {
     int mask, cell0;
 };
 constexpr int kBigThreads = 256;
-constexpr int kRsrcWord3 = 0x00020000;             // buffer descriptor word 3 for raw dword access
 constexpr int kBigCells = 8;                       // cells a thread
 constexpr int kBigTile = kBigThreads * kBigCells;  // cells a workgroup
 

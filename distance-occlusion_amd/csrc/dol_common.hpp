@@ -13,6 +13,7 @@ namespace dolhip {
 constexpr int kMaxDelta2 = 32767;   // DistanceOcclusionLib.h:56
 constexpr int kMaxDelta3 = 25000;   // DistanceOcclusionLib.h:57
 constexpr int32_t kEdtInf = 0x7fffffff;
+constexpr int kRsrcWord3 = 0x00020000;   // buffer descriptor word 3 for raw dword access (gfx9)
 
 __host__ __device__ inline int mask_stride(int w) { return (w + 31) >> 5; }
 
@@ -69,6 +70,22 @@ __device__ __forceinline__ float sqrt_d2(int v)
     r = em <= 0.0f ? rm : r;
     r = ep > 0.0f ? rp : r;
     return r;
+}
+
+// The same for squared distances below 2^20 (or kEdtInf).  On gfx950 the
+// hardware root of every integer below 2^20 is either correctly rounded or
+// one ulp low, never high (every such integer checked on the device; the
+// tests compare every root this path produces with IEEE sqrt), so one residual
+// decides: the upper neighbour rp is nearer iff x > (r + ulp/2)^2, i.e.
+// x - r * rp > ulp^2 / 4, and x - r * rp is a multiple of ulp^2 whose sign the
+// fma keeps.
+__device__ __forceinline__ float sqrt_d2_small(int v)
+{
+    const float x = (float)v;
+    const float r = __builtin_amdgcn_sqrtf(x);
+    const float rp = __uint_as_float(__float_as_uint(r) + 1u);
+    const float root = __builtin_fmaf(-rp, r, x) > 0.0f ? rp : r;
+    return v == kEdtInf ? __uint_as_float(0x5f7fffffu) : root;   // sqrt_rn(FLT_MAX)
 }
 
 // delta2 packed in a uint32: x in the low half, y in the high half

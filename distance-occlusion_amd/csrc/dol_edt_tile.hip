@@ -373,7 +373,9 @@ struct FieldBothOut {   // exact value into f32, clamped into f16
     }
 };
 
-struct FinalOut {   // int32 + float at cell index base(q) + x (< 2^31 on this path)
+// Every squared distance of this path is at most 3 * 511^2 < 2^20 (or
+// kEdtInf), the domain of sqrt_d2_small.
+struct FinalOut {   // int32 + float at cell index base(q) + x (< 2^27 on this path)
     int32_t* d2;
     float* fl;
     uint32_t row_stride;   // elements between consecutive q
@@ -384,19 +386,27 @@ struct FinalOut {   // int32 + float at cell index base(q) + x (< 2^31 on this p
     {
         const uint32_t i = base + (uint32_t)q * row_stride + (uint32_t)c;
         if (d2) d2[i] = v;
-        if (fl) fl[i] = sqrt_d2(v);
+        if (fl) fl[i] = sqrt_d2_small(v);
     }
     // columns c, c + 1 (c even; base and row_stride even, outputs 8-byte
-    // aligned).  Nontemporal: the outputs are not read again here, and
-    // these stores leave the caches to the z pass's input.
+    // aligned).  Buffer stores take the 32-bit byte offset as it is (no 64-bit
+    // address per store); nontemporal: the outputs are not read again here,
+    // and these stores leave the caches to the z pass's input.
     __device__ void pair(int c, int q, int v0, int v1) const
     {
-        typedef int i32x2 __attribute__((ext_vector_type(2)));
-        typedef float f32x2 __attribute__((ext_vector_type(2)));
+        typedef unsigned u32x2 __attribute__((__vector_size__(8)));
+        constexpr int kNontemporal = 2;
         const uint32_t i = base + (uint32_t)q * row_stride + (uint32_t)c;
         if (c + 1 < valid) {
-            if (d2) __builtin_nontemporal_store((i32x2){v0, v1}, (i32x2*)(d2 + i));
-            if (fl) __builtin_nontemporal_store((f32x2){sqrt_d2(v0), sqrt_d2(v1)}, (f32x2*)(fl + i));
+            const int off = (int)(i * 4u);
+            if (d2)
+                __builtin_amdgcn_raw_buffer_store_b64((u32x2){(uint32_t)v0, (uint32_t)v1},
+                                                      __builtin_amdgcn_make_buffer_rsrc(d2, 0, -1, kRsrcWord3),
+                                                      off, 0, kNontemporal);
+            if (fl)
+                __builtin_amdgcn_raw_buffer_store_b64(
+                    (u32x2){__float_as_uint(sqrt_d2_small(v0)), __float_as_uint(sqrt_d2_small(v1))},
+                    __builtin_amdgcn_make_buffer_rsrc(fl, 0, -1, kRsrcWord3), off, 0, kNontemporal);
         } else if (c < valid) {
             (*this)(c, q, v0);
         }
@@ -513,7 +523,22 @@ __device__ __forceinline__ uint32_t pk_add_sat(uint32_t a, uint32_t b)
     return __builtin_bit_cast(uint32_t, __builtin_elementwise_add_sat(__builtin_bit_cast(u16x2, a),
                                                                       __builtin_bit_cast(u16x2, b)));
 }
-__device__ __forceinline__ uint32_t clamp16(uint32_t v) { return min(v, kClamp); }
+__device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b)   // modular (v_pk_add_u16)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) + __builtin_bit_cast(u16x2, b));
+}
+__device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b)   // modular (v_pk_sub_u16)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) - __builtin_bit_cast(u16x2, b));
+}
+__device__ __forceinline__ uint32_t pk_mul(uint32_t a, uint32_t b)   // low halves of the products (v_pk_mul_lo_u16)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) * __builtin_bit_cast(u16x2, b));
+}
+// the packed x pass clamps x distances at 255, so the y scan's results are
+// exact below 255^2 (a clamped sample's true value is at least that); the z
+// scan's input is exact below kClamp
+constexpr uint32_t kXExact = 255u * 255u;
 
 // A lane computes kLaneRows consecutive samples of its two lines: the rows
 // its samples read at steps k .. k + kStepGroup - 1 overlap, so each group
@@ -551,9 +576,14 @@ __device__ __forceinline__ void window_minima(int n, const uint32_t* hs, uint32_
     for (int e = tid; e < kWinWords; e += kThreads) {
         const int b = e / kPairs, i = e % kPairs;
         uint32_t m = 0xffffffffu;
+        if (8 * b + 8 <= n) {   // the whole block inside the line (every block of a full line)
 #pragma unroll
-        for (int r = 0; r < 8; r++)
-            if (8 * b + r < n) m = pk_min(m, hs[(8 * b + r) * kPairs + i]);
+            for (int r = 0; r < 8; r++) m = pk_min(m, hs[(8 * b + r) * kPairs + i]);
+        } else {
+#pragma unroll
+            for (int r = 0; r < 8; r++)
+                if (8 * b + r < n) m = pk_min(m, hs[(8 * b + r) * kPairs + i]);
+        }
         mwin[e] = m;
     }
     __syncthreads();
@@ -562,9 +592,9 @@ __device__ __forceinline__ void window_minima(int n, const uint32_t* hs, uint32_
     for (int j = 0; j < kPer; j++) {
         const int e = tid + j * kThreads, b = e / kPairs, i = e % kPairs;
         uint32_t m = mwin[e];
+        // blocks past the last repeat it (a minimum taken twice is the same)
 #pragma unroll
-        for (int s = 1; s < 4; s++)
-            if (b + s < kWinBlocks) m = pk_min(m, mwin[(b + s) * kPairs + i]);
+        for (int s = 1; s < 4; s++) m = pk_min(m, mwin[min(b + s, kWinBlocks - 1) * kPairs + i]);
         v[j] = m;
     }
     __syncthreads();
@@ -593,7 +623,7 @@ __device__ __forceinline__ void pair_rows(const uint32_t* hs, int row0, int n, i
     }
 }
 
-template <class Out>
+template <uint32_t kExact, class Out>
 __device__ __forceinline__ bool scan_pairs(int n, const uint32_t* hs, const uint32_t* mwin, Out& out,
                                            int* abort_flag)
 {
@@ -682,7 +712,7 @@ __device__ __forceinline__ bool scan_pairs(int n, const uint32_t* hs, const uint
         }
 #pragma unroll
         for (int r = 0; r < R; r++)
-            if (max(best[r] & 0xffffu, best[r] >> 16) >= kClamp) fail = true;
+            if (max(best[r] & 0xffffu, best[r] >> 16) >= kExact) fail = true;
         if (fail) {
             __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         } else if (active) {
@@ -740,14 +770,18 @@ __device__ __forceinline__ bool xpass(int w, int h, int z, int x0, const uint32_
 {
     constexpr int kN = PACKED ? kWideCols : kCols;
     constexpr uint32_t kFar = 0x10000u;                 // no set bit on that side
-    constexpr uint32_t kCap = PACKED ? 256u : (uint32_t)kMaxN;
+    constexpr uint32_t kCap = kMaxN;
     static_assert(kPairs == 16 && kCols == 16, "a row of LDS words holds the row's mask words");
     const int ws = mask_stride(w);
     const int j = x0 >> 5, boff = PACKED ? 0 : (x0 & 31);
     const int tail = w - 32 * (ws - 1);
     const uint32_t last_valid = tail >= 32 ? 0xffffffffu : ((1u << tail) - 1u);
     int has_site = 0;
-    for (int y = tid; y < h; y += kThreads) {
+    static_assert(kThreads >= kMaxN, "one row a thread");
+    // one row a thread (h <= kMaxN): a plain branch rather than a loop, so no
+    // per-word condition is hoisted into registers that live across it
+    if (tid < h) {
+        const int y = tid;
         const uint32_t* row = mask + ((size_t)z * h + y) * ws;
         uint32_t* srow = fs + y * 16;
         uint32_t words[16];
@@ -763,12 +797,22 @@ __device__ __forceinline__ bool xpass(int w, int h, int z, int x0, const uint32_
             for (int k = 0; k < 16; k++) words[k] = k < ws ? row[k] : 0u;
         }
         uint32_t nz = 0;   // bit k: word k has a set bit
+        if (ws == 16) {    // a full row of words: no per-word range test
 #pragma unroll
-        for (int k = 0; k < 16; k++) {
-            uint32_t m = k < ws ? words[k] ^ invert : 0u;
-            if (k == ws - 1) m &= last_valid;
-            words[k] = m;
-            nz |= (m != 0u ? 1u : 0u) << k;
+            for (int k = 0; k < 16; k++) {
+                uint32_t m = words[k] ^ invert;
+                if (k == 15) m &= last_valid;
+                words[k] = m;
+                nz |= min(m, 1u) << k;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                uint32_t m = k < ws ? words[k] ^ invert : 0u;
+                if (k == ws - 1) m &= last_valid;
+                words[k] = m;
+                nz |= min(m, 1u) << k;
+            }
         }
 #pragma unroll
         for (int k = 0; k < 16; k += 4)
@@ -785,29 +829,45 @@ __device__ __forceinline__ bool xpass(int w, int h, int z, int x0, const uint32_
             dR0 = (uint32_t)(32 * (kr - j) + lo_bit(srow[kr]));
         }
         has_site |= nz != 0u;
-        uint32_t v[kN];
-#pragma unroll
-        for (int t = 0; t < kN; t++) {
-            const int ii = boff + t;   // bit in word j
-            const uint32_t a = clz_or_max(wj << (31 - ii));
-            const uint32_t b = ctz_or_max(wj >> ii);
-            const uint32_t e = min(min(dL0 + ii, dR0 - ii), kCap);
-            const uint32_t dd = min(a, min(b, e));
-            if (PACKED) v[t] = min(dd * dd, 0xffffu);
-            else v[t] = dd >= kCap ? kNoSite : dd * dd;
-        }
-        if (PACKED && 32 * j + kN > w) {   // the last tile of a row: columns past w hold 0
-#pragma unroll
-            for (int t = 0; t < kN; t++)
-                if (32 * j + t >= w) v[t] = 0u;
-        }
         uint4* out = (uint4*)srow;
-        if (PACKED) {
+        if constexpr (PACKED) {
+            // two columns a word from the start: the in-word distances of
+            // columns t, t + 1 (leading / trailing zeros, 0xffffffff for none)
+            // packed by v_perm into 16-bit halves, the distances through the
+            // neighbouring words formed in both halves at once, the least of
+            // them clamped to 255 and squared in 16 bits.  255^2 = 65025 then
+            // stands for "65025 or more", so the y scan takes results below
+            // kXExact as exact.
+            const uint32_t DL = min(dL0, 0x3fffu) * 0x10001u, DR = min(dR0, 0x3fffu) * 0x10001u;
+            uint32_t v[kPairs];
 #pragma unroll
-            for (int t = 0; t < kN; t += 8)
-                out[t >> 3] = make_uint4(v[t] | v[t + 1] << 16, v[t + 2] | v[t + 3] << 16,
-                                         v[t + 4] | v[t + 5] << 16, v[t + 6] | v[t + 7] << 16);
+            for (int q = 0; q < kPairs; q++) {
+                const int t = 2 * q;
+                const uint32_t a = __builtin_amdgcn_perm(clz_or_max(wj << (30 - t)), clz_or_max(wj << (31 - t)),
+                                                         0x05040100u);
+                const uint32_t b = __builtin_amdgcn_perm(ctz_or_max(wj >> (t + 1)), ctz_or_max(wj >> t), 0x05040100u);
+                const uint32_t ii = (uint32_t)t | (uint32_t)(t + 1) << 16;
+                const uint32_t dd = pk_min(pk_min(pk_min(a, b), pk_min(pk_add(DL, ii), pk_sub(DR, ii))), 0x00ff00ffu);
+                v[q] = pk_mul(dd, dd);
+            }
+            if (32 * j + kN > w) {   // the last tile of a row: columns past w (w even) hold 0
+#pragma unroll
+                for (int q = 0; q < kPairs; q++)
+                    if (32 * j + 2 * q >= w) v[q] = 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < kPairs; q += 4) out[q >> 2] = make_uint4(v[q], v[q + 1], v[q + 2], v[q + 3]);
         } else {
+            uint32_t v[kN];
+#pragma unroll
+            for (int t = 0; t < kN; t++) {
+                const int ii = boff + t;   // bit in word j
+                const uint32_t a = clz_or_max(wj << (31 - ii));
+                const uint32_t b = ctz_or_max(wj >> ii);
+                const uint32_t e = min(min(dL0 + ii, dR0 - ii), kCap);
+                const uint32_t dd = min(a, min(b, e));
+                v[t] = dd >= kCap ? kNoSite : dd * dd;
+            }
 #pragma unroll
             for (int t = 0; t < kN; t += 4) out[t >> 2] = make_uint4(v[t], v[t + 1], v[t + 2], v[t + 3]);
         }
@@ -865,18 +925,20 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
 }
 
 // the same over wide tiles (kWideCols lines, one mask word): packed scan
-// into the 16-bit field, and if that fails, each half again exactly (x pass
-// into 32-bit words, then the envelope) into both fields.  flags[z * xt +
-// jt] records which field the z pass reads for this tile: 0 the 16-bit one
-// (every value below 0xffff), 1 the exact one, 2 none (no site: every value
-// is kEdtInf).  w even, f16 4-byte aligned.
+// into the 16-bit field.  flags[z * xt + jt] records which field the z pass
+// reads for this tile: 0 the 16-bit one (every value below 0xffff), 1 the
+// exact one, 2 none (no site: every value is kEdtInf); a tile whose scan fails
+// is marked kRedo here and redone exactly by k_edt3d_xy_redo, which sets 1.
+// (The redo is a kernel of its own so that this one carries none of the
+// envelope's registers.)  w even, f16 4-byte aligned.
+constexpr uint8_t kRedo = 3;
+
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) k_edt3d_xy_wide(int w, int h, int d, const uint32_t* __restrict__ mask,
                                                             uint32_t invert, uint16_t* __restrict__ f16,
-                                                            int32_t* __restrict__ f32,
                                                             uint8_t* __restrict__ flags)
 {
     __shared__ __align__(16) uint32_t fs[kWords];
-    __shared__ __align__(16) uint32_t bm[kBmWords > kWinWords ? kBmWords : kWinWords];   // window minima, then the envelope's bitmask
+    __shared__ __align__(16) uint32_t mwin[kWinWords];
     __shared__ int abort_flag;
     if (threadIdx.x == 0) abort_flag = 0;
     const int xt = (w + kWideCols - 1) / kWideCols;
@@ -889,21 +951,49 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
     if (!any) {
         no_sites_pairs(h, o);
         flag = 2;
-    } else if (window_minima(h, fs, bm, threadIdx.x), !scan_pairs(h, fs, bm, o, &abort_flag)) {
-        Bits bits{bm};
+    } else if (window_minima(h, fs, mwin, threadIdx.x), !scan_pairs<kXExact>(h, fs, mwin, o, &abort_flag)) {
+        flag = kRedo;
+    }
+    if (threadIdx.x == 0) flags[z * xt + jt] = flag;
+}
+
+// the wide tiles marked kRedo, each half again exactly (x pass into 32-bit
+// words, then the envelope) into both fields.  Workgroup g owns tiles g,
+// g + gridDim.x, ...: its threads read those flags at once (one each) and
+// list the marked tiles in LDS, so a volume with no such tile costs one read.
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) k_edt3d_xy_redo(int w, int h, int d, const uint32_t* __restrict__ mask,
+                                                            uint32_t invert, uint16_t* __restrict__ f16,
+                                                            int32_t* __restrict__ f32,
+                                                            uint8_t* __restrict__ flags)
+{
+    __shared__ __align__(16) uint32_t fs[kWords];
+    __shared__ __align__(16) uint32_t bm[kBmWords];
+    __shared__ int list[kThreads];
+    __shared__ int count;
+    const int xt = (w + kWideCols - 1) / kWideCols;
+    const int tiles = xt * d;   // <= kThreads * gridDim.x (checked by the launcher)
+    if (threadIdx.x == 0) count = 0;
+    __syncthreads();
+    const int mine = blockIdx.x + threadIdx.x * gridDim.x;
+    if (mine < tiles && flags[mine] == kRedo) list[atomicAdd(&count, 1)] = mine;
+    __syncthreads();
+    const int n = count;
+    Bits bits{bm};
+    for (int k = 0; k < n; k++) {
+        const int t = list[k];
+        const int z = t / xt, x0 = (t - z * xt) * kWideCols;
         for (int half = 0; half < 2 && x0 + kCols * half < w; half++) {
             // the thread index is opaque to the compiler here, so nothing of
-            // the half's work is hoisted out of the loop (register pressure)
+            // the half's work is hoisted out of the loops (register pressure)
             int tid = threadIdx.x;
             asm volatile("" : "+v"(tid));
-            __syncthreads();
+            __syncthreads();   // the previous half's (or tile's) reads of fs are done
             FieldBothOut oh{f32, f16, w, h, x0 + kCols * half, z};
             xpass<false>(w, h, z, x0 + kCols * half, mask, invert, fs, tid);
             envelope(h, fs, oh, bits, tid);
         }
-        flag = 1;
+        if (threadIdx.x == 0) flags[t] = 1;
     }
-    if (threadIdx.x == 0) flags[z * xt + jt] = flag;
 }
 
 // an f2 sample (< 2^20, or kEdtInf) as an LDS word
@@ -1058,10 +1148,12 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
             return;
         }
     }
-    if (!known || (window_minima(d, fs, bm, threadIdx.x), !scan_pairs(d, fs, bm, o, &abort_flag))) {
+    if (!known || (window_minima(d, fs, bm, threadIdx.x), !scan_pairs<kClamp>(d, fs, bm, o, &abort_flag))) {
         Bits bits{bm};
         for (int half = 0; half < 2 && kCols * half < valid; half++) {
-            int tid = threadIdx.x;   // opaque, as in k_edt3d_xy_wide
+            // the thread index is opaque to the compiler here, so nothing of
+            // the half's work is hoisted out of the loop (register pressure)
+            int tid = threadIdx.x;
             asm volatile("" : "+v"(tid));
             __syncthreads();
             const int xh = x0 + kCols * half, vh = min(kCols, valid - kCols * half);
@@ -1077,6 +1169,9 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8
 bool edt_tile_fits(int w, int h, int d) { return w <= kMaxN && h <= kMaxN && d <= kMaxN; }
 
 static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// workgroups of the redo pass (each checks its share of the tiles' flags)
+constexpr int kRedoGroups = 1024;
 
 // 3D: the exact int32 field, the 16-bit field and the wide tiles' flags
 size_t edt_tile_workspace_3d(int w, int h, int d)
@@ -1108,7 +1203,13 @@ void launch_edt_tile_3d(int w, int h, int d, const uint32_t* mask, int invert, i
         {
             ProfileScope p("edt3d_xy", s);
             hipLaunchKernelGGL(k_edt3d_xy_wide, dim3((unsigned)(xt * d)), dim3(kThreads), 0, s, w,
-                               h, d, mask, invert ? 0xffffffffu : 0u, f16, f2, flags);
+                               h, d, mask, invert ? 0xffffffffu : 0u, f16, flags);
+        }
+        static_assert((kMaxN / kWideCols) * kMaxN <= kThreads * kRedoGroups, "one flag a redo thread");
+        {
+            ProfileScope p("edt3d_redo", s);
+            hipLaunchKernelGGL(k_edt3d_xy_redo, dim3((unsigned)(xt * d < kRedoGroups ? xt * d : kRedoGroups)), dim3(kThreads), 0, s,
+                               w, h, d, mask, invert ? 0xffffffffu : 0u, f16, f2, flags);
         }
         {
             ProfileScope p("edt3d_z", s);

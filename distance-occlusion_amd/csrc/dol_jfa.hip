@@ -194,7 +194,6 @@ constexpr int kKeyScale = 27;
 constexpr uint32_t kPoisonB = 2700000000u;          // 27 * 1e8
 constexpr uint32_t kSelfMaxKey = 27u * 67108864u;   // 27 * 2^26
 constexpr int kFastMaxOff = 606;                    // 54 * 606 <= 32767
-constexpr int kRsrcWord3 = 0x00020000;              // buffer descriptor word 3 for raw dword access (gfx9)
 
 typedef unsigned short ushort2v __attribute__((ext_vector_type(2)));
 typedef short short2v __attribute__((ext_vector_type(2)));

@@ -464,6 +464,28 @@ def test_edt_3d_far_sites(gpu, oracle, rng, dims):
         np.testing.assert_array_equal(fl[finite], np.sqrt(exp[finite].astype(np.float32)))
 
 
+@pytest.mark.parametrize("dims", [(512, 512, 512), (511, 384, 300), (512, 512), (301, 512)])
+def test_edt_corner_site_every_root(gpu, dims):
+    """One site at the origin: d2 = x^2 + y^2 (+ z^2) at every cell, a closed
+    form, so the outputs hold every squared distance a grid of this size can
+    produce, and each float must be IEEE sqrt of it bit for bit.  (The tiled
+    path's roots rest on the hardware root of an integer below 2^20 being
+    correctly rounded or one ulp low; this covers every value it meets.)"""
+    shape = tuple(reversed(dims))
+    bits = np.zeros(shape, bool)
+    bits[(0,) * len(dims)] = True
+    m = pack_mask(bits)
+    sq = [np.arange(n, dtype=np.int32) ** 2 for n in shape]
+    if len(dims) == 3:
+        d2, fl = gpu.edt_3d(*dims, m)
+        exp = sq[0][:, None, None] + sq[1][None, :, None] + sq[2][None, None, :]
+    else:
+        d2, fl = gpu.edt_2d(*dims, m)
+        exp = sq[0][:, None] + sq[1][None, :]
+    np.testing.assert_array_equal(d2, exp)
+    np.testing.assert_array_equal(fl.view(np.uint32), np.sqrt(exp.astype(np.float32)).view(np.uint32))
+
+
 @pytest.mark.parametrize("dims", [(64, 40, 24), (33, 20, 9), (64, 64)])
 def test_edt_one_output(gpu, oracle, rng, dims):
     """Either output may be omitted (a null pointer)."""
