@@ -69,6 +69,22 @@ def timed(fn, steps, torch):
     return start.elapsed_time(end) / steps
 
 
+def profile_run(L, fn, steps, torch, names):
+    """Per-kernel mean durations (HIP events around each launch) from a run of
+    their own: the per-step times are taken with no such events in the timed
+    region, which would add their own cost to every launch."""
+    L.profile_enable(True)
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    ks = {}
+    for k in names:
+        avg, n = L.profile_query(k)
+        ks[k] = {"avg_ms": round(avg, 4), "launches": n}
+    L.profile_enable(False)
+    return ks
+
+
 def load_traffic(kernel):
     """HBM bytes per launch measured with rocprofv3 counters (profiles/), if any."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
@@ -131,13 +147,8 @@ def bench_jfa3d(L, torch, dev, mask_t, w, h, d, fl, cpu=True):
         L.call_dev("signed_distance_3d", w, h, d, mask_t, delta, fl)
 
     jstep()
-    L.profile_enable(True)
     jms = timed(jstep, 2, torch)
-    jk = {}
-    for k in ("border_3d", "jfa3d_pass", "signed_dist_3d"):
-        avg, n = L.profile_query(k)
-        jk[k] = {"avg_ms": round(avg, 4), "launches": n}
-    L.profile_enable(False)
+    jk = profile_run(L, jstep, 2, torch, ("border_3d", "jfa3d_pass", "signed_dist_3d"))
     passes = jk["jfa3d_pass"]["launches"] // 2
     bpv = 0.125 + 6 + passes * 12 + 4
     out = {
@@ -240,13 +251,8 @@ def bench_jfa2d(L, torch, dev, cpu=True):
         L.call_dev("signed_distance_2d", w, h, m_t, delta, out)
 
     step()
-    L.profile_enable(True)
     ms = timed(step, 3, torch)
-    ks = {}
-    for k in ("border_2d", "jfa2d_pass", "signed_dist_2d"):
-        avg, n = L.profile_query(k)
-        ks[k] = {"avg_ms": round(avg, 4), "launches": n}
-    L.profile_enable(False)
+    ks = profile_run(L, step, 3, torch, ("border_2d", "jfa2d_pass", "signed_dist_2d"))
     passes = ks["jfa2d_pass"]["launches"] // 3
     bpp = 0.125 + 4 + passes * 8 + 4
     res = {
@@ -284,13 +290,8 @@ def bench_two_field(L, torch, dev, cpu=True):
         L.call_dev("encode_two_field_2d", w, h, d0, d1, 0.25 * w, 0, out)
 
     step()
-    L.profile_enable(True)
     ms = timed(step, 3, torch)
-    ks = {}
-    for k in ("init_deltas_2d", "jfa2d_pass", "encode_two_field_2d"):
-        avg, n = L.profile_query(k)
-        ks[k] = {"avg_ms": round(avg, 4), "launches": n}
-    L.profile_enable(False)
+    ks = profile_run(L, step, 3, torch, ("init_deltas_2d", "jfa2d_pass", "encode_two_field_2d"))
     passes = ks["jfa2d_pass"]["launches"] // 6
     # per field: mask in, init out, the passes' reads and writes; then two
     # fields in and one byte out
@@ -384,21 +385,11 @@ def bench_felzenszwalb(L, torch, dev, cpu=True):
         felz()
         edt()
         reps = 20 if w == 512 else 5
-        L.profile_enable(True)
         fms = timed(felz, reps, torch)
-        ks = {}
-        for k in ("init_distances", "felzenszwalb_cols", "felzenszwalb_rows"):
-            avg, n = L.profile_query(k)
-            ks[k] = {"avg_ms": round(avg, 4), "launches": n}
-        L.profile_enable(False)
-        L.profile_enable(True)
+        ks = profile_run(L, felz, reps, torch, ("init_distances", "felzenszwalb_cols", "felzenszwalb_rows"))
         ems = timed(edt, reps, torch)
-        ek = {}
-        for k in ("edt2d_xy", "edt_rows", "edt_cols"):
-            avg, n = L.profile_query(k)
-            if n:
-                ek[k] = {"avg_ms": round(avg, 4), "launches": n}
-        L.profile_enable(False)
+        ek = {k: v for k, v in profile_run(L, edt, reps, torch, ("edt2d_xy", "edt_rows", "edt_cols")).items()
+              if v["launches"]}
         out[f"felzenszwalb_{name}"] = {"ms": round(fms, 4), "gpixels_per_s": round(w * h / (fms * 1e-3) / 1e9, 3),
                                        "kernels": ks, "digest": digest(f)}
         out[f"edt2d_{name}"] = {"ms": round(ems, 4), "gpixels_per_s": round(w * h / (ems * 1e-3) / 1e9, 3),
@@ -581,10 +572,8 @@ def bench_sweep(L, torch, dev, mask3_t=None, cpu=True):
         L.call_dev("fast_sweep_2d", w, h, delta, 2)
 
     step2()
-    L.profile_enable(True)
     ms = timed(step2, 3, torch)
-    sweep_ms, _ = L.profile_query("fast_sweep_2d")
-    L.profile_enable(False)
+    sweep_ms = profile_run(L, step2, 3, torch, ("fast_sweep_2d",))["fast_sweep_2d"]["avg_ms"]
     launches = 4 * (w // 64 + -(-h // 63) - 1)   # tiles of 64 columns x 63 rows, one launch a diagonal
     out["fast_sweep_2d_8192"] = {"ms": round(ms, 3), "sweep_ms": round(sweep_ms, 3),
                                  "gpixels_per_s": round(w * h / (sweep_ms * 1e-3) / 1e9, 3),
@@ -620,10 +609,8 @@ def bench_sweep(L, torch, dev, mask3_t=None, cpu=True):
         L.call_dev("fast_sweep_3d", n, n, n, d3, 2)
 
     step3()
-    L.profile_enable(True)
     ms = timed(step3, 2, torch)
-    sweep_ms, _ = L.profile_query("fast_sweep_3d")
-    L.profile_enable(False)
+    sweep_ms = profile_run(L, step3, 2, torch, ("fast_sweep_3d",))["fast_sweep_3d"]["avg_ms"]
     launches = 8 * (n // 8 + -(-n // 15) + -(-(n - 1) // 2) - 2)   # tiles of 8 x 15 x 2 slices, four a wavefront
     out["fast_sweep_3d_512"] = {"ms": round(ms, 3), "sweep_ms": round(sweep_ms, 3),
                                 "gvoxels_per_s": round(n ** 3 / (sweep_ms * 1e-3) / 1e9, 3),
@@ -664,11 +651,9 @@ def bench_occlusion(L, torch, dev, mask3_t=None, cpu=True):
         L.call_dev("encode_dirw_2d", w, h, dw, rgba)
 
     step2()
-    L.profile_enable(True)
     ms = timed(step2, 3, torch)
-    sweep_ms, _ = L.profile_query("occlusion_sweep")
-    enc_ms, _ = L.profile_query("encode_ao")
-    L.profile_enable(False)
+    pk = profile_run(L, step2, 3, torch, ("occlusion_sweep", "encode_ao"))
+    sweep_ms, enc_ms = pk["occlusion_sweep"]["avg_ms"], pk["encode_ao"]["avg_ms"]
     out["occlusion_2d_8192"] = {"ms": round(ms, 3), "sweep_ms": round(sweep_ms, 3), "encode_ms": round(enc_ms, 4),
                                 "gpixels_per_s": round(w * h / (sweep_ms * 1e-3) / 1e9, 3),
                                 "launches": w // 64 + -(-h // 63) - 1}
@@ -699,10 +684,8 @@ def bench_occlusion(L, torch, dev, mask3_t=None, cpu=True):
         L.call_dev("occlusion_sweep_3d", n, n, n, dw)
 
     step3()
-    L.profile_enable(True)
     ms = timed(step3, 2, torch)
-    sweep_ms, _ = L.profile_query("occlusion_sweep")
-    L.profile_enable(False)
+    sweep_ms = profile_run(L, step3, 2, torch, ("occlusion_sweep",))["occlusion_sweep"]["avg_ms"]
     out["occlusion_3d_512"] = {"ms": round(ms, 3), "sweep_ms": round(sweep_ms, 3),
                                "gvoxels_per_s": round(n ** 3 / (sweep_ms * 1e-3) / 1e9, 3),
                                "launches": n // 16 + -(-n // 15) + -(-n // 2) - 2}
@@ -730,8 +713,11 @@ def mask_cpu_view(L, torch, dev, pos, idx, bbox, n):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # about 25 ms of untimed steps first: the engine clock takes a few tens of
+    # steps of this workload to settle, and the first ones run up to 10 %
+    # slower
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
@@ -807,19 +793,19 @@ def main():
         step()
     torch.cuda.synchronize()
 
-    L.profile_enable(True)
     ms = timed(step, args.steps, torch)
-    kernels = {}
-    for k in EDT3D_BYTES_PER_VOXEL:
-        avg, n = L.profile_query(k)
-        if n:
-            kernels[k] = {"avg_ms": avg, "launches": n}
-    redo_ms, redo_n = L.profile_query("edt3d_redo")   # exact redo of failed wide tiles (none here: flags only)
-    L.profile_enable(False)
+    pk = profile_run(L, step, args.steps, torch, (*EDT3D_BYTES_PER_VOXEL, "edt3d_redo"))
+    kernels = {k: v for k, v in pk.items() if v["launches"] and k in EDT3D_BYTES_PER_VOXEL}
+    # exact redo of failed wide tiles (none in this scene: one read of the flags)
+    redo_ms, redo_n = pk["edt3d_redo"]["avg_ms"], pk["edt3d_redo"]["launches"]
     print(f"# edt3d {ms:.3f} ms/step, kernels {kernels}", file=sys.stderr, flush=True)
 
+    # the tiled path runs in column chunks (several launches of each kernel
+    # a step): a launch covers cells * steps / launches voxels
+    for v in kernels.values():
+        v["cells"] = cells * args.steps // v["launches"]
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
-    dom_bytes = EDT3D_BYTES_PER_VOXEL[dom] * cells
+    dom_bytes = EDT3D_BYTES_PER_VOXEL[dom] * kernels[dom]["cells"]
     achieved = dom_bytes / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
     roofline = {
         "kernel": dom,
@@ -856,16 +842,16 @@ def main():
         "hbm_roofline_pct_pipeline": round(100 * pipeline_pct, 2),
         "pipeline_bytes_per_voxel": bpv,
         "hbm_roofline_pct_of_3pass_bytes": round(100 * pct_3pass, 2),
-        "kernels": {k: round(v["avg_ms"], 4) for k, v in kernels.items()},
+        "kernels": {k: round(v["avg_ms"], 4) for k, v in kernels.items()},   # mean per launch
         "redo_ms": round(redo_ms, 4) if redo_n else None,
         "roofline": roofline,
         # both kernels against HBM peak: edt3d_z is HBM-bound, edt3d_xy VALU-bound
         # (its scan steps), so its HBM fraction is low by construction
         "roofline_by_kernel": {
             k: {"bound": "hbm" if k == "edt3d_z" else "valu",
-                "hbm_gbs": round(EDT3D_BYTES_PER_VOXEL[k] * cells / (v["avg_ms"] * 1e-3) / 1e9, 1),
-                "hbm_frac": round(EDT3D_BYTES_PER_VOXEL[k] * cells / (v["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                "avg_ms": round(v["avg_ms"], 4)}
+                "hbm_gbs": round(EDT3D_BYTES_PER_VOXEL[k] * v["cells"] / (v["avg_ms"] * 1e-3) / 1e9, 1),
+                "hbm_frac": round(EDT3D_BYTES_PER_VOXEL[k] * v["cells"] / (v["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "avg_ms": round(v["avg_ms"], 4), "launches_per_step": v["launches"] // args.steps}
             for k, v in kernels.items()},
     }
 
