@@ -368,7 +368,7 @@ constexpr int jfa3d_lanes() { return W == 512 ? 256 : kRowMax / 2; }
 // z - step, z, z + step), so consecutive rows share two of their three
 // y-rows.  The ring holds 3 y-rows x 3 z-rows of staged cells: row i + 1's
 // new y-row replaces row i - 2's in slot (i + 1) mod 3.  A staged cell
-// holds its raw (x, y), its z and
+// holds its raw (x, y) and
 //     Bb = 27 * (sx^2 + sy^2 + (sz + kc*off)^2)
 // which, unlike the per-row B of jfa3d_row, does not depend on the y offset
 // the cell is read with, so it is formed once per run; a tap adds
@@ -381,7 +381,7 @@ constexpr int jfa3d_lanes() { return W == 512 ? 256 : kRowMax / 2; }
 // flags then come from k_jfa3d_classify).  The loop is unrolled by the
 // ring's three phases so every LDS offset is an immediate.
 template <bool IN_AOS, bool OUT_AOS, int W>
-__global__ void __launch_bounds__(jfa3d_lanes<W>()) __attribute__((amdgpu_waves_per_eu(W == 512 ? 3 : 2)))
+__global__ void __launch_bounds__(jfa3d_lanes<W>()) __attribute__((amdgpu_waves_per_eu(W == 512 ? 4 : 2)))
 k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __restrict__ in,
              int16_t* __restrict__ out, int cw, const uint8_t* __restrict__ flags_in,
              uint8_t* __restrict__ flags_out, int run, int residues)
@@ -475,21 +475,24 @@ k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __r
     }
     const bool any_max = (fl & kRowHasMax) != 0;
 
-    // LDS: ring [3 z][3 slots][w + 1] (xy, Bb) pairs, the same of int16 z,
-    // then the decoding tables of the three phases
+    // LDS: ring [3 z][3 slots][w + 1] (xy, Bb) pairs, then the decoding
+    // tables of the three phases.  The winner's z is read from the source
+    // field itself (a cell of the 9 rows just loaded for staging), which keeps
+    // the ring at 8 bytes a cell so that 4 workgroups share a CU.
     const int stride = w + 1;
     u32x2* pb = (u32x2*)lds;
-    int16_t* zs = (int16_t*)(pb + 9 * stride);
-    uint4* tbl = (uint4*)(lds + ((9 * stride * 10 + 15) / 16) * 4);
+    uint4* tbl = (uint4*)(lds + ((9 * stride * 8 + 15) / 16) * 4);
     for (int e = t; e < 81; e += T) {
         // phase p = i mod 3, rank: the winner's cell index relative to
-        // (slot 0, column x) and the offsets added to its (x, y) and z
+        // (slot 0, column x), the offsets added to its (x, y) and z, and its
+        // cell in the source field relative to the output cell
         const int p = e / 27, rk = e % 27;
         const int tt = rk == 0 ? 13 : (rk <= 13 ? rk - 1 : rk);
         const int r = tt / 3, ic = tt % 3 - 1, jc = r % 3 - 1, kc = r / 3 - 1;
         const int slot = (kc + 1) * 3 + (p + jc + 3) % 3;
         const uint32_t dxy = (uint32_t)(uint16_t)(ic * off) | (uint32_t)(uint16_t)(jc * off) << 16;
-        tbl[e] = make_uint4((uint32_t)(slot * stride + ic * step), dxy, (uint32_t)(kc * off), 0u);
+        const int src = (kc * step * h + jc * step) * w + ic * step;
+        tbl[e] = make_uint4((uint32_t)(slot * stride + ic * step), dxy, (uint32_t)(kc * off), (uint32_t)src);
     }
     if (t < 9) pb[t * stride + w] = u32x2{0u, kPoisonB};   // the poison column
     // stage this lane's cells of y-row g of every z-row into ring slot g mod 3;
@@ -518,10 +521,7 @@ k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __r
                     X = m ? 0u : X;
                     B = m ? kPoisonB : B;
                 }
-                if (live[c]) {
-                    pb[(k * 3 + slot) * stride + xc[c]] = u32x2{X, B};
-                    zs[(k * 3 + slot) * stride + xc[c]] = (int16_t)vz[k][c];
-                }
+                if (live[c]) pb[(k * 3 + slot) * stride + xc[c]] = u32x2{X, B};
             }
         }
     };
@@ -639,7 +639,11 @@ k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __r
             ushort2v v = __builtin_bit_cast(ushort2v, pb[idx].x);
             v += __builtin_bit_cast(ushort2v, e.y);
             bxy[c] = rank ? __builtin_bit_cast(uint32_t, v) : self_xy[0][c];
-            bz[c] = rank ? zs[idx] + (int)e.z : self_z[0][c];
+            // the winner's z from its source cell (in range: a winner is never
+            // a tap outside the volume); the cell's own for rank 0
+            const size_t si = ((size_t)z * h + y0 + i * step) * w + x + (int)e.w;
+            const int sz = IN_AOS ? in[3 * si + 2] : in[2 * cells + si];
+            bz[c] = rank ? sz + (int)e.z : self_z[0][c];
         }
         const size_t row = (size_t)z * h + y;
 #pragma unroll
@@ -742,12 +746,12 @@ void launch_jfa3d_pass(int step, int w, int h, int d, const int16_t* in, int16_t
 }
 
 // dynamic LDS of the run kernel: the larger of the keyed path's ring of
-// (xy, Bb) pairs and z with its decoding tables, and the general path's
-// planes
+// (xy, Bb) pairs with its decoding tables (38,232 bytes for 512-wide rows:
+// four workgroups a CU), and the general path's planes
 static size_t jfa3d_rows_lds(int w)
 {
     const size_t general = (size_t)9 * w * 6;
-    const size_t keyed = ((size_t)9 * (w + 1) * 10 + 15) / 16 * 16 + 81 * 16;
+    const size_t keyed = ((size_t)9 * (w + 1) * 8 + 15) / 16 * 16 + 81 * 16;
     return general > keyed ? general : keyed;
 }
 
