@@ -545,7 +545,7 @@ constexpr uint32_t kXExact = 255u * 255u;
 // reads kStepGroup + kLaneRows - 1 rows on either side instead of
 // kStepGroup * kLaneRows.
 #ifndef DOL_LANE_ROWS
-#define DOL_LANE_ROWS 3
+#define DOL_LANE_ROWS 4   // a pass of 8 wavefronts x 16 rows: 512-sample lines in 4 passes
 #endif
 #ifndef DOL_PAIR_GROUP
 #define DOL_PAIR_GROUP 8
