@@ -235,12 +235,13 @@ __device__ __forceinline__ uint32_t fold_flags(uint64_t f)
 
 // One output row (y, z) by the general path, which follows the reference's
 // arithmetic including its int16 and int32 wraps: lane t takes columns
-// x = t + c T (c < COLS, T lanes) from its cells cxy[c][r], cz[c][r] of
+// x = t + c T (c < COLS, T lanes; with PAIR x = COLS t + c) from its cells
+// cxy[c][r], cz[c][r] of
 // staged row r = 3 (kc+1) + (jc+1), source row (z + kc*step, y + jc*step),
 // present where rv[r] (uniform).  Stores the result (interleaved if OUT_AOS)
 // and the row's flag bytes.  Uses LDS from `lds` on; the caller
 // synchronises before using it again.
-template <bool OUT_AOS, int W, int COLS>
+template <bool OUT_AOS, int W, int COLS, bool PAIR>
 __device__ __forceinline__ void jfa3d_row_general(uint32_t* lds, int w, int t, int T, size_t row,
                                                   int step, int cw, const bool (&rv)[9],
                                                   const uint32_t (&cxy)[COLS][9],
@@ -254,7 +255,7 @@ __device__ __forceinline__ void jfa3d_row_general(uint32_t* lds, int w, int t, i
     bool live[COLS];
 #pragma unroll
     for (int c = 0; c < COLS; c++) {
-        xc[c] = t + c * T;
+        xc[c] = PAIR ? COLS * t + c : t + c * T;
         live[c] = kAllLive || xc[c] < w;
     }
     uint32_t bxy[COLS];   // the results
@@ -352,6 +353,8 @@ __device__ __forceinline__ void jfa3d_row_general(uint32_t* lds, int w, int t, i
     }
 }
 
+constexpr int kRun = 8;   // output rows (at most) a workgroup of k_jfa3d_ring takes
+
 // columns a lane takes: two (half the lanes, so the scalar work of a row is
 // shared by twice the cells)
 template <int W>
@@ -384,7 +387,7 @@ template <bool IN_AOS, bool OUT_AOS, int W>
 __global__ void __launch_bounds__(jfa3d_lanes<W>()) __attribute__((amdgpu_waves_per_eu(W == 512 ? 4 : 2)))
 k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __restrict__ in,
              int16_t* __restrict__ out, int cw, const uint8_t* __restrict__ flags_in,
-             uint8_t* __restrict__ flags_out, int run, int residues)
+             uint8_t* __restrict__ flags_out, int residues)
 {
     constexpr int COLS = jfa3d_cols<W>();
     constexpr bool kAllLive = W > 0 && W % (64 * COLS) == 0;
@@ -395,17 +398,21 @@ k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __r
     const int z = q % d;
     const int qq = q / d;
     const int r0 = qq % residues, ch = qq / residues;
-    const int y0 = r0 + ch * run * step;
+    const int y0 = r0 + ch * kRun * step;
     if (y0 >= h) return;
-    const int n = min(run, (h - 1 - y0) / step + 1);
+    const int n = min(kRun, (h - 1 - y0) / step + 1);
     const int t = threadIdx.x, T = blockDim.x;
     const int off = cw * step;
+    // The first pass of 512-wide rows reads the caller's interleaved cells
+    // two adjacent ones a lane, 12 bytes in one load, so a lane's columns are
+    // 2t and 2t + 1 there; otherwise t and t + T.
+    constexpr bool kPairAos = IN_AOS && W == 512 && COLS == 2;
     int xc[COLS];
     bool live[COLS];
     unsigned xl[COLS];
 #pragma unroll
     for (int c = 0; c < COLS; c++) {
-        xc[c] = t + c * T;
+        xc[c] = kPairAos ? COLS * t + c : t + c * T;
         live[c] = kAllLive || xc[c] < w;
         xl[c] = live[c] ? xc[c] : w - 1;
     }
@@ -414,26 +421,53 @@ k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __r
     // this lane's cells of source row sr (rows outside the volume are never
     // read): the row's address in a buffer descriptor, the lane's offset in
     // one register
-    auto load = [&](int sr, uint32_t (&vxy)[COLS], int (&vz)[COLS]) {
-        sr = __builtin_amdgcn_readfirstlane(sr);
-        if (IN_AOS) {
-            const auto rr = __builtin_amdgcn_make_buffer_rsrc((void*)(in + (size_t)sr * w * 3), 0, w * 6, kRsrcWord3);
+    // vz gets the raw 16 bits (zero-extended): the sign extension waits for
+    // the first use, so that nothing needs a prefetched value early.
+    // With ok false the loads are still issued, through a descriptor of no
+    // records (they read nothing and return 0), so that every path issues
+    // the same loads and waits on them can count exactly.
+    auto load = [&](int sr, uint32_t (&vxy)[COLS], int (&vz)[COLS], bool ok = true) {
+        sr = __builtin_amdgcn_readfirstlane(ok ? sr : 0);
+        const int nb = ok ? 1 : 0;
+        if (kPairAos) {
+            // raw words d0 d1 d2 = (x0 y0) (z0 x1) (y1 z1) in vxy[0], vz[0],
+            // vxy[1]; unpack() sorts them out where they are first used
+            const auto rr = __builtin_amdgcn_make_buffer_rsrc((void*)(in + (size_t)sr * w * 3), 0, nb * w * 6, kRsrcWord3);
+            const auto v = __builtin_amdgcn_raw_buffer_load_b96(rr, t * 12, 0, 0);
+            vxy[0] = v[0];
+            vz[0] = (int)v[1];
+            vxy[1] = v[2];
+            vz[1] = 0;
+        } else if (IN_AOS) {
+            const auto rr = __builtin_amdgcn_make_buffer_rsrc((void*)(in + (size_t)sr * w * 3), 0, nb * w * 6, kRsrcWord3);
 #pragma unroll
             for (int c = 0; c < COLS; c++) {
-                const uint32_t lo = __builtin_amdgcn_raw_buffer_load_b16(rr, xl[c] * 6, 0, 0);
-                const uint32_t hi = __builtin_amdgcn_raw_buffer_load_b16(rr, xl[c] * 6 + 2, 0, 0);
-                vxy[c] = lo | hi << 16;
-                vz[c] = (int16_t)__builtin_amdgcn_raw_buffer_load_b16(rr, xl[c] * 6 + 4, 0, 0);
+                // x and y loaded into the two halves of one register (no
+                // instruction combines them, so nothing waits on the loads)
+                ushort2v v;
+                v.x = __builtin_amdgcn_raw_buffer_load_b16(rr, xl[c] * 6, 0, 0);
+                v.y = __builtin_amdgcn_raw_buffer_load_b16(rr, xl[c] * 6 + 2, 0, 0);
+                vxy[c] = __builtin_bit_cast(uint32_t, v);
+                vz[c] = (int)__builtin_amdgcn_raw_buffer_load_b16(rr, xl[c] * 6 + 4, 0, 0);
             }
         } else {
             const size_t base = (size_t)sr * w;
-            const auto rxy = __builtin_amdgcn_make_buffer_rsrc((void*)((const uint32_t*)in + base), 0, w * 4, kRsrcWord3);
-            const auto rz = __builtin_amdgcn_make_buffer_rsrc((void*)(in + 2 * cells + base), 0, w * 2, kRsrcWord3);
+            const auto rxy = __builtin_amdgcn_make_buffer_rsrc((void*)((const uint32_t*)in + base), 0, nb * w * 4, kRsrcWord3);
+            const auto rz = __builtin_amdgcn_make_buffer_rsrc((void*)(in + 2 * cells + base), 0, nb * w * 2, kRsrcWord3);
 #pragma unroll
             for (int c = 0; c < COLS; c++) {
                 vxy[c] = __builtin_amdgcn_raw_buffer_load_b32(rxy, xl[c] * 4, 0, 0);
-                vz[c] = (int16_t)__builtin_amdgcn_raw_buffer_load_b16(rz, xl[c] * 2, 0, 0);
+                vz[c] = (int)__builtin_amdgcn_raw_buffer_load_b16(rz, xl[c] * 2, 0, 0);
             }
+        }
+    };
+    // the loaded cells as (x, y) pairs and 16-bit z, from what load() left
+    auto unpack = [&](uint32_t (&vxy)[COLS], int (&vz)[COLS]) __attribute__((always_inline)) {
+        if constexpr (kPairAos) {
+            const uint32_t d1 = (uint32_t)vz[0], d2 = vxy[1];
+            vz[0] = (int)(d1 & 0xffffu);
+            vxy[1] = (d1 >> 16) | (d2 << 16);
+            vz[1] = (int)(d2 >> 16);
         }
     };
     auto y_ok = [&](int g) { const int yy = y0 + g * step; return yy >= 0 && yy < h; };
@@ -461,13 +495,14 @@ k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __r
                 uint32_t vxy[COLS] = {};
                 int vz[COLS] = {};
                 if (rv[r]) load((z + (k - 1) * step) * h + y + jc * step, vxy, vz);
+                unpack(vxy, vz);
 #pragma unroll
                 for (int c = 0; c < COLS; c++) {
                     cxy[c][r] = vxy[c];
-                    cz[c][r] = vz[c];
+                    cz[c][r] = (int16_t)vz[c];
                 }
             }
-            jfa3d_row_general<OUT_AOS, W, COLS>(lds, w, t, T, (size_t)z * h + y, step, cw, rv, cxy, cz,
+            jfa3d_row_general<OUT_AOS, W, COLS, kPairAos>(lds, w, t, T, (size_t)z * h + y, step, cw, rv, cxy, cz,
                                                 cells, out, flags_out);
             __syncthreads();
         }
@@ -497,8 +532,16 @@ k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __r
     if (t < 9) pb[t * stride + w] = u32x2{0u, kPoisonB};   // the poison column
     // stage this lane's cells of y-row g of every z-row into ring slot g mod 3;
     // a row outside the volume is staged as poison, so the taps need no test
-    auto stage = [&](int g, const uint32_t (&vxy)[3][COLS], const int (&vz)[3][COLS]) {
+    auto stage = [&](int g, uint32_t (&vxy)[3][COLS], int (&vz)[3][COLS]) {
         const int slot = (g + 3) % 3;
+        // the loads' values are first used here, not where the scheduler
+        // would like to convert them (right after the loads)
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int c = 0; c < COLS; c++) asm volatile("" : "+v"(vxy[k][c]), "+v"(vz[k][c]));
+#pragma unroll
+        for (int k = 0; k < 3; k++) unpack(vxy[k], vz[k]);
         const bool gy = y_ok(g);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -511,7 +554,7 @@ k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __r
             const int kc = k - 1;
 #pragma unroll
             for (int c = 0; c < COLS; c++) {
-                const int czr = vz[k][c] + kc * off;
+                const int czr = (int16_t)vz[k][c] + kc * off;
                 const int len = __builtin_amdgcn_sdot2(__builtin_bit_cast(short2v, vxy[k][c]),
                                                        __builtin_bit_cast(short2v, vxy[k][c]), __mul24(czr, czr), false);
                 uint32_t B = __umul24((uint32_t)len, kKeyScale);
@@ -544,7 +587,7 @@ k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __r
 #pragma unroll
                 for (int c = 0; c < COLS; c++) {
                     self_xy[g][c] = vxy[1][c];
-                    self_z[g][c] = vz[1][c];
+                    self_z[g][c] = (int16_t)vz[1][c];
                 }
             }
         }
@@ -557,36 +600,77 @@ k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __r
     uint32_t nxy[3][COLS] = {};
     int nz[3][COLS] = {};
     auto prefetch = [&](int g) {
-        if (g >= n + 1 || !y_ok(g)) return;
+        const bool gy = g < n + 1 && y_ok(g);
 #pragma unroll
         for (int k = 0; k < 3; k++)
-            if (zv[k]) load((z + (k - 1) * step) * h + y0 + g * step, nxy[k], nz[k]);
+            load((z + (k - 1) * step) * h + y0 + g * step, nxy[k], nz[k], gy && zv[k]);
     };
     prefetch(2);
     __syncthreads();
-    auto body = [&](int i, auto P) {
+    // A row's results wait in registers until the next row has put its
+    // prefetch in flight: the winner's z is loaded from
+    // the field, and a wait for that load right after it would also wait for
+    // the prefetch issued before it (loads complete in order), stalling every
+    // row for a full memory latency.  pxy: the winners' (x, y); pz: the loaded z of each
+    // winner's source cell; pdz: the offset added to it.
+    uint32_t pxy[COLS];
+    int pz[COLS], pdz[COLS];
+#pragma unroll
+    for (int c = 0; c < COLS; c++) {
+        pxy[c] = 0;
+        pz[c] = 0;
+        pdz[c] = 0;
+    }
+    auto flush = [&](int i) __attribute__((always_inline)) {
+        const size_t row = (size_t)z * h + y0 + i * step;
+#pragma unroll
+        for (int c = 0; c < COLS; c++) {
+            const uint32_t bxy = pxy[c];
+            const int bz = (int16_t)((uint32_t)pz[c] >> ((pdz[c] & 1) * 16)) + (pdz[c] >> 1);
+            if (live[c]) {
+                const size_t o = row * w + xc[c];
+                if (OUT_AOS) {
+                    int16_t* op = out + 3 * o;
+                    op[0] = (int16_t)bxy; op[1] = (int16_t)(bxy >> 16); op[2] = (int16_t)bz;
+                } else {
+                    ((uint32_t*)out)[o] = bxy;
+                    out[2 * cells + o] = (int16_t)bz;
+                }
+            }
+            if (flags_out) {
+                const int ox = (int16_t)bxy, oy = (int16_t)(bxy >> 16), oz = (int16_t)bz;
+                const bool m = ox == kMaxDelta3;
+                const bool unb = !m && (abs(ox) > kFastBound || abs(oy) > kFastBound || abs(oz) > kFastBound);
+                const uint64_t bu = __ballot(live[c] && unb), bmx = __ballot(live[c] && m);
+                if ((t & 63) == 0 && xc[c] < w)
+                    flags_out[row * kFlagSlots + (xc[c] >> 6)] =
+                        (uint8_t)((bu ? kRowUnbounded : 0u) | (bmx ? kRowHasMax : 0u));
+            }
+        }
+    };
+    auto body = [&](int i, auto P) __attribute__((always_inline)) {
         constexpr int p = decltype(P)::value;   // i mod 3
         if (i > 0) {
             // y-row i + 1 into the slot of y-row i - 2, and the row after
-            // it into flight
+            // it into flight; then row i - 1's stores
             if (i + 1 <= n) stage(i + 1, nxy, nz);
 #pragma unroll
             for (int c = 0; c < COLS; c++) {
                 self_xy[0][c] = self_xy[1][c];
                 self_z[0][c] = self_z[1][c];
                 self_xy[1][c] = nxy[1][c];
-                self_z[1][c] = nz[1][c];
+                self_z[1][c] = (int16_t)nz[1][c];
             }
             prefetch(i + 2);
+            // keep the prefetch's loads ahead of the stores' wait on the z
+            // loads (the scheduler would otherwise hoist their first use)
+#pragma unroll
+            for (int c = 0; c < COLS; c++) asm volatile("" : "+v"(pz[c])::"memory");
+            flush(i - 1);
             __syncthreads();
         }
-        const int y = y0 + i * step;
-        uint32_t bxy[COLS];
-        int bz[COLS];
 #pragma unroll
         for (int c = 0; c < COLS; c++) {
-            bxy[c] = 0;
-            bz[c] = 0;
             if (!live[c]) continue;
             const int x = xc[c];
             uint32_t best;
@@ -638,43 +722,50 @@ k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __r
             const int idx = x + (int)e.x;
             ushort2v v = __builtin_bit_cast(ushort2v, pb[idx].x);
             v += __builtin_bit_cast(ushort2v, e.y);
-            bxy[c] = rank ? __builtin_bit_cast(uint32_t, v) : self_xy[0][c];
+            pxy[c] = rank ? __builtin_bit_cast(uint32_t, v) : self_xy[0][c];
             // the winner's z from its source cell (in range: a winner is never
-            // a tap outside the volume); the cell's own for rank 0
+            // a tap outside the volume); for rank 0 the entry's offsets are 0,
+            // so this is the cell's own z
+            // (the 32-bit word holding it is loaded, so that nothing needs
+            // the value before the next row's flush)
             const size_t si = ((size_t)z * h + y0 + i * step) * w + x + (int)e.w;
-            const int sz = IN_AOS ? in[3 * si + 2] : in[2 * cells + si];
-            bz[c] = rank ? sz + (int)e.z : self_z[0][c];
-        }
-        const size_t row = (size_t)z * h + y;
-#pragma unroll
-        for (int c = 0; c < COLS; c++) {
-            if (live[c]) {
-                const size_t o = row * w + xc[c];
-                if (OUT_AOS) {
-                    int16_t* op = out + 3 * o;
-                    op[0] = (int16_t)bxy[c]; op[1] = (int16_t)(bxy[c] >> 16); op[2] = (int16_t)bz[c];
-                } else {
-                    ((uint32_t*)out)[o] = bxy[c];
-                    out[2 * cells + o] = (int16_t)bz[c];
-                }
-            }
-            if (flags_out) {
-                const int ox = (int16_t)bxy[c], oy = (int16_t)(bxy[c] >> 16), oz = (int16_t)bz[c];
-                const bool m = ox == kMaxDelta3;
-                const bool unb = !m && (abs(ox) > kFastBound || abs(oy) > kFastBound || abs(oz) > kFastBound);
-                const uint64_t bu = __ballot(live[c] && unb), bmx = __ballot(live[c] && m);
-                if ((t & 63) == 0 && xc[c] < w)
-                    flags_out[row * kFlagSlots + (xc[c] >> 6)] =
-                        (uint8_t)((bu ? kRowUnbounded : 0u) | (bmx ? kRowHasMax : 0u));
-            }
+            const size_t zi = IN_AOS ? 3 * si + 2 : 2 * cells + si;
+            pz[c] = (int)((const uint32_t*)in)[zi >> 1];
+            pdz[c] = (int)e.z * 2 + (int)(zi & 1);   // the offset, and which half
         }
         __syncthreads();
     };
-    for (int i = 0; i < n; i += 3) {
-        body(i, std::integral_constant<int, 0>{});
-        if (i + 1 < n) body(i + 1, std::integral_constant<int, 1>{});
-        if (i + 2 < n) body(i + 2, std::integral_constant<int, 2>{});
+    // the run's rows fully unrolled (n <= kRun): values in flight pass from
+    // one row to the next in straight-line code, with no loop to carry them
+    // around (whose register copies would wait for the loads)
+    auto row = [&](auto I) __attribute__((always_inline)) {
+        constexpr int i = decltype(I)::value;
+        body(i, std::integral_constant<int, i % 3>{});
+        if (i + 1 < n) return false;
+        flush(i);
+        return true;
+    };
+    static_assert(kRun == 8, "the run's rows are unrolled for eight");
+    if constexpr (IN_AOS && !kPairAos) {
+        // the first pass (three 16-bit loads a cell) keeps a loop: unrolled,
+        // its loads in flight would need more registers than four workgroups
+        // a CU leave
+        for (int i = 0; i < n; i += 3) {
+            body(i, std::integral_constant<int, 0>{});
+            if (i + 1 < n) body(i + 1, std::integral_constant<int, 1>{});
+            if (i + 2 < n) body(i + 2, std::integral_constant<int, 2>{});
+        }
+        flush(n - 1);
+        return;
     }
+    if (row(std::integral_constant<int, 0>{})) return;
+    if (row(std::integral_constant<int, 1>{})) return;
+    if (row(std::integral_constant<int, 2>{})) return;
+    if (row(std::integral_constant<int, 3>{})) return;
+    if (row(std::integral_constant<int, 4>{})) return;
+    if (row(std::integral_constant<int, 5>{})) return;
+    if (row(std::integral_constant<int, 6>{})) return;
+    row(std::integral_constant<int, 7>{});
 }
 
 // Flag bytes of the caller's interleaved field, in the passes' format, for
@@ -755,7 +846,6 @@ static size_t jfa3d_rows_lds(int w)
     return general > keyed ? general : keyed;
 }
 
-constexpr int kRun = 8;   // output rows a workgroup of k_jfa3d_runs takes
 
 static void launch_jfa3d_rows(bool first, bool out_aos, int step, int w, int h, int d,
                               const int16_t* in, int16_t* out, int cw, const uint8_t* flags_in,
@@ -792,7 +882,7 @@ static void launch_jfa3d_rows(bool first, bool out_aos, int step, int w, int h, 
                         : (out_aos ? k_jfa3d_ring<false, true, W> : k_jfa3d_ring<false, false, W>);  \
         want_lds((const void*)fn);                                                                   \
         hipLaunchKernelGGL(fn, grid_runs, dim3(threads), lds, s, step, w, h, d, cells, in, out, cw,  \
-                           flags_in, flags_out, kRun, residues);                                     \
+                           flags_in, flags_out, residues);                                           \
     }
     if (w == 512) {
         DOL_JFA3D_ROWS(512)
