@@ -623,24 +623,31 @@ k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __r
     }
     auto flush = [&](int i) __attribute__((always_inline)) {
         const size_t row = (size_t)z * h + y0 + i * step;
+        // planar output through descriptors of this row (32-bit lane
+        // offsets; a column past the row falls outside and stores nothing)
+        const auto rxy = __builtin_amdgcn_make_buffer_rsrc((void*)((uint32_t*)out + row * w), 0, w * 4, kRsrcWord3);
+        const auto rz = __builtin_amdgcn_make_buffer_rsrc((void*)(out + 2 * cells + row * w), 0, w * 2, kRsrcWord3);
 #pragma unroll
         for (int c = 0; c < COLS; c++) {
             const uint32_t bxy = pxy[c];
             const int bz = (int16_t)((uint32_t)pz[c] >> ((pdz[c] & 1) * 16)) + (pdz[c] >> 1);
-            if (live[c]) {
-                const size_t o = row * w + xc[c];
-                if (OUT_AOS) {
-                    int16_t* op = out + 3 * o;
+            if (OUT_AOS) {
+                if (live[c]) {
+                    int16_t* op = out + 3 * (row * w + xc[c]);
                     op[0] = (int16_t)bxy; op[1] = (int16_t)(bxy >> 16); op[2] = (int16_t)bz;
-                } else {
-                    ((uint32_t*)out)[o] = bxy;
-                    out[2 * cells + o] = (int16_t)bz;
                 }
+            } else {
+                __builtin_amdgcn_raw_buffer_store_b32(bxy, rxy, xc[c] * 4, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b16((uint16_t)bz, rz, xc[c] * 2, 0, 0);
             }
             if (flags_out) {
-                const int ox = (int16_t)bxy, oy = (int16_t)(bxy >> 16), oz = (int16_t)bz;
-                const bool m = ox == kMaxDelta3;
-                const bool unb = !m && (abs(ox) > kFastBound || abs(oy) > kFastBound || abs(oz) > kFastBound);
+                // |component| > kFastBound, each component biased by
+                // kFastBound and compared unsigned
+                const uint32_t u = __builtin_bit_cast(uint32_t, __builtin_bit_cast(ushort2v, bxy) +
+                                                                    ushort2v{kFastBound, kFastBound});
+                const uint32_t uz = (uint32_t)(bz + kFastBound);
+                const bool m = (bxy & 0xffffu) == (uint32_t)kMaxDelta3;
+                const bool unb = !m && max(max(u & 0xffffu, u >> 16), uz) > 2u * kFastBound;
                 const uint64_t bu = __ballot(live[c] && unb), bmx = __ballot(live[c] && m);
                 if ((t & 63) == 0 && xc[c] < w)
                     flags_out[row * kFlagSlots + (xc[c] >> 6)] =
