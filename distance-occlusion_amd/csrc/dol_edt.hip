@@ -3,9 +3,9 @@
 // Two families:
 //  * Felzenszwalb float (DistanceOcclusionLib.cpp:1018-1085), bit-identical to
 //    the reference: the same float operations in the same order, no FMA
-//    contraction, correctly rounded division.  One thread per column (then
-//    per row), envelope stacks in a workspace laid out [k][line] so that
-//    lanes at equal depth touch consecutive words.
+//    contraction, correctly rounded division.  Each pass is an envelope
+//    kernel (one lane a line, the stack's top in LDS) and an output kernel
+//    (one workgroup a line, one output a thread).
 //  * Exact integer EDT (an extension; the reference has only the float 2D
 //    form): a per-row nearest-set-bit pass straight from the bitmask, then the
 //    lower envelope of parabolas along each further axis with intersections
@@ -17,101 +17,210 @@
 namespace dolhip {
 
 // ------------------------------------------------------ Felzenszwalb float
+//
+// DistanceOcclusionLib.cpp:1026-1062 runs each line in two loops: the lower
+// envelope of the parabolas (a stack v, z whose pops depend on float
+// comparisons, so a line is one sequential chain), then the outputs, each
+// taking the first envelope entry k with z[k + 1] >= q.  Here the two loops
+// are two kernels:
+//  * k_felz_envelope: one lane a line, the same float operations in the same
+//    order.  The stack's top entries live in an LDS ring of kRing entries a
+//    line ((v, z) and the key F = f[v] + v*v the reference recomputes on
+//    each pop, the same float); every push is also written to the line's
+//    stack in device memory, which a pop deeper than the ring reads.  The
+//    line's samples are loaded kFelzAhead steps ahead.  A workgroup is one
+//    wavefront with LPW lanes (lines) in use, so the lines spread over the
+//    SIMDs and a wavefront's lanes wait for each other's pops less.
+//  * k_felz_output: one workgroup a line, one output a thread.  z[1..D] rises
+//    strictly (a push follows a failed "s <= z[k]"), so the entry the
+//    reference's running scan reaches for q is the number of z[1..D] below
+//    q, found by binary search over the line's z in LDS.  A NaN in z breaks
+//    the order; such a line (marked by the envelope kernel) runs the
+//    reference's scan in one thread.
+// Layouts: element q of line l at f2[l + q * m]; outputs at out[l * n + q];
+// the stacks at V[l * n + k], Z[l * (n + 1) + k]; D[l] the final depth, or
+// -1 - depth for a line with a NaN in z.
 
-// 1D envelope of DistanceOcclusionLib.cpp:1026-1062 over n samples of f at
-// stride fs, writing n outputs at stride os.  v and z are this line's stacks
-// at stride `ls` (number of lines).
-__device__ void felz_line(int n, const float* __restrict__ f, size_t fs, float* __restrict__ out,
-                          size_t os, int* __restrict__ v, float* __restrict__ z, size_t ls)
+constexpr int kRing = 64;        // stack entries a line keeps in LDS (half with 64 lines)
+constexpr int kFelzAhead = 16;   // samples a line has in flight
+
+template <int LPW>
+__global__ void __launch_bounds__(LPW) k_felz_envelope(int n, int lines, const float* __restrict__ f2, size_t m,
+                                                       int* __restrict__ V, float* __restrict__ Z,
+                                                       int* __restrict__ D)
 {
-    int k = 0;
-    v[0] = 0;
-    z[0] = -FLT_MAX;
-    z[ls] = +FLT_MAX;
+    // the ring: entry (v, z, F) of depth k at slot k mod R, one 16-byte
+    // record a lane (one LDS write or read a push or pop)
+    constexpr int R = LPW >= 64 ? kRing / 2 : kRing;
+    __shared__ uint4 ring[R][LPW];
+    const int t = threadIdx.x;
+    const int l = blockIdx.x * LPW + t;
+    if (l >= lines) return;
+    const float* f = f2 + l;
+    // the stacks through descriptors based at the workgroup's first line,
+    // with 32-bit lane offsets (below 2^32 for lines of up to 2^24 samples)
+    const size_t l0 = (size_t)blockIdx.x * LPW;
+    const auto rV = __builtin_amdgcn_make_buffer_rsrc((void*)(V + l0 * n), 0, 0xffffffffu, kRsrcWord3);
+    const auto rZ = __builtin_amdgcn_make_buffer_rsrc((void*)(Z + l0 * (n + 1)), 0, 0xffffffffu, kRsrcWord3);
+    const uint32_t oV = (uint32_t)t * (uint32_t)n * 4u, oZ = (uint32_t)t * (uint32_t)(n + 1) * 4u;
+    int k = 0, lo = 0;   // stack depth; the lowest depth the ring still holds
     int vk = 0;
     float zk = -FLT_MAX;
-    for (int q = 1; q < n; q++) {
-        const float fq = __fadd_rn(f[fs * q], (float)(q * q));
-        float fv = __fadd_rn(f[fs * vk], (float)(vk * vk));
-        float s = __fdiv_rn(__fsub_rn(fq, fv), (float)(2 * (q - vk)));
+    float Fk = __fadd_rn(f[0], 0.0f);
+    auto put = [&](int d, int v, float z, float F) __attribute__((always_inline)) {
+        ring[d & (R - 1)][t] = make_uint4((uint32_t)v, __float_as_uint(z), __float_as_uint(F), 0u);
+        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)v, rV, oV + 4u * (uint32_t)d, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(z), rZ, oZ + 4u * (uint32_t)d, 0, 0);
+    };
+    put(0, 0, -FLT_MAX, Fk);
+    bool nan = false;
+    auto step = [&](int q, float fq) __attribute__((always_inline)) {
+        const float fqt = __fadd_rn(fq, (float)(q * q));
+        float s = __fdiv_rn(__fsub_rn(fqt, Fk), (float)(2 * (q - vk)));
         while (s <= zk && k > 0) {
             k--;
-            vk = v[ls * k];
-            zk = z[ls * k];
-            fv = __fadd_rn(f[fs * vk], (float)(vk * vk));
-            s = __fdiv_rn(__fsub_rn(fq, fv), (float)(2 * (q - vk)));
+            if (k >= lo) {
+                const uint4 e = ring[k & (R - 1)][t];
+                vk = (int)e.x;
+                zk = __uint_as_float(e.y);
+                Fk = __uint_as_float(e.z);
+            } else {
+                vk = (int)__builtin_amdgcn_raw_buffer_load_b32(rV, oV + 4u * (uint32_t)k, 0, 0);
+                zk = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rZ, oZ + 4u * (uint32_t)k, 0, 0));
+                Fk = __fadd_rn(f[m * (size_t)vk], (float)(vk * vk));
+                // wait for these loads here, in the rare branch: left to
+                // the loop, the wait would cover the samples in flight too
+                asm volatile("" : "+v"(vk), "+v"(zk), "+v"(Fk));
+            }
+            s = __fdiv_rn(__fsub_rn(fqt, Fk), (float)(2 * (q - vk)));
         }
         k++;
-        v[ls * k] = q;
-        z[ls * k] = s;
-        z[ls * (k + 1)] = +FLT_MAX;
+        put(k, q, s, fqt);
+        lo = max(lo, k - R + 1);
+        nan |= s != s;
         vk = q;
         zk = s;
+        Fk = fqt;
+    };
+    // samples kFelzAhead steps ahead; every load is issued (past the line's
+    // end it reads the last sample), so that the waits on them count exactly
+    auto sample = [&](int q) { return f[m * (size_t)min(q, n - 1)]; };
+    float cur[kFelzAhead], nxt[kFelzAhead];
+#pragma unroll
+    for (int j = 0; j < kFelzAhead; j++) cur[j] = sample(1 + j);
+    int q0 = 1;
+    for (; q0 + kFelzAhead <= n; q0 += kFelzAhead) {
+#pragma unroll
+        for (int j = 0; j < kFelzAhead; j++) nxt[j] = sample(q0 + kFelzAhead + j);
+#pragma unroll
+        for (int j = 0; j < kFelzAhead; j++) step(q0 + j, cur[j]);
+#pragma unroll
+        for (int j = 0; j < kFelzAhead; j++) cur[j] = nxt[j];
     }
-    k = 0;
-    vk = v[0];
-    float znext = z[ls];
-    for (int q = 0; q < n; q++) {
-        while (znext < (float)q) {
-            k++;
-            vk = v[ls * k];
-            znext = z[ls * (k + 1)];
+#pragma unroll
+    for (int j = 0; j < kFelzAhead; j++)
+        if (q0 + j < n) step(q0 + j, cur[j]);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(+FLT_MAX), rZ, oZ + 4u * (uint32_t)(k + 1), 0, 0);
+    D[l] = nan ? -1 - k : k;
+}
+
+constexpr int kFelzOutThreads = 256;
+
+// outputs of line blockIdx.x; zs holds z[1..D] (n floats at most)
+__global__ void __launch_bounds__(kFelzOutThreads) k_felz_output(int n, const float* __restrict__ f2, size_t m,
+                                                                 const int* __restrict__ V,
+                                                                 const float* __restrict__ Z,
+                                                                 const int* __restrict__ D,
+                                                                 float* __restrict__ out)
+{
+    extern __shared__ float zs[];
+    const int l = blockIdx.x;
+    const float* f = f2 + l;
+    const int* Vl = V + (size_t)l * n;
+    const float* Zl = Z + (size_t)l * (n + 1);
+    float* o = out + (size_t)l * n;
+    const int dd = D[l];
+    if (dd < 0) {
+        // a NaN in z: the reference's scan as it is
+        if (threadIdx.x == 0) {
+            int k = 0;
+            for (int q = 0; q < n; q++) {
+                while (Zl[k + 1] < (float)q) k++;
+                const int vk = Vl[k];
+                o[q] = __fadd_rn((float)((q - vk) * (q - vk)), f[m * (size_t)vk]);
+            }
         }
-        out[os * q] = __fadd_rn((float)((q - vk) * (q - vk)), f[fs * vk]);
+        return;
+    }
+    for (int j = threadIdx.x; j < dd; j += kFelzOutThreads) zs[j] = Zl[1 + j];
+    __syncthreads();
+    for (int q = threadIdx.x; q < n; q += kFelzOutThreads) {
+        // k = #{j in 1..dd : z[j] < q}
+        const float fq = (float)q;
+        int lo = 0, hi = dd;   // the count lies in [lo, hi]
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (zs[mid] < fq)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        const int vk = Vl[lo];
+        o[q] = __fadd_rn((float)((q - vk) * (q - vk)), f[m * (size_t)vk]);
     }
 }
 
-// one wavefront a workgroup: the lines (one a thread) spread over as many CUs
-// as there are wavefronts, each with an L1 of its own for its dependent reads
-constexpr int kFelzThreads = 64;
-
-// column pass: line x reads dist[q*w + x], writes td[x*h + q]
-__global__ void __launch_bounds__(kFelzThreads) k_felz_cols(int w, int h, const float* __restrict__ dist,
-                                                   float* __restrict__ td, int* __restrict__ v,
-                                                   float* __restrict__ z)
-{
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= w) return;
-    felz_line(h, dist + x, (size_t)w, td + (size_t)x * h, 1, v + x, z + x, (size_t)w);
-}
-
-// row pass: line y reads td[q*h + y], writes dist[y*w + q]
-__global__ void __launch_bounds__(kFelzThreads) k_felz_rows(int w, int h, const float* __restrict__ td,
-                                                   float* __restrict__ dist, int* __restrict__ v,
-                                                   float* __restrict__ z)
-{
-    const int y = blockIdx.x * blockDim.x + threadIdx.x;
-    if (y >= h) return;
-    felz_line(w, td + y, (size_t)h, dist + (size_t)y * w, 1, v + y, z + y, (size_t)h);
-}
-
-// transposed field td (w*h floats), stack v (w*h ints: a column's h entries
-// at stride w, or a row's w entries at stride h) and stack z (one more entry
-// per line than v)
+// the transposed field td (w*h floats), the stacks V (w*h ints) and Z
+// (w*h + max(w, h) floats), the depths D (max(w, h) ints)
 static inline size_t felz_part(size_t b) { return (b + 255) & ~(size_t)255; }
 
 size_t felz_workspace(int w, int h)
 {
     const size_t cells = (size_t)w * h, lines = (size_t)(w > h ? w : h);
-    return felz_part(cells * 4) * 2 + felz_part((cells + lines) * 4);
+    return felz_part(cells * 4) * 2 + felz_part((cells + lines) * 4) + felz_part(lines * 4);
+}
+
+// one 1D pass over `lines` lines of n samples
+static void felz_pass(int n, int lines, const float* f2, size_t m, float* out, int* V, float* Z, int* D,
+                      hipStream_t s)
+{
+    // wavefronts of LPW lines: about one a SIMD (1024 on the chip) or more.
+    // At least two lanes, so that the line index is per lane and every read
+    // of the stack in device memory (written by this kernel) is a vector load.
+#define DOL_FELZ_ENV(LPW)                                                                              \
+    hipLaunchKernelGGL(k_felz_envelope<LPW>, dim3(grid1d((size_t)lines, LPW)), dim3(LPW), 0, s, n, lines, \
+                       f2, m, V, Z, D)
+    if (lines >= 64 * 1024)
+        DOL_FELZ_ENV(64);
+    else if (lines >= 16 * 1024)
+        DOL_FELZ_ENV(16);
+    else if (lines >= 4 * 1024)
+        DOL_FELZ_ENV(4);
+    else
+        DOL_FELZ_ENV(2);
+#undef DOL_FELZ_ENV
+    const size_t lds = (size_t)n * 4;
+    if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_felz_output, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_felz_output, dim3((unsigned)lines), dim3(kFelzOutThreads), lds, s, n, f2, m, V, Z, D, out);
 }
 
 void launch_felzenszwalb_2d(int w, int h, float* dist, void* work, hipStream_t s)
 {
-    const size_t cells = (size_t)w * h;
+    const size_t cells = (size_t)w * h, lines = (size_t)(w > h ? w : h);
     char* p = (char*)work;
     float* td = (float*)p;
-    int* v = (int*)(p + felz_part(cells * 4));
-    float* z = (float*)(p + 2 * felz_part(cells * 4));
+    int* V = (int*)(p + felz_part(cells * 4));
+    float* Z = (float*)(p + 2 * felz_part(cells * 4));
+    int* D = (int*)(p + 2 * felz_part(cells * 4) + felz_part((cells + lines) * 4));
     {
+        // columns: line x reads dist[q*w + x], writes td[x*h + q]
         ProfileScope ps("felzenszwalb_cols", s);
-        hipLaunchKernelGGL(k_felz_cols, dim3(grid1d((size_t)w, kFelzThreads)), dim3(kFelzThreads), 0, s, w, h,
-                           dist, td, v, z);
+        felz_pass(h, w, dist, (size_t)w, td, V, Z, D, s);
     }
     {
+        // rows: line y reads td[q*h + y], writes dist[y*w + q]
         ProfileScope ps("felzenszwalb_rows", s);
-        hipLaunchKernelGGL(k_felz_rows, dim3(grid1d((size_t)h, kFelzThreads)), dim3(kFelzThreads), 0, s, w, h,
-                           td, dist, v, z);
+        felz_pass(w, h, td, (size_t)h, dist, V, Z, D, s);
     }
 }
 

@@ -204,6 +204,42 @@ def test_felzenszwalb_float(gpu, oracle, reference, rng, dims):
             np.testing.assert_array_equal(got.view(np.uint32), reference.felzenszwalb_2d(w, h, f0).view(np.uint32))
 
 
+def _felz_fields(rng, w, h):
+    """Float fields for the Felzenszwalb envelope: random values, small
+    integers (many ties), a ramp down each column whose last sample pops a
+    stack deeper than the envelope kernel's LDS ring, and FLT_MAX with zeros."""
+    fmax = np.finfo(np.float32).max
+    yield rng.uniform(0, 1e4, (h, w)).astype(np.float32)
+    yield rng.integers(0, 8, (h, w)).astype(np.float32)
+    ramp = (1e5 - 100.0 * np.arange(h, dtype=np.float64))[:, None] * np.ones((1, w))
+    ramp[-1, :] = 0.0
+    yield ramp.astype(np.float32)
+    yield np.where(rng.random((h, w)) < 0.02, 0.0, fmax).astype(np.float32)
+
+
+@pytest.mark.parametrize("dims", [(1, 1), (1, 300), (300, 1), (97, 130), (513, 67), (64, 700), (4100, 8)])
+def test_felzenszwalb_float_any_field(gpu, oracle, reference, rng, dims):
+    """DOL::Felzenszwalb on arbitrary float fields, bit for bit (the deep-pop
+    ramp exercises the stack reads from device memory)."""
+    w, h = dims
+    for f0 in _felz_fields(rng, w, h):
+        got = gpu.felzenszwalb_2d(w, h, f0)
+        exp = oracle.felzenszwalb_2d(w, h, f0)
+        np.testing.assert_array_equal(got.view(np.uint32), exp.view(np.uint32))
+        if reference is not None:
+            np.testing.assert_array_equal(got.view(np.uint32), reference.felzenszwalb_2d(w, h, f0).view(np.uint32))
+
+
+def test_felzenszwalb_float_nan(gpu, oracle, rng):
+    """NaN samples: the envelope's z loses its order, and the lines that hold
+    a NaN take the reference's running scan; compared as floats (NaN equal to
+    NaN; the payload bits are not the reference's concern)."""
+    w, h = 150, 90
+    f0 = rng.uniform(0, 1e3, (h, w)).astype(np.float32)
+    f0[rng.random((h, w)) < 0.01] = np.nan
+    np.testing.assert_array_equal(gpu.felzenszwalb_2d(w, h, f0), oracle.felzenszwalb_2d(w, h, f0))
+
+
 # ------------------------------------------------------------- exact EDT
 
 
