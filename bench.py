@@ -560,7 +560,8 @@ def bench_sweep(L, torch, dev, mask3_t=None, cpu=True):
     """FastSweep (the tool's -f method) after border init, cw = 2: the 8192^2
     disc/box scene and the 512^3 scene.  CPU figures: the oracle (one thread)
     on the full 8192^2 field, which also checks the GPU result cell for cell,
-    and on a 128^3 corner of the 3D start field."""
+    and on a 128^3 corner of the 3D start field, which the GPU also sweeps
+    for a voxel-for-voxel check."""
     out = {}
     w = h = 8192
     mask = L.scene_shapes_2d(w, h, 2000, seed=2)
@@ -621,11 +622,17 @@ def bench_sweep(L, torch, dev, mask3_t=None, cpu=True):
         torch.cuda.synchronize()
         corner = np.ascontiguousarray(d3.cpu().numpy().reshape(n, n, n, 3)[:k, :k, :k])
         t0 = time.perf_counter()
-        O.fast_sweep_3d(k, k, k, corner, 2)
+        ref = O.fast_sweep_3d(k, k, k, corner, 2)
         cpu_s = time.perf_counter() - t0
+        # the GPU on the same corner, checked voxel for voxel
+        c_t = torch.from_numpy(corner.reshape(-1)).to(dev)
+        L.call_dev("fast_sweep_3d", k, k, k, c_t, 2)
+        torch.cuda.synchronize()
+        mism = int((c_t.cpu().numpy().reshape(k, k, k, 3) != ref).any(axis=-1).sum())
         out["fast_sweep_3d_512"]["cpu_baseline"] = {
             "ms_128cube": round(cpu_s * 1e3, 1), "gvoxels_per_s": round(k ** 3 / cpu_s / 1e9, 4), "cores": 1,
             "kind": "port", "sample": "FastSweep of the 128^3 corner of the 512^3 start field"}
+        out["fast_sweep_3d_512"]["parity"] = {"corner_128cube_mismatches_vs_oracle": mism}
     del d3
     torch.cuda.empty_cache()
     return out
@@ -635,7 +642,8 @@ def bench_occlusion(L, torch, dev, mask3_t=None, cpu=True):
     """Occlusion direction fields and sweeps (the tool's ambient occlusion
     input): the 8192^2 disc/box scene, 4 planes; the 512^3 scene, 8 planes in
     place.  CPU figures: the oracle (one thread) on a 2048^2 corner and a
-    128^3 corner, same work per cell."""
+    128^3 corner, same work per cell; the GPU runs the same corners and its
+    fields are compared with the oracle's bit for bit."""
     out = {}
     w = h = 8192
     mask = L.scene_shapes_2d(w, h, 2000, seed=2)
@@ -666,11 +674,19 @@ def bench_occlusion(L, torch, dev, mask3_t=None, cpu=True):
         corner = np.ascontiguousarray(mask.reshape(h, w // 32)[:k, : k // 32])
         start = O.init_dirw_2d(k, k, corner)
         t0 = time.perf_counter()
-        O.occlusion_sweep_2d(k, k, start)
+        ref = O.occlusion_sweep_2d(k, k, start)
         cpu_s = time.perf_counter() - t0
+        # the GPU on the same corner, checked bit for bit
+        cm_t = torch.from_numpy(corner.view(np.int32)).to(dev)
+        cw_t = torch.empty(4 * k * k, dtype=torch.float32, device=dev)
+        L.call_dev("init_dirw_2d", k, k, cm_t, cw_t)
+        L.call_dev("occlusion_sweep_2d", k, k, cw_t)
+        torch.cuda.synchronize()
+        mism = int((cw_t.cpu().numpy().view(np.uint32) != ref.reshape(-1).view(np.uint32)).sum())
         out["occlusion_2d_8192"]["cpu_baseline"] = {
             "ms_2048sq": round(cpu_s * 1e3, 1), "gpixels_per_s": round(k * k / cpu_s / 1e9, 4), "cores": 1,
             "kind": "port", "sample": "the 2048^2 corner of the scene, 4 planes"}
+        out["occlusion_2d_8192"]["parity"] = {"corner_2048sq_float_mismatches_vs_oracle": mism}
     del dw, ao, rgba
     torch.cuda.empty_cache()
 
@@ -695,11 +711,19 @@ def bench_occlusion(L, torch, dev, mask3_t=None, cpu=True):
         corner = np.ascontiguousarray(m3[:k, :k, : k // 32])
         start = O.init_dirw_3d(k, k, k, corner)
         t0 = time.perf_counter()
-        O.occlusion_sweep_3d(k, k, k, start)
+        ref = O.occlusion_sweep_3d(k, k, k, start)
         cpu_s = time.perf_counter() - t0
+        # the GPU on the same corner, checked bit for bit
+        cm_t = torch.from_numpy(corner.view(np.int32)).to(dev)
+        cw_t = torch.empty(8 * k ** 3, dtype=torch.float32, device=dev)
+        L.call_dev("init_dirw_3d", k, k, k, cm_t, cw_t)
+        L.call_dev("occlusion_sweep_3d", k, k, k, cw_t)
+        torch.cuda.synchronize()
+        mism = int((cw_t.cpu().numpy().view(np.uint32) != ref.reshape(-1).view(np.uint32)).sum())
         out["occlusion_3d_512"]["cpu_baseline"] = {
             "ms_128cube": round(cpu_s * 1e3, 1), "gvoxels_per_s": round(k ** 3 / cpu_s / 1e9, 4), "cores": 1,
             "kind": "port", "sample": "the 128^3 corner of the scene, 8 planes"}
+        out["occlusion_3d_512"]["parity"] = {"corner_128cube_float_mismatches_vs_oracle": mism}
     del dw
     torch.cuda.empty_cache()
     return out
