@@ -4,8 +4,10 @@
 //
 // Forward:
 //   1. each seed marks the cell nearest to it (lowest index wins a cell);
-//   2. JumpFlood on the seed-index field (steps w/2 ... 1, then one more
-//      step-1 pass) gives every cell a candidate seed;
+//   2. the library's 2D JumpFlood (cw = 1) from the marked cells gives every
+//      cell a candidate: the marked cell its delta points to, and that
+//      cell's seed (a delta that leads nowhere, as the reference's wrap
+//      rules can make one, leaves the cell without a candidate);
 //   3. exact fix-up: seeds are bucketed on a coarse grid, and each cell
 //      compares, in float64, every seed of the buckets that meet the square
 //      of half-side |c - candidate| around it.  Every seed at least as near
@@ -25,6 +27,7 @@ namespace dolhip {
 
 int check_launch(const char* what);
 void set_error(const char* fmt, ...);
+void launch_jfa2d_pass(int step, int w, int h, const int16_t* in, int16_t* out, int cw, hipStream_t s);
 
 namespace seeds {
 
@@ -68,33 +71,17 @@ __device__ __forceinline__ float d2f(int x, int y, const float* s, int k)
     return dx * dx + dy * dy;
 }
 
-// one JumpFlood pass on the index field (float distances; only a candidate)
-__global__ void __launch_bounds__(256) k_seed_jfa(int w, int h, int step, const float* __restrict__ s,
-                                                  const int* __restrict__ in, int* __restrict__ out)
+// JumpFlood start field: delta 0 at a marked cell, kMaxDelta2 elsewhere
+__global__ void k_seed_start(size_t cells, const int* __restrict__ idx0, uint32_t* __restrict__ delta)
 {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= w || y >= h) return;
-    int best = in[(size_t)y * w + x];
-    float bd = best == kNone ? FLT_MAX : d2f(x, y, s, best);
-    for (int dy = -1; dy <= 1; dy++) {
-        const int yy = y + dy * step;
-        if (yy < 0 || yy >= h) continue;
-        for (int dx = -1; dx <= 1; dx++) {
-            const int xx = x + dx * step;
-            if (xx < 0 || xx >= w || (dx == 0 && dy == 0)) continue;
-            const int k = in[(size_t)yy * w + xx];
-            if (k == kNone) continue;
-            const float d = d2f(x, y, s, k);
-            if (d < bd || (d == bd && k < best)) { bd = d; best = k; }
-        }
-    }
-    out[(size_t)y * w + x] = best;
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < cells) delta[c] = idx0[c] == kNone ? ((uint32_t)kMaxDelta2 * 0x10001u) : 0u;
 }
 
-// counting sort of the seeds into buckets, one workgroup (nb <= 4096)
+// counting sort of the seeds into buckets, one workgroup (nb <= 4096); the
+// list holds each seed as (x, y, index) so the fix-up reads a seed in one go
 __global__ void __launch_bounds__(1024) k_bucket(int n, Grid g, const float* __restrict__ s,
-                                                 int* __restrict__ start, int* __restrict__ list)
+                                                 int* __restrict__ start, float4* __restrict__ list)
 {
     __shared__ int cnt[4096 + 1];
     const int nb = g.nbx * g.nby;
@@ -108,34 +95,63 @@ __global__ void __launch_bounds__(1024) k_bucket(int n, Grid g, const float* __r
     };
     for (int k = threadIdx.x; k < n; k += blockDim.x) atomicAdd(&cnt[bucket_of(k)], 1);
     __syncthreads();
-    if (threadIdx.x == 0) {   // exclusive scan (nb <= 4096, once per call)
-        int acc = 0;
-        for (int b = 0; b < nb; b++) {
-            const int c = cnt[b];
-            cnt[b] = acc;
-            start[b] = acc;
-            acc += c;
-        }
-        start[nb] = acc;
+    // exclusive scan of cnt[0 .. nb) (nb <= 4096): each thread sums a run of
+    // four, the runs' sums are scanned across the workgroup (wavefront
+    // prefix sums, then the wavefronts' totals), then each run is rewritten
+    __shared__ int wsum[1024 / 64];
+    const int t = threadIdx.x;
+    int run[4], tot = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int b = 4 * t + j;
+        run[j] = b < nb ? cnt[b] : 0;
+        tot += run[j];
     }
+    int incl = tot;   // inclusive prefix of the run sums within the wavefront
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if ((t & 63) >= o) incl += v;
+    }
+    if ((t & 63) == 63) wsum[t >> 6] = incl;
+    __syncthreads();
+    int before = incl - tot;
+    for (int v = 0; v < (t >> 6); v++) before += wsum[v];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int b = 4 * t + j;
+        if (b < nb) {
+            cnt[b] = before;
+            start[b] = before;
+        }
+        before += run[j];
+    }
+    if (t == blockDim.x - 1) start[nb] = before;   // the total (every run ends by the last thread's)
     __syncthreads();
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
         const int pos = atomicAdd(&cnt[bucket_of(k)], 1);
-        list[pos] = k;
+        list[pos] = make_float4(s[2 * k], s[2 * k + 1], __int_as_float(k), 0.0f);
     }
 }
 
 // exact nearest seed: float64 comparison, ties to the lowest index
 __global__ void __launch_bounds__(256) k_fixup(int w, int h, Grid g, const float* __restrict__ s,
-                                               const int* __restrict__ cand, const int* __restrict__ start,
-                                               const int* __restrict__ list, int32_t* __restrict__ idx,
-                                               float* __restrict__ dist)
+                                               const int* __restrict__ idx0, const uint32_t* __restrict__ delta,
+                                               const int* __restrict__ start, const float4* __restrict__ list,
+                                               int32_t* __restrict__ idx, float* __restrict__ dist)
 {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (x >= w || y >= h) return;
     const size_t c = (size_t)y * w + x;
-    int best = cand[c];
+    // the candidate: the seed of the marked cell the delta points to
+    int best = kNone;
+    {
+        const uint32_t d = delta[c];
+        const int sx = x + (int16_t)(d & 0xffffu), sy = y + (int16_t)(d >> 16);
+        if (sx >= 0 && sx < w && sy >= 0 && sy < h) best = idx0[(size_t)sy * w + sx];
+    }
     double bd = DBL_MAX;
     int bx0 = 0, bx1 = g.nbx - 1, by0 = 0, by1 = g.nby - 1;
     if (best != kNone) {
@@ -153,8 +169,9 @@ __global__ void __launch_bounds__(256) k_fixup(int w, int h, Grid g, const float
         for (int bx = bx0; bx <= bx1; bx++) {
             const int b = by * g.nbx + bx;
             for (int p = start[b]; p < start[b + 1]; p++) {
-                const int k = list[p];
-                const double dx = (double)x - (double)s[2 * k], dy = (double)y - (double)s[2 * k + 1];
+                const float4 e = list[p];
+                const int k = __float_as_int(e.z);
+                const double dx = (double)x - (double)e.x, dy = (double)y - (double)e.y;
                 const double d = dx * dx + dy * dy;
                 if (d < bd || (d == bd && k < best)) { bd = d; best = k; }
             }
@@ -163,7 +180,8 @@ __global__ void __launch_bounds__(256) k_fixup(int w, int h, Grid g, const float
     dist[c] = (float)sqrt(bd);
 }
 
-// backward: per-workgroup LDS table of (seed, gx, gy), flushed with atomics
+// backward: per-workgroup LDS table of (seed, gx, gy), filled with one sum a
+// wavefront and seed, flushed with atomics
 constexpr int kSlots = 128;
 
 __global__ void __launch_bounds__(256) k_seed_grad(int w, int h, const float* __restrict__ s,
@@ -182,29 +200,52 @@ __global__ void __launch_bounds__(256) k_seed_grad(int w, int h, const float* __
     __syncthreads();
     const int x = blockIdx.x * 16 + (threadIdx.x & 15);
     const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    int mk = -1;   // this cell's seed, or -1 (no term)
+    float vx = 0.0f, vy = 0.0f;
     if (x < w && y < h) {
         const size_t c = (size_t)y * w + x;
         const int k = idx[c];
         const float d = dist[c];
         if (k >= 0 && d > 0.0f) {
             const float a = -gout[c] / d;
-            const float vx = a * ((float)x - s[2 * k]), vy = a * ((float)y - s[2 * k + 1]);
-            // open addressing in LDS; a full table falls back to global atomics
-            int slot = (k * 40503) & (kSlots - 1);
+            mk = k;
+            vx = a * ((float)x - s[2 * k]);
+            vy = a * ((float)y - s[2 * k + 1]);
+        }
+    }
+    // The cells of a wavefront belong to few seeds: for each seed in turn,
+    // the wavefront sums its cells' terms (butterfly over the lanes) and one
+    // lane adds the sum to the workgroup's LDS table (open addressing; a full
+    // table falls back to global atomics).
+    const int lane = threadIdx.x & 63;
+    uint64_t left = __ballot(mk >= 0);
+    while (left) {
+        const int lead = __ffsll((unsigned long long)left) - 1;
+        const int kl = __shfl(mk, lead, 64);
+        const bool mine = mk == kl;
+        left &= ~__ballot(mine);
+        float sx = mine ? vx : 0.0f, sy = mine ? vy : 0.0f;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            sx += __shfl_xor(sx, o, 64);
+            sy += __shfl_xor(sy, o, 64);
+        }
+        if (lane == lead) {
+            int slot = (kl * 40503) & (kSlots - 1);
             bool done = false;
             for (int probe = 0; probe < kSlots; probe++) {
-                const int prev = atomicCAS(&key[slot], -1, k);
-                if (prev == -1 || prev == k) {
-                    atomicAdd(&gx[slot], vx);
-                    atomicAdd(&gy[slot], vy);
+                const int prev = atomicCAS(&key[slot], -1, kl);
+                if (prev == -1 || prev == kl) {
+                    atomicAdd(&gx[slot], sx);
+                    atomicAdd(&gy[slot], sy);
                     done = true;
                     break;
                 }
                 slot = (slot + 1) & (kSlots - 1);
             }
             if (!done) {
-                atomicAdd(&grad[2 * k], vx);
-                atomicAdd(&grad[2 * k + 1], vy);
+                atomicAdd(&grad[2 * kl], sx);
+                atomicAdd(&grad[2 * kl + 1], sy);
             }
         }
     }
@@ -243,7 +284,7 @@ size_t dolhip_seed_workspace(int w, int h, int n)
     if (w < 1 || h < 1 || n < 1) return 0;
     const Grid g = make_grid(w, h);
     const size_t cells = (size_t)w * h;
-    return 2 * al(cells * 4) + al(((size_t)g.nbx * g.nby + 1) * 4) + al((size_t)n * 4);
+    return 3 * al(cells * 4) + al(((size_t)g.nbx * g.nby + 1) * 4) + al((size_t)n * 16);
 }
 
 int dolhip_seed_nearest_2d_dev(int w, int h, int n, const float* seeds, int32_t* idx, float* dist,
@@ -257,34 +298,36 @@ int dolhip_seed_nearest_2d_dev(int w, int h, int n, const float* seeds, int32_t*
     if (g.nbx * g.nby > 4096) return bad("bucket grid too large");
     const size_t cells = (size_t)w * h;
     char* p = (char*)work;
-    int* fa = (int*)p;
-    int* fb = (int*)(p + al(cells * 4));
-    int* start = (int*)(p + 2 * al(cells * 4));
-    int* list = (int*)(p + 2 * al(cells * 4) + al(((size_t)g.nbx * g.nby + 1) * 4));
+    int* idx0 = (int*)p;
+    uint32_t* da = (uint32_t*)(p + al(cells * 4));
+    uint32_t* db = (uint32_t*)(p + 2 * al(cells * 4));
+    int* start = (int*)(p + 3 * al(cells * 4));
+    float4* list = (float4*)(p + 3 * al(cells * 4) + al(((size_t)g.nbx * g.nby + 1) * 4));
     {
         ProfileScope ps("seed_splat", s);
-        hipLaunchKernelGGL(k_fill, dim3(grid1d(cells, 256)), dim3(256), 0, s, fa, cells, kNone);
-        hipLaunchKernelGGL(k_splat, dim3(grid1d((size_t)n, 256)), dim3(256), 0, s, w, h, n, seeds, fa);
+        hipLaunchKernelGGL(k_fill, dim3(grid1d(cells, 256)), dim3(256), 0, s, idx0, cells, kNone);
+        hipLaunchKernelGGL(k_splat, dim3(grid1d((size_t)n, 256)), dim3(256), 0, s, w, h, n, seeds, idx0);
         hipLaunchKernelGGL(k_bucket, dim3(1), dim3(1024), 0, s, n, g, seeds, start, list);
+        hipLaunchKernelGGL(k_seed_start, dim3(grid1d(cells, 256)), dim3(256), 0, s, cells, idx0, da);
+    }
+    {
+        // the reference's pass schedule (steps FloorPow2(max - 1) .. 1, and
+        // one more step-1 pass when the count is odd), cw = 1
+        ProfileScope ps("seed_jfa", s);
+        int m = (w >= h ? w : h) - 1, step = 1;
+        while (step * 2 <= m) step *= 2;
+        uint32_t* a = da;
+        uint32_t* b = db;
+        for (; step > 0 && m > 0; step /= 2) {
+            launch_jfa2d_pass(step, w, h, (const int16_t*)a, (int16_t*)b, 1, s);
+            uint32_t* t = a; a = b; b = t;
+        }
+        if (a != da) launch_jfa2d_pass(1, w, h, (const int16_t*)a, (int16_t*)da, 1, s);
     }
     const dim3 grid((w + 63) / 64, (h + 3) / 4);
-    int m = w >= h ? w : h;
-    int step = 1;
-    while (step * 2 < m) step *= 2;
-    int* a = fa;
-    int* b = fb;
-    {
-        ProfileScope ps("seed_jfa", s);
-        for (; step >= 1; step /= 2) {
-            hipLaunchKernelGGL(k_seed_jfa, grid, dim3(256), 0, s, w, h, step, seeds, a, b);
-            int* t = a; a = b; b = t;
-        }
-        hipLaunchKernelGGL(k_seed_jfa, grid, dim3(256), 0, s, w, h, 1, seeds, a, b);
-        int* t = a; a = b; b = t;
-    }
     {
         ProfileScope ps("seed_fixup", s);
-        hipLaunchKernelGGL(k_fixup, grid, dim3(256), 0, s, w, h, g, seeds, a, start, list, idx, dist);
+        hipLaunchKernelGGL(k_fixup, grid, dim3(256), 0, s, w, h, g, seeds, idx0, da, start, list, idx, dist);
     }
     return check_launch("seed_nearest");
 }
