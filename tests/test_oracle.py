@@ -260,6 +260,25 @@ def test_live_reference(oracle, reference, rng):
         np.testing.assert_array_equal(oracle.jfa_3d(w, h, d, b, 2), reference.jfa_3d(w, h, d, b, 2))
 
 
+def test_felzenszwalb_any_field_live_reference(oracle, reference, rng):
+    """The oracle's Felzenszwalb on arbitrary float fields (random values,
+    small integers with many ties, a ramp whose last sample pops a deep
+    stack, NaNs) against the reference compiled from its sources."""
+    if reference is None:
+        pytest.skip("oracle/_ref not built (no reference sources here)")
+    for w, h in [(1, 1), (1, 300), (300, 1), (97, 130), (64, 700)]:
+        ramp = (1e5 - 100.0 * np.arange(h, dtype=np.float64))[:, None] * np.ones((1, w))
+        ramp[-1, :] = 0.0
+        fields = [rng.uniform(0, 1e4, (h, w)), rng.integers(0, 8, (h, w)), ramp]
+        for f0 in fields:
+            f0 = np.ascontiguousarray(f0, dtype=np.float32)
+            np.testing.assert_array_equal(oracle.felzenszwalb_2d(w, h, f0).view(np.uint32),
+                                          reference.felzenszwalb_2d(w, h, f0).view(np.uint32))
+        f0 = rng.uniform(0, 1e3, (h, w)).astype(np.float32)
+        f0[rng.random((h, w)) < 0.01] = np.nan
+        np.testing.assert_array_equal(oracle.felzenszwalb_2d(w, h, f0), reference.felzenszwalb_2d(w, h, f0))
+
+
 def test_threaded_jfa_equals_serial(oracle, rng):
     w, h = 97, 61
     m = pack_mask(rng.random((h, w)) < 0.01)
