@@ -199,6 +199,7 @@ static void felz_pass(int n, int lines, const float* f2, size_t m, float* out, i
     else
         DOL_FELZ_ENV(2);
 #undef DOL_FELZ_ENV
+    // a line's z in LDS: at most 32767 samples (the entry points' bound), 128 KiB
     const size_t lds = (size_t)n * 4;
     if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_felz_output, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k_felz_output, dim3((unsigned)lines), dim3(kFelzOutThreads), lds, s, n, f2, m, V, Z, D, out);

@@ -217,7 +217,8 @@ def _felz_fields(rng, w, h):
     yield np.where(rng.random((h, w)) < 0.02, 0.0, fmax).astype(np.float32)
 
 
-@pytest.mark.parametrize("dims", [(1, 1), (1, 300), (300, 1), (97, 130), (513, 67), (64, 700), (4100, 8)])
+@pytest.mark.parametrize("dims", [(1, 1), (1, 300), (300, 1), (97, 130), (513, 67), (64, 700), (4100, 8),
+                                  (32767, 3)])
 def test_felzenszwalb_float_any_field(gpu, oracle, reference, rng, dims):
     """DOL::Felzenszwalb on arbitrary float fields, bit for bit (the deep-pop
     ramp exercises the stack reads from device memory)."""
