@@ -50,7 +50,7 @@ void launch_sqrt_f32(size_t n, const float* v, float* out, hipStream_t s);
 void launch_jfa2d_pass(int step, int w, int h, const int16_t* in, int16_t* out, int cw,
                        hipStream_t s);
 size_t jfa3d_workspace(int w, int h, int d);
-void jfa3d_run(int w, int h, int d, int16_t* delta, int cw, void* work, hipStream_t s);
+void jfa3d_run(int w, int h, int d, int16_t* delta, int cw, void* work, hipStream_t s, bool bounded = false);
 void launch_jfa3d_pass(int step, int w, int h, int d, const int16_t* in, int16_t* out, int cw,
                        hipStream_t s);
 size_t felz_workspace(int w, int h);
@@ -1644,7 +1644,7 @@ int dolhip_signed_border_3d_dev(int w, int h, int d, const uint32_t* mask, int16
     void* tmp = nullptr;
     TRY(resolve_work(work, dolhip_jump_flood_3d_workspace(w, h, d), stream, &tmp));
     launch_border_3d(w, h, d, mask, delta, as_stream(stream));
-    jfa3d_run(w, h, d, delta, 2, tmp, as_stream(stream));
+    jfa3d_run(w, h, d, delta, 2, tmp, as_stream(stream), true);
     return finish(as_stream(stream), "signed_border_3d");
 }
 
@@ -1719,8 +1719,8 @@ int dolhip_two_field_deltas_3d_dev(int w, int h, int d, const uint32_t* mask, in
     const hipStream_t s = as_stream(stream);
     launch_init_deltas_3d(w, h, d, mask, delta0, 0, s);
     launch_init_deltas_3d(w, h, d, mask, delta1, 1, s);
-    jfa3d_run(w, h, d, delta0, cw, tmp, s);
-    jfa3d_run(w, h, d, delta1, cw, tmp, s);
+    jfa3d_run(w, h, d, delta0, cw, tmp, s, true);
+    jfa3d_run(w, h, d, delta1, cw, tmp, s, true);
     return finish(s, "two_field_deltas_3d");
 }
 

@@ -854,7 +854,16 @@ static size_t jfa3d_rows_lds(int w)
 }
 
 
-static void launch_jfa3d_rows(bool first, bool out_aos, int step, int w, int h, int d,
+// every flag byte of the first pass set to kRowHasMax, for a start field
+// known to hold only kMaxDelta3 cells and cells within kFastBound
+__global__ void __launch_bounds__(256) k_jfa3d_flags_max(size_t words, uint4* __restrict__ flags)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < words) flags[i] = make_uint4(kRowHasMax * 0x01010101u, kRowHasMax * 0x01010101u,
+                                         kRowHasMax * 0x01010101u, kRowHasMax * 0x01010101u);
+}
+
+static void launch_jfa3d_rows(bool first, bool bounded, bool out_aos, int step, int w, int h, int d,
                               const int16_t* in, int16_t* out, int cw, const uint8_t* flags_in,
                               uint8_t* flags_out, hipStream_t s)
 {
@@ -871,7 +880,14 @@ static void launch_jfa3d_rows(bool first, bool out_aos, int step, int w, int h, 
     const int residues = step < h ? step : h;
     const int runs = ((h + step - 1) / step + kRun - 1) / kRun;
     const dim3 grid_runs((unsigned)((size_t)d * residues * runs));
-    if (first) {
+    if (first && bounded) {
+        // a start field of the library's own init kernels: no cell beyond
+        // kFastBound, so flagging every slot "has a kMaxDelta3 cell" (the
+        // keyed path with the skip, always exact) replaces reading it
+        const size_t words = (size_t)kFlagSlots * h * d / 16;
+        hipLaunchKernelGGL(k_jfa3d_flags_max, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, words,
+                           (uint4*)flags_in);
+    } else if (first) {
         // the caller's field has no flags: classify it first
         if (w % 64 == 0 && ((uintptr_t)in & 15) == 0) {
             hipLaunchKernelGGL(k_jfa3d_classify64, dim3((unsigned)((cells / 8 + 255) / 256)), dim3(256), 0, s,
@@ -918,8 +934,9 @@ size_t jfa3d_workspace(int w, int h, int d)
 // The pass schedule of DistanceOcclusionLib.cpp:2450-2472: steps
 // FloorPow2(max dim - 1) down to 1, ping-ponging between delta and the
 // workspace's field, plus one step-1 pass when the count is odd so the
-// result ends in delta.
-void jfa3d_run(int w, int h, int d, int16_t* delta, int cw, void* work, hipStream_t s)
+// result ends in delta.  bounded: the start field comes from the library's
+// border or unsigned init (every cell kMaxDelta3 or within kFastBound).
+void jfa3d_run(int w, int h, int d, int16_t* delta, int cw, void* work, hipStream_t s, bool bounded)
 {
     int m = w >= h ? w : h;
     m = m >= d ? m : d;
@@ -936,7 +953,7 @@ void jfa3d_run(int w, int h, int d, int16_t* delta, int cw, void* work, hipStrea
     int16_t* b = tmp;
     for (int k = 0; k < n; k++) {
         if (rows)
-            launch_jfa3d_rows(k == 0, k == n - 1, steps[k], w, h, d, a, b, cw, flags[(k + 1) & 1],
+            launch_jfa3d_rows(k == 0, bounded, k == n - 1, steps[k], w, h, d, a, b, cw, flags[(k + 1) & 1],
                               k == n - 1 ? nullptr : flags[k & 1], s);
         else
             launch_jfa3d_pass(steps[k], w, h, d, a, b, cw, s);
