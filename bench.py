@@ -10,7 +10,8 @@ second in Gvoxels/s.
 Also reported on the same line:
   roofline      dominant kernel of the step: algorithmic bytes per launch /
                 its mean duration (HIP events on the launch stream, recorded
-                during the timed steps) against 8 TB/s
+                around each launch in a run of the same steps after the
+                timed ones) against 8 TB/s
   cpu_baseline  the oracle's exact separable EDT (oracle/dol_oracle.c) on
                 the same volume, one thread, on this host (median of 3, CPU
                 model and cores stated); its output is also compared with
