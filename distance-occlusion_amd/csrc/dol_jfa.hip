@@ -169,7 +169,8 @@ __global__ void __launch_bounds__(256) k_jfa3d(int step, int w, int h, int d, in
 //
 // Keyed path.  With |components| <= kFastBound and cw * step <= 606 no
 // candidate wraps and every candidate length is below 2^24, so a tap's
-// outcome is the order of its key 27 * len + rank, where rank is the tap's
+// outcome is the order of its key 27 * len + rank (the scale 27 stands for
+// the key's scale KS below), where rank is the tap's
 // place in the reference's order with the cell itself first (rank 0; the
 // strict ">" keeps the earliest of equal lengths).  The least key wins.  A
 // staged source s on z-row kc holds
@@ -179,10 +180,10 @@ __global__ void __launch_bounds__(256) k_jfa3d(int step, int w, int h, int d, in
 // and the packed constant (54 ic off, 54 jc off):
 //     key = 27 * ((sx + ic*off)^2 + (sy + jc*off)^2 + (sz + kc*off)^2) + rank.
 // A kMaxDelta3 source (which the reference skips) is staged with Bb =
-// kPoisonB and (sx, sy) = 0, so its key is above every other; so are rows
+// poison_b and (sx, sy) = 0, so its key is above every other; so are rows
 // outside the volume and the poison column at index w that taps beyond
 // either end of the row read.  A kMaxDelta3 cell itself has the key
-// kSelfMaxKey (above every real candidate, below every skipped one), or 0
+// self_max_key (above every real candidate, below every skipped one), or 0
 // where its own 32-bit length wraps negative (then nothing replaces it, as
 // in the reference).  The winner is decoded from the key's rank and rebuilt
 // from its source.
@@ -190,10 +191,17 @@ constexpr int kRowMax = 1024;
 constexpr int kFlagSlots = 16;        // flag bytes a row: one per wave of up to kRowMax cells
 constexpr int kFastBound = 2048;
 constexpr uint32_t kRowUnbounded = 1, kRowHasMax = 2;
-constexpr int kKeyScale = 27;
-constexpr uint32_t kPoisonB = 2700000000u;          // 27 * 1e8
-constexpr uint32_t kSelfMaxKey = 27u * 67108864u;   // 27 * 2^26
-constexpr int kFastMaxOff = 606;                    // 54 * 606 <= 32767
+// The key's scale KS is 27 (ranks 0..26 fill the residues) or 32 (the rank
+// is the key's low 5 bits, one AND to decode) where the cross-term constant
+// 2 * 32 * off still fits 16 bits.
+template <int KS> constexpr uint32_t poison_b() { return (uint32_t)KS * 100000000u; }   // above every real key
+template <int KS> constexpr uint32_t self_max_key() { return (uint32_t)KS * 67108864u; }  // KS * 2^26
+template <int KS> constexpr int fast_max_off() { return 32767 / (2 * KS); }              // 606 for 27, 511 for 32
+template <int KS> __device__ __forceinline__ uint32_t key_rank(uint32_t key)
+{
+    return KS == 32 ? key & 31u : key % (uint32_t)KS;
+}
+static_assert(poison_b<32>() + 32u * 2u * 511u * 511u + 26u < 0xffffffffu, "poisoned keys stay in 32 bits");
 
 typedef unsigned short ushort2v __attribute__((ext_vector_type(2)));
 typedef short short2v __attribute__((ext_vector_type(2)));
@@ -377,13 +385,13 @@ constexpr int jfa3d_lanes() { return W == 512 ? 256 : kRowMax / 2; }
 // the cell is read with, so it is formed once per run; a tap adds
 //     K = 27 * ((ic*off)^2 + (jc*off)^2) + rank
 // and the cross term (54 ic off, 54 jc off) . (sx, sy) with one v_dot2.  A
-// kMaxDelta3 cell is staged with (x, y) = 0 and Bb = kPoisonB.  The whole
+// kMaxDelta3 cell is staged with (x, y) = 0 and Bb = poison_b.  The whole
 // run takes this path when the flags of all its source rows allow it;
 // otherwise the workgroup runs its rows through jfa3d_row's general path.
 // With IN_AOS the cells are read from the caller's interleaved field (the
 // flags then come from k_jfa3d_classify).  The loop is unrolled by the
 // ring's three phases so every LDS offset is an immediate.
-template <bool IN_AOS, bool OUT_AOS, int W>
+template <bool IN_AOS, bool OUT_AOS, int W, int KS>
 __global__ void __launch_bounds__(jfa3d_lanes<W>()) __attribute__((amdgpu_waves_per_eu(W == 512 ? 4 : 2)))
 k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __restrict__ in,
              int16_t* __restrict__ out, int cw, const uint8_t* __restrict__ flags_in,
@@ -481,7 +489,7 @@ k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __r
             if (zv[k]) f |= row_flags(flags_in, (z + (k - 1) * step) * h + y0 + g * step, nw);
     }
     const uint32_t fl = fold_flags(f);
-    if ((fl & kRowUnbounded) || off > kFastMaxOff) {
+    if ((fl & kRowUnbounded) || off > fast_max_off<KS>()) {
         // general path, one row at a time
         for (int i = 0; i < n; i++) {
             const int y = y0 + i * step;
@@ -529,7 +537,7 @@ k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __r
         const int src = (kc * step * h + jc * step) * w + ic * step;
         tbl[e] = make_uint4((uint32_t)(slot * stride + ic * step), dxy, (uint32_t)(kc * off), (uint32_t)src);
     }
-    if (t < 9) pb[t * stride + w] = u32x2{0u, kPoisonB};   // the poison column
+    if (t < 9) pb[t * stride + w] = u32x2{0u, poison_b<KS>()};   // the poison column
     // stage this lane's cells of y-row g of every z-row into ring slot g mod 3;
     // a row outside the volume is staged as poison, so the taps need no test
     auto stage = [&](int g, uint32_t (&vxy)[3][COLS], int (&vz)[3][COLS]) {
@@ -548,7 +556,7 @@ k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __r
             if (!zv[k] || !gy) {
 #pragma unroll
                 for (int c = 0; c < COLS; c++)
-                    if (live[c]) pb[(k * 3 + slot) * stride + xc[c]] = u32x2{0u, kPoisonB};
+                    if (live[c]) pb[(k * 3 + slot) * stride + xc[c]] = u32x2{0u, poison_b<KS>()};
                 continue;
             }
             const int kc = k - 1;
@@ -557,12 +565,12 @@ k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __r
                 const int czr = (int16_t)vz[k][c] + kc * off;
                 const int len = __builtin_amdgcn_sdot2(__builtin_bit_cast(short2v, vxy[k][c]),
                                                        __builtin_bit_cast(short2v, vxy[k][c]), __mul24(czr, czr), false);
-                uint32_t B = __umul24((uint32_t)len, kKeyScale);
+                uint32_t B = __umul24((uint32_t)len, (uint32_t)KS);
                 uint32_t X = vxy[k][c];
                 if (any_max) {
                     const bool m = (int16_t)X == kMaxDelta3;
                     X = m ? 0u : X;
-                    B = m ? kPoisonB : B;
+                    B = m ? poison_b<KS>() : B;
                 }
                 if (live[c]) pb[(k * 3 + slot) * stride + xc[c]] = u32x2{X, B};
             }
@@ -593,9 +601,9 @@ k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __r
         }
     }
     // the uniform per-tap constants
-    const uint32_t c54 = (uint32_t)__builtin_amdgcn_readfirstlane(2 * kKeyScale * off) & 0xffffu;
+    const uint32_t c54 = (uint32_t)__builtin_amdgcn_readfirstlane(2 * KS * off) & 0xffffu;
     const uint32_t c54n = (0x10000u - c54) & 0xffffu;
-    const uint32_t k1 = (uint32_t)__builtin_amdgcn_readfirstlane(kKeyScale * off * off);
+    const uint32_t k1 = (uint32_t)__builtin_amdgcn_readfirstlane(KS * off * off);
     // the next y-row's cells, in flight during a row
     uint32_t nxy[3][COLS] = {};
     int nz[3][COLS] = {};
@@ -684,7 +692,7 @@ k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __r
             {
                 const int sx = (int16_t)self_xy[0][c];
                 const int l = len2_packed(self_xy[0][c], self_z[0][c]);
-                best = sx == kMaxDelta3 ? (l < 0 ? 0u : kSelfMaxKey) : __umul24((uint32_t)l, kKeyScale);
+                best = sx == kMaxDelta3 ? (l < 0 ? 0u : self_max_key<KS>()) : __umul24((uint32_t)l, (uint32_t)KS);
             }
             const int colm = x - step >= 0 ? x - step : w;
             const int colp = x + step < w ? x + step : w;
@@ -708,7 +716,7 @@ k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __r
                     const int r = 3 * k + j;
                     const int slot = k * 3 + (p + j - 1 + 3) % 3;
                     nb[j][0] = bm[slot * stride];
-                    nb[j][1] = r != 4 ? b0[slot * stride] : u32x2{0u, kPoisonB};
+                    nb[j][1] = r != 4 ? b0[slot * stride] : u32x2{0u, poison_b<KS>()};
                     nb[j][2] = bp[slot * stride];
                 }
 #pragma unroll
@@ -724,21 +732,26 @@ k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __r
             }
             // the winner, rebuilt from its source (rank 0: the cell itself,
             // whose staged copy may have been cleared as a skipped source)
-            const uint32_t rank = best % (uint32_t)kKeyScale;
+            const uint32_t rank = key_rank<KS>(best);
             const uint4 e = tbl[p * 27 + rank];
             const int idx = x + (int)e.x;
+            // (read for every lane, rank 0 included, and then selected: a
+            // select rather than a branch around the read)
             ushort2v v = __builtin_bit_cast(ushort2v, pb[idx].x);
             v += __builtin_bit_cast(ushort2v, e.y);
             pxy[c] = rank ? __builtin_bit_cast(uint32_t, v) : self_xy[0][c];
             // the winner's z from its source cell (in range: a winner is never
             // a tap outside the volume); for rank 0 the entry's offsets are 0,
-            // so this is the cell's own z
-            // (the 32-bit word holding it is loaded, so that nothing needs
-            // the value before the next row's flush)
-            const size_t si = ((size_t)z * h + y0 + i * step) * w + x + (int)e.w;
-            const size_t zi = IN_AOS ? 3 * si + 2 : 2 * cells + si;
-            pz[c] = (int)((const uint32_t*)in)[zi >> 1];
-            pdz[c] = (int)e.z * 2 + (int)(zi & 1);   // the offset, and which half
+            // so this is the cell's own z.  The 32-bit word holding it is
+            // loaded, so that nothing needs the value before the next row's
+            // flush; the row's part of the 16-bit index, U, is even (w is),
+            // so the word is U / 2 + (vz >> 1) and the half vz & 1.
+            const size_t rowoff = ((size_t)z * h + y0 + i * step) * w;
+            const int vs = x + (int)e.w;
+            const int vz = IN_AOS ? 3 * vs : vs;
+            const size_t U = IN_AOS ? 3 * rowoff + 2 : 2 * cells + rowoff;
+            pz[c] = (int)((const uint32_t*)in + U / 2)[vz >> 1];
+            pdz[c] = (int)e.z * 2 + (vz & 1);   // the offset, and which half
         }
         __syncthreads();
     };
@@ -899,18 +912,20 @@ static void launch_jfa3d_rows(bool first, bool bounded, bool out_aos, int step, 
                                in, (uint8_t*)flags_in);
         }
     }
-#define DOL_JFA3D_ROWS(W)                                                                            \
-    {                                                                                                \
-        auto fn = first ? (out_aos ? k_jfa3d_ring<true, true, W> : k_jfa3d_ring<true, false, W>)     \
-                        : (out_aos ? k_jfa3d_ring<false, true, W> : k_jfa3d_ring<false, false, W>);  \
-        want_lds((const void*)fn);                                                                   \
-        hipLaunchKernelGGL(fn, grid_runs, dim3(threads), lds, s, step, w, h, d, cells, in, out, cw,  \
-                           flags_in, flags_out, residues);                                           \
+#define DOL_JFA3D_ROWS(W, KS)                                                                                \
+    {                                                                                                        \
+        auto fn = first ? (out_aos ? k_jfa3d_ring<true, true, W, KS> : k_jfa3d_ring<true, false, W, KS>)     \
+                        : (out_aos ? k_jfa3d_ring<false, true, W, KS> : k_jfa3d_ring<false, false, W, KS>);  \
+        want_lds((const void*)fn);                                                                           \
+        hipLaunchKernelGGL(fn, grid_runs, dim3(threads), lds, s, step, w, h, d, cells, in, out, cw,          \
+                           flags_in, flags_out, residues);                                                   \
     }
+    // the key's scale: 32 where the cross term allows it
+    const bool k32 = (long long)cw * step <= fast_max_off<32>();
     if (w == 512) {
-        DOL_JFA3D_ROWS(512)
+        if (k32) DOL_JFA3D_ROWS(512, 32) else DOL_JFA3D_ROWS(512, 27)
     } else {
-        DOL_JFA3D_ROWS(0)
+        if (k32) DOL_JFA3D_ROWS(0, 32) else DOL_JFA3D_ROWS(0, 27)
     }
 #undef DOL_JFA3D_ROWS
 }
