@@ -373,24 +373,25 @@ constexpr int jfa3d_lanes() { return W == 512 ? 256 : kRowMax / 2; }
 
 // The 3D passes on runs of rows with the source rows in an LDS ring.
 //
-// Workgroup b takes a run of up to `run` output rows (y0 + i*step, z),
-// i = 0, 1, ..., of one residue y0 mod step, as k_jfa3d_runs.  Row i reads
+// Workgroup b takes a run of up to kRun output rows (y0 + i*step, z),
+// i = 0, 1, ..., of one residue y0 mod step.  Row i reads
 // source y-rows y0 + (i-1) step, y0 + i step and y0 + (i+1) step (each at
 // z - step, z, z + step), so consecutive rows share two of their three
 // y-rows.  The ring holds 3 y-rows x 3 z-rows of staged cells: row i + 1's
 // new y-row replaces row i - 2's in slot (i + 1) mod 3.  A staged cell
 // holds its raw (x, y) and
 //     Bb = 27 * (sx^2 + sy^2 + (sz + kc*off)^2)
-// which, unlike the per-row B of jfa3d_row, does not depend on the y offset
-// the cell is read with, so it is formed once per run; a tap adds
+// (27 standing for the key's scale KS) which does not depend on the y
+// offset the cell is read with, so it is formed once per run; a tap adds
 //     K = 27 * ((ic*off)^2 + (jc*off)^2) + rank
 // and the cross term (54 ic off, 54 jc off) . (sx, sy) with one v_dot2.  A
 // kMaxDelta3 cell is staged with (x, y) = 0 and Bb = poison_b.  The whole
 // run takes this path when the flags of all its source rows allow it;
-// otherwise the workgroup runs its rows through jfa3d_row's general path.
-// With IN_AOS the cells are read from the caller's interleaved field (the
-// flags then come from k_jfa3d_classify).  The loop is unrolled by the
-// ring's three phases so every LDS offset is an immediate.
+// otherwise the workgroup runs its rows through jfa3d_row_general.  With
+// IN_AOS the cells are read from the caller's interleaved field (the flags
+// then come from k_jfa3d_classify, or are all kRowHasMax after the library's
+// own init).  The rows are unrolled, so every LDS offset of the ring's three
+// phases is an immediate.
 template <bool IN_AOS, bool OUT_AOS, int W, int KS>
 __global__ void __launch_bounds__(jfa3d_lanes<W>()) __attribute__((amdgpu_waves_per_eu(W == 512 ? 4 : 2)))
 k_jfa3d_ring(int step, int w_arg, int h, int d, size_t cells, const int16_t* __restrict__ in,
