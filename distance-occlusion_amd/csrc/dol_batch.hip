@@ -105,6 +105,16 @@ __host__ __device__ inline uint32_t group_word(int slot, int w, int h, int passe
            (uint32_t)l << 28;
 }
 
+// The packed squared length as len2_packed, through the compiler's dot
+// product (v_dot2c with a zeroed accumulator, which the compiler schedules
+// freely) rather than the inline-asm three-operand form: measured faster in
+// these LDS-bound glyph kernels, slower in the streaming 2D pass.
+__device__ __forceinline__ int len2_packed_b(uint32_t xy)
+{
+    const jfa_i16x2 v = __builtin_bit_cast(jfa_i16x2, xy);
+    return __builtin_amdgcn_sdot2(v, v, 0, false);
+}
+
 // One JumpFlood pass for the cell at c, (x, y) of a w x h mask, in half
 // cells (DistanceOcclusionLib.cpp:782-873, the taps of jfa2_cell in its
 // order), without branches: a tap outside the mask reads the cell itself and
@@ -115,7 +125,7 @@ __device__ __forceinline__ void jfa2_tap_if(uint32_t& best, int& bl, uint32_t v,
     jfa_u16x2 c = __builtin_bit_cast(jfa_u16x2, v);
     c += jfa_u16x2{(unsigned short)ox, (unsigned short)oy};
     const uint32_t cv = __builtin_bit_cast(uint32_t, c);
-    const int d = len2_packed(cv);
+    const int d = len2_packed_b(cv);
     const int cl = ok ? d : 0x7fffffff;
     best = bl > cl ? cv : best;
     bl = min(bl, cl);
@@ -130,7 +140,7 @@ __device__ __forceinline__ uint32_t jfa2_taps(Load ld, int w, int h, int x, int 
     const bool l = x - step >= 0, r = x + step < w, u = y - step >= 0, d = y + step < h;
     const int dl = l ? -step : 0, dr = r ? step : 0, du = u ? -sw : 0, dd = d ? sw : 0;
     uint32_t best = ld(0);
-    int bl = len2_packed(best);
+    int bl = len2_packed_b(best);
     jfa2_tap_if(best, bl, ld(dl), -off, 0, l);
     jfa2_tap_if(best, bl, ld(du + dl), -off, -off, l && u);
     jfa2_tap_if(best, bl, ld(dd + dl), -off, +off, l && d);
