@@ -105,14 +105,20 @@ __host__ __device__ inline uint32_t group_word(int slot, int w, int h, int passe
            (uint32_t)l << 28;
 }
 
-// The packed squared length as len2_packed, through the compiler's dot
-// product (v_dot2c with a zeroed accumulator, which the compiler schedules
-// freely) rather than the inline-asm three-operand form: measured faster in
-// these LDS-bound glyph kernels, slower in the streaming 2D pass.
+// The packed squared length as len2_packed; with BUILTIN through the
+// compiler's dot product (v_dot2c with a zeroed accumulator, which the
+// compiler schedules freely) rather than the inline-asm three-operand form:
+// measured faster in the large-mask kernels (taps from device memory),
+// slower in the bins (taps from LDS) and in the streaming 2D pass.
+template <bool BUILTIN>
 __device__ __forceinline__ int len2_packed_b(uint32_t xy)
 {
-    const jfa_i16x2 v = __builtin_bit_cast(jfa_i16x2, xy);
-    return __builtin_amdgcn_sdot2(v, v, 0, false);
+    if constexpr (BUILTIN) {
+        const jfa_i16x2 v = __builtin_bit_cast(jfa_i16x2, xy);
+        return __builtin_amdgcn_sdot2(v, v, 0, false);
+    } else {
+        return len2_packed(xy);
+    }
 }
 
 // One JumpFlood pass for the cell at c, (x, y) of a w x h mask, in half
@@ -120,19 +126,20 @@ __device__ __forceinline__ int len2_packed_b(uint32_t xy)
 // order), without branches: a tap outside the mask reads the cell itself and
 // its length is replaced by INT_MAX, which never passes the strict test.
 // best takes a candidate iff bl > cl, and then bl = cl, so bl = min(bl, cl).
+template <bool BUILTIN>
 __device__ __forceinline__ void jfa2_tap_if(uint32_t& best, int& bl, uint32_t v, int ox, int oy, bool ok)
 {
     jfa_u16x2 c = __builtin_bit_cast(jfa_u16x2, v);
     c += jfa_u16x2{(unsigned short)ox, (unsigned short)oy};
     const uint32_t cv = __builtin_bit_cast(uint32_t, c);
-    const int d = len2_packed_b(cv);
+    const int d = len2_packed_b<BUILTIN>(cv);
     const int cl = ok ? d : 0x7fffffff;
     best = bl > cl ? cv : best;
     bl = min(bl, cl);
 }
 
 // ld(o) reads the cell o cells after this one (o = 0: the cell itself)
-template <class Load>
+template <bool BUILTIN, class Load>
 __device__ __forceinline__ uint32_t jfa2_taps(Load ld, int w, int h, int x, int y, int step)
 {
     const int off = 2 * step;
@@ -140,21 +147,21 @@ __device__ __forceinline__ uint32_t jfa2_taps(Load ld, int w, int h, int x, int 
     const bool l = x - step >= 0, r = x + step < w, u = y - step >= 0, d = y + step < h;
     const int dl = l ? -step : 0, dr = r ? step : 0, du = u ? -sw : 0, dd = d ? sw : 0;
     uint32_t best = ld(0);
-    int bl = len2_packed_b(best);
-    jfa2_tap_if(best, bl, ld(dl), -off, 0, l);
-    jfa2_tap_if(best, bl, ld(du + dl), -off, -off, l && u);
-    jfa2_tap_if(best, bl, ld(dd + dl), -off, +off, l && d);
-    jfa2_tap_if(best, bl, ld(dr), +off, 0, r);
-    jfa2_tap_if(best, bl, ld(du + dr), +off, -off, r && u);
-    jfa2_tap_if(best, bl, ld(dd + dr), +off, +off, r && d);
-    jfa2_tap_if(best, bl, ld(du), 0, -off, u);
-    jfa2_tap_if(best, bl, ld(dd), 0, +off, d);
+    int bl = len2_packed_b<BUILTIN>(best);
+    jfa2_tap_if<BUILTIN>(best, bl, ld(dl), -off, 0, l);
+    jfa2_tap_if<BUILTIN>(best, bl, ld(du + dl), -off, -off, l && u);
+    jfa2_tap_if<BUILTIN>(best, bl, ld(dd + dl), -off, +off, l && d);
+    jfa2_tap_if<BUILTIN>(best, bl, ld(dr), +off, 0, r);
+    jfa2_tap_if<BUILTIN>(best, bl, ld(du + dr), +off, -off, r && u);
+    jfa2_tap_if<BUILTIN>(best, bl, ld(dd + dr), +off, +off, r && d);
+    jfa2_tap_if<BUILTIN>(best, bl, ld(du), 0, -off, u);
+    jfa2_tap_if<BUILTIN>(best, bl, ld(dd), 0, +off, d);
     return best;
 }
 
 __device__ __forceinline__ uint32_t jfa2_cell_at(const uint32_t* c, int w, int h, int x, int y, int step)
 {
-    return jfa2_taps([&](int o) { return c[o]; }, w, h, x, y, step);
+    return jfa2_taps<false>([&](int o) { return c[o]; }, w, h, x, y, step);
 }
 
 // v, made opaque to the compiler: values derived from it are recomputed
@@ -377,7 +384,7 @@ __global__ void __launch_bounds__(kBigThreads) k_big_pass(int step, const BigTil
         // past the mask's end a lane computes its last cell again, unstored
         const int i = min(t.cell0 + (int)threadIdx.x + c * kBigThreads, area - 1);
         const int y = div_by(i, m.magic), x = i - y * m.w;
-        v[c] = jfa2_taps([&](int o) { return __builtin_amdgcn_raw_buffer_load_b32(rs, (i + o) * 4, 0, 0); }, m.w, m.h,
+        v[c] = jfa2_taps<true>([&](int o) { return __builtin_amdgcn_raw_buffer_load_b32(rs, (i + o) * 4, 0, 0); }, m.w, m.h,
                          x, y, step);
     }
 #pragma unroll
@@ -404,7 +411,7 @@ __global__ void __launch_bounds__(kBigThreads) k_big_last(const BigTile* __restr
     for (int c = 0; c < kBigCells; c++) {
         const int i = min(t.cell0 + (int)threadIdx.x + c * kBigThreads, area - 1);
         const int y = div_by(i, m.magic), x = i - y * m.w;
-        v[c] = jfa2_taps([&](int o) { return __builtin_amdgcn_raw_buffer_load_b32(rs, (i + o) * 4, 0, 0); }, m.w, m.h,
+        v[c] = jfa2_taps<true>([&](int o) { return __builtin_amdgcn_raw_buffer_load_b32(rs, (i + o) * 4, 0, 0); }, m.w, m.h,
                          x, y, 1);
     }
 #pragma unroll
