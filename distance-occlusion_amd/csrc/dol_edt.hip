@@ -23,14 +23,15 @@ namespace dolhip {
 // comparisons, so a line is one sequential chain), then the outputs, each
 // taking the first envelope entry k with z[k + 1] >= q.  Here the two loops
 // are two kernels:
-//  * k_felz_envelope: one lane a line, the same float operations in the same
-//    order.  The stack's top entries live in an LDS ring of kRing entries a
-//    line ((v, z) and the key F = f[v] + v*v the reference recomputes on
-//    each pop, the same float); every push is also written to the line's
-//    stack in device memory, which a pop deeper than the ring reads.  The
-//    line's samples are loaded kFelzAhead steps ahead.  A workgroup is one
-//    wavefront with LPW lanes (lines) in use, so the lines spread over the
-//    SIMDs and a wavefront's lanes wait for each other's pops less.
+//  * k_felz_envelope: one lane a line, 64 lines a wavefront, the same float
+//    operations in the same order.  The stack's top entries live in an LDS
+//    ring ((v, z) and the key F = f[v] + v*v the reference recomputes on
+//    each pop, the same float), the top two also in registers, so that the
+//    first pop of a step is a select beside the test; the stack in device
+//    memory, read by pops deeper than the ring, is brought up to date from
+//    the ring once a block of steps.  A block after one in which no line
+//    popped first takes its tests as if none pops, and if so pushes with
+//    no branch.  The line's samples are loaded kFelzAhead steps ahead.
 //  * k_felz_output: one workgroup a line, one output a thread.  z[1..D] rises
 //    strictly (a push follows a failed "s <= z[k]"), so the entry the
 //    reference's running scan reaches for q is the number of z[1..D] below
@@ -44,14 +45,29 @@ namespace dolhip {
 constexpr int kRing = 64;        // stack entries a line keeps in LDS (half with 64 lines)
 constexpr int kFelzAhead = 16;   // samples a line has in flight
 
+// A line's stacks in device memory: V[l * felz_vs(n) + k], Z[l * felz_zs(n) + k],
+// each line's rows 16-byte aligned, with 4 entries past the deepest one a
+// copy of the ring can write
+__host__ __device__ inline int felz_vs(int n) { return ((n + 3) & ~3) + 4; }
+__host__ __device__ inline int felz_zs(int n) { return ((n + 4) & ~3) + 4; }
+
 template <int LPW>
 __global__ void __launch_bounds__(LPW) k_felz_envelope(int n, int lines, const float* __restrict__ f2, size_t m,
                                                        int* __restrict__ V, float* __restrict__ Z,
                                                        int* __restrict__ D)
 {
     // the ring: entry (v, z, F) of depth k at slot k mod R, one 16-byte
-    // record a lane (one LDS write or read a push or pop)
+    // record a lane (one LDS write or read a push or pop); it holds depths
+    // lo .. k (a push below lo lowers lo).  Pushes go to the ring only;
+    // after every block of kFelzAhead steps each line copies the entries it
+    // pushed in the block that are still on the stack (depths head .. k) to
+    // its stack in device memory, mostly with 16-byte stores.  Depths below
+    // head did not change in the block, so after the copy device memory
+    // holds the whole stack.  An entry a pop reads from device memory (a
+    // depth below lo) was pushed in an earlier block: overwriting its slot
+    // within its own block would take R > kFelzAhead more pushes.
     constexpr int R = LPW >= 64 ? kRing / 2 : kRing;
+    static_assert(R >= 2 * kFelzAhead, "a block's pushes stay in the ring until it is copied");
     __shared__ uint4 ring[R][LPW];
     const int t = threadIdx.x;
     const int l = blockIdx.x * LPW + t;
@@ -60,23 +76,79 @@ __global__ void __launch_bounds__(LPW) k_felz_envelope(int n, int lines, const f
     // the stacks through descriptors based at the workgroup's first line,
     // with 32-bit lane offsets (below 2^32 for lines of up to 2^24 samples)
     const size_t l0 = (size_t)blockIdx.x * LPW;
-    const auto rV = __builtin_amdgcn_make_buffer_rsrc((void*)(V + l0 * n), 0, 0xffffffffu, kRsrcWord3);
-    const auto rZ = __builtin_amdgcn_make_buffer_rsrc((void*)(Z + l0 * (n + 1)), 0, 0xffffffffu, kRsrcWord3);
-    const uint32_t oV = (uint32_t)t * (uint32_t)n * 4u, oZ = (uint32_t)t * (uint32_t)(n + 1) * 4u;
+    const int vs = felz_vs(n), zs = felz_zs(n);
+    const auto rV = __builtin_amdgcn_make_buffer_rsrc((void*)(V + l0 * vs), 0, 0xffffffffu, kRsrcWord3);
+    const auto rZ = __builtin_amdgcn_make_buffer_rsrc((void*)(Z + l0 * zs), 0, 0xffffffffu, kRsrcWord3);
+    const uint32_t oV = (uint32_t)t * (uint32_t)vs * 4u, oZ = (uint32_t)t * (uint32_t)zs * 4u;
     int k = 0, lo = 0;   // stack depth; the lowest depth the ring still holds
-    int vk = 0;
-    float zk = -FLT_MAX;
-    float Fk = __fadd_rn(f[0], 0.0f);
+    // the top entry (depth k) and the one below it (depth k - 1, when k > 0)
+    // in registers: the test after popping the top is computed beside the
+    // test against it, and the first pop is a select, not a branch
+    int vk = 0, v1 = 0;
+    float zk = -FLT_MAX, z1 = -FLT_MAX;
+    float Fk = __fadd_rn(f[0], 0.0f), F1 = Fk;
     auto put = [&](int d, int v, float z, float F) __attribute__((always_inline)) {
         ring[d & (R - 1)][t] = make_uint4((uint32_t)v, __float_as_uint(z), __float_as_uint(F), 0u);
-        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)v, rV, oV + 4u * (uint32_t)d, 0, 0);
-        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(z), rZ, oZ + 4u * (uint32_t)d, 0, 0);
+    };
+    // the ring's entries of depths head .. top (head >= top - kFelzAhead + 1,
+    // all at or above lo) to the stacks in device memory: the first quad's
+    // entries one at a time, then whole 16-byte aligned quads (at most 4),
+    // whose depths above the top carry stale values (depths above the top
+    // are not on the stack; the highest, top + 3, is below felz_vs(n)).
+    // With last, depth top + 1 of Z gets the terminator +FLT_MAX.
+    auto copy = [&](int top, int head, bool last) __attribute__((always_inline)) {
+        const int hq = head & ~3;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int d = hq + c;
+            if (d >= head && d <= top) {
+                const uint4 e = ring[d & (R - 1)][t];
+                __builtin_amdgcn_raw_buffer_store_b32(e.x, rV, oV + 4u * (uint32_t)d, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b32(e.y, rZ, oZ + 4u * (uint32_t)d, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int i = 1; i <= 4; i++) {
+            const int d = hq + 4 * i;
+            if (d <= top) {
+                typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+                u32x4 vv, zz;
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const uint4 e = ring[(d + c) & (R - 1)][t];
+                    vv[c] = e.x;
+                    zz[c] = last && d + c == top + 1 ? __float_as_uint(+FLT_MAX) : e.y;
+                }
+                __builtin_amdgcn_raw_buffer_store_b128(vv, rV, oV + 4u * (uint32_t)d, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b128(zz, rZ, oZ + 4u * (uint32_t)d, 0, 0);
+            }
+        }
+        // the terminator, unless a quad above wrote it
+        if (last && !(top + 1 >= hq + 4 && ((top + 1) & 3) != 0))
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(+FLT_MAX), rZ, oZ + 4u * (uint32_t)(top + 1), 0, 0);
     };
     put(0, 0, -FLT_MAX, Fk);
+    __builtin_amdgcn_raw_buffer_store_b32(0u, rV, oV, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(-FLT_MAX), rZ, oZ, 0, 0);
+    int head = 1;   // the lowest depth pushed since the last copy
     bool nan = false;
+    bool popped = false;   // this line popped in the current block
     auto step = [&](int q, float fq) __attribute__((always_inline)) {
         const float fqt = __fadd_rn(fq, (float)(q * q));
         float s = __fdiv_rn(__fsub_rn(fqt, Fk), (float)(2 * (q - vk)));
+        // the same test against the entry below (v1 < q: no zero divisor;
+        // used only if the top is popped), computed beside the first: the
+        // first pop is a select, not a branch
+        const float s1 = __fdiv_rn(__fsub_rn(fqt, F1), (float)(2 * (q - v1)));
+        const bool pop = s <= zk && k > 0;
+        popped |= pop;
+        k -= pop;
+        s = pop ? s1 : s;
+        vk = pop ? v1 : vk;
+        zk = pop ? z1 : zk;
+        Fk = pop ? F1 : Fk;
+        // further pops read the stack's next entry from the ring, or, for a
+        // depth the ring no longer holds, from the stack in device memory
         while (s <= zk && k > 0) {
             k--;
             if (k >= lo) {
@@ -96,8 +168,12 @@ __global__ void __launch_bounds__(LPW) k_felz_envelope(int n, int lines, const f
         }
         k++;
         put(k, q, s, fqt);
-        lo = max(lo, k - R + 1);
+        head = min(head, k);
+        lo = max(min(lo, k), k - R + 1);
         nan |= s != s;
+        v1 = vk;
+        z1 = zk;
+        F1 = Fk;
         vk = q;
         zk = s;
         Fk = fqt;
@@ -109,18 +185,60 @@ __global__ void __launch_bounds__(LPW) k_felz_envelope(int n, int lines, const f
 #pragma unroll
     for (int j = 0; j < kFelzAhead; j++) cur[j] = sample(1 + j);
     int q0 = 1;
+    // A block of kFelzAhead steps after a block in which no line popped
+    // first takes the tests as if none pops: the top is then always sample
+    // q - 1 (a step that does not pop pushes q), so each test is the
+    // reference's with the divisor 2 * (q - (q - 1)) = 2.  If no line of
+    // the wavefront pops in the block, those are its steps, and the pushes
+    // follow with no branch; otherwise the block runs step by step.
+    bool hot = true;
     for (; q0 + kFelzAhead <= n; q0 += kFelzAhead) {
 #pragma unroll
         for (int j = 0; j < kFelzAhead; j++) nxt[j] = sample(q0 + kFelzAhead + j);
+        bool done = false;
+        if (!hot) {
+            float ft[kFelzAhead], sj[kFelzAhead];
+            bool any = false;
 #pragma unroll
-        for (int j = 0; j < kFelzAhead; j++) step(q0 + j, cur[j]);
+            for (int j = 0; j < kFelzAhead; j++) {
+                const int q = q0 + j;
+                ft[j] = __fadd_rn(cur[j], (float)(q * q));
+                sj[j] = __fdiv_rn(__fsub_rn(ft[j], j ? ft[j - 1] : Fk), 2.0f);
+                any |= sj[j] <= (j ? sj[j - 1] : zk) && (j || k > 0);
+            }
+            if (!__builtin_amdgcn_ballot_w64(any)) {
+#pragma unroll
+                for (int j = 0; j < kFelzAhead; j++) {
+                    put(k + 1 + j, q0 + j, sj[j], ft[j]);
+                    nan |= sj[j] != sj[j];
+                }
+                head = min(head, k + 1);
+                lo = max(min(lo, k + 1), k + kFelzAhead - R + 1);
+                k += kFelzAhead;
+                v1 = q0 + kFelzAhead - 2;
+                z1 = sj[kFelzAhead - 2];
+                F1 = ft[kFelzAhead - 2];
+                vk = q0 + kFelzAhead - 1;
+                zk = sj[kFelzAhead - 1];
+                Fk = ft[kFelzAhead - 1];
+                done = true;
+            }
+        }
+        if (!done) {
+            popped = false;
+#pragma unroll
+            for (int j = 0; j < kFelzAhead; j++) step(q0 + j, cur[j]);
+            hot = __builtin_amdgcn_ballot_w64(popped) != 0;
+        }
+        copy(k, head, false);
+        head = k + 1;
 #pragma unroll
         for (int j = 0; j < kFelzAhead; j++) cur[j] = nxt[j];
     }
 #pragma unroll
     for (int j = 0; j < kFelzAhead; j++)
         if (q0 + j < n) step(q0 + j, cur[j]);
-    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(+FLT_MAX), rZ, oZ + 4u * (uint32_t)(k + 1), 0, 0);
+    copy(k, head, true);
     D[l] = nan ? -1 - k : k;
 }
 
@@ -134,10 +252,13 @@ __global__ void __launch_bounds__(kFelzOutThreads) k_felz_output(int n, const fl
                                                                  float* __restrict__ out)
 {
     extern __shared__ float zs[];
-    const int l = blockIdx.x;
+    // workgroups are dealt to the 8 XCDs in turn: each XCD takes a run of
+    // consecutive lines, whose sample gathers f[m * v] share cache lines
+    int l = blockIdx.x;
+    if ((gridDim.x & 7) == 0) l = (l & 7) * (gridDim.x >> 3) + (l >> 3);
     const float* f = f2 + l;
-    const int* Vl = V + (size_t)l * n;
-    const float* Zl = Z + (size_t)l * (n + 1);
+    const int* Vl = V + (size_t)l * felz_vs(n);
+    const float* Zl = Z + (size_t)l * felz_zs(n);
     float* o = out + (size_t)l * n;
     const int dd = D[l];
     if (dd < 0) {
@@ -170,35 +291,34 @@ __global__ void __launch_bounds__(kFelzOutThreads) k_felz_output(int n, const fl
     }
 }
 
-// the transposed field td (w*h floats), the stacks V (w*h ints) and Z
-// (w*h + max(w, h) floats), the depths D (max(w, h) ints)
+// the transposed field td (w*h floats), the stacks V and Z (felz_vs and
+// felz_zs entries a line, for the larger of the two passes), the depths D
+// (max(w, h) ints)
 static inline size_t felz_part(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static inline size_t felz_stack(int w, int h, int (*stride)(int))
+{
+    const size_t cols = (size_t)w * stride(h), rows = (size_t)h * stride(w);
+    return felz_part((cols > rows ? cols : rows) * 4);
+}
 
 size_t felz_workspace(int w, int h)
 {
     const size_t cells = (size_t)w * h, lines = (size_t)(w > h ? w : h);
-    return felz_part(cells * 4) * 2 + felz_part((cells + lines) * 4) + felz_part(lines * 4);
+    return felz_part(cells * 4) + felz_stack(w, h, felz_vs) + felz_stack(w, h, felz_zs) + felz_part(lines * 4);
 }
 
 // one 1D pass over `lines` lines of n samples
 static void felz_pass(int n, int lines, const float* f2, size_t m, float* out, int* V, float* Z, int* D,
                       hipStream_t s)
 {
-    // wavefronts of LPW lines: about one a SIMD (1024 on the chip) or more.
-    // At least two lanes, so that the line index is per lane and every read
-    // of the stack in device memory (written by this kernel) is a vector load.
-#define DOL_FELZ_ENV(LPW)                                                                              \
-    hipLaunchKernelGGL(k_felz_envelope<LPW>, dim3(grid1d((size_t)lines, LPW)), dim3(LPW), 0, s, n, lines, \
-                       f2, m, V, Z, D)
-    if (lines >= 64 * 1024)
-        DOL_FELZ_ENV(64);
-    else if (lines >= 16 * 1024)
-        DOL_FELZ_ENV(16);
-    else if (lines >= 4 * 1024)
-        DOL_FELZ_ENV(4);
-    else
-        DOL_FELZ_ENV(2);
-#undef DOL_FELZ_ENV
+    // 64 lines a wavefront: the lines' samples at each step are one
+    // contiguous read, and each line is one chain of dependent steps whose
+    // length does not shrink with fewer lines a wavefront (measured 2, 4, 8,
+    // 16 and 64 lines on 512^2 to 16384 x 2048 fields: 64 fastest or within
+    // 2 % everywhere)
+    hipLaunchKernelGGL(k_felz_envelope<64>, dim3(grid1d((size_t)lines, 64)), dim3(64), 0, s, n, lines, f2, m,
+                       V, Z, D);
     // a line's z in LDS: at most 32767 samples (the entry points' bound), 128 KiB
     const size_t lds = (size_t)n * 4;
     if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_felz_output, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -207,12 +327,12 @@ static void felz_pass(int n, int lines, const float* f2, size_t m, float* out, i
 
 void launch_felzenszwalb_2d(int w, int h, float* dist, void* work, hipStream_t s)
 {
-    const size_t cells = (size_t)w * h, lines = (size_t)(w > h ? w : h);
+    const size_t cells = (size_t)w * h;
     char* p = (char*)work;
     float* td = (float*)p;
     int* V = (int*)(p + felz_part(cells * 4));
-    float* Z = (float*)(p + 2 * felz_part(cells * 4));
-    int* D = (int*)(p + 2 * felz_part(cells * 4) + felz_part((cells + lines) * 4));
+    float* Z = (float*)((char*)V + felz_stack(w, h, felz_vs));
+    int* D = (int*)((char*)Z + felz_stack(w, h, felz_zs));
     {
         // columns: line x reads dist[q*w + x], writes td[x*h + q]
         ProfileScope ps("felzenszwalb_cols", s);

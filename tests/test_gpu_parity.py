@@ -207,13 +207,21 @@ def test_felzenszwalb_float(gpu, oracle, reference, rng, dims):
 def _felz_fields(rng, w, h):
     """Float fields for the Felzenszwalb envelope: random values, small
     integers (many ties), a ramp down each column whose last sample pops a
-    stack deeper than the envelope kernel's LDS ring, and FLT_MAX with zeros."""
+    stack deeper than the envelope kernel's LDS ring, a sawtooth of such
+    ramps (each column its own period and phase: the stack grows past the
+    ring, a zero pops far below it, and the next ramp pushes again over the
+    depths the ring no longer held), and FLT_MAX with zeros."""
     fmax = np.finfo(np.float32).max
     yield rng.uniform(0, 1e4, (h, w)).astype(np.float32)
     yield rng.integers(0, 8, (h, w)).astype(np.float32)
     ramp = (1e5 - 100.0 * np.arange(h, dtype=np.float64))[:, None] * np.ones((1, w))
     ramp[-1, :] = 0.0
     yield ramp.astype(np.float32)
+    period = rng.integers(40, 200, w)
+    r = (np.arange(h)[:, None] + rng.integers(0, 200, w)[None, :]) % period[None, :]
+    saw = 1e5 - 100.0 * r
+    saw[r == period[None, :] - 1] = 0.0
+    yield saw.astype(np.float32)
     yield np.where(rng.random((h, w)) < 0.02, 0.0, fmax).astype(np.float32)
 
 
