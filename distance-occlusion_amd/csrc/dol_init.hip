@@ -129,11 +129,42 @@ __global__ void __launch_bounds__(256) k_init_dist(int w, int xb, const uint32_t
     dist[(size_t)row * w + x] = bit ? (T)0 : max_d;
 }
 
+// The same, 16 bytes of a row a thread (4 float or int32 cells, 8 int16
+// cells, all from one mask word) with one nontemporal store: w a multiple
+// of the cells a thread, dist 16-byte aligned (checked by the launcher).
+template <class T>
+__global__ void __launch_bounds__(256) k_init_dist_vec(int w, int xb, const uint32_t* __restrict__ mask,
+                                                       T* __restrict__ dist, T max_d)
+{
+    constexpr int C = 16 / sizeof(T);
+    static_assert(32 % C == 0, "a thread's cells lie in one mask word");
+    const int row = blockIdx.x / xb;
+    const int x = ((blockIdx.x - row * xb) * 256 + threadIdx.x) * C;
+    if (x >= w) return;
+    const uint32_t bits = mask[(size_t)row * mask_stride(w) + (x >> 5)] >> (x & 31);
+    union {
+        T v[C];
+        uint32_t u[4];
+    } out;
+#pragma unroll
+    for (int c = 0; c < C; c++) out.v[c] = ((bits >> c) & 1u) ? (T)0 : max_d;
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 q = {out.u[0], out.u[1], out.u[2], out.u[3]};
+    __builtin_nontemporal_store(q, (u32x4*)(dist + (size_t)row * w + x));
+}
+
 template <class T>
 void launch_init_dist(int w, int h, const uint32_t* mask, T* dist, T max_d, hipStream_t s)
 {
-    int xb = (w + 255) / 256;
+    constexpr int C = 16 / sizeof(T);
     ProfileScope p("init_distances", s);
+    if (w % C == 0 && ((uintptr_t)dist & 15) == 0) {
+        const int xb = (w / C + 255) / 256;
+        hipLaunchKernelGGL(k_init_dist_vec<T>, dim3((unsigned)(xb * h)), dim3(256), 0, s, w, xb, mask,
+                           dist, max_d);
+        return;
+    }
+    int xb = (w + 255) / 256;
     hipLaunchKernelGGL(k_init_dist<T>, dim3((unsigned)(xb * h)), dim3(256), 0, s, w, xb, mask,
                        dist, max_d);
 }

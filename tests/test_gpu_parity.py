@@ -105,7 +105,10 @@ def test_border_3d_word_rows(gpu, oracle, reference, rng, dims):
 
 
 def test_init_distances(gpu, oracle, rng):
-    for w, h in [(1, 1), (33, 17), (100, 64)]:
+    # widths that are multiples of 4 (and of 8) take the 16-byte-a-thread
+    # kernel for 4-byte (and 2-byte) cells; (2056, 5) leaves threads past
+    # the row's end in its last workgroup
+    for w, h in [(1, 1), (33, 17), (100, 64), (4, 3), (8, 2), (1024, 9), (2056, 5)]:
         m = rand_mask(rng, (w, h), 0.3)
         np.testing.assert_array_equal(gpu.init_distances_f32(w, h, m), oracle.init_distances_f32(w, h, m))
         exp = unpack_mask(m, w, h)
