@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(LPW) k_felz_envelope(int n, int lines, const f
     D[l] = nan ? -1 - k : k;
 }
 
-constexpr int kFelzOutThreads = 512;   // 32 waves a CU beside the line's z in LDS (256: 3.9 % slower on 8192^2; 1024: within 1.5 %)
+constexpr int kFelzOutThreads = 512;   // 32 waves a CU beside the line's z in LDS (256: 8 % slower on 8192^2; 1024: within 1.5 %)
 
 // outputs of line blockIdx.x; zs holds z[1..D] (n floats at most)
 __global__ void __launch_bounds__(kFelzOutThreads) k_felz_output(int n, const float* __restrict__ f2, size_t m,
